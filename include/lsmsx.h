@@ -1,0 +1,112 @@
+/*
+ * lsmsx.h -- C ABI of the MI355X tau00 path: KKR matrix assembly
+ * M = 1 - t G(E) over a local interaction zone and the central-block
+ * inversion tau00 = [M^-1 [t0; 0]]_00, on one GPU.
+ *
+ * Plain pointers and sizes only.  Complex numbers are interleaved
+ * (re, im) doubles, the layout of std::complex<double>; matrices are
+ * column-major (element (i,j) at a[i + j*ld]).  Host buffers are
+ * caller-owned; device memory is owned by the context.
+ *
+ * Return codes: 0 ok; > 0 a LAPACK-style singular-pivot position; < 0 an
+ * argument or HIP error (lsmsx_last_error() describes it).  No entry point
+ * exits the process.  One context serves one host thread at a time.
+ */
+#ifndef LSMSX_H
+#define LSMSX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lsmsx_ctx lsmsx_ctx;
+
+/* precision modes for the solve */
+#define LSMSX_PREC_FP64 0 /* all arithmetic complex FP64 */
+
+/* linear-solver selector ids for the reference's 16-bit
+ * lsms.global.linearSolver field (linearSolvers.hpp:18-27,196-199;
+ * buildKKRMatrix.hpp:22-30): solver bits 0x0fff, build bits 0xf000. */
+#define LSMSX_LINEAR_SOLVER_TAU00_MI355X 0x040
+#define LSMSX_BUILD_KKR_MATRIX_MI355X 0x4000
+
+/* Last error message of the calling thread ("" if none). */
+const char *lsmsx_last_error(void);
+
+/* Create / destroy a context on HIP device `device`.  Owns all device
+ * buffers, like DeviceStorage::allocate (Accelerator/DeviceStorage_HIP.cpp:123-233). */
+int lsmsx_ctx_create(int device, lsmsx_ctx **out);
+int lsmsx_ctx_destroy(lsmsx_ctx *ctx);
+
+/* Angular-momentum tables for maxlmax <= 3: Gaunt numbers cgnt with dims
+ * (lmax+1, kkr, kkr), illp(kkr, kkr) = i^(l-l'), ilp1(0..2lmax) = i^(l+1).
+ * Replaces DeviceConstants::allocate (Accelerator/DeviceStorage_HIP.cpp:402-)
+ * fed from GauntCoeficients::cgnt / IFactors (Misc/Coeficients.hpp:42-106). */
+int lsmsx_set_tables(lsmsx_ctx *ctx, int maxlmax, const double *cgnt,
+                     const double *illp, const double *ilp1);
+
+/* Compute the same tables on the host (independent of any caller). */
+int lsmsx_compute_tables(int maxlmax, double *cgnt, double *illp, double *ilp1);
+
+/* Upload the t-matrix store: tmatStore(r, s) at tmat[2*(r + s*ldTmatStore)],
+ * blocks of blkSizeTmatStore complex per energy slot, groupSize slots.
+ * Replaces DeviceStorage::copyTmatStoreToDevice (DeviceStorage_HIP.cpp:282-309). */
+int lsmsx_upload_tmats(lsmsx_ctx *ctx, const double *tmat, long ldTmatStore,
+                       int numStore, long blkSizeTmatStore, int groupSize);
+
+/* Register the LIZ of local atom `atom_id`: LIZPos (3 x numLIZ, relative to
+ * the central site 0), LIZlmax, LIZStoreIdx (AtomData fields,
+ * SingleSite/AtomData.hpp:441-467).  Replaces DeviceAtom::copyFromAtom
+ * (DeviceStorage_HIP.cpp:380-390). */
+int lsmsx_set_liz(lsmsx_ctx *ctx, int atom_id, int numLIZ, const double *LIZPos,
+                  const int *LIZlmax, const int *LIZStoreIdx);
+
+/* Assemble the KKR matrix of one atom/energy in the reference's site order
+ * into host memory m (nrmat_ns x nrmat_ns, leading dimension ldm).
+ * Replaces buildKKRMatrix / buildKKRMatrixCPU / buildKKRMatrixHip
+ * (MultipleScattering/buildKKRMatrix.hpp:27-42; calculateTauMatrix.cpp:57). */
+int lsmsx_build_kkr(lsmsx_ctx *ctx, int atom_id, int ispin, int iie,
+                    const double *prel, int n_spin_cant, int n_spin_pola,
+                    double *m, int ldm);
+
+/* Solver-level tau00 = [M^-1 [t0;0]]_00 (kkrsz_ns x kkrsz_ns) for one
+ * atom/energy, assembling M on the device.  Replaces the
+ * calculateTauMatrix build+solve pair for solver ids < 0xf00
+ * (calculateTauMatrix.cpp:359-427,619-686; solveTau00zgetrf,
+ * linearSolvers_CPU.cpp:98-123).  The caller then applies
+ * calculateTau00MinusT and the local-frame rotation as today. */
+int lsmsx_tau00(lsmsx_ctx *ctx, int atom_id, int ispin, int iie,
+                const double *prel, int n_spin_cant, int n_spin_pola,
+                int precision, double *tau00, int *info);
+
+/* Batched form: nsolve independent (atom, spin, energy slot) solves in one
+ * grouped launch sequence.  prel holds 2*nsolve doubles; tau00 receives
+ * nsolve blocks of kkrsz_ns^2 complex; info nsolve ints.  Replaces the
+ * OpenMP loop of calculateAllTauMatrices (calculateTauMatrix.cpp:841-945). */
+int lsmsx_tau00_batched(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
+                        const int *ispins, const int *iies, const double *prel,
+                        int n_spin_cant, int n_spin_pola, int precision,
+                        double *tau00, int *info);
+
+/* tau00 from a caller-built matrix m (n x n, central kkrsz_ns block first)
+ * and t0 (k x k).  m is not modified.  Same contract as solveTau00zgetrf
+ * (linearSolvers_CPU.cpp:98-123) with the matrix on the host. */
+int lsmsx_solve_tau00_matrix(lsmsx_ctx *ctx, int n, int k, const double *m,
+                             int ldm, const double *t0, double *tau00, int *info);
+
+/* Workspace cap in bytes for the batched matrix pool (0 = automatic:
+ * a fraction of the free device memory).  Larger batches run in chunks. */
+int lsmsx_set_workspace_limit(lsmsx_ctx *ctx, long bytes);
+
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on
+ * the context's stream; lsmsx_get_profile returns the accumulated time (ms)
+ * and launch count of one kernel kind (0 assemble, 1 diagonal-tile LU,
+ * 2 panel, 3 trailing update, 4 finish).  lsmsx_reset_profile clears them. */
+int lsmsx_set_profiling(lsmsx_ctx *ctx, int on);
+int lsmsx_reset_profile(lsmsx_ctx *ctx);
+int lsmsx_get_profile(lsmsx_ctx *ctx, int kind, double *ms, long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,59 @@
+"""Build the native library lsms_amd/liblsmsx.so for gfx950 with hipcc.
+
+    python -m lsms_amd.build
+
+Objects go to lsms_amd/_build/; the shared library is written in-tree next
+to this file so it travels with the source tree.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "liblsmsx.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_SOURCES = ["kkr_assembly.hip", "tau_lu.hip", "lsmsx_capi.hip"]
+CXX_SOURCES = ["tables.cpp"]
+
+HIP_FLAGS = [
+    "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+    # packed FP32 instructions are kept out of every device code object
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build(verbose=False):
+    os.makedirs(OUT, exist_ok=True)
+    jobs = []
+    for s in HIP_SOURCES:
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(SRC, s), "-o",
+                     os.path.join(OUT, s + ".o")])
+    for s in CXX_SOURCES:
+        jobs.append(["g++", "-O2", "-fPIC", "-std=c++17", "-c", os.path.join(SRC, s), "-o",
+                     os.path.join(OUT, s + ".o")])
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        for r in ex.map(_run, jobs):
+            if verbose and (r.stdout or r.stderr):
+                print(r.stdout, r.stderr, file=sys.stderr)
+    objs = [os.path.join(OUT, s + ".o") for s in HIP_SOURCES + CXX_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

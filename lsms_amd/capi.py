@@ -1,0 +1,181 @@
+"""Python binding of the C ABI in include/lsmsx.h (ctypes).
+
+The native library lsms_amd/liblsmsx.so is the only compute path: if it is
+missing this module raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblsmsx.so")
+
+PREC_FP64 = 0
+KIND_ASSEMBLE, KIND_DIAG, KIND_PANEL, KIND_UPDATE, KIND_FINISH = range(5)
+
+EXPORTED = [
+    "lsmsx_last_error", "lsmsx_ctx_create", "lsmsx_ctx_destroy", "lsmsx_set_tables",
+    "lsmsx_compute_tables", "lsmsx_upload_tmats", "lsmsx_set_liz", "lsmsx_build_kkr",
+    "lsmsx_tau00", "lsmsx_tau00_batched", "lsmsx_solve_tau00_matrix",
+    "lsmsx_set_workspace_limit", "lsmsx_set_profiling", "lsmsx_reset_profile",
+    "lsmsx_get_profile",
+]
+
+_lib = None
+_vp = C.c_void_p
+
+
+def load():
+    """Load liblsmsx.so; raises RuntimeError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"native library missing: {LIB_PATH}; build it with "
+                           "`python -m lsms_amd.build` (no fallback path exists)")
+    L = C.CDLL(LIB_PATH)
+    L.lsmsx_last_error.restype = C.c_char_p
+    L.lsmsx_ctx_create.argtypes = [C.c_int, C.POINTER(_vp)]
+    L.lsmsx_ctx_destroy.argtypes = [_vp]
+    L.lsmsx_set_tables.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
+    L.lsmsx_compute_tables.argtypes = [C.c_int, _vp, _vp, _vp]
+    L.lsmsx_upload_tmats.argtypes = [_vp, _vp, C.c_long, C.c_int, C.c_long, C.c_int]
+    L.lsmsx_set_liz.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp]
+    L.lsmsx_build_kkr.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
+                                  C.c_int]
+    L.lsmsx_tau00.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
+                              _vp, _vp]
+    L.lsmsx_tau00_batched.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
+                                      C.c_int, _vp, _vp]
+    L.lsmsx_solve_tau00_matrix.argtypes = [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]
+    L.lsmsx_set_workspace_limit.argtypes = [_vp, C.c_long]
+    L.lsmsx_set_profiling.argtypes = [_vp, C.c_int]
+    L.lsmsx_reset_profile.argtypes = [_vp]
+    L.lsmsx_get_profile.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+    _lib = L
+    return L
+
+
+class LsmsxError(RuntimeError):
+    pass
+
+
+def _ck(rc):
+    if rc != 0:
+        msg = load().lsmsx_last_error().decode()
+        raise LsmsxError(f"lsmsx error {rc}: {msg}")
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp)
+
+
+def compute_tables(lmax):
+    """Gaunt numbers cgnt (lmax+1, kkr, kkr), illp (kkr, kkr), ilp1 (2lmax+1)."""
+    kkr = (lmax + 1) ** 2
+    cg = np.zeros((lmax + 1, kkr, kkr), order="F")
+    illp = np.zeros((kkr, kkr), dtype=np.complex128, order="F")
+    ilp1 = np.zeros(2 * lmax + 1, dtype=np.complex128)
+    _ck(load().lsmsx_compute_tables(lmax, _p(cg), _p(illp), _p(ilp1)))
+    return cg, illp, ilp1
+
+
+class Context:
+    """One device context: tables, t-matrix store and LIZ data stay resident."""
+
+    def __init__(self, device=0):
+        self._h = _vp()
+        _ck(load().lsmsx_ctx_create(device, C.byref(self._h)))
+        self.nsc = 1
+        self.nspola = 1
+
+    def close(self):
+        if self._h:
+            load().lsmsx_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tables(self, lmax, cgnt=None, illp=None, ilp1=None):
+        if cgnt is None:
+            cgnt, illp, ilp1 = compute_tables(lmax)
+        cg = np.asfortranarray(cgnt, dtype=np.float64)
+        il = np.asfortranarray(illp, dtype=np.complex128)
+        i1 = np.ascontiguousarray(ilp1, dtype=np.complex128)
+        _ck(load().lsmsx_set_tables(self._h, lmax, _p(cg), _p(il), _p(i1)))
+
+    def upload_tmats(self, tmat_store, blk, group):
+        ts = np.asfortranarray(tmat_store, dtype=np.complex128)
+        _ck(load().lsmsx_upload_tmats(self._h, _p(ts), ts.shape[0], ts.shape[1], blk, group))
+
+    def set_liz(self, atom_id, liz_pos, liz_lmax, liz_store):
+        pos = np.asfortranarray(liz_pos, dtype=np.float64)
+        lm = np.ascontiguousarray(liz_lmax, dtype=np.int32)
+        st = np.ascontiguousarray(liz_store, dtype=np.int32)
+        _ck(load().lsmsx_set_liz(self._h, atom_id, pos.shape[1], _p(pos), _p(lm), _p(st)))
+
+    def load_system(self, system):
+        """Upload tables, t-matrix store and every atom's LIZ of a configs.System."""
+        self.set_tables(system.maxlmax)
+        self.upload_tmats(system.tmatStore, system.blkSizeTmatStore, len(system.energies))
+        for a, at in enumerate(system.atoms):
+            self.set_liz(a, at.LIZPos, at.LIZlmax, at.LIZStoreIdx)
+        self.nsc = system.n_spin_cant
+        self.nspola = system.n_spin_pola
+
+    def build_kkr(self, atom_id, iie, prel, nrmat_ns, ispin=0):
+        m = np.zeros((nrmat_ns, nrmat_ns), dtype=np.complex128, order="F")
+        pr = np.array([complex(prel)], dtype=np.complex128)
+        _ck(load().lsmsx_build_kkr(self._h, atom_id, ispin, iie, _p(pr), self.nsc, self.nspola,
+                                   _p(m), nrmat_ns))
+        return m
+
+    def tau00(self, atom_id, iie, prel, kkrsz_ns, ispin=0):
+        tau = np.zeros((kkrsz_ns, kkrsz_ns), dtype=np.complex128, order="F")
+        pr = np.array([complex(prel)], dtype=np.complex128)
+        info = C.c_int(0)
+        _ck(load().lsmsx_tau00(self._h, atom_id, ispin, iie, _p(pr), self.nsc, self.nspola,
+                               PREC_FP64, _p(tau), C.byref(info)))
+        return tau, info.value
+
+    def tau00_batched(self, atom_ids, iies, prels, kkrsz_ns, ispins=None):
+        n = len(atom_ids)
+        a = np.ascontiguousarray(atom_ids, dtype=np.int32)
+        e = np.ascontiguousarray(iies, dtype=np.int32)
+        s = np.zeros(n, dtype=np.int32) if ispins is None else np.ascontiguousarray(ispins, dtype=np.int32)
+        pr = np.ascontiguousarray(prels, dtype=np.complex128)
+        tau = np.zeros((n, kkrsz_ns, kkrsz_ns), dtype=np.complex128)
+        info = np.zeros(n, dtype=np.int32)
+        _ck(load().lsmsx_tau00_batched(self._h, n, _p(a), _p(s), _p(e), _p(pr), self.nsc,
+                                       self.nspola, PREC_FP64, _p(tau), _p(info)))
+        # each block is column-major kkrsz_ns x kkrsz_ns
+        return np.ascontiguousarray(np.transpose(tau, (0, 2, 1))), info
+
+    def solve_tau00_matrix(self, m, t0):
+        mm = np.asfortranarray(m, dtype=np.complex128)
+        tt = np.asfortranarray(t0, dtype=np.complex128)
+        n, k = mm.shape[0], tt.shape[0]
+        tau = np.zeros((k, k), dtype=np.complex128, order="F")
+        info = C.c_int(0)
+        _ck(load().lsmsx_solve_tau00_matrix(self._h, n, k, _p(mm), n, _p(tt), _p(tau),
+                                            C.byref(info)))
+        return tau, info.value
+
+    def set_workspace_limit(self, nbytes):
+        _ck(load().lsmsx_set_workspace_limit(self._h, int(nbytes)))
+
+    def set_profiling(self, on):
+        _ck(load().lsmsx_set_profiling(self._h, 1 if on else 0))
+
+    def reset_profile(self):
+        _ck(load().lsmsx_reset_profile(self._h))
+
+    def profile(self, kind):
+        ms = C.c_double(0)
+        n = C.c_long(0)
+        _ck(load().lsmsx_get_profile(self._h, kind, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,268 @@
+"""Synthetic inputs for the tau00 path: lattices, local interaction zones
+(LIZ), the energy contour and per-species t-matrix stores.
+
+These mirror what the reference's setup code produces for the path:
+  - LIZ: sphere cut around each local atom over periodic images, sorted by
+    distance, central site first (Main/buildLIZandCommLists.cpp:136-190,
+    377-400); per-site lmax reduced by `rsteps` (:393-399);
+  - tmatStore: one column per t-matrix type, blocks of (2 kkrsz)^2 complex
+    per energy slot (buildLIZandCommLists.cpp:553-555);
+  - contour: Gauss-Legendre semicircle (Misc/congauss_c.f:3-124), without the
+    zero-weight extra point the reference appends.
+The t-matrices are synthetic: t_l = -sin(d_l) e^{i d_l} / kappa with seeded
+phase shifts d_l per (species, l), diagonal in (l, m).
+"""
+from dataclasses import dataclass, field
+import numpy as np
+
+C_PHOT = 2.0 * 137.035999206  # include/PhysicalConstants.hpp:5-8
+
+
+def prel_of(e):
+    """kappa = sqrt(E (1 + E/c^2)) (calculateTauMatrix.cpp:846)."""
+    e = np.asarray(e, dtype=np.complex128)
+    return np.sqrt(e * (1.0 + e / (C_PHOT * C_PHOT)))
+
+
+def contour_semicircle(ebot, etop, npts):
+    """Gauss-Legendre semicircle contour, nodes ascending from ebot to etop."""
+    x, w = np.polynomial.legendre.leggauss(npts)
+    erad = 0.5 * (etop - ebot)
+    zz = erad * np.exp(1j * 0.5 * np.pi * (1.0 - x))
+    egrd = 0.5 * (etop + ebot) + zz
+    dele = -1j * 0.5 * np.pi * zz * w
+    return egrd.astype(np.complex128), dele.astype(np.complex128)
+
+
+def lattice_sites(kind, a, ncell):
+    """Cartesian sites of an n1 x n2 x n3 supercell of the conventional cubic
+    cell; returns (positions (nat,3), box lengths (3,))."""
+    if kind == "bcc":
+        basis = np.array([[0, 0, 0], [0.5, 0.5, 0.5]])
+    elif kind == "fcc":
+        basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    elif kind == "sc":
+        basis = np.array([[0, 0, 0]])
+    else:
+        raise ValueError(kind)
+    n1, n2, n3 = ncell
+    pts = []
+    for i in range(n1):
+        for j in range(n2):
+            for k in range(n3):
+                for b in basis:
+                    pts.append((np.array([i, j, k]) + b) * a)
+    return np.array(pts, dtype=np.float64), np.array([n1, n2, n3], dtype=np.float64) * a
+
+
+def build_liz(pos, box, center, rcut, occupied=None):
+    """Sites within rcut of pos[center] over periodic images, sorted by
+    distance (ties broken by x, y, z), central site first.
+    Returns (rel_pos (nliz,3), site_index (nliz,))."""
+    if occupied is None:
+        occupied = np.ones(len(pos), dtype=bool)
+    c = pos[center]
+    nimg = np.ceil(rcut / box).astype(int) + 1
+    out = []
+    r2 = rcut * rcut
+    for i in range(-nimg[0], nimg[0] + 1):
+        for j in range(-nimg[1], nimg[1] + 1):
+            for k in range(-nimg[2], nimg[2] + 1):
+                d = pos + np.array([i, j, k]) * box - c
+                dd = np.einsum("ij,ij->i", d, d)
+                sel = np.nonzero((dd <= r2) & occupied)[0]
+                for s in sel:
+                    out.append((dd[s], d[s, 0], d[s, 1], d[s, 2], s))
+    out.sort(key=lambda t: (round(t[0], 9), round(t[1], 9), round(t[2], 9), round(t[3], 9)))
+    rel = np.array([[t[1], t[2], t[3]] for t in out], dtype=np.float64)
+    idx = np.array([t[4] for t in out], dtype=np.int64)
+    assert np.allclose(rel[0], 0.0), "central site must come first"
+    return rel, idx
+
+
+def liz_lmax(rel, lmax, rsteps=None):
+    """Per-site lmax: one step down per rsteps radius exceeded
+    (buildLIZandCommLists.cpp:393-399)."""
+    d = np.sqrt(np.sum(rel * rel, axis=1))
+    out = np.full(len(rel), lmax, dtype=np.int32)
+    if rsteps is not None:
+        for rs in rsteps:
+            out -= (d > rs).astype(np.int32)
+    return out
+
+
+def tmat_diag(deltas, kappa, lmax):
+    """Diagonal t_lm = -sin(d_l) e^{i d_l} / kappa for one spin channel."""
+    kkr = (lmax + 1) ** 2
+    t = np.zeros(kkr, dtype=np.complex128)
+    for l in range(lmax + 1):
+        d = deltas[l]
+        t[l * l:(l + 1) * (l + 1)] = -np.sin(d) * np.exp(1j * d) / kappa
+    return t
+
+
+@dataclass
+class Atom:
+    """Mirror of the AtomData fields the path reads (SingleSite/AtomData.hpp:441-467)."""
+    numLIZ: int
+    LIZPos: np.ndarray        # (3, numLIZ) column-major, relative to site 0
+    LIZlmax: np.ndarray       # (numLIZ,) int32
+    LIZStoreIdx: np.ndarray   # (numLIZ,) int32
+    lmax: int
+    kkrsz: int
+    nrmat: int
+    evec: np.ndarray = field(default_factory=lambda: np.array([0.0, 0.0, 1.0]))
+
+
+@dataclass
+class System:
+    """A set of local atoms, the energy group and the t-matrix store."""
+    name: str
+    maxlmax: int
+    n_spin_cant: int
+    n_spin_pola: int
+    atoms: list
+    energies: np.ndarray      # (ne,) complex
+    weights: np.ndarray       # (ne,) complex contour weights (zero where not on a contour)
+    tmatStore: np.ndarray     # (lDim, num_store) complex, Fortran order
+    blkSizeTmatStore: int
+    deltas: np.ndarray        # (num_store, nspin, lmax+1) phase shifts
+
+    @property
+    def kkrsz_ns(self):
+        return self.n_spin_cant * (self.maxlmax + 1) ** 2
+
+    def nrmat_ns(self, a):
+        return self.n_spin_cant * self.atoms[a].nrmat
+
+    def prel(self, ie):
+        return complex(prel_of(self.energies[ie]))
+
+
+def make_tmat_store(deltas, energies, lmax, n_spin_cant, n_spin_pola, evecs_by_store=None):
+    """tmatStore(iie*blk + i + j*kkrsz_ns, s) for every store column s and energy.
+    Layout of one block: kkrsz_ns x kkrsz_ns with ld kkrsz_ns for
+    n_spin_pola == n_spin_cant; collinear: the spin-up and spin-down
+    kkrsz x kkrsz blocks one after the other (offset kkrsz^2 * ispin)."""
+    kkr = (lmax + 1) ** 2
+    blk = (2 * kkr) ** 2
+    nstore = deltas.shape[0]
+    ne = len(energies)
+    store = np.zeros((blk * ne, nstore), dtype=np.complex128, order="F")
+    for s in range(nstore):
+        for ie, e in enumerate(energies):
+            kap = complex(prel_of(e))
+            base = ie * blk
+            if n_spin_cant == 1 and n_spin_pola == 1:
+                t = np.diag(tmat_diag(deltas[s, 0], kap, lmax))
+                store[base:base + kkr * kkr, s] = t.flatten(order="F")
+            elif n_spin_cant == 1 and n_spin_pola == 2:
+                for isp in range(2):
+                    t = np.diag(tmat_diag(deltas[s, isp], kap, lmax))
+                    store[base + isp * kkr * kkr:base + (isp + 1) * kkr * kkr, s] = t.flatten(order="F")
+            else:
+                tu = np.diag(tmat_diag(deltas[s, 0], kap, lmax))
+                td = np.diag(tmat_diag(deltas[s, 1], kap, lmax))
+                ev = np.array([0.0, 0.0, 1.0]) if evecs_by_store is None else evecs_by_store[s]
+                tg = local_to_global_t(tu, td, ev)
+                store[base:base + 4 * kkr * kkr, s] = tg.flatten(order="F")
+    return store, blk
+
+
+def spin_rotation(evec):
+    """u, u^dagger for a moment direction (Misc/spin_trafo.f)."""
+    u1 = np.sqrt(0.5 * (1.0 + evec[2])) + 0j
+    if abs(u1) < 1e-8:
+        u = np.array([0.0, 1.0, 1.0, 0.0], dtype=np.complex128)
+    else:
+        u2 = -0.5 * (evec[0] + 1j * evec[1]) / u1
+        u3 = 0.5 * (evec[0] - 1j * evec[1]) / u1
+        u = np.array([u1, u2, u3, u1], dtype=np.complex128)
+    ud = np.array([np.conj(u[0]), np.conj(u[2]), np.conj(u[1]), np.conj(u[3])])
+    return u, ud
+
+
+def local_to_global_t(tu, td, evec):
+    """Global-frame 2kkr x 2kkr t from local spin-diagonal (tu, td):
+    t_g = (U^dagger (x) 1) diag(tu, td) (U (x) 1) with U = [[u1,u3],[u2,u4]],
+    the inverse of the local-frame rotation applied to tau00."""
+    u, ud = spin_rotation(evec)
+    U = np.array([[u[0], u[2]], [u[1], u[3]]])
+    UD = np.array([[ud[0], ud[2]], [ud[1], ud[3]]])
+    kkr = tu.shape[0]
+    tl = np.zeros((2 * kkr, 2 * kkr), dtype=np.complex128)
+    tl[:kkr, :kkr] = tu
+    tl[kkr:, kkr:] = td
+    I = np.eye(kkr)
+    return np.kron(UD, I) @ tl @ np.kron(U, I)
+
+
+def make_system(name, kind, a, ncell, rcut, lmax, energies, weights, nspecies=2,
+                seed=20261202, local_atoms=None, n_spin_cant=1, n_spin_pola=1,
+                vacancy=0.0, rcut_per_atom=None, rsteps=None, delta_range=1.2):
+    """Build a System from a lattice; species are drawn per site (seeded)."""
+    rng = np.random.default_rng(seed)
+    pos, box = lattice_sites(kind, a, ncell)
+    nat = len(pos)
+    if nat <= nspecies:
+        species = np.arange(nat) % nspecies
+    else:
+        species = rng.integers(0, nspecies, nat)
+    occupied = np.ones(nat, dtype=bool)
+    if vacancy > 0:
+        occupied = rng.random(nat) >= vacancy
+    nspin = 2 if (n_spin_pola == 2) else 1
+    deltas = rng.uniform(-delta_range, delta_range, size=(nspecies, nspin, lmax + 1))
+    evecs = None
+    if n_spin_cant == 2:
+        v = rng.normal(size=(nspecies, 3))
+        evecs = v / np.linalg.norm(v, axis=1, keepdims=True)
+    if local_atoms is None:
+        local_atoms = list(range(nat))
+    local_atoms = [i for i in local_atoms if occupied[i]]
+    atoms = []
+    for n, i in enumerate(local_atoms):
+        rc = rcut if rcut_per_atom is None else rcut_per_atom[n % len(rcut_per_atom)]
+        rel, idx = build_liz(pos, box, i, rc, occupied)
+        lm = liz_lmax(rel, lmax, rsteps)
+        ev = np.array([0.0, 0.0, 1.0]) if evecs is None else evecs[species[i]]
+        atoms.append(Atom(numLIZ=len(rel), LIZPos=np.asfortranarray(rel.T),
+                          LIZlmax=lm.astype(np.int32), LIZStoreIdx=species[idx].astype(np.int32),
+                          lmax=lmax, kkrsz=(lmax + 1) ** 2,
+                          nrmat=int(np.sum((lm + 1) ** 2)), evec=ev))
+    store, blk = make_tmat_store(deltas, energies, lmax, n_spin_cant, n_spin_pola, evecs)
+    return System(name=name, maxlmax=lmax, n_spin_cant=n_spin_cant, n_spin_pola=n_spin_pola,
+                  atoms=atoms, energies=np.asarray(energies, dtype=np.complex128),
+                  weights=np.asarray(weights, dtype=np.complex128), tmatStore=store,
+                  blkSizeTmatStore=blk, deltas=deltas)
+
+
+def config_c1(npts=31, seed=20261202):
+    """bcc a=5.42 bohr, 2-atom cubic cell, LIZ r=6.0 (15 sites, N=240), lmax=3,
+    31-point semicircle contour -0.5..0.7 Ry."""
+    e, w = contour_semicircle(-0.5, 0.7, npts)
+    return make_system("c1_bcc15", "bcc", 5.42, (1, 1, 1), 6.0, 3, e, w, nspecies=2, seed=seed)
+
+
+def config_c2(npts=31, seed=20261202, natoms=32):
+    """fcc a=6.82 bohr, 2x2x2 conventional supercell (32 atoms), LIZ r=13.0
+    (135 sites, N=2160), lmax=3, 31-point semicircle contour."""
+    e, w = contour_semicircle(-0.5, 0.7, npts)
+    return make_system("c2_fcc135", "fcc", 6.82, (2, 2, 2), 13.0, 3, e, w, nspecies=2,
+                       seed=seed, local_atoms=list(range(natoms)))
+
+
+def config_c3_energies():
+    """32 semicircle points plus 16 near the real axis: Re E in [-0.4, 0.7],
+    Im E log-spaced 0.001..0.01 Ry (weights zero for the latter)."""
+    e, w = contour_semicircle(-0.5, 0.7, 32)
+    re = np.linspace(-0.4, 0.7, 16)
+    im = np.logspace(-3, -2, 16)
+    e2 = re + 1j * im
+    return np.concatenate([e, e2]), np.concatenate([w, np.zeros(16, dtype=np.complex128)])
+
+
+def config_c3(seed=20261202, natoms=32):
+    e, w = config_c3_energies()
+    return make_system("c3_fcc135_mixed", "fcc", 6.82, (2, 2, 2), 13.0, 3, e, w, nspecies=2,
+                       seed=seed, local_atoms=list(range(natoms)))

@@ -1,0 +1,628 @@
+// lsmsx_capi.hip -- context, device residency and the C ABI of the MI355X
+// tau00 path (declared in include/lsmsx.h).
+//
+// Device residency mirrors the reference's DeviceStorage
+// (Accelerator/DeviceStorage.hpp:56-184): t-matrix store, LIZ geometry and
+// angular-momentum tables are uploaded once and stay on the device; the
+// batched call assembles and factors every (atom, energy) matrix of a chunk
+// without host round trips, and only the kkrsz_ns^2 tau00 blocks come back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/lsmsx.h"
+#include "lsmsx_device.hpp"
+
+namespace lsmsx {
+__global__ void k_assemble(const SolveDesc *, const double2 *, long, const double *,
+                           const double2 *, int);
+__global__ void k_pad_identity(const SolveDesc *);
+__global__ void k_lu_diag(const SolveDesc *, int);
+__global__ void k_lu_panel(const SolveDesc *, int);
+__global__ void k_lu_update(const SolveDesc *, int);
+__global__ void k_lu_finish(const SolveDesc *);
+
+// Copy a caller matrix (central block first, n x n) into the solver layout.
+__global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
+                                                    const double2 *__restrict__ src, long lds,
+                                                    int n, int k) {
+  const SolveDesc &d = descs[0];
+  const int Ns = d.Ns, pad = d.pad;
+  const int c = blockIdx.x;  // solver-layout column
+  if (c >= Ns) return;
+  double2 *col = d.M + d.ld * (long)c;
+  for (int r = threadIdx.x; r < Ns; r += blockDim.x) {
+    double2 v;
+    if (r < pad || c < pad) {
+      v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+    } else {
+      int xr = r - pad, xc = c - pad;
+      int orr = xr < n - k ? xr + k : xr - (n - k);
+      int oc = xc < n - k ? xc + k : xc - (n - k);
+      v = src[orr + lds * oc];
+    }
+    col[r] = v;
+  }
+}
+
+__global__ void k_init_status(const SolveDesc *__restrict__ descs, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    *descs[i].info = 0;
+    *descs[i].minpiv = 1e300;
+  }
+}
+}  // namespace lsmsx
+
+using namespace lsmsx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(-100 - (int)e_, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct AtomDev {
+  int numLIZ = 0;
+  int nrmat = 0;
+  int kkrsz = 0;
+  std::vector<int> lmax, store;
+  double *d_pos = nullptr;
+  int *d_lmax = nullptr, *d_store = nullptr;
+  int *d_off[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [solver layout?][nsc-1]
+};
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int ensure(size_t want) {
+    if (want <= n) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(-100 - (int)e, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    n = want;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+int roundup(int x, int m) { return ((x + m - 1) / m) * m; }
+
+}  // namespace
+
+struct lsmsx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int lmax_tab = -1;
+  DevBuf<double> cgnt;
+  DevBuf<double2> illp;
+  DevBuf<double2> tmat;
+  long ldT = 0, blk = 0;
+  int nstore = 0, group = 0;
+  std::map<int, AtomDev> atoms;
+  // batch pool
+  DevBuf<double2> M, inv, tau, tmpM, t0buf;
+  DevBuf<int> perm, info;
+  DevBuf<double> minpiv;
+  DevBuf<SolveDesc> desc;
+  long ws_limit = 0;
+  // profiling
+  bool profiling = false;
+  double prof_ms[5] = {0, 0, 0, 0, 0};
+  long prof_n[5] = {0, 0, 0, 0, 0};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+hipEvent_t get_event(lsmsx_ctx *c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct ProfScope {
+  lsmsx_ctx *c;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(lsmsx_ctx *c_, int k) : c(c_), kind(k) {
+    if (c->profiling) {
+      a = get_event(c);
+      b = get_event(c);
+      if (a) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~ProfScope() {
+    if (c->profiling && a && b) {
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back({kind, {a, b}});
+    }
+  }
+};
+
+int drain_profile(lsmsx_ctx *c) {
+  for (auto &p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+      c->prof_ms[p.first] += ms;
+      c->prof_n[p.first] += 1;
+    }
+    c->event_pool.push_back(p.second.first);
+    c->event_pool.push_back(p.second.second);
+  }
+  c->pending.clear();
+  return 0;
+}
+
+int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(-100 - (int)e, std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+// Offsets of the site blocks: reference order (site 0 first, no padding) or
+// solver order (padding, sites 1..n-1, site 0 last).
+std::vector<int> site_offsets(const AtomDev &a, int nsc, bool solver) {
+  std::vector<int> off(a.numLIZ);
+  int N = nsc * a.nrmat;
+  if (!solver) {
+    int o = 0;
+    for (int s = 0; s < a.numLIZ; s++) {
+      off[s] = o;
+      o += nsc * (a.lmax[s] + 1) * (a.lmax[s] + 1);
+    }
+  } else {
+    int pad = roundup(N, kTile) - N;
+    int o = pad;
+    for (int s = 1; s < a.numLIZ; s++) {
+      off[s] = o;
+      o += nsc * (a.lmax[s] + 1) * (a.lmax[s] + 1);
+    }
+    off[0] = o;
+  }
+  return off;
+}
+
+// Launch the blocked LU over a chunk of nb solves already in solver layout.
+int launch_lu(lsmsx_ctx *c, int nb, int nt_max) {
+  SolveDesc *dd = c->desc.p;
+  for (int s = 0; s < nt_max - 1; s++) {
+    int nrest = nt_max - 1 - s;
+    {
+      ProfScope p(c, 1);
+      hipLaunchKernelGGL(k_lu_diag, dim3(nb), dim3(256), 0, c->stream, dd, s);
+    }
+    if (int r = check_launch("k_lu_diag")) return r;
+    {
+      ProfScope p(c, 2);
+      hipLaunchKernelGGL(k_lu_panel, dim3(2 * nrest, nb), dim3(256), 0, c->stream, dd, s);
+    }
+    if (int r = check_launch("k_lu_panel")) return r;
+    {
+      ProfScope p(c, 3);
+      hipLaunchKernelGGL(k_lu_update, dim3(nrest * nrest, nb), dim3(256), 0, c->stream, dd, s);
+    }
+    if (int r = check_launch("k_lu_update")) return r;
+  }
+  {
+    ProfScope p(c, 4);
+    hipLaunchKernelGGL(k_lu_finish, dim3(nb), dim3(256), 0, c->stream, dd);
+  }
+  return check_launch("k_lu_finish");
+}
+
+int validate_common(lsmsx_ctx *c, int nsc, int nspola) {
+  if (!c) return fail(-1, "null context");
+  if (c->lmax_tab < 0) return fail(-2, "tables not set");
+  if (!c->tmat.p) return fail(-3, "t-matrix store not uploaded");
+  if (nsc != 1 && nsc != 2) return fail(-4, "n_spin_cant must be 1 or 2");
+  if (nspola != 1 && nspola != 2) return fail(-4, "n_spin_pola must be 1 or 2");
+  if (nspola < nsc) return fail(-4, "n_spin_pola < n_spin_cant");
+  return 0;
+}
+
+// Batched solve of nsolve (atom, spin, slot) problems; results to host.
+int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins,
+                const int *iies, const double *prel, int nsc, int nspola, double *tau_out,
+                int *info_out) {
+  if (int r = validate_common(c, nsc, nspola)) return r;
+  if (nsolve <= 0) return 0;
+  HIPCHK(hipSetDevice(c->device));
+  // per-solve sizes
+  std::vector<long> bytes(nsolve);
+  int kc = -1;
+  for (int i = 0; i < nsolve; i++) {
+    auto it = c->atoms.find(atom_ids[i]);
+    if (it == c->atoms.end()) return fail(-5, "unknown atom id " + std::to_string(atom_ids[i]));
+    const AtomDev &a = it->second;
+    int kci = nsc * a.kkrsz;
+    if (kc < 0) kc = kci;
+    if (kci != kc) return fail(-6, "all atoms of a batch must share kkrsz");
+    if (kc > kTile) return fail(-6, "kkrsz_ns > 64 not supported");
+    if (iies[i] < 0 || iies[i] >= c->group) return fail(-7, "energy slot out of range");
+    if (ispins[i] < 0 || ispins[i] >= nspola) return fail(-7, "spin index out of range");
+    for (int s = 0; s < a.numLIZ; s++)
+      if (a.store[s] < 0 || a.store[s] >= c->nstore) return fail(-7, "LIZStoreIdx out of range");
+    long Ns = roundup(nsc * a.nrmat, kTile);
+    bytes[i] = Ns * Ns;
+  }
+  long cap = c->ws_limit;
+  if (cap <= 0) {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    cap = (long)((double)(fr + c->M.n * sizeof(double2)) * 0.7);
+  }
+  long cap_elems = cap / (long)sizeof(double2);
+  int done = 0;
+  while (done < nsolve) {
+    // chunk [done, end)
+    long acc = 0;
+    int end = done;
+    while (end < nsolve && (end == done || acc + bytes[end] <= cap_elems)) acc += bytes[end++];
+    int nb = end - done;
+    if (int r = c->M.ensure(acc)) return r;
+    if (int r = c->inv.ensure((size_t)nb * kTile * kTile)) return r;
+    if (int r = c->perm.ensure((size_t)nb * kTile)) return r;
+    if (int r = c->info.ensure((size_t)nb)) return r;
+    if (int r = c->minpiv.ensure((size_t)nb)) return r;
+    if (int r = c->tau.ensure((size_t)nb * kc * kc)) return r;
+    if (int r = c->desc.ensure((size_t)nb)) return r;
+    std::vector<SolveDesc> hd(nb);
+    long moff = 0;
+    int maxliz = 1, nt_max = 1;
+    for (int b = 0; b < nb; b++) {
+      int i = done + b;
+      AtomDev &a = c->atoms[atom_ids[i]];
+      SolveDesc &d = hd[b];
+      int N = nsc * a.nrmat, Ns = roundup(N, kTile);
+      d.pos = a.d_pos;
+      d.lmax = a.d_lmax;
+      d.store = a.d_store;
+      d.site_off = a.d_off[1][nsc - 1];
+      d.numLIZ = a.numLIZ;
+      d.n_spin_cant = nsc;
+      d.kkrsz = a.kkrsz;
+      d.kc = kc;
+      d.prel_re = prel[2 * i];
+      d.prel_im = prel[2 * i + 1];
+      long jsm = (nspola != nsc) ? (long)kc * kc * ispins[i] : 0;
+      d.t_off = (long)iies[i] * c->blk + jsm;
+      d.ldt = kc;
+      d.M = c->M.p + moff;
+      d.ld = Ns;
+      d.Ns = Ns;
+      d.pad = Ns - N;
+      d.nt = Ns / kTile;
+      d.inv = c->inv.p + (long)b * kTile * kTile;
+      d.perm = c->perm.p + (long)b * kTile;
+      d.info = c->info.p + b;
+      d.minpiv = c->minpiv.p + b;
+      d.tau00 = c->tau.p + (long)b * kc * kc;
+      d.t0 = c->tmat.p + c->ldT * a.store[0] + d.t_off;
+      moff += (long)Ns * Ns;
+      maxliz = std::max(maxliz, a.numLIZ);
+      nt_max = std::max(nt_max, d.nt);
+    }
+    HIPCHK(hipMemcpyAsync(c->desc.p, hd.data(), nb * sizeof(SolveDesc), hipMemcpyHostToDevice,
+                          c->stream));
+    hipLaunchKernelGGL(k_init_status, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->desc.p, nb);
+    if (int r = check_launch("k_init_status")) return r;
+    {
+      ProfScope p(c, 0);
+      hipLaunchKernelGGL(k_pad_identity, dim3(nb), dim3(256), 0, c->stream, c->desc.p);
+      hipLaunchKernelGGL(k_assemble, dim3(maxliz, nb), dim3(256), 0, c->stream, c->desc.p,
+                         c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
+    }
+    if (int r = check_launch("k_assemble")) return r;
+    if (int r = launch_lu(c, nb, nt_max)) return r;
+    HIPCHK(hipMemcpyAsync(tau_out + (long)done * kc * kc * 2, c->tau.p,
+                          (size_t)nb * kc * kc * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    std::vector<int> hinfo(nb);
+    HIPCHK(hipMemcpyAsync(hinfo.data(), c->info.p, nb * sizeof(int), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_profile(c);
+    for (int b = 0; b < nb; b++) info_out[done + b] = hinfo[b];
+    done = end;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lsmsx_last_error(void) { return g_err.c_str(); }
+
+int lsmsx_ctx_create(int device, lsmsx_ctx **out) {
+  if (!out) return fail(-1, "null out pointer");
+  *out = nullptr;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(-1, "invalid device index");
+  HIPCHK(hipSetDevice(device));
+  lsmsx_ctx *c = new lsmsx_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(-100 - (int)e, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return 0;
+}
+
+int lsmsx_ctx_destroy(lsmsx_ctx *c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  drain_profile(c);
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  for (auto &kv : c->atoms) {
+    AtomDev &a = kv.second;
+    (void)hipFree(a.d_pos);
+    (void)hipFree(a.d_lmax);
+    (void)hipFree(a.d_store);
+    for (int l = 0; l < 2; l++)
+      for (int s = 0; s < 2; s++) (void)hipFree(a.d_off[l][s]);
+  }
+  c->cgnt.release();
+  c->illp.release();
+  c->tmat.release();
+  c->M.release();
+  c->inv.release();
+  c->tau.release();
+  c->tmpM.release();
+  c->t0buf.release();
+  c->perm.release();
+  c->info.release();
+  c->minpiv.release();
+  c->desc.release();
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int lsmsx_set_tables(lsmsx_ctx *c, int maxlmax, const double *cgnt, const double *illp,
+                     const double *ilp1) {
+  (void)ilp1;
+  if (!c) return fail(-1, "null context");
+  if (maxlmax < 0 || maxlmax > kMaxLmax) return fail(-2, "maxlmax must be in 0..3");
+  HIPCHK(hipSetDevice(c->device));
+  int kkr = (maxlmax + 1) * (maxlmax + 1);
+  if (int r = c->cgnt.ensure((size_t)(maxlmax + 1) * kkr * kkr)) return r;
+  if (int r = c->illp.ensure((size_t)kkr * kkr)) return r;
+  HIPCHK(hipMemcpy(c->cgnt.p, cgnt, sizeof(double) * (maxlmax + 1) * kkr * kkr,
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->illp.p, illp, sizeof(double2) * kkr * kkr, hipMemcpyHostToDevice));
+  c->lmax_tab = maxlmax;
+  return 0;
+}
+
+int lsmsx_upload_tmats(lsmsx_ctx *c, const double *tmat, long ldT, int numStore, long blk,
+                       int groupSize) {
+  if (!c) return fail(-1, "null context");
+  if (ldT < blk * groupSize || numStore <= 0 || blk <= 0 || groupSize <= 0)
+    return fail(-2, "inconsistent t-matrix store dimensions");
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->tmat.ensure((size_t)ldT * numStore)) return r;
+  HIPCHK(hipMemcpy(c->tmat.p, tmat, sizeof(double2) * ldT * numStore, hipMemcpyHostToDevice));
+  c->ldT = ldT;
+  c->blk = blk;
+  c->nstore = numStore;
+  c->group = groupSize;
+  return 0;
+}
+
+int lsmsx_set_liz(lsmsx_ctx *c, int atom_id, int numLIZ, const double *pos, const int *lmax,
+                  const int *store) {
+  if (!c) return fail(-1, "null context");
+  if (numLIZ <= 0) return fail(-2, "numLIZ must be positive");
+  HIPCHK(hipSetDevice(c->device));
+  AtomDev &a = c->atoms[atom_id];
+  (void)hipFree(a.d_pos);
+  (void)hipFree(a.d_lmax);
+  (void)hipFree(a.d_store);
+  for (int l = 0; l < 2; l++)
+    for (int s = 0; s < 2; s++) {
+      (void)hipFree(a.d_off[l][s]);
+      a.d_off[l][s] = nullptr;
+    }
+  a.numLIZ = numLIZ;
+  a.lmax.assign(lmax, lmax + numLIZ);
+  a.store.assign(store, store + numLIZ);
+  a.nrmat = 0;
+  for (int s = 0; s < numLIZ; s++) {
+    if (lmax[s] < 0 || lmax[s] > kMaxLmax) return fail(-2, "LIZlmax must be in 0..3");
+    if (lmax[s] > lmax[0]) return fail(-2, "LIZlmax exceeds the central site's lmax");
+    a.nrmat += (lmax[s] + 1) * (lmax[s] + 1);
+  }
+  a.kkrsz = (lmax[0] + 1) * (lmax[0] + 1);
+  HIPCHK(hipMalloc((void **)&a.d_pos, sizeof(double) * 3 * numLIZ));
+  HIPCHK(hipMalloc((void **)&a.d_lmax, sizeof(int) * numLIZ));
+  HIPCHK(hipMalloc((void **)&a.d_store, sizeof(int) * numLIZ));
+  HIPCHK(hipMemcpy(a.d_pos, pos, sizeof(double) * 3 * numLIZ, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(a.d_lmax, lmax, sizeof(int) * numLIZ, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(a.d_store, store, sizeof(int) * numLIZ, hipMemcpyHostToDevice));
+  for (int l = 0; l < 2; l++)
+    for (int s = 0; s < 2; s++) {
+      std::vector<int> off = site_offsets(a, s + 1, l == 1);
+      HIPCHK(hipMalloc((void **)&a.d_off[l][s], sizeof(int) * numLIZ));
+      HIPCHK(hipMemcpy(a.d_off[l][s], off.data(), sizeof(int) * numLIZ, hipMemcpyHostToDevice));
+    }
+  return 0;
+}
+
+int lsmsx_build_kkr(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *prel, int nsc,
+                    int nspola, double *m, int ldm) {
+  if (int r = validate_common(c, nsc, nspola)) return r;
+  auto it = c->atoms.find(atom_id);
+  if (it == c->atoms.end()) return fail(-5, "unknown atom id");
+  AtomDev &a = it->second;
+  if (iie < 0 || iie >= c->group) return fail(-7, "energy slot out of range");
+  if (ispin < 0 || ispin >= nspola) return fail(-7, "spin index out of range");
+  int N = nsc * a.nrmat;
+  if (ldm < N) return fail(-2, "ldm < nrmat_ns");
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->M.ensure((size_t)N * N)) return r;
+  if (int r = c->desc.ensure(1)) return r;
+  SolveDesc d{};
+  int kc = nsc * a.kkrsz;
+  d.pos = a.d_pos;
+  d.lmax = a.d_lmax;
+  d.store = a.d_store;
+  d.site_off = a.d_off[0][nsc - 1];
+  d.numLIZ = a.numLIZ;
+  d.n_spin_cant = nsc;
+  d.kkrsz = a.kkrsz;
+  d.kc = kc;
+  d.prel_re = prel[0];
+  d.prel_im = prel[1];
+  d.t_off = (long)iie * c->blk + ((nspola != nsc) ? (long)kc * kc * ispin : 0);
+  d.ldt = kc;
+  d.M = c->M.p;
+  d.ld = N;
+  d.Ns = N;
+  d.pad = 0;
+  d.nt = 0;
+  HIPCHK(hipMemcpyAsync(c->desc.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+  {
+    ProfScope p(c, 0);
+    hipLaunchKernelGGL(k_assemble, dim3(a.numLIZ, 1), dim3(256), 0, c->stream, c->desc.p,
+                       c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
+  }
+  if (int r = check_launch("k_assemble")) return r;
+  HIPCHK(hipMemcpy2DAsync(m, sizeof(double2) * ldm, c->M.p, sizeof(double2) * N,
+                          sizeof(double2) * N, N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drain_profile(c);
+  return 0;
+}
+
+int lsmsx_tau00(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *prel, int nsc,
+                int nspola, int precision, double *tau00, int *info) {
+  if (precision != LSMSX_PREC_FP64) return fail(-8, "unsupported precision mode");
+  int inf = 0;
+  int r = run_batched(c, 1, &atom_id, &ispin, &iie, prel, nsc, nspola, tau00, &inf);
+  if (info) *info = inf;
+  return r;
+}
+
+int lsmsx_tau00_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins,
+                        const int *iies, const double *prel, int nsc, int nspola, int precision,
+                        double *tau00, int *info) {
+  if (precision != LSMSX_PREC_FP64) return fail(-8, "unsupported precision mode");
+  return run_batched(c, nsolve, atom_ids, ispins, iies, prel, nsc, nspola, tau00, info);
+}
+
+int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ldm,
+                             const double *t0, double *tau00, int *info) {
+  if (!c) return fail(-1, "null context");
+  if (k <= 0 || k > kTile || n < k || ldm < n) return fail(-2, "bad matrix dimensions");
+  HIPCHK(hipSetDevice(c->device));
+  int Ns = roundup(n, kTile);
+  if (int r = c->tmpM.ensure((size_t)n * n)) return r;
+  if (int r = c->M.ensure((size_t)Ns * Ns)) return r;
+  if (int r = c->inv.ensure((size_t)kTile * kTile)) return r;
+  if (int r = c->perm.ensure(kTile)) return r;
+  if (int r = c->info.ensure(1)) return r;
+  if (int r = c->minpiv.ensure(1)) return r;
+  if (int r = c->tau.ensure((size_t)k * k)) return r;
+  if (int r = c->t0buf.ensure((size_t)k * k)) return r;
+  if (int r = c->desc.ensure(1)) return r;
+  HIPCHK(hipMemcpy2DAsync(c->tmpM.p, sizeof(double2) * n, m, sizeof(double2) * ldm,
+                          sizeof(double2) * n, n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->t0buf.p, t0, sizeof(double2) * k * k, hipMemcpyHostToDevice,
+                        c->stream));
+  SolveDesc d{};
+  d.kc = k;
+  d.ldt = k;
+  d.M = c->M.p;
+  d.ld = Ns;
+  d.Ns = Ns;
+  d.pad = Ns - n;
+  d.nt = Ns / kTile;
+  d.inv = c->inv.p;
+  d.perm = c->perm.p;
+  d.info = c->info.p;
+  d.minpiv = c->minpiv.p;
+  d.tau00 = c->tau.p;
+  d.t0 = c->t0buf.p;
+  HIPCHK(hipMemcpyAsync(c->desc.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_init_status, dim3(1), dim3(64), 0, c->stream, c->desc.p, 1);
+  hipLaunchKernelGGL(k_permute_in, dim3(Ns), dim3(256), 0, c->stream, c->desc.p, c->tmpM.p,
+                     (long)n, n, k);
+  if (int r = check_launch("k_permute_in")) return r;
+  if (int r = launch_lu(c, 1, d.nt)) return r;
+  int hinfo = 0;
+  HIPCHK(hipMemcpyAsync(tau00, c->tau.p, sizeof(double2) * k * k, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(&hinfo, c->info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drain_profile(c);
+  if (info) *info = hinfo;
+  return 0;
+}
+
+int lsmsx_set_workspace_limit(lsmsx_ctx *c, long bytes) {
+  if (!c) return fail(-1, "null context");
+  c->ws_limit = bytes;
+  return 0;
+}
+
+int lsmsx_set_profiling(lsmsx_ctx *c, int on) {
+  if (!c) return fail(-1, "null context");
+  c->profiling = on != 0;
+  return 0;
+}
+
+int lsmsx_reset_profile(lsmsx_ctx *c) {
+  if (!c) return fail(-1, "null context");
+  for (int i = 0; i < 5; i++) {
+    c->prof_ms[i] = 0;
+    c->prof_n[i] = 0;
+  }
+  return 0;
+}
+
+int lsmsx_get_profile(lsmsx_ctx *c, int kind, double *ms, long *launches) {
+  if (!c) return fail(-1, "null context");
+  if (kind < 0 || kind > 4) return fail(-2, "bad kernel kind");
+  if (ms) *ms = c->prof_ms[kind];
+  if (launches) *launches = c->prof_n[kind];
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,46 @@
+// lsmsx_device.hpp -- shared device-side types of the MI355X tau00 path.
+//
+// Matrix layout used by the solver ("solver layout"): an Ns x Ns column-major
+// complex FP64 matrix, Ns = roundup(N, 64).  The first pad = Ns - N rows and
+// columns hold an identity block (eliminated trivially), then the LIZ sites
+// 1..numLIZ-1 in LIZ order, then the central site 0 last, so the central
+// kkrsz_ns x kkrsz_ns block is the trailing corner of the last 64-tile and
+// tau00 comes from the Schur complement of everything before it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lsmsx {
+
+constexpr int kTile = 64;     // LU tile / panel width
+constexpr int kMaxLmax = 3;   // tables and per-site blocks sized for lmax <= 3
+constexpr int kMaxKkr = (kMaxLmax + 1) * (kMaxLmax + 1);
+
+// One solve (local atom, spin, energy slot) of a batch.
+struct SolveDesc {
+  // LIZ of the atom (device pointers)
+  const double *pos;      // 3 x numLIZ, relative to site 0
+  const int *lmax;        // numLIZ
+  const int *store;       // numLIZ: t-matrix store column of each site
+  const int *site_off;    // numLIZ: first row/col of each site's block in M
+  int numLIZ;
+  int n_spin_cant;
+  int kkrsz;              // (maxlmax+1)^2 of the central atom
+  int kc;                 // kkrsz_ns: central block size
+  double prel_re, prel_im;
+  long t_off;             // offset of this energy slot (and collinear spin block) in a tmatStore column
+  int ldt;                // leading dimension of one t block (n_spin_cant * kkrsz)
+  // matrix and solver state
+  double2 *M;
+  int ld;                 // leading dimension of M
+  int Ns;                 // matrix dimension in this layout
+  int pad;                // identity rows/cols at the front
+  int nt;                 // Ns / kTile (solver layout)
+  double2 *inv;           // kTile x kTile: L11^-1 (strict lower) and U11^-1 (upper)
+  int *perm;              // kTile: row permutation of the current diagonal tile
+  int *info;              // first singular pivot (1-based, solver order) or 0
+  double *minpiv;         // smallest |pivot| seen
+  double2 *tau00;         // kc x kc output
+  const double2 *t0;      // central t-matrix (kc x kc, leading dim ldt)
+};
+
+}  // namespace lsmsx

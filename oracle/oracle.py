@@ -1,0 +1,193 @@
+"""ctypes bindings to the CPU oracle (oracle/liblsms_oracle.so).
+
+TEST INFRASTRUCTURE ONLY.  The oracle is the checker for the HIP path: only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module.  The product package (lsms_amd) never imports it.
+
+Each wrapper names the reference routine its C function restates; the C
+source carries the file:line citations.
+"""
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblsms_oracle.so")
+_lib = None
+
+_zp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="F_CONTIGUOUS")
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+
+
+class _Z(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+def _z(v):
+    v = complex(v)
+    return _Z(v.real, v.imag)
+
+
+def lib():
+    """Load the oracle library (build it with `make -C oracle`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run make -C oracle)")
+    L = C.CDLL(_LIB_PATH)
+    L.or_prel.restype = _Z
+    L.or_prel.argtypes = [_Z]
+    L.or_gauss_legendre.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp]
+    L.or_plm_normalized.argtypes = [C.c_int, C.c_double, _dp]
+    L.or_cgaunt.argtypes = [C.c_int, _dp]
+    L.or_ifacts.argtypes = [C.c_int, _zp, _zp]
+    L.or_hankel.argtypes = [_Z, C.c_double, C.c_int, _zp]
+    L.or_makegij.argtypes = [C.c_int, C.c_int, C.c_int, _Z, _dp, _dp, _zp, _zp, _zp]
+    L.or_build_kkr.restype = C.c_int
+    L.or_build_kkr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip,
+                               _zp, C.c_long, C.c_long, C.c_int, _Z, _zp, C.c_int]
+    L.or_t0.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _zp, C.c_long,
+                        C.c_long, C.c_int, _zp]
+    L.or_zgetrf.restype = C.c_int
+    L.or_zgetrf.argtypes = [C.c_int, C.c_int, _zp, C.c_int, _ip]
+    L.or_zgetrs.argtypes = [C.c_int, C.c_int, _zp, C.c_int, _ip, _zp, C.c_int]
+    L.or_solve_tau00_zgetrf.restype = C.c_int
+    L.or_solve_tau00_zgetrf.argtypes = [C.c_int, C.c_int, _zp, C.c_int, _zp, _zp]
+    L.or_solve_tau00_zblocklu.restype = C.c_int
+    L.or_solve_tau00_zblocklu.argtypes = [C.c_int, C.c_int, C.c_int, _zp, C.c_int, _zp, _zp]
+    L.or_tau00_minus_t.argtypes = [C.c_int, _zp, _zp, _zp]
+    L.or_spin_trafo.argtypes = [_dp, _zp, _zp]
+    L.or_trgtol.argtypes = [C.c_int, C.c_int, _zp, _zp, _zp, _zp]
+    L.or_congauss.argtypes = [C.c_double, C.c_double, C.c_int, _zp, _zp]
+    _lib = L
+    return L
+
+
+def _f(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def prel(energy):
+    """kappa = sqrt(E(1+E/c^2)) (calculateTauMatrix.cpp:846)."""
+    r = lib().or_prel(_z(energy))
+    return complex(r.re, r.im)
+
+
+def gauss_legendre(n, x1=-1.0, x2=1.0):
+    x = np.zeros(n, order="F")
+    w = np.zeros(n, order="F")
+    lib().or_gauss_legendre(n, x1, x2, x, w)
+    return x, w
+
+
+def plm_normalized(lmax, x):
+    p = np.zeros((lmax + 1) * (lmax + 2) // 2, order="F")
+    lib().or_plm_normalized(lmax, float(x), p)
+    return p
+
+
+def cgaunt(lmax):
+    """cgnt(l3/2, j1, j2) as an array of shape (lmax+1, kkr, kkr)."""
+    kkr = (lmax + 1) ** 2
+    c = np.zeros((lmax + 1, kkr, kkr), order="F")
+    lib().or_cgaunt(lmax, c)
+    return c
+
+
+def ifacts(lmax):
+    kkr = (lmax + 1) ** 2
+    illp = np.zeros((kkr, kkr), dtype=np.complex128, order="F")
+    ilp1 = np.zeros(2 * lmax + 1, dtype=np.complex128, order="F")
+    lib().or_ifacts(lmax, illp, ilp1)
+    return illp, ilp1
+
+
+def hankel(prel_, r, lend):
+    h = np.zeros(max(lend, 1) + 1, dtype=np.complex128, order="F")
+    lib().or_hankel(_z(prel_), float(r), lend, h)
+    return h[: lend + 1]
+
+
+def makegij(lmaxi, lmaxj, lmax_tab, prel_, rij, cgnt=None, illp=None, ilp1=None):
+    if cgnt is None:
+        cgnt = cgaunt(lmax_tab)
+    if illp is None:
+        illp, ilp1 = ifacts(lmax_tab)
+    g = np.zeros(((lmaxi + 1) ** 2, (lmaxj + 1) ** 2), dtype=np.complex128, order="F")
+    lib().or_makegij(lmaxi, lmaxj, lmax_tab, _z(prel_), _f(rij, np.float64),
+                     _f(cgnt, np.float64), _f(illp, np.complex128),
+                     _f(ilp1, np.complex128), g)
+    return g
+
+
+def build_kkr(maxlmax, liz_pos, liz_lmax, liz_store, tmat_store, blk_size, iie, prel_,
+              n_spin_cant=1, n_spin_pola=1, ispin=0):
+    """KKR matrix M = 1 - t G, F77 route (calculateTauMatrix.cpp:57-315)."""
+    liz_pos = _f(liz_pos, np.float64)
+    liz_lmax = _f(liz_lmax, np.int32)
+    liz_store = _f(liz_store, np.int32)
+    ts = _f(tmat_store, np.complex128)
+    n = n_spin_cant * int(np.sum((liz_lmax + 1) ** 2))
+    m = np.zeros((n, n), dtype=np.complex128, order="F")
+    lib().or_build_kkr(maxlmax, n_spin_cant, n_spin_pola, ispin, liz_pos.shape[1], liz_pos,
+                       liz_lmax, liz_store, ts, ts.shape[0], blk_size, iie, _z(prel_), m, n)
+    return m
+
+
+def t0(maxlmax, store0, tmat_store, blk_size, iie, n_spin_cant=1, n_spin_pola=1, ispin=0):
+    k = n_spin_cant * (maxlmax + 1) ** 2
+    ts = _f(tmat_store, np.complex128)
+    out = np.zeros((k, k), dtype=np.complex128, order="F")
+    lib().or_t0(maxlmax, n_spin_cant, n_spin_pola, ispin, store0, ts, ts.shape[0], blk_size,
+                iie, out)
+    return out
+
+
+def zgetrf(a):
+    a = np.array(a, dtype=np.complex128, order="F", copy=True)
+    m, n = a.shape
+    ipiv = np.zeros(min(m, n), dtype=np.int32, order="F")
+    info = lib().or_zgetrf(m, n, a, m, ipiv)
+    return a, ipiv, info
+
+
+def solve_tau00_zgetrf(m, t0_):
+    """tau00 = [M^-1 [t0;0]]_00 (linearSolvers_CPU.cpp:98-123)."""
+    a = np.array(m, dtype=np.complex128, order="F", copy=True)
+    k = t0_.shape[0]
+    tau = np.zeros((k, k), dtype=np.complex128, order="F")
+    info = lib().or_solve_tau00_zgetrf(a.shape[0], k, a, a.shape[0], _f(t0_, np.complex128), tau)
+    return tau, info
+
+
+def solve_tau00_zblocklu(m, t0_, nblk=3):
+    """Block-inverse route (zblock_lu_cpp.cpp, linearSolvers_CPU.cpp:261-331)."""
+    a = np.array(m, dtype=np.complex128, order="F", copy=True)
+    k = t0_.shape[0]
+    tau = np.zeros((k, k), dtype=np.complex128, order="F")
+    info = lib().or_solve_tau00_zblocklu(a.shape[0], k, nblk, a, a.shape[0],
+                                         _f(t0_, np.complex128), tau)
+    return tau, info
+
+
+def spin_trafo(evec):
+    u = np.zeros(4, dtype=np.complex128, order="F")
+    ud = np.zeros(4, dtype=np.complex128, order="F")
+    lib().or_spin_trafo(_f(evec, np.float64), u, ud)
+    return u, ud
+
+
+def trgtol(nr, nc, u, ud, mg):
+    ml = np.zeros((nr, nc, 4), dtype=np.complex128, order="F")
+    lib().or_trgtol(nr, nc, _f(u, np.complex128), _f(ud, np.complex128),
+                    _f(mg, np.complex128), ml)
+    return ml
+
+
+def congauss(ebot, etop, ngauss):
+    e = np.zeros(ngauss, dtype=np.complex128, order="F")
+    w = np.zeros(ngauss, dtype=np.complex128, order="F")
+    lib().or_congauss(ebot, etop, ngauss, e, w)
+    return e, w

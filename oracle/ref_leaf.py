@@ -1,0 +1,161 @@
+"""ctypes bindings to oracle/_ref/liblsms_ref.so: the reference's own leaf
+routines of the tau00 path (makegij_c.f, setgij.f, cgaunt_c.f90, gaunt.f,
+ifacts_c.f, congauss_c.f, associatedLegendreFunction.cpp, quadrature.cpp),
+compiled from their sources by `make -C oracle ref`.
+
+TEST INFRASTRUCTURE ONLY: used to pin the oracle's restatement against the
+reference.  The library exists only where the reference tree was available
+at build time; callers skip when `available()` is False.
+
+Fortran character arguments carry a hidden trailing length (size_t).
+"""
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_ref", "liblsms_ref.so")
+_lib = None
+
+_zp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="F_CONTIGUOUS")
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+_ISTOP = C.create_string_buffer(b" " * 32, 33)
+
+
+def available():
+    return os.path.exists(_LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(_LIB_PATH)
+    return _lib
+
+
+def _i(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _d(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def _zarr(*vals):
+    return np.asfortranarray(np.array(vals, dtype=np.complex128))
+
+
+def lofk_mofk(lmax):
+    lofk, mofk = [], []
+    for l in range(lmax + 1):
+        for m in range(-l, l + 1):
+            lofk.append(l)
+            mofk.append(m)
+    return (np.asfortranarray(np.array(lofk, dtype=np.int32)),
+            np.asfortranarray(np.array(mofk, dtype=np.int32)))
+
+
+def gauss_legendre_points(n, x1=-1.0, x2=1.0):
+    x = np.zeros(n, order="F")
+    w = np.zeros(n, order="F")
+    lib().gauss_legendre_points_(_d(x1), _d(x2), x.ctypes.data_as(C.c_void_p),
+                                 w.ctypes.data_as(C.c_void_p), _i(n))
+    return x, w
+
+
+def plm_normalized(lmax, x):
+    p = np.zeros((lmax + 1) * (lmax + 2) // 2, order="F")
+    lib().plm_normalized_(_i(lmax), _d(x), p.ctypes.data_as(C.c_void_p))
+    return p
+
+
+def cgaunt(lmax):
+    """Misc/cgaunt_c.f90 as called by GauntCoeficients::init (Coeficients.hpp:42-91)."""
+    kkr = (lmax + 1) ** 2
+    nd = (2 * lmax + 1) * (2 * lmax + 2) // 2
+    clm = np.ones(nd, order="F")
+    plmg = np.zeros((nd, 2 * lmax + 1), order="F")
+    tg = np.zeros(2 * (2 * lmax + 1), order="F")
+    wg = np.zeros(2 * (2 * lmax + 1), order="F")
+    cg = np.zeros((lmax + 1, kkr, kkr), order="F")
+    lofk, mofk = lofk_mofk(2 * lmax)
+    iprint = C.c_int(-1)
+    f = lib().cgaunt_
+    f(_i(lmax), clm.ctypes.data_as(C.c_void_p), plmg.ctypes.data_as(C.c_void_p),
+      tg.ctypes.data_as(C.c_void_p), wg.ctypes.data_as(C.c_void_p),
+      cg.ctypes.data_as(C.c_void_p), lofk.ctypes.data_as(C.c_void_p),
+      mofk.ctypes.data_as(C.c_void_p), C.byref(iprint), _ISTOP, C.c_size_t(32))
+    return cg
+
+
+def ifacts(lmax):
+    """Misc/ifacts_c.f."""
+    kkr = (lmax + 1) ** 2
+    illp = np.zeros((kkr, kkr), dtype=np.complex128, order="F")
+    ilp1 = np.zeros(2 * lmax + 1, dtype=np.complex128, order="F")
+    iprint = C.c_int(-1)
+    lib().ifacts_(_i(lmax), illp.ctypes.data_as(C.c_void_p), ilp1.ctypes.data_as(C.c_void_p),
+                  C.byref(iprint), _ISTOP, C.c_size_t(32))
+    return illp, ilp1
+
+
+def makegij(lmaxi, lmaxj, lmax, prel, rij, cgnt=None, illp=None, ilp1=None):
+    """MultipleScattering/makegij_c.f, with the tables of the reference."""
+    if cgnt is None:
+        cgnt = cgaunt(lmax)
+    if illp is None:
+        illp, ilp1 = ifacts(lmax)
+    kkri, kkrj = (lmaxi + 1) ** 2, (lmaxj + 1) ** 2
+    kkrsz = (lmax + 1) ** 2
+    ndlj = (2 * lmax + 1) ** 2
+    ndlm = (2 * lmax + 1) * (2 * lmax + 2) // 2
+    lofk, mofk = lofk_mofk(2 * lmax)
+    lend = lmaxi + lmaxj
+    sinmp = np.zeros(lend + 1, order="F")
+    cosmp = np.zeros(lend + 1, order="F")
+    clm = np.ones(ndlm, order="F")
+    plm = np.zeros(ndlm, order="F")
+    hfn = np.zeros(lend + 2, dtype=np.complex128, order="F")
+    dlm = np.zeros(ndlj, dtype=np.complex128, order="F")
+    gij = np.zeros((kkri, kkrj), dtype=np.complex128, order="F")
+    r = np.asfortranarray(np.asarray(rij, dtype=np.float64))
+    pz = _zarr(prel)
+    pi4 = C.c_double(4.0 * 2.0 * np.arcsin(1.0))
+    iprint = C.c_int(-1)
+    cg = np.asfortranarray(cgnt)
+    il = np.asfortranarray(illp)
+    i1 = np.asfortranarray(ilp1)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib().makegij_(_i(lmaxi), _i(kkri), _i(lmaxj), _i(kkrj), _i(lmax), _i(kkrsz), _i(ndlj),
+                   _i(ndlm), vp(pz), vp(r), vp(sinmp), vp(cosmp), vp(clm), vp(plm), vp(cg),
+                   _i(lmax), vp(lofk), vp(mofk), vp(i1), vp(il), vp(hfn), vp(dlm), vp(gij),
+                   C.byref(pi4), C.byref(iprint), _ISTOP, C.c_size_t(32))
+    return gij
+
+
+def setgij(gij, n_spin_cant, energy):
+    """MultipleScattering/setgij.f, non-relativistic branch."""
+    kkr1, kkr2 = gij.shape
+    b = np.zeros((kkr1 * n_spin_cant, kkr2 * n_spin_cant), dtype=np.complex128, order="F")
+    g = np.asfortranarray(gij, dtype=np.complex128)
+    psq = _zarr(0.0)
+    ce = _zarr(energy)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib().setgij_(vp(g), vp(b), _i(kkr1), _i(kkr1 * n_spin_cant), _i(kkr2),
+                  _i(kkr2 * n_spin_cant), _i(n_spin_cant), _i(0), vp(psq), vp(ce))
+    return b
+
+
+def congauss(ebot, etop, eibot, npts):
+    """Misc/congauss_c.f as called by buildEnergyContour (energyContourIntegration.cpp:106-112)."""
+    ip = npts + 2
+    egrd = np.zeros(ip, dtype=np.complex128, order="F")
+    dele = np.zeros(ip, dtype=np.complex128, order="F")
+    nume = C.c_int(0)
+    pi = C.c_double(2.0 * np.arcsin(1.0))
+    iprint = C.c_int(-1)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib().congauss_(_d(ebot), _d(etop), _d(eibot), vp(egrd), vp(dele), _i(npts), C.byref(nume),
+                    C.byref(pi), _i(ip), C.byref(iprint), _ISTOP, C.c_size_t(32))
+    return egrd[: nume.value], dele[: nume.value]
