@@ -1,0 +1,184 @@
+"""Throughput of the tau00 path on one MI355X.
+
+One step = every (local atom, contour energy) tau00 solve of the workload:
+KKR assembly M = 1 - t G(E) on the device plus the central-block inversion
+tau00 = [M^-1 [t0;0]]_00, complex FP64.  Workload: fcc a = 6.82 bohr, LIZ of
+135 sites (N = 2160), lmax = 3, 32 local atoms x 31 semicircle energies
+(992 solves per step).  t-matrices, LIZ geometry and tables are resident on
+the device before timing; only the 16x16 tau00 blocks come back.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "tau00 solves/sec at lmax=3 LIZ=135: mixed+batched vs FP64, <=1e-10 rel err"
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix) peak
+
+
+def solve_flops(n, k):
+    """Algorithmic work of one tau00 solve: complex LU + two triangular solves
+    with k right-hand sides, F = 8/3 N^3 + 8 N^2 k real flops."""
+    return (8.0 / 3.0) * n ** 3 + 8.0 * n * n * k
+
+
+def update_kernel_flops(nt_list):
+    """Algorithmic flops of all trailing-update launches of one batch:
+    sum over steps s of (nt-1-s)^2 tiles of 8*64^3 per solve."""
+    total = 0.0
+    for nt in nt_list:
+        for s in range(nt - 1):
+            r = nt - 1 - s
+            total += r * r * 8.0 * 64 ** 3
+    return total
+
+
+def cpu_baseline(system, gpu_tau, atoms, iies, budget_s=15.0):
+    """Host-core baseline: the oracle's C assembly of M plus LAPACK zgetrf/zgetrs
+    (scipy's LAPACK, all host threads) on a bounded sample of the same solves."""
+    import numpy as np
+    import scipy.linalg as sl
+    from oracle import oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max(d.get("num_threads", 1) for d in threadpool_info()) if threadpool_info() else 1
+    except Exception:
+        threads = os.cpu_count() or 1
+    t_start = time.perf_counter()
+    n_done = 0
+    max_err = 0.0
+    idx = 0
+    while n_done < 2 or (time.perf_counter() - t_start < budget_s and idx < len(atoms)):
+        a, ie = int(atoms[idx]), int(iies[idx])
+        at = system.atoms[a]
+        M = O.build_kkr(3, at.LIZPos, at.LIZlmax, at.LIZStoreIdx, system.tmatStore,
+                        system.blkSizeTmatStore, ie, system.prel(ie))
+        t0 = O.t0(3, int(at.LIZStoreIdx[0]), system.tmatStore, system.blkSizeTmatStore, ie)
+        lu, piv, info = sl.lapack.zgetrf(M, overwrite_a=True)
+        rhs = np.zeros((M.shape[0], t0.shape[0]), complex)
+        rhs[:t0.shape[0]] = t0
+        x, info = sl.lapack.zgetrs(lu, piv, rhs)
+        ref = x[:t0.shape[0]]
+        max_err = max(max_err, float(np.linalg.norm(gpu_tau[idx] - ref) / np.linalg.norm(ref)))
+        n_done += 1
+        idx += 37 % len(atoms) if len(atoms) > 37 else 1
+        idx %= len(atoms)
+    dt = time.perf_counter() - t_start
+    return n_done / dt, threads, n_done, dt, max_err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--natoms", type=int, default=32)
+    ap.add_argument("--nenergies", type=int, default=31)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by independent (atom, energy) solves; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from lsms_amd import capi
+    from lsms_amd import configs as CF
+
+    if not os.path.exists(capi.LIB_PATH):
+        import __graft_entry__
+        __graft_entry__.build()
+
+    s = CF.config_c2(npts=args.nenergies, natoms=args.natoms)
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    na, ne = len(s.atoms), len(s.energies)
+    atoms = np.repeat(np.arange(na), ne)
+    iies = np.tile(np.arange(ne), na)
+    prels = np.array([s.prel(i) for i in iies])
+    nsolve = len(atoms)
+    N = s.nrmat_ns(0)
+    k = s.kkrsz_ns
+
+    for _ in range(args.warmup):
+        tau, info = ctx.tau00_batched(atoms, iies, prels, k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tau, info = ctx.tau00_batched(atoms, iies, prels, k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert np.all(info == 0), "singular pivot reported"
+    ms_per_step = 1e3 * elapsed / args.steps
+    solves_per_s = nsolve * args.steps / elapsed
+
+    # per-kernel timing with HIP events on the library's stream (one extra step)
+    ctx.set_profiling(True)
+    ctx.reset_profile()
+    ctx.tau00_batched(atoms, iies, prels, k)
+    ctx.set_profiling(False)
+    prof = {name: ctx.profile(kind) for name, kind in
+            (("assemble", capi.KIND_ASSEMBLE), ("diag", capi.KIND_DIAG),
+             ("panel", capi.KIND_PANEL), ("update", capi.KIND_UPDATE),
+             ("finish", capi.KIND_FINISH))}
+    upd_ms, upd_n = prof["update"]
+    nt = (N + 63) // 64
+    upd_flops = update_kernel_flops([nt] * nsolve)
+    achieved_tf = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    path_tf = solve_flops(N, k) * solves_per_s / 1e12
+
+    cpu = None
+    err = None
+    if not args.no_cpu:
+        cps, threads, nsamp, dt, err = cpu_baseline(s, tau, atoms, iies)
+        cpu = {"value": round(cps, 4), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"{nsamp} of the {nsolve} solves (N={N}) in {dt:.1f} s: oracle C "
+                         f"assembly + LAPACK zgetrf/zgetrs (scipy OpenBLAS, {threads} threads)"}
+    ctx.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(solves_per_s, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"fcc a=6.82 bohr, LIZ r=13.0 (135 sites, N={N}), lmax=3, "
+                        f"{na} atoms x {ne} contour energies, plain FP64 batched",
+            "solves_per_step": nsolve, "N": N, "kkrsz_ns": k, "precision": "fp64",
+        },
+        "roofline": {
+            "bound": "mfma", "kernel": "k_lu_update",
+            "achieved": round(achieved_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 4), "traffic": None,
+            "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4), "launches": upd_n,
+        },
+        "path_roofline": {"achieved": round(path_tf, 3), "unit": "TFLOP/s",
+                          "frac": round(path_tf / FP64_PEAK_TFLOPS, 4),
+                          "flops_per_solve": solve_flops(N, k)},
+        "kernel_ms_per_step": {kname: round(v[0], 3) for kname, v in prof.items()},
+        "max_rel_err_vs_lapack": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

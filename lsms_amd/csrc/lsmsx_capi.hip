@@ -271,6 +271,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     if (kc < 0) kc = kci;
     if (kci != kc) return fail(-6, "all atoms of a batch must share kkrsz");
     if (kc > kTile) return fail(-6, "kkrsz_ns > 64 not supported");
+    if (a.kkrsz > (c->lmax_tab + 1) * (c->lmax_tab + 1))
+      return fail(-6, "atom lmax exceeds the lmax of the uploaded tables");
     if (iies[i] < 0 || iies[i] >= c->group) return fail(-7, "energy slot out of range");
     if (ispins[i] < 0 || ispins[i] >= nspola) return fail(-7, "spin index out of range");
     for (int s = 0; s < a.numLIZ; s++)
@@ -495,6 +497,10 @@ int lsmsx_build_kkr(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double 
   if (ispin < 0 || ispin >= nspola) return fail(-7, "spin index out of range");
   int N = nsc * a.nrmat;
   if (ldm < N) return fail(-2, "ldm < nrmat_ns");
+  if (a.kkrsz > (c->lmax_tab + 1) * (c->lmax_tab + 1))
+    return fail(-6, "atom lmax exceeds the lmax of the uploaded tables");
+  for (int s = 0; s < a.numLIZ; s++)
+    if (a.store[s] < 0 || a.store[s] >= c->nstore) return fail(-7, "LIZStoreIdx out of range");
   HIPCHK(hipSetDevice(c->device));
   if (int r = c->M.ensure((size_t)N * N)) return r;
   if (int r = c->desc.ensure(1)) return r;

@@ -190,7 +190,8 @@ __global__ __launch_bounds__(256) void k_lu_diag(const SolveDesc *__restrict__ d
 // Wave w computes rows 32*(w&1).., cols 32*(w>>1).. as 2x2 16x16 blocks with
 // 4 real MFMAs per complex block (re += ar br - ai bi, im += ar bi + ai br).
 // MFMA 16x16x4 f64 operand layout: a = A[lane%16][lane/16],
-// b = B[lane/16][lane%16]; accumulator lane holds D[4*(lane/16)+r][lane%16].
+// b = B[lane/16][lane%16]; accumulator register r of a lane holds
+// D[lane/16 + 4r][lane%16] (rows interleaved across the 4 lane groups).
 constexpr int kKc = 32;
 constexpr int kBs = kTile + 1;  // padded stride of Bs rows
 
@@ -281,10 +282,10 @@ __device__ void tile_gemm(GemmSmem &s, double2 *__restrict__ Cg, long ldc,
 #pragma unroll
     for (int jb = 0; jb < 2; jb++) {
       if (ACCUM) {
-        const double2 *cp = Cg + (wr + 16 * ib + 4 * kq) + ldc * (wc + 16 * jb + li);
+        const double2 *cp = Cg + (wr + 16 * ib + kq) + ldc * (wc + 16 * jb + li);
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          double2 v = cp[r];
+          double2 v = cp[4 * r];
           cre[ib][jb][r] = v.x;
           cim[ib][jb][r] = v.y;
         }
@@ -303,9 +304,9 @@ __device__ void tile_gemm(GemmSmem &s, double2 *__restrict__ Cg, long ldc,
   for (int ib = 0; ib < 2; ib++)
 #pragma unroll
     for (int jb = 0; jb < 2; jb++) {
-      double2 *cp = Cg + (wr + 16 * ib + 4 * kq) + ldc * (wc + 16 * jb + li);
+      double2 *cp = Cg + (wr + 16 * ib + kq) + ldc * (wc + 16 * jb + li);
 #pragma unroll
-      for (int r = 0; r < 4; r++) cp[r] = make_double2(cre[ib][jb][r], cim[ib][jb][r]);
+      for (int r = 0; r < 4; r++) cp[4 * r] = make_double2(cre[ib][jb][r], cim[ib][jb][r]);
     }
 }
 
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__
 __global__ __launch_bounds__(256) void k_lu_finish(const SolveDesc *__restrict__ descs) {
   const SolveDesc &d = descs[blockIdx.x];
   __shared__ double2 A[kTile][kTile + 1];
-  __shared__ double2 X[2 * kMaxKkr][2 * kMaxKkr + 1];
+  __shared__ double2 X[kTile][kTile + 1];
   __shared__ int piv[kTile], perm[kTile];
   __shared__ int sinfo;
   __shared__ double sminpiv;

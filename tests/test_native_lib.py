@@ -1,0 +1,43 @@
+"""The C-ABI library loads and exports every symbol include/lsmsx.h declares
+(CPU only: no compute call that needs a GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from lsms_amd import capi
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "lsmsx.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(lsmsx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_expected_entry_points():
+    syms = declared_symbols()
+    assert set(capi.EXPORTED) == set(syms), (set(capi.EXPORTED) ^ set(syms))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    for s in declared_symbols():
+        assert hasattr(lib, s), s
+
+
+def test_host_tables_match_oracle():
+    for lmax in (1, 2, 3):
+        cg, illp, ilp1 = capi.compute_tables(lmax)
+        assert np.abs(cg - O.cgaunt(lmax)).max() < 1e-14
+        a, b = O.ifacts(lmax)
+        assert np.array_equal(illp, a) and np.array_equal(ilp1, b)
+
+
+def test_null_context_is_an_error_not_a_crash():
+    lib = capi.load()
+    assert lib.lsmsx_set_profiling(None, 1) < 0
+    assert b"null" in lib.lsmsx_last_error()
