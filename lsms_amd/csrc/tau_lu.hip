@@ -184,173 +184,172 @@ __global__ __launch_bounds__(256) void k_lu_diag(const SolveDesc *__restrict__ d
 }
 
 // ---------------------------------------------------------------------------
-// 64x64x64 complex tile product on FP64 MFMA (v_mfma_f64_16x16x4f64).
-// Operands staged in LDS as split re/im planes, k-major:
-//   As[k][i] (A(i,k)),  Bs[k][j] (B(k,j)), in two K-chunks of 32.
-// Wave w computes rows 32*(w&1).., cols 32*(w>>1).. as 2x2 16x16 blocks with
-// 4 real MFMAs per complex block (re += ar br - ai bi, im += ar bi + ai br).
-// MFMA 16x16x4 f64 operand layout: a = A[lane%16][lane/16],
-// b = B[lane/16][lane%16]; accumulator register r of a lane holds
-// D[lane/16 + 4r][lane%16] (rows interleaved across the 4 lane groups).
-constexpr int kKc = 32;
-constexpr int kBs = kTile + 1;  // padded stride of Bs rows
+// Complex 32x32x64 wave product on FP64 MFMA (v_mfma_f64_16x16x4f64), 3M form:
+//   P1 = Ar Br, P2 = Ai Bi, P3 = (Ar+Ai)(Br+Bi);  re = P1 - P2, im = P3 - P1 - P2
+// (three real MFMAs per complex 16x16x4 block instead of four).
+// The MFMA is applied to the transposed product, D'(j,i) = sum_k B(k,j) A(i,k),
+// so the accumulator of a lane covers consecutive rows i of C: register r of
+// lane (li = lane%16, kq = lane/16) holds C(i0+li, j0+kq+4r), and every C and
+// A access is a run of 16 consecutive rows (256 B).  Operands are read straight
+// from L2/LDS into registers in MFMA layout (a-operand lane value B(k0+kq, j0+li),
+// b-operand lane value A(i0+li, k0+kq)), one k-step ahead; no LDS staging and
+// no barriers, so the waves of a CU run independently.
+typedef d4 Acc[2][2];
 
-struct GemmSmem {
-  double are[kKc][kTile], aim[kKc][kTile];
-  double bre[kKc][kBs], bim[kKc][kBs];
-};
-
-// Operand sources.  amode: 0 global tile, 1 packed L^-1 (unit lower).
-// bmode: 0 global tile, 1 packed U^-1 (upper), 2 global tile with rows gathered by perm.
-template <int AMODE, int BMODE, bool NEGA>
-__device__ inline void stage_chunk(GemmSmem &s, const double2 *__restrict__ Ag, long lda,
-                                   const double2 *__restrict__ Bg, long ldb, const int *perm,
-                                   int k0) {
-  const int tid = threadIdx.x;
+template <class FA, class FB>
+__device__ __forceinline__ void wave_gemm3m(FA fa, FB fb, Acc &p1, Acc &p2, Acc &p3) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+  double2 a[2], b[2], an[2], bn[2];
 #pragma unroll
-  for (int it = 0; it < (kTile * kKc) / 256; it++) {
-    int e = tid + 256 * it;
-    int i = e & 63, k = e >> 6;  // A(i, k0+k)
-    double2 v;
-    if (AMODE == 0) {
-      v = Ag[i + lda * (k0 + k)];
-    } else {
-      int kk = k0 + k;
-      v = (i > kk) ? Ag[i + kTile * kk] : make_double2(i == kk ? 1.0 : 0.0, 0.0);
-    }
-    if (NEGA) {
-      v.x = -v.x;
-      v.y = -v.y;
-    }
-    s.are[k][i] = v.x;
-    s.aim[k][i] = v.y;
+  for (int t = 0; t < 2; t++) {
+    a[t] = fa(16 * t + li, kq);
+    b[t] = fb(kq, 16 * t + li);
   }
+#pragma unroll 4
+  for (int k0 = 0; k0 < kTile; k0 += 4) {
+    if (k0 + 4 < kTile) {
 #pragma unroll
-  for (int it = 0; it < (kTile * kKc) / 256; it++) {
-    int e = tid + 256 * it;
-    int k = e & 31, jcol = e >> 5;  // B(k0+k, jcol)
-    int kk = k0 + k;
-    double2 v;
-    if (BMODE == 0) {
-      v = Bg[kk + ldb * jcol];
-    } else if (BMODE == 1) {
-      v = (kk <= jcol) ? Bg[kk + kTile * jcol] : make_double2(0.0, 0.0);
-    } else {
-      v = Bg[perm[kk] + ldb * jcol];
-    }
-    s.bre[k][jcol] = v.x;
-    s.bim[k][jcol] = v.y;
-  }
-}
-
-__device__ inline void mma_chunk(const GemmSmem &s, d4 (&cre)[2][2], d4 (&cim)[2][2], int wr,
-                                 int wc, int lane) {
-  const int li = lane & 15, kq = lane >> 4;
-#pragma unroll
-  for (int kk = 0; kk < kKc; kk += 4) {
-    double ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      ar[b] = s.are[kk + kq][wr + 16 * b + li];
-      ai[b] = s.aim[kk + kq][wr + 16 * b + li];
-      br[b] = s.bre[kk + kq][wc + 16 * b + li];
-      bi[b] = s.bim[kk + kq][wc + 16 * b + li];
-    }
-#pragma unroll
-    for (int ib = 0; ib < 2; ib++)
-#pragma unroll
-      for (int jb = 0; jb < 2; jb++) {
-        cre[ib][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ib], br[jb], cre[ib][jb], 0, 0, 0);
-        cre[ib][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[ib], bi[jb], cre[ib][jb], 0, 0, 0);
-        cim[ib][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ib], bi[jb], cim[ib][jb], 0, 0, 0);
-        cim[ib][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[ib], br[jb], cim[ib][jb], 0, 0, 0);
+      for (int t = 0; t < 2; t++) {
+        an[t] = fa(16 * t + li, k0 + 4 + kq);
+        bn[t] = fb(k0 + 4 + kq, 16 * t + li);
       }
+    }
+    double as[2], bs[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      as[t] = a[t].x + a[t].y;
+      bs[t] = b[t].x + b[t].y;
+    }
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+        p1[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[bj].x, a[bi].x, p1[bj][bi], 0, 0, 0);
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+        p2[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[bj].y, a[bi].y, p2[bj][bi], 0, 0, 0);
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+        p3[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(bs[bj], as[bi], p3[bj][bi], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      a[t] = an[t];
+      b[t] = bn[t];
+    }
   }
 }
 
-// C (global, 64x64) = [C +] (+-A) B.
-template <int AMODE, int BMODE, bool ACCUM>
-__device__ void tile_gemm(GemmSmem &s, double2 *__restrict__ Cg, long ldc,
-                          const double2 *__restrict__ Ag, long lda,
-                          const double2 *__restrict__ Bg, long ldb, const int *perm) {
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
-  const int li = lane & 15, kq = lane >> 4;
-  d4 cre[2][2], cim[2][2];
+__device__ __forceinline__ void zero_acc(Acc &p) {
 #pragma unroll
-  for (int ib = 0; ib < 2; ib++)
+  for (int bj = 0; bj < 2; bj++)
 #pragma unroll
-    for (int jb = 0; jb < 2; jb++) {
-      if (ACCUM) {
-        const double2 *cp = Cg + (wr + 16 * ib + kq) + ldc * (wc + 16 * jb + li);
+    for (int bi = 0; bi < 2; bi++) p[bj][bi] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+// C (wave's 32x32 block at Cg, leading dim ldc) = [C -] A B from the 3M sums.
+template <bool SUBTRACT>
+__device__ __forceinline__ void wave_store3m(double2 *__restrict__ Cg, long ldc, const Acc &p1,
+                                             const Acc &p2, const Acc &p3) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          double2 v = cp[4 * r];
-          cre[ib][jb][r] = v.x;
-          cim[ib][jb][r] = v.y;
+  for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        double2 *cp = Cg + (16 * bi + li) + ldc * (16 * bj + kq + 4 * r);
+        double re = p1[bj][bi][r] - p2[bj][bi][r];
+        double im = p3[bj][bi][r] - p1[bj][bi][r] - p2[bj][bi][r];
+        if (SUBTRACT) {
+          double2 c = *cp;
+          *cp = make_double2(c.x - re, c.y - im);
+        } else {
+          *cp = make_double2(re, im);
         }
-      } else {
-        cre[ib][jb] = d4{0.0, 0.0, 0.0, 0.0};
-        cim[ib][jb] = d4{0.0, 0.0, 0.0, 0.0};
       }
-    }
-  for (int k0 = 0; k0 < kTile; k0 += kKc) {
-    __syncthreads();
-    stage_chunk<AMODE, BMODE, ACCUM>(s, Ag, lda, Bg, ldb, perm, k0);
-    __syncthreads();
-    mma_chunk(s, cre, cim, wr, wc, lane);
-  }
-#pragma unroll
-  for (int ib = 0; ib < 2; ib++)
-#pragma unroll
-    for (int jb = 0; jb < 2; jb++) {
-      double2 *cp = Cg + (wr + 16 * ib + kq) + ldc * (wc + 16 * jb + li);
-#pragma unroll
-      for (int r = 0; r < 4; r++) cp[4 * r] = make_double2(cre[ib][jb][r], cim[ib][jb][r]);
-    }
 }
 
 // Panel of step s: x < nrest -> L_is = A_is U^-1 ; else U_sj = L^-1 P A_sj.
-// The tile is read completely into LDS before it is overwritten in place.
+// The operand tile is staged in LDS (rows gathered by the pivot permutation
+// for U_sj) before it is overwritten in place.
 __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ descs, int step) {
   const SolveDesc &d = descs[blockIdx.y];
   const int nrest = d.nt - 1 - step;
   const int x = blockIdx.x;
   if (nrest <= 0 || x >= 2 * nrest) return;
-  __shared__ GemmSmem s;
-  __shared__ double2 T[kTile][kTile];  // copy of the operand tile (column-major)
-  __shared__ int perm[kTile];
-  const int tid = threadIdx.x;
+  __shared__ double2 T[kTile][kTile + 1];  // T[col][row]
+  const int tid = threadIdx.x, w = tid >> 6;
+  const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
   const int c0 = step * kTile;
   double2 *M = d.M;
   const long ld = d.ld;
-  if (tid < kTile) perm[tid] = d.perm[tid];
-  double2 *tile;
-  if (x < nrest) tile = M + (long)(c0 + kTile * (1 + x)) + ld * c0;                 // A_is
-  else tile = M + (long)c0 + ld * (c0 + kTile * (1 + x - nrest));                   // A_sj
-  for (int e = tid; e < kTile * kTile; e += 256) {
-    int r = e & 63, c = e >> 6;
-    T[c][r] = tile[r + ld * c];
+  const double2 *__restrict__ inv = d.inv;
+  const bool lower = x < nrest;
+  double2 *tile = lower ? M + (long)(c0 + kTile * (1 + x)) + ld * c0
+                        : M + (long)c0 + ld * (c0 + kTile * (1 + x - nrest));
+  if (lower) {
+    for (int e = tid; e < kTile * kTile; e += 256) {
+      int r = e & 63, c = e >> 6;
+      T[c][r] = tile[r + ld * c];
+    }
+  } else {
+    const int *perm = d.perm;
+    for (int e = tid; e < kTile * kTile; e += 256) {
+      int r = e & 63, c = e >> 6;
+      T[c][r] = tile[perm[r] + ld * c];
+    }
   }
   __syncthreads();
-  if (x < nrest) tile_gemm<0, 1, false>(s, tile, ld, &T[0][0], kTile, d.inv, kTile, perm);
-  else tile_gemm<1, 2, false>(s, tile, ld, d.inv, kTile, &T[0][0], kTile, perm);
+  Acc p1, p2, p3;
+  zero_acc(p1);
+  zero_acc(p2);
+  zero_acc(p3);
+  if (lower) {
+    // A(i,k) = T(wr+i, k); B(k,j) = U^-1(k, wc+j), upper incl. diagonal
+    wave_gemm3m([&](int i, int k) { return T[k][wr + i]; },
+                [&](int k, int j) {
+                  int jj = wc + j;
+                  return (k <= jj) ? inv[k + kTile * jj] : make_double2(0.0, 0.0);
+                },
+                p1, p2, p3);
+  } else {
+    // A(i,k) = L^-1(wr+i, k), unit lower; B(k,j) = (P A_sj)(k, wc+j)
+    wave_gemm3m([&](int i, int k) {
+                  int ii = wr + i;
+                  return (ii > k) ? inv[ii + kTile * k]
+                                  : make_double2(ii == k ? 1.0 : 0.0, 0.0);
+                },
+                [&](int k, int j) { return T[wc + j][k]; }, p1, p2, p3);
+  }
+  wave_store3m<false>(tile + wr + ld * wc, ld, p1, p2, p3);
 }
 
-// Trailing update of step s: A_ij -= L_is U_sj for all i, j > s.
+// Trailing update of step s: A_ij -= L_is U_sj for all i, j > s.  One 64x64
+// tile per workgroup, a 32x32 quarter per wave.
 __global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__ descs, int step) {
   const SolveDesc &d = descs[blockIdx.y];
   const int nrest = d.nt - 1 - step;
   const int x = blockIdx.x;
   if (nrest <= 0 || x >= nrest * nrest) return;
-  __shared__ GemmSmem s;
   const int ti = step + 1 + x % nrest, tj = step + 1 + x / nrest;
+  const int w = threadIdx.x >> 6;
+  const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
   const int c0 = step * kTile;
   double2 *M = d.M;
   const long ld = d.ld;
-  tile_gemm<0, 0, true>(s, M + (long)kTile * ti + ld * (kTile * tj), ld,
-                        M + (long)kTile * ti + ld * c0, ld, M + (long)c0 + ld * (kTile * tj), ld,
-                        nullptr);
+  const double2 *__restrict__ A = M + (long)(kTile * ti + wr) + ld * c0;
+  const double2 *__restrict__ B = M + (long)c0 + ld * (kTile * tj + wc);
+  Acc p1, p2, p3;
+  zero_acc(p1);
+  zero_acc(p2);
+  zero_acc(p3);
+  wave_gemm3m([&](int i, int k) { return A[i + ld * k]; }, [&](int k, int j) { return B[k + ld * j]; },
+              p1, p2, p3);
+  wave_store3m<true>(M + (long)(kTile * ti + wr) + ld * (kTile * tj + wc), ld, p1, p2, p3);
 }
 
 // Last tile: eliminate its non-central part (pivot rows restricted to it),
