@@ -22,6 +22,14 @@
 namespace lsmsx {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+// Pointers read from the solve descriptors are generic; casting them to the
+// global address space lets the compiler emit global (not flat) loads/stores.
+// (The host pass only type-checks kernel bodies, so it sees plain double2.)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) double2 gdouble2;
+#else
+typedef double2 gdouble2;
+#endif
 
 __device__ inline double cabs1(double2 z) { return fabs(z.x) + fabs(z.y); }
 __device__ inline double2 cmul2(double2 a, double2 b) {
@@ -195,8 +203,11 @@ __global__ __launch_bounds__(256) void k_lu_diag(const SolveDesc *__restrict__ d
 // b-operand lane value A(i0+li, k0+kq)), one k-step ahead; no LDS staging and
 // no barriers, so the waves of a CU run independently.
 typedef d4 Acc[2][2];
+#ifndef LSMSX_KUNROLL
+#define LSMSX_KUNROLL 2
+#endif
 
-template <class FA, class FB>
+template <bool NEGA, class FA, class FB>
 __device__ __forceinline__ void wave_gemm3m(FA fa, FB fb, Acc &p1, Acc &p2, Acc &p3) {
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
   double2 a[2], b[2], an[2], bn[2];
@@ -205,7 +216,7 @@ __device__ __forceinline__ void wave_gemm3m(FA fa, FB fb, Acc &p1, Acc &p2, Acc 
     a[t] = fa(16 * t + li, kq);
     b[t] = fb(kq, 16 * t + li);
   }
-#pragma unroll 4
+#pragma unroll LSMSX_KUNROLL
   for (int k0 = 0; k0 < kTile; k0 += 4) {
     if (k0 + 4 < kTile) {
 #pragma unroll
@@ -217,6 +228,10 @@ __device__ __forceinline__ void wave_gemm3m(FA fa, FB fb, Acc &p1, Acc &p2, Acc 
     double as[2], bs[2];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
+      if (NEGA) {
+        a[t].x = -a[t].x;
+        a[t].y = -a[t].y;
+      }
       as[t] = a[t].x + a[t].y;
       bs[t] = b[t].x + b[t].y;
     }
@@ -287,7 +302,7 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
   const int c0 = step * kTile;
   double2 *M = d.M;
   const long ld = d.ld;
-  const double2 *__restrict__ inv = d.inv;
+  const gdouble2 *__restrict__ inv = (const gdouble2 *)d.inv;
   const bool lower = x < nrest;
   double2 *tile = lower ? M + (long)(c0 + kTile * (1 + x)) + ld * c0
                         : M + (long)c0 + ld * (c0 + kTile * (1 + x - nrest));
@@ -310,18 +325,21 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
   zero_acc(p3);
   if (lower) {
     // A(i,k) = T(wr+i, k); B(k,j) = U^-1(k, wc+j), upper incl. diagonal
-    wave_gemm3m([&](int i, int k) { return T[k][wr + i]; },
+    wave_gemm3m<false>([&](int i, int k) { return T[k][wr + i]; },
                 [&](int k, int j) {
                   int jj = wc + j;
-                  return (k <= jj) ? inv[k + kTile * jj] : make_double2(0.0, 0.0);
+                  double2 v = make_double2(0.0, 0.0);
+                  if (k <= jj) v = inv[k + kTile * jj];
+                  return v;
                 },
                 p1, p2, p3);
   } else {
     // A(i,k) = L^-1(wr+i, k), unit lower; B(k,j) = (P A_sj)(k, wc+j)
-    wave_gemm3m([&](int i, int k) {
+    wave_gemm3m<false>([&](int i, int k) {
                   int ii = wr + i;
-                  return (ii > k) ? inv[ii + kTile * k]
-                                  : make_double2(ii == k ? 1.0 : 0.0, 0.0);
+                  double2 v = make_double2(ii == k ? 1.0 : 0.0, 0.0);
+                  if (ii > k) v = inv[ii + kTile * k];
+                  return v;
                 },
                 [&](int k, int j) { return T[wc + j][k]; }, p1, p2, p3);
   }
@@ -329,27 +347,49 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
 }
 
 // Trailing update of step s: A_ij -= L_is U_sj for all i, j > s.  One 64x64
-// tile per workgroup, a 32x32 quarter per wave.
+// tile per workgroup, a 32x32 quarter per wave.  C enters the 3M sums at the
+// start (q1 = Cre, q2 = 0, q3 = Cre + Cim, products accumulated with -A), so
+// C' = (q1 - q2, q3 - q1 - q2) and no load sits in the epilogue.
 __global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__ descs, int step) {
   const SolveDesc &d = descs[blockIdx.y];
   const int nrest = d.nt - 1 - step;
   const int x = blockIdx.x;
   if (nrest <= 0 || x >= nrest * nrest) return;
   const int ti = step + 1 + x % nrest, tj = step + 1 + x / nrest;
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
   const int w = threadIdx.x >> 6;
   const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
   const int c0 = step * kTile;
   double2 *M = d.M;
   const long ld = d.ld;
-  const double2 *__restrict__ A = M + (long)(kTile * ti + wr) + ld * c0;
-  const double2 *__restrict__ B = M + (long)c0 + ld * (kTile * tj + wc);
-  Acc p1, p2, p3;
-  zero_acc(p1);
-  zero_acc(p2);
-  zero_acc(p3);
-  wave_gemm3m([&](int i, int k) { return A[i + ld * k]; }, [&](int k, int j) { return B[k + ld * j]; },
-              p1, p2, p3);
-  wave_store3m<true>(M + (long)(kTile * ti + wr) + ld * (kTile * tj + wc), ld, p1, p2, p3);
+  const gdouble2 *__restrict__ A = (const gdouble2 *)(M + (long)(kTile * ti + wr) + ld * c0);
+  const gdouble2 *__restrict__ B = (const gdouble2 *)(M + (long)c0 + ld * (kTile * tj + wc));
+  gdouble2 *__restrict__ C = (gdouble2 *)(M + (long)(kTile * ti + wr) + ld * (kTile * tj + wc));
+  Acc q1, q2, q3;
+#pragma unroll
+  for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) {
+      q2[bj][bi] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        double2 c = C[(16 * bi + li) + ld * (16 * bj + kq + 4 * r)];  // global load
+        q1[bj][bi][r] = c.x;
+        q3[bj][bi][r] = c.x + c.y;
+      }
+    }
+  wave_gemm3m<true>([&](int i, int k) { return A[i + ld * k]; },
+                    [&](int k, int j) { return B[k + ld * j]; }, q1, q2, q3);
+#pragma unroll
+  for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        double re = q1[bj][bi][r] - q2[bj][bi][r];
+        double im = q3[bj][bi][r] - q1[bj][bi][r] - q2[bj][bi][r];
+        C[(16 * bi + li) + ld * (16 * bj + kq + 4 * r)] = make_double2(re, im);
+      }
 }
 
 // Last tile: eliminate its non-central part (pivot rows restricted to it),
