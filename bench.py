@@ -41,6 +41,22 @@ def update_kernel_flops(nt_list):
     return total
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_lu_update_fcc135_N2160.md")
+
+
+def profiled_traffic():
+    """HBM bytes per k_lu_update launch measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (copy-calibrated) on this workload; the command and the
+    calibration are recorded in the profile summary."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(system, gpu_tau, atoms, iies, budget_s=15.0):
     """Host-core baseline: the oracle's C assembly of M plus LAPACK zgetrf/zgetrs
     (scipy's LAPACK, all host threads) on a bounded sample of the same solves."""
@@ -180,7 +196,8 @@ def main():
         "roofline": {
             "bound": "mfma", "kernel": "k_lu_update",
             "achieved": round(achieved_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 4),
+            "traffic": profiled_traffic() if (args.natoms, args.nenergies) == (32, 31) else None,
             "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4), "launches": upd_n,
         },
         "path_roofline": {"achieved": round(path_tf, 3), "unit": "TFLOP/s",

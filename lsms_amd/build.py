@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblsmsx.so")
+DRIVER = os.path.join(HERE, "_build", "host_mirror_driver")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -55,6 +56,12 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # C++ driver of the LSMS-shaped host interface (run by the GPU tests)
+    drv = os.path.join(os.path.dirname(HERE), "tests", "cpp", "host_mirror_driver.cpp")
+    if os.path.exists(drv):
+        _run(["g++", "-O2", "-std=c++17", "-I/opt/rocm/include", drv, "-o", DRIVER,
+              f"-L{HERE}", "-llsmsx", "-Wl,-rpath,$ORIGIN/..",
+              "-L/opt/rocm/lib", "-lamdhip64"])
     return LIB
 
 

@@ -1,0 +1,102 @@
+"""The LSMS-shaped C++ host interface (lsms_amd/host/lsmsx_lsms.hpp), driven
+from C++ exactly as calculateTauMatrix / calculateAllTauMatrices would call
+it, against the oracle -- needs a GPU.
+
+Checked: calculateAllTauMatricesMI355X output tau00_l = rotate(tau00 - t0)
+for every atom and energy (collinear, spin-polarized collinear and
+non-collinear systems), buildKKRMatrixMI355X, solveTau00MI355X and
+calculateTauMatrixMI355X.  tau00 tolerance 1e-10 relative (plain FP64).
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from lsms_amd import build as B
+from lsms_amd import configs as CF
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def write_input(path, s):
+    nsc, nsp = s.n_spin_cant, s.n_spin_pola
+    ts = np.asfortranarray(s.tmatStore)
+    with open(path, "wb") as f:
+        np.array([s.maxlmax, nsc, nsp, len(s.atoms), ts.shape[1], len(s.energies),
+                  s.blkSizeTmatStore, ts.shape[0]], dtype=np.int32).tofile(f)
+        ts.flatten(order="F").astype(np.complex128).tofile(f)
+        s.energies.astype(np.complex128).tofile(f)
+        for at in s.atoms:
+            np.array([at.numLIZ], dtype=np.int32).tofile(f)
+            np.asfortranarray(at.LIZPos).flatten(order="F").tofile(f)
+            at.LIZlmax.astype(np.int32).tofile(f)
+            at.LIZStoreIdx.astype(np.int32).tofile(f)
+            u, ud = O.spin_trafo(at.evec)
+            u.astype(np.complex128).tofile(f)
+            ud.astype(np.complex128).tofile(f)
+
+
+def oracle_tau00_l(s, a, ie, ispin):
+    at = s.atoms[a]
+    M = O.build_kkr(s.maxlmax, at.LIZPos, at.LIZlmax, at.LIZStoreIdx, s.tmatStore,
+                    s.blkSizeTmatStore, ie, s.prel(ie), s.n_spin_cant, s.n_spin_pola, ispin)
+    t0 = O.t0(s.maxlmax, int(at.LIZStoreIdx[0]), s.tmatStore, s.blkSizeTmatStore, ie,
+              s.n_spin_cant, s.n_spin_pola, ispin)
+    tau, _ = O.solve_tau00_zgetrf(M, t0)
+    d = tau - t0
+    if s.n_spin_cant == 2:
+        u, ud = O.spin_trafo(at.evec)
+        kk = (s.maxlmax + 1) ** 2
+        return O.trgtol(kk, kk, u, ud, d).flatten(order="F"), tau, M
+    return d.flatten(order="F"), tau, M
+
+
+def run_driver(s):
+    if not os.path.exists(B.DRIVER):
+        pytest.fail("host_mirror_driver not built (python -m lsms_amd.build)")
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_input(fin, s)
+        r = subprocess.run([B.DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        return np.fromfile(fout, dtype=np.complex128)
+
+
+def systems():
+    e, w = CF.contour_semicircle(-0.5, 0.7, 3)
+    out = [("c1", CF.config_c1(npts=3))]
+    out.append(("collinear", CF.make_system("col", "bcc", 5.42, (1, 1, 1), 6.0, 3, e, w, seed=5,
+                                            n_spin_cant=1, n_spin_pola=2)))
+    out.append(("noncollinear", CF.make_system("nc", "bcc", 5.42, (1, 1, 1), 6.0, 3, e, w, seed=6,
+                                               n_spin_cant=2, n_spin_pola=2)))
+    return out
+
+
+@pytest.mark.parametrize("name,s", systems(), ids=[n for n, _ in systems()])
+def test_host_mirror_matches_oracle(name, s):
+    out = run_driver(s)
+    na, ne = len(s.atoms), len(s.energies)
+    nspin = 2 if s.n_spin_pola != s.n_spin_cant else 1
+    k = s.kkrsz_ns
+    pos = 0
+    for ie in range(ne):
+        blk = out[pos:pos + k * k * na * nspin].reshape((k * k, na * nspin), order="F")
+        pos += k * k * na * nspin
+        for sp in range(nspin):
+            for a in range(na):
+                ref, _, _ = oracle_tau00_l(s, a, ie, sp)
+                got = blk[:, a + sp * na]
+                assert np.linalg.norm(got - ref) <= 1e-10 * np.linalg.norm(ref), (ie, a, sp)
+    n = s.nrmat_ns(0)
+    M = out[pos:pos + n * n].reshape((n, n), order="F")
+    pos += n * n
+    ref_l, ref_tau, ref_M = oracle_tau00_l(s, 0, 0, 0)
+    assert np.abs(M - ref_M).max() <= 1e-12 * np.abs(ref_M).max()
+    tau = out[pos:pos + k * k].reshape((k, k), order="F")
+    pos += k * k
+    assert np.linalg.norm(tau - ref_tau) <= 1e-10 * np.linalg.norm(ref_tau)
+    tl = out[pos:pos + k * k]
+    assert np.linalg.norm(tl - ref_l) <= 1e-10 * np.linalg.norm(ref_l)
