@@ -22,7 +22,10 @@ extern "C" {
 typedef struct lsmsx_ctx lsmsx_ctx;
 
 /* precision modes for the solve */
-#define LSMSX_PREC_FP64 0 /* all arithmetic complex FP64 */
+#define LSMSX_PREC_FP64 0       /* all arithmetic complex FP64 */
+#define LSMSX_PREC_MIXED_FP32 1 /* complex FP32 factors (FP32 MFMA) + FP64 iterative
+                                   refinement against the FP64 matrix; solves whose
+                                   refinement stalls are redone in FP64 */
 
 /* linear-solver selector ids for the reference's 16-bit
  * lsms.global.linearSolver field (linearSolvers.hpp:18-27,196-199;
@@ -94,6 +97,18 @@ int lsmsx_tau00_batched(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
 int lsmsx_solve_tau00_matrix(lsmsx_ctx *ctx, int n, int k, const double *m,
                              int ldm, const double *t0, double *tau00, int *info);
 
+/* Refinement settings of LSMSX_PREC_MIXED_FP32: at most max_sweeps
+ * residual/correction sweeps (default 12), converged when the central rows
+ * satisfy ||dX||_F <= tol ||X||_F (default 1e-12).  Solves that do not
+ * converge are redone in FP64. */
+int lsmsx_set_refinement(lsmsx_ctx *ctx, int max_sweeps, double tol);
+
+/* Per-solve statistics of the last batched / single call on this context:
+ * iters[i] = refinement sweeps (0 on the plain FP64 path, -1 when a mixed
+ * solve fell back to FP64), minpiv[i] = smallest |pivot| of the factorization.
+ * Either pointer may be NULL. */
+int lsmsx_last_batch_stats(lsmsx_ctx *ctx, int nsolve, int *iters, double *minpiv);
+
 /* Workspace cap in bytes for the batched matrix pool (0 = automatic:
  * a fraction of the free device memory).  Larger batches run in chunks. */
 int lsmsx_set_workspace_limit(lsmsx_ctx *ctx, long bytes);
@@ -101,7 +116,8 @@ int lsmsx_set_workspace_limit(lsmsx_ctx *ctx, long bytes);
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on
  * the context's stream; lsmsx_get_profile returns the accumulated time (ms)
  * and launch count of one kernel kind (0 assemble, 1 diagonal-tile LU,
- * 2 panel, 3 trailing update, 4 finish).  lsmsx_reset_profile clears them. */
+ * 2 panel, 3 trailing update, 4 last tile, 5 refinement residual,
+ * 6 refinement solve).  lsmsx_reset_profile clears them. */
 int lsmsx_set_profiling(lsmsx_ctx *ctx, int on);
 int lsmsx_reset_profile(lsmsx_ctx *ctx);
 int lsmsx_get_profile(lsmsx_ctx *ctx, int kind, double *ms, long *launches);

@@ -11,14 +11,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblsmsx.so")
 
 PREC_FP64 = 0
-KIND_ASSEMBLE, KIND_DIAG, KIND_PANEL, KIND_UPDATE, KIND_FINISH = range(5)
+PREC_MIXED_FP32 = 1
+(KIND_ASSEMBLE, KIND_DIAG, KIND_PANEL, KIND_UPDATE, KIND_FINISH, KIND_IR_RESIDUAL,
+ KIND_IR_SOLVE) = range(7)
 
 EXPORTED = [
     "lsmsx_last_error", "lsmsx_ctx_create", "lsmsx_ctx_destroy", "lsmsx_set_tables",
     "lsmsx_compute_tables", "lsmsx_upload_tmats", "lsmsx_set_liz", "lsmsx_build_kkr",
     "lsmsx_tau00", "lsmsx_tau00_batched", "lsmsx_solve_tau00_matrix",
     "lsmsx_set_workspace_limit", "lsmsx_set_profiling", "lsmsx_reset_profile",
-    "lsmsx_get_profile",
+    "lsmsx_get_profile", "lsmsx_set_refinement", "lsmsx_last_batch_stats",
 ]
 
 _lib = None
@@ -52,6 +54,8 @@ def load():
     L.lsmsx_set_profiling.argtypes = [_vp, C.c_int]
     L.lsmsx_reset_profile.argtypes = [_vp]
     L.lsmsx_get_profile.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+    L.lsmsx_set_refinement.argtypes = [_vp, C.c_int, C.c_double]
+    L.lsmsx_last_batch_stats.argtypes = [_vp, C.c_int, _vp, _vp]
     _lib = L
     return L
 
@@ -134,15 +138,15 @@ class Context:
                                    _p(m), nrmat_ns))
         return m
 
-    def tau00(self, atom_id, iie, prel, kkrsz_ns, ispin=0):
+    def tau00(self, atom_id, iie, prel, kkrsz_ns, ispin=0, precision=PREC_FP64):
         tau = np.zeros((kkrsz_ns, kkrsz_ns), dtype=np.complex128, order="F")
         pr = np.array([complex(prel)], dtype=np.complex128)
         info = C.c_int(0)
         _ck(load().lsmsx_tau00(self._h, atom_id, ispin, iie, _p(pr), self.nsc, self.nspola,
-                               PREC_FP64, _p(tau), C.byref(info)))
+                               precision, _p(tau), C.byref(info)))
         return tau, info.value
 
-    def tau00_batched(self, atom_ids, iies, prels, kkrsz_ns, ispins=None):
+    def tau00_batched(self, atom_ids, iies, prels, kkrsz_ns, ispins=None, precision=PREC_FP64):
         n = len(atom_ids)
         a = np.ascontiguousarray(atom_ids, dtype=np.int32)
         e = np.ascontiguousarray(iies, dtype=np.int32)
@@ -151,7 +155,7 @@ class Context:
         tau = np.zeros((n, kkrsz_ns, kkrsz_ns), dtype=np.complex128)
         info = np.zeros(n, dtype=np.int32)
         _ck(load().lsmsx_tau00_batched(self._h, n, _p(a), _p(s), _p(e), _p(pr), self.nsc,
-                                       self.nspola, PREC_FP64, _p(tau), _p(info)))
+                                       self.nspola, precision, _p(tau), _p(info)))
         # each block is column-major kkrsz_ns x kkrsz_ns
         return np.ascontiguousarray(np.transpose(tau, (0, 2, 1))), info
 
@@ -164,6 +168,16 @@ class Context:
         _ck(load().lsmsx_solve_tau00_matrix(self._h, n, k, _p(mm), n, _p(tt), _p(tau),
                                             C.byref(info)))
         return tau, info.value
+
+    def set_refinement(self, max_sweeps=12, tol=1e-12):
+        _ck(load().lsmsx_set_refinement(self._h, int(max_sweeps), float(tol)))
+
+    def last_stats(self, n):
+        """(refinement sweeps per solve, -1 = FP64 fallback; smallest |pivot|)."""
+        it = np.zeros(n, dtype=np.int32)
+        mp = np.zeros(n, dtype=np.float64)
+        _ck(load().lsmsx_last_batch_stats(self._h, n, _p(it), _p(mp)))
+        return it, mp
 
     def set_workspace_limit(self, nbytes):
         _ck(load().lsmsx_set_workspace_limit(self._h, int(nbytes)))

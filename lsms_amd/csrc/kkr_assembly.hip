@@ -109,6 +109,8 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
   const bool prel0 = (prel.x == 0.0 && prel.y == 0.0);
   const double pi4 = 4.0 * 3.141592653589793238462643383279502884;
   double2 *__restrict__ M = d.M;
+  // mixed precision: the solver factors an FP32 copy written alongside M
+  float2 *__restrict__ W = (d.A != (void *)d.M) ? (float2 *)d.A : nullptr;
   const long ld = d.ld;
 
   for (int i0 = 0; i0 < d.numLIZ; i0 += 4) {
@@ -199,11 +201,13 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
           acc.y += tv.x * gv.y + tv.y * gv.x;
         }
         M[(offi + rr) + ld * (offj + cc)] = make_double2(-acc.x, -acc.y);
+        if (W) W[(offi + rr) + ld * (offj + cc)] = make_float2((float)-acc.x, (float)-acc.y);
       }
     } else if (act) {
       for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
         int rr = e % kkri_ns, cc = e / kkri_ns;
         M[(offi + rr) + ld * (offj + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
+        if (W) W[(offi + rr) + ld * (offj + cc)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
       }
     }
     __syncthreads();
@@ -216,15 +220,18 @@ __global__ __launch_bounds__(256) void k_pad_identity(const SolveDesc *__restric
   const int pad = d.pad;
   if (pad == 0) return;
   double2 *M = d.M;
+  float2 *W = (d.A != (void *)d.M) ? (float2 *)d.A : nullptr;
   const long ld = d.ld;
   // columns 0..pad-1: full columns; columns pad..Ns-1: rows 0..pad-1
   for (long e = threadIdx.x; e < (long)pad * d.Ns; e += blockDim.x) {
     int rr = (int)(e % d.Ns), cc = (int)(e / d.Ns);
     M[rr + ld * cc] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
+    if (W) W[rr + ld * cc] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
   }
   for (long e = threadIdx.x; e < (long)pad * (d.Ns - pad); e += blockDim.x) {
     int rr = (int)(e % pad), cc = pad + (int)(e / pad);
     M[rr + ld * cc] = make_double2(0.0, 0.0);
+    if (W) W[rr + ld * cc] = make_float2(0.f, 0.f);
   }
 }
 

@@ -23,10 +23,19 @@ namespace lsmsx {
 __global__ void k_assemble(const SolveDesc *, const double2 *, long, const double *,
                            const double2 *, int);
 __global__ void k_pad_identity(const SolveDesc *);
+template <class T>
 __global__ void k_lu_diag(const SolveDesc *, int);
+template <class T>
 __global__ void k_lu_panel(const SolveDesc *, int);
+template <class T>
 __global__ void k_lu_update(const SolveDesc *, int);
-__global__ void k_lu_finish(const SolveDesc *);
+template <class T>
+__global__ void k_lu_last(const SolveDesc *);
+__global__ void k_ir_init(const SolveDesc *);
+__global__ void k_ir_residual(const SolveDesc *);
+__global__ void k_ir_solve(const SolveDesc *, double);
+__global__ void k_ir_extract(const SolveDesc *);
+__global__ void k_count_conv(const SolveDesc *, int, int *);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout.
 __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
@@ -56,6 +65,7 @@ __global__ void k_init_status(const SolveDesc *__restrict__ descs, int n) {
   if (i < n) {
     *descs[i].info = 0;
     *descs[i].minpiv = 1e300;
+    if (descs[i].conv) *descs[i].conv = 0;
   }
 }
 }  // namespace lsmsx
@@ -127,15 +137,20 @@ struct lsmsx_ctx {
   int nstore = 0, group = 0;
   std::map<int, AtomDev> atoms;
   // batch pool
-  DevBuf<double2> M, inv, tau, tmpM, t0buf;
-  DevBuf<int> perm, info;
-  DevBuf<double> minpiv;
-  DevBuf<SolveDesc> desc;
+  DevBuf<double2> M, inv, tau, tmpM, t0buf, X;
+  DevBuf<float2> W, R;
+  DevBuf<int> perm, info, conv, iters, count;
+  DevBuf<double> minpiv, dxn;
+  DevBuf<SolveDesc> desc, desc2;
+  std::vector<int> last_iters;
+  std::vector<double> last_minpiv;
   long ws_limit = 0;
+  int ir_max_sweeps = 12;
+  double ir_tol = 1e-12;
   // profiling
   bool profiling = false;
-  double prof_ms[5] = {0, 0, 0, 0, 0};
-  long prof_n[5] = {0, 0, 0, 0, 0};
+  double prof_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+  long prof_n[7] = {0, 0, 0, 0, 0, 0, 0};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;
 };
@@ -216,31 +231,65 @@ std::vector<int> site_offsets(const AtomDev &a, int nsc, bool solver) {
 }
 
 // Launch the blocked LU over a chunk of nb solves already in solver layout.
-int launch_lu(lsmsx_ctx *c, int nb, int nt_max) {
-  SolveDesc *dd = c->desc.p;
+template <class T>
+int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max) {
   for (int s = 0; s < nt_max - 1; s++) {
     int nrest = nt_max - 1 - s;
     {
       ProfScope p(c, 1);
-      hipLaunchKernelGGL(k_lu_diag, dim3(nb), dim3(256), 0, c->stream, dd, s);
+      hipLaunchKernelGGL(k_lu_diag<T>, dim3(nb), dim3(256), 0, c->stream, dd, s);
     }
     if (int r = check_launch("k_lu_diag")) return r;
     {
       ProfScope p(c, 2);
-      hipLaunchKernelGGL(k_lu_panel, dim3(2 * nrest, nb), dim3(256), 0, c->stream, dd, s);
+      hipLaunchKernelGGL(k_lu_panel<T>, dim3(2 * nrest, nb), dim3(256), 0, c->stream, dd, s);
     }
     if (int r = check_launch("k_lu_panel")) return r;
     {
       ProfScope p(c, 3);
-      hipLaunchKernelGGL(k_lu_update, dim3(nrest * nrest, nb), dim3(256), 0, c->stream, dd, s);
+      hipLaunchKernelGGL(k_lu_update<T>, dim3(nrest * nrest, nb), dim3(256), 0, c->stream, dd, s);
     }
     if (int r = check_launch("k_lu_update")) return r;
   }
   {
     ProfScope p(c, 4);
-    hipLaunchKernelGGL(k_lu_finish, dim3(nb), dim3(256), 0, c->stream, dd);
+    hipLaunchKernelGGL(k_lu_last<T>, dim3(nb), dim3(256), 0, c->stream, dd);
   }
-  return check_launch("k_lu_finish");
+  return check_launch("k_lu_last");
+}
+
+// Refinement sweeps of the mixed path on a chunk whose FP32 factors are in
+// place.  Returns the number of unconverged solves left in *left.
+int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int max_sweeps, double tol,
+           int *left) {
+  hipLaunchKernelGGL(k_ir_init, dim3(nb), dim3(256), 0, c->stream, dd);
+  if (int r = check_launch("k_ir_init")) return r;
+  int done = 0;
+  for (int it = 0; it < max_sweeps; it++) {
+    {
+      ProfScope p(c, 5);
+      hipLaunchKernelGGL(k_ir_residual, dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
+    }
+    if (int r = check_launch("k_ir_residual")) return r;
+    {
+      ProfScope p(c, 6);
+      hipLaunchKernelGGL(k_ir_solve, dim3(nb), dim3(256), 0, c->stream, dd, tol);
+    }
+    if (int r = check_launch("k_ir_solve")) return r;
+    if (it >= 1) {  // the first correction never meets the tolerance
+      hipLaunchKernelGGL(k_count_conv, dim3(1), dim3(256), 0, c->stream, dd, nb, c->count.p);
+      HIPCHK(hipMemcpyAsync(&done, c->count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (done == nb) break;
+    }
+  }
+  hipLaunchKernelGGL(k_ir_extract, dim3(nb), dim3(256), 0, c->stream, dd);
+  if (int r = check_launch("k_ir_extract")) return r;
+  hipLaunchKernelGGL(k_count_conv, dim3(1), dim3(256), 0, c->stream, dd, nb, c->count.p);
+  HIPCHK(hipMemcpyAsync(&done, c->count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *left = nb - done;
+  return 0;
 }
 
 int validate_common(lsmsx_ctx *c, int nsc, int nspola) {
@@ -255,12 +304,17 @@ int validate_common(lsmsx_ctx *c, int nsc, int nspola) {
 
 // Batched solve of nsolve (atom, spin, slot) problems; results to host.
 int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins,
-                const int *iies, const double *prel, int nsc, int nspola, double *tau_out,
-                int *info_out) {
+                const int *iies, const double *prel, int nsc, int nspola, int precision,
+                double *tau_out, int *info_out) {
   if (int r = validate_common(c, nsc, nspola)) return r;
+  if (precision != LSMSX_PREC_FP64 && precision != LSMSX_PREC_MIXED_FP32)
+    return fail(-8, "unsupported precision mode");
+  const bool mixed = precision == LSMSX_PREC_MIXED_FP32;
   if (nsolve <= 0) return 0;
   HIPCHK(hipSetDevice(c->device));
-  // per-solve sizes
+  c->last_iters.assign(nsolve, 0);
+  c->last_minpiv.assign(nsolve, 0.0);
+  // per-solve sizes (bytes of all per-solve device buffers)
   std::vector<long> bytes(nsolve);
   int kc = -1;
   for (int i = 0; i < nsolve; i++) {
@@ -271,43 +325,66 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     if (kc < 0) kc = kci;
     if (kci != kc) return fail(-6, "all atoms of a batch must share kkrsz");
     if (kc > kTile) return fail(-6, "kkrsz_ns > 64 not supported");
+    if (mixed && kc != 16 && kc != 32) return fail(-6, "mixed precision needs kkrsz_ns = 16 or 32");
     if (a.kkrsz > (c->lmax_tab + 1) * (c->lmax_tab + 1))
       return fail(-6, "atom lmax exceeds the lmax of the uploaded tables");
     if (iies[i] < 0 || iies[i] >= c->group) return fail(-7, "energy slot out of range");
     if (ispins[i] < 0 || ispins[i] >= nspola) return fail(-7, "spin index out of range");
     for (int s = 0; s < a.numLIZ; s++)
       if (a.store[s] < 0 || a.store[s] >= c->nstore) return fail(-7, "LIZStoreIdx out of range");
-    long Ns = roundup(nsc * a.nrmat, kTile);
-    bytes[i] = Ns * Ns;
+    long Ns = roundup(nsc * a.nrmat, kTile), nt = Ns / kTile;
+    bytes[i] = Ns * Ns * 16;
+    if (mixed) bytes[i] += Ns * Ns * 8 + nt * kTile * kTile * 8 + Ns * kc * 24;
+    else bytes[i] += kTile * kTile * 16;
   }
   long cap = c->ws_limit;
   if (cap <= 0) {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
-    cap = (long)((double)(fr + c->M.n * sizeof(double2)) * 0.7);
+    long held = (long)(c->M.n * sizeof(double2) + c->W.n * sizeof(float2) + c->inv.n * sizeof(double2));
+    cap = (long)((double)(fr + held) * 0.7);
   }
-  long cap_elems = cap / (long)sizeof(double2);
   int done = 0;
   while (done < nsolve) {
     // chunk [done, end)
     long acc = 0;
     int end = done;
-    while (end < nsolve && (end == done || acc + bytes[end] <= cap_elems)) acc += bytes[end++];
+    while (end < nsolve && (end == done || acc + bytes[end] <= cap)) acc += bytes[end++];
     int nb = end - done;
-    if (int r = c->M.ensure(acc)) return r;
-    if (int r = c->inv.ensure((size_t)nb * kTile * kTile)) return r;
-    if (int r = c->perm.ensure((size_t)nb * kTile)) return r;
+    long mtot = 0, xtot = 0, ivtot = 0;
+    int maxliz = 1, nt_max = 1;
+    for (int b = 0; b < nb; b++) {
+      const AtomDev &a = c->atoms[atom_ids[done + b]];
+      long Ns = roundup(nsc * a.nrmat, kTile);
+      mtot += Ns * Ns;
+      xtot += Ns * kc;
+      ivtot += mixed ? (Ns / kTile) * kTile * kTile : kTile * kTile;
+      maxliz = std::max(maxliz, a.numLIZ);
+      nt_max = std::max(nt_max, (int)(Ns / kTile));
+    }
+    if (int r = c->M.ensure(mtot)) return r;
+    if (int r = c->inv.ensure(mixed ? (ivtot + 1) / 2 : ivtot)) return r;
+    if (int r = c->perm.ensure((size_t)nb * nt_max * kTile)) return r;
     if (int r = c->info.ensure((size_t)nb)) return r;
     if (int r = c->minpiv.ensure((size_t)nb)) return r;
     if (int r = c->tau.ensure((size_t)nb * kc * kc)) return r;
     if (int r = c->desc.ensure((size_t)nb)) return r;
+    if (mixed) {
+      if (int r = c->W.ensure(mtot)) return r;
+      if (int r = c->X.ensure(xtot)) return r;
+      if (int r = c->R.ensure(xtot)) return r;
+      if (int r = c->conv.ensure((size_t)nb)) return r;
+      if (int r = c->iters.ensure((size_t)nb)) return r;
+      if (int r = c->dxn.ensure((size_t)nb)) return r;
+      if (int r = c->count.ensure(1)) return r;
+    }
     std::vector<SolveDesc> hd(nb);
-    long moff = 0;
-    int maxliz = 1, nt_max = 1;
+    long moff = 0, xoff = 0, ioff = 0;
     for (int b = 0; b < nb; b++) {
       int i = done + b;
       AtomDev &a = c->atoms[atom_ids[i]];
       SolveDesc &d = hd[b];
+      d = SolveDesc{};
       int N = nsc * a.nrmat, Ns = roundup(N, kTile);
       d.pos = a.d_pos;
       d.lmax = a.d_lmax;
@@ -327,15 +404,28 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       d.Ns = Ns;
       d.pad = Ns - N;
       d.nt = Ns / kTile;
-      d.inv = c->inv.p + (long)b * kTile * kTile;
-      d.perm = c->perm.p + (long)b * kTile;
+      d.perm = c->perm.p + (long)b * nt_max * kTile;
       d.info = c->info.p + b;
       d.minpiv = c->minpiv.p + b;
       d.tau00 = c->tau.p + (long)b * kc * kc;
       d.t0 = c->tmat.p + c->ldT * a.store[0] + d.t_off;
+      if (mixed) {
+        d.A = c->W.p + moff;
+        d.inv = (float2 *)c->inv.p + ioff;
+        d.inv_stride = kTile * kTile;
+        ioff += (long)d.nt * kTile * kTile;
+        d.X = c->X.p + xoff;
+        d.R = c->R.p + xoff;
+        d.conv = c->conv.p + b;
+        d.iters = c->iters.p + b;
+        d.dxn = c->dxn.p + b;
+        xoff += (long)Ns * kc;
+      } else {
+        d.A = d.M;
+        d.inv = c->inv.p + (long)b * kTile * kTile;
+        d.inv_stride = 0;
+      }
       moff += (long)Ns * Ns;
-      maxliz = std::max(maxliz, a.numLIZ);
-      nt_max = std::max(nt_max, d.nt);
     }
     HIPCHK(hipMemcpyAsync(c->desc.p, hd.data(), nb * sizeof(SolveDesc), hipMemcpyHostToDevice,
                           c->stream));
@@ -348,15 +438,59 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
                          c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
     }
     if (int r = check_launch("k_assemble")) return r;
-    if (int r = launch_lu(c, nb, nt_max)) return r;
+    std::vector<int> hiters(nb, 0);
+    if (!mixed) {
+      if (int r = launch_lu<double>(c, c->desc.p, nb, nt_max)) return r;
+    } else {
+      if (int r = launch_lu<float>(c, c->desc.p, nb, nt_max)) return r;
+      int left = 0;
+      if (int r = refine(c, c->desc.p, nb, nt_max, c->ir_max_sweeps, c->ir_tol, &left)) return r;
+      std::vector<int> hconv(nb);
+      HIPCHK(hipMemcpyAsync(hiters.data(), c->iters.p, nb * sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream));
+      HIPCHK(hipMemcpyAsync(hconv.data(), c->conv.p, nb * sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (left > 0) {
+        // FP64 fallback for the solves whose refinement stalled: factor the
+        // untouched FP64 matrix with the plain path.
+        std::vector<SolveDesc> fb;
+        std::vector<int> fidx;
+        for (int b = 0; b < nb; b++)
+          if (!hconv[b]) {
+            SolveDesc d = hd[b];
+            d.A = d.M;
+            d.inv = c->inv.p + (long)fb.size() * kTile * kTile;
+            d.inv_stride = 0;
+            d.conv = nullptr;
+            fb.push_back(d);
+            fidx.push_back(b);
+            hiters[b] = -1;
+          }
+        int nf = (int)fb.size();
+        int fnt = 1;
+        for (auto &d : fb) fnt = std::max(fnt, d.nt);
+        if (int r = c->desc2.ensure((size_t)nf)) return r;
+        HIPCHK(hipMemcpyAsync(c->desc2.p, fb.data(), nf * sizeof(SolveDesc),
+                              hipMemcpyHostToDevice, c->stream));
+        if (int r = launch_lu<double>(c, c->desc2.p, nf, fnt)) return r;
+      }
+    }
     HIPCHK(hipMemcpyAsync(tau_out + (long)done * kc * kc * 2, c->tau.p,
                           (size_t)nb * kc * kc * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
     std::vector<int> hinfo(nb);
+    std::vector<double> hmin(nb);
     HIPCHK(hipMemcpyAsync(hinfo.data(), c->info.p, nb * sizeof(int), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(hmin.data(), c->minpiv.p, nb * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     drain_profile(c);
-    for (int b = 0; b < nb; b++) info_out[done + b] = hinfo[b];
+    for (int b = 0; b < nb; b++) {
+      info_out[done + b] = hinfo[b];
+      c->last_iters[done + b] = hiters[b];
+      c->last_minpiv[done + b] = hmin[b];
+    }
     done = end;
   }
   return 0;
@@ -412,6 +546,14 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->info.release();
   c->minpiv.release();
   c->desc.release();
+  c->desc2.release();
+  c->X.release();
+  c->W.release();
+  c->R.release();
+  c->conv.release();
+  c->iters.release();
+  c->count.release();
+  c->dxn.release();
   (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -523,6 +665,7 @@ int lsmsx_build_kkr(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double 
   d.Ns = N;
   d.pad = 0;
   d.nt = 0;
+  d.A = d.M;
   HIPCHK(hipMemcpyAsync(c->desc.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
   {
     ProfScope p(c, 0);
@@ -539,9 +682,8 @@ int lsmsx_build_kkr(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double 
 
 int lsmsx_tau00(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *prel, int nsc,
                 int nspola, int precision, double *tau00, int *info) {
-  if (precision != LSMSX_PREC_FP64) return fail(-8, "unsupported precision mode");
   int inf = 0;
-  int r = run_batched(c, 1, &atom_id, &ispin, &iie, prel, nsc, nspola, tau00, &inf);
+  int r = run_batched(c, 1, &atom_id, &ispin, &iie, prel, nsc, nspola, precision, tau00, &inf);
   if (info) *info = inf;
   return r;
 }
@@ -549,8 +691,7 @@ int lsmsx_tau00(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *pre
 int lsmsx_tau00_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins,
                         const int *iies, const double *prel, int nsc, int nspola, int precision,
                         double *tau00, int *info) {
-  if (precision != LSMSX_PREC_FP64) return fail(-8, "unsupported precision mode");
-  return run_batched(c, nsolve, atom_ids, ispins, iies, prel, nsc, nspola, tau00, info);
+  return run_batched(c, nsolve, atom_ids, ispins, iies, prel, nsc, nspola, precision, tau00, info);
 }
 
 int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ldm,
@@ -562,7 +703,7 @@ int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ld
   if (int r = c->tmpM.ensure((size_t)n * n)) return r;
   if (int r = c->M.ensure((size_t)Ns * Ns)) return r;
   if (int r = c->inv.ensure((size_t)kTile * kTile)) return r;
-  if (int r = c->perm.ensure(kTile)) return r;
+  if (int r = c->perm.ensure((size_t)(Ns / kTile) * kTile)) return r;
   if (int r = c->info.ensure(1)) return r;
   if (int r = c->minpiv.ensure(1)) return r;
   if (int r = c->tau.ensure((size_t)k * k)) return r;
@@ -576,6 +717,7 @@ int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ld
   d.kc = k;
   d.ldt = k;
   d.M = c->M.p;
+  d.A = d.M;
   d.ld = Ns;
   d.Ns = Ns;
   d.pad = Ns - n;
@@ -591,7 +733,7 @@ int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ld
   hipLaunchKernelGGL(k_permute_in, dim3(Ns), dim3(256), 0, c->stream, c->desc.p, c->tmpM.p,
                      (long)n, n, k);
   if (int r = check_launch("k_permute_in")) return r;
-  if (int r = launch_lu(c, 1, d.nt)) return r;
+  if (int r = launch_lu<double>(c, c->desc.p, 1, d.nt)) return r;
   int hinfo = 0;
   HIPCHK(hipMemcpyAsync(tau00, c->tau.p, sizeof(double2) * k * k, hipMemcpyDeviceToHost,
                         c->stream));
@@ -599,6 +741,24 @@ int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ld
   HIPCHK(hipStreamSynchronize(c->stream));
   drain_profile(c);
   if (info) *info = hinfo;
+  return 0;
+}
+
+int lsmsx_set_refinement(lsmsx_ctx *c, int max_sweeps, double tol) {
+  if (!c) return fail(-1, "null context");
+  if (max_sweeps < 1 || max_sweeps > 100 || !(tol > 0.0)) return fail(-2, "bad refinement settings");
+  c->ir_max_sweeps = max_sweeps;
+  c->ir_tol = tol;
+  return 0;
+}
+
+int lsmsx_last_batch_stats(lsmsx_ctx *c, int nsolve, int *iters, double *minpiv) {
+  if (!c) return fail(-1, "null context");
+  if (nsolve < 0 || nsolve > (int)c->last_iters.size()) return fail(-2, "nsolve exceeds the last batch");
+  for (int i = 0; i < nsolve; i++) {
+    if (iters) iters[i] = c->last_iters[i];
+    if (minpiv) minpiv[i] = c->last_minpiv[i];
+  }
   return 0;
 }
 
@@ -616,7 +776,7 @@ int lsmsx_set_profiling(lsmsx_ctx *c, int on) {
 
 int lsmsx_reset_profile(lsmsx_ctx *c) {
   if (!c) return fail(-1, "null context");
-  for (int i = 0; i < 5; i++) {
+  for (int i = 0; i < 7; i++) {
     c->prof_ms[i] = 0;
     c->prof_n[i] = 0;
   }
@@ -625,7 +785,7 @@ int lsmsx_reset_profile(lsmsx_ctx *c) {
 
 int lsmsx_get_profile(lsmsx_ctx *c, int kind, double *ms, long *launches) {
   if (!c) return fail(-1, "null context");
-  if (kind < 0 || kind > 4) return fail(-2, "bad kernel kind");
+  if (kind < 0 || kind > 6) return fail(-2, "bad kernel kind");
   if (ms) *ms = c->prof_ms[kind];
   if (launches) *launches = c->prof_n[kind];
   return 0;

@@ -1,7 +1,7 @@
 // lsmsx_device.hpp -- shared device-side types of the MI355X tau00 path.
 //
 // Matrix layout used by the solver ("solver layout"): an Ns x Ns column-major
-// complex FP64 matrix, Ns = roundup(N, 64).  The first pad = Ns - N rows and
+// complex matrix, Ns = roundup(N, 64).  The first pad = Ns - N rows and
 // columns hold an identity block (eliminated trivially), then the LIZ sites
 // 1..numLIZ-1 in LIZ order, then the central site 0 last, so the central
 // kkrsz_ns x kkrsz_ns block is the trailing corner of the last 64-tile and
@@ -25,22 +25,31 @@ struct SolveDesc {
   int numLIZ;
   int n_spin_cant;
   int kkrsz;              // (maxlmax+1)^2 of the central atom
-  int kc;                 // kkrsz_ns: central block size
+  int kc;                 // kkrsz_ns: central block size (= number of right-hand sides)
   double prel_re, prel_im;
   long t_off;             // offset of this energy slot (and collinear spin block) in a tmatStore column
   int ldt;                // leading dimension of one t block (n_spin_cant * kkrsz)
-  // matrix and solver state
-  double2 *M;
-  int ld;                 // leading dimension of M
+  // matrices
+  double2 *M;             // FP64 KKR matrix (assembly target; residuals of the mixed path)
+  void *A;                // matrix the LU kernels factor: M itself (FP64) or a float2 copy
+  int ld;                 // leading dimension of M and A
   int Ns;                 // matrix dimension in this layout
   int pad;                // identity rows/cols at the front
   int nt;                 // Ns / kTile (solver layout)
-  double2 *inv;           // kTile x kTile: L11^-1 (strict lower) and U11^-1 (upper)
-  int *perm;              // kTile: row permutation of the current diagonal tile
+  // factorization state
+  void *inv;              // per diagonal tile: L^-1 (strict lower) and U^-1 (upper), kTile^2
+  long inv_stride;        // elements between the tiles of consecutive steps (0: one reused tile)
+  int *perm;              // nt x kTile row permutations of the diagonal tiles
   int *info;              // first singular pivot (1-based, solver order) or 0
   double *minpiv;         // smallest |pivot| seen
   double2 *tau00;         // kc x kc output
   const double2 *t0;      // central t-matrix (kc x kc, leading dim ldt)
+  // iterative refinement (mixed precision)
+  double2 *X;             // Ns x kc solution, FP64
+  float2 *R;              // Ns x kc residual / correction, FP32
+  int *conv;              // 1 once converged
+  int *iters;             // refinement sweeps used
+  double *dxn;            // ||dX_c||^2 of the previous sweep
 };
 
 }  // namespace lsmsx
