@@ -6,10 +6,11 @@
 //   D_lm   = hfn_l Pbar_l|m|(cos t) e^{-+i|m|phi} (makegij_c.f:186-214)
 //   G(L2,L1) = 4pi i^(l2-l1) sum_l3 cgnt(l3/2,L1,L2) D_{l3,m2-m1}
 //   M(i,j) = -tpack_i * diag_spin(G_ij),  M(i,i) = 1
-// Design: one workgroup per (column site j, solve); each wave builds one
-// (i, j) block at a time.  D is computed lane-parallel (one (l,m) per lane),
-// the Gaunt/i^l tables sit in LDS, and every lane writes consecutive rows of
-// a column so the HBM stores (the kernel's roofline) are coalesced.
+// Design: one workgroup per (row site i, solve) with t_i and the angular
+// tables in LDS; each wave builds (i, j) blocks for its own column sites with
+// wave-scope synchronisation only.  D is computed lane-parallel (one (l,m)
+// per lane), and every lane writes consecutive rows of a column so the HBM
+// stores (the kernel's roofline) are coalesced.
 #include "lsmsx_device.hpp"
 
 namespace lsmsx {
@@ -79,66 +80,153 @@ __device__ double plm_bar(int l, int m, double x) {
   return p1;
 }
 
-// Assemble all blocks of column site j = blockIdx.x for solve blockIdx.y.
-// Off-diagonal blocks: -tpack_i diag_spin(G_ij); diagonal blocks: identity.
+// Wave-scope ordering of LDS traffic: every lane's earlier LDS writes are
+// visible to the wave's later LDS reads.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int kLmaxD = 2 * kMaxLmax;                       // l <= lend of D
+constexpr int kNd = (kLmaxD + 1) * (kLmaxD + 1);           // 49 (l,m) of D
+
+// Assemble row-site block row i = blockIdx.x of solve blockIdx.y:
+// M(i,j) = -tpack_i diag_spin(G_ij) for j != i, identity for j == i.
+// The workgroup stages the packed t_i and the angular tables in LDS; each
+// wave then builds blocks (i, j), j = w, w+4, ..., independently.
 __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ descs,
                                                   const double2 *__restrict__ tmat, long ldT,
                                                   const double *__restrict__ cgnt_g,
                                                   const double2 *__restrict__ illp_g,
                                                   int lmax_tab) {
   const SolveDesc &d = descs[blockIdx.y];
-  const int j = blockIdx.x;
-  if (j >= d.numLIZ) return;
+  const int i = blockIdx.x;
+  if (i >= d.numLIZ) return;
 
   __shared__ double cg[(kMaxLmax + 1) * kMaxKkr * kMaxKkr];
   __shared__ double2 il[kMaxKkr * kMaxKkr];
-  __shared__ double2 dl[4][(2 * kMaxLmax + 1) * (2 * kMaxLmax + 1)];
+  __shared__ double2 tn[2 * kMaxKkr][2 * kMaxKkr + 1];   // t_i in 16-aligned spin blocks, [row][col]
+  __shared__ double pa[kLmaxD + 1][kLmaxD + 1], pb[kLmaxD + 1][kLmaxD + 1];
+  __shared__ double pmmf[kLmaxD + 1], pm1f[kLmaxD + 1];
+  __shared__ int lofk[kNd];
+  __shared__ double2 dl[4][kNd];
   __shared__ double2 gs[4][kMaxKkr * kMaxKkr];
 
+  const int tid = threadIdx.x;
   const int kkr_tab = (lmax_tab + 1) * (lmax_tab + 1);
-  for (int t = threadIdx.x; t < (lmax_tab + 1) * kkr_tab * kkr_tab; t += blockDim.x) cg[t] = cgnt_g[t];
-  for (int t = threadIdx.x; t < kkr_tab * kkr_tab; t += blockDim.x) il[t] = illp_g[t];
+  const int nsc = d.n_spin_cant, kkrsz = d.kkrsz, ldt = d.ldt;
+  const int lmi = d.lmax[i], kkri = (lmi + 1) * (lmi + 1), kkri_ns = kkri * nsc;
+  for (int t = tid; t < (lmax_tab + 1) * kkr_tab * kkr_tab; t += 256) cg[t] = cgnt_g[t];
+  for (int t = tid; t < kkr_tab * kkr_tab; t += 256) il[t] = illp_g[t];
+  // t_i packed as the F77 route does (calculateTauMatrix.cpp:155-183: the
+  // leading kkri block of each spin block), each spin block placed at a
+  // 16-aligned offset and zero-padded for the MFMA product below.
+  {
+    const double2 *__restrict__ T = tmat + ldT * d.store[i] + d.t_off;
+    for (int t = tid; t < 4 * kMaxKkr * kMaxKkr; t += 256) {
+      int r = t % (2 * kMaxKkr), c = t / (2 * kMaxKkr);
+      int rs = r / kMaxKkr, ri = r % kMaxKkr, cs = c / kMaxKkr, ci = c % kMaxKkr;
+      double2 v = make_double2(0.0, 0.0);
+      if (rs < nsc && cs < nsc && ri < kkri && ci < kkri)
+        v = T[(rs * kkrsz + ri) + (long)ldt * (cs * kkrsz + ci)];
+      tn[r][c] = v;
+    }
+  }
+  // Pbar recursion coefficients (Misc/associatedLegendreFunction.hpp:33-69)
+  if (tid < (kLmaxD + 1) * (kLmaxD + 1)) {
+    int l = tid / (kLmaxD + 1), m = tid % (kLmaxD + 1);
+    double a = 0.0, b = 0.0;
+    if (l >= m + 2) {
+      a = sqrt((4.0 * l * l - 1.0) / (double)(l * l - m * m));
+      b = sqrt(((l - 1.0) * (l - 1.0) - m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+    }
+    pa[l][m] = a;
+    pb[l][m] = b;
+  }
+  if (tid < kLmaxD + 1) {
+    pmmf[tid] = tid >= 1 ? sqrt((2.0 * tid + 1.0) / (2.0 * tid)) : 1.0;
+    pm1f[tid] = sqrt(2.0 * tid + 3.0);
+  }
+  if (tid < kNd) {
+    int l = 0;
+    while ((l + 1) * (l + 1) <= tid) l++;
+    lofk[tid] = l;
+  }
   __syncthreads();
 
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nsc = d.n_spin_cant;
-  const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1), kkrj_ns = kkrj * nsc;
-  const int offj = d.site_off[j];
-  const double xj = d.pos[3 * j], yj = d.pos[3 * j + 1], zj = d.pos[3 * j + 2];
+  const int w = tid >> 6, lane = tid & 63;
+  const double xi = d.pos[3 * i], yi = d.pos[3 * i + 1], zi = d.pos[3 * i + 2];
   const double2 prel = make_double2(d.prel_re, d.prel_im);
   const bool prel0 = (prel.x == 0.0 && prel.y == 0.0);
   const double pi4 = 4.0 * 3.141592653589793238462643383279502884;
+  const double y00 = 0.28209479177387814347;  // 1/sqrt(4 pi)
   double2 *__restrict__ M = d.M;
   // mixed precision: the solver factors an FP32 copy written alongside M
   float2 *__restrict__ W = (d.A != (void *)d.M) ? (float2 *)d.A : nullptr;
   const long ld = d.ld;
+  const int offi = d.site_off[i];
 
-  for (int i0 = 0; i0 < d.numLIZ; i0 += 4) {
-    const int i = i0 + w;
-    const bool act = i < d.numLIZ;
-    int lmi = 0, kkri = 1, kkri_ns = 1, offi = 0, lend = 0;
-    double rx = 0, ry = 0, rz = 0, r = 1;
-    if (act) {
-      lmi = d.lmax[i];
-      kkri = (lmi + 1) * (lmi + 1);
-      kkri_ns = kkri * nsc;
-      offi = d.site_off[i];
-      rx = d.pos[3 * i] - xj;
-      ry = d.pos[3 * i + 1] - yj;
-      rz = d.pos[3 * i + 2] - zj;
-      r = sqrt(rx * rx + ry * ry + rz * rz);
-      lend = lmi + lmj;
+  for (int j = w; j < d.numLIZ; j += 4) {
+    const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1), kkrj_ns = kkrj * nsc;
+    const int offj = d.site_off[j];
+    if (j == i) {
+      for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
+        int rr = e % kkri_ns, cc = e / kkri_ns;
+        M[(offi + rr) + ld * (offj + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
+        if (W) W[(offi + rr) + ld * (offj + cc)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
+      }
+      continue;
     }
-    const bool offdiag = act && (i != j);
-    // --- D_{l,m}, one (l,m) per lane (makegij_c.f:186-214)
-    if (offdiag && lane < (lend + 1) * (lend + 1)) {
-      int l = (int)sqrt((double)lane + 0.5);
-      while (l * l > lane) l--;
-      while ((l + 1) * (l + 1) <= lane) l++;
-      int m = lane - l * (l + 1), am = m < 0 ? -m : m;
-      double2 h = hankel_l(prel, r, l);
-      double p = plm_bar(l, am, rz / r);
-      double pmag = sqrt(rx * rx + ry * ry);
+    const double rx = xi - d.pos[3 * j], ry = yi - d.pos[3 * j + 1], rz = zi - d.pos[3 * j + 2];
+    const double r = sqrt(rx * rx + ry * ry + rz * rz);
+    const int lend = lmi + lmj;
+    // --- D_{l,m}, one (l,m) per lane (makegij_c.f:139-214)
+    if (lane < (lend + 1) * (lend + 1)) {
+      const int l = lofk[lane], m = lane - l * (l + 1), am = m < 0 ? -m : m;
+      double2 h;
+      if (prel0) {  // static limit: hfn_l = i^l (2l-1)!! / r^(l+1)
+        double v = 1.0 / r;
+        for (int k = 1; k <= l; k++) v *= (2.0 * k - 1.0) / r;
+        h = cmul(ipow_c(l), make_double2(v, 0.0));
+      } else {
+        // p_0 = -i, p_1 = -1 - i/z, p_{k+1} = (2k+1) p_k / z - p_{k-1};
+        // hfn_l = -p_l e^{iz}/r i^{l+1}
+        const double2 z = make_double2(prel.x * r, prel.y * r);
+        const double zn = 1.0 / (z.x * z.x + z.y * z.y);
+        const double2 iz = make_double2(z.x * zn, -z.y * zn);  // 1/z
+        double2 pm = make_double2(0.0, -1.0);
+        double2 p = make_double2(-1.0 + iz.y, -iz.x);          // -1 - i/z
+        if (l == 0) p = pm;
+        for (int k = 1; k < l; k++) {
+          double2 t = cscale(cmul(p, iz), 2.0 * k + 1.0);
+          double2 pn = make_double2(t.x - pm.x, t.y - pm.y);
+          pm = p;
+          p = pn;
+        }
+        const double ex = exp(-z.y) / r;
+        double sn, cs;
+        sincos(z.x, &sn, &cs);
+        h = cmul(cmul(p, make_double2(ex * cs, ex * sn)), ipow_c(l + 1));
+        h = make_double2(-h.x, -h.y);
+      }
+      // Pbar_{l,|m|}(cos theta)
+      const double x = rz / r, y = sqrt(1.0 - x * x);
+      double pmm = y00;
+      for (int k = 1; k <= am; k++) pmm = -pmmf[k] * y * pmm;
+      double pv = pmm;
+      if (l > am) {
+        double p0 = pmm, p1 = pm1f[am] * x * pmm;
+        for (int ll = am + 2; ll <= l; ll++) {
+          double pn = pa[ll][am] * (x * p1 - pb[ll][am] * p0);
+          p0 = p1;
+          p1 = pn;
+        }
+        pv = p1;
+      }
+      // cos/sin(|m| phi), zero azimuth on the z axis
+      const double pmag = sqrt(rx * rx + ry * ry);
       double c1 = 0.0, s1 = 0.0;
       if (pmag >= 1.0e-6) {
         c1 = rx / pmag;
@@ -147,70 +235,73 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
       double cm = 1.0, sm = 0.0;
       for (int k = 1; k <= am; k++) {
         double cn = cm * c1 - sm * s1;
-        double sn = sm * c1 + cm * s1;
+        double snn = sm * c1 + cm * s1;
         cm = cn;
-        sm = sn;
+        sm = snn;
       }
-      double2 fac = make_double2(p * cm, p * sm);
+      const double2 fac = make_double2(pv * cm, pv * sm);
       double2 v;
-      if (m == 0) v = cscale(h, p);
+      if (m == 0) v = cscale(h, pv);
       else if (m < 0) v = cscale(cmul(h, fac), (am & 1) ? -1.0 : 1.0);
       else v = cmul(h, make_double2(fac.x, -fac.y));
       dl[w][lane] = v;
     }
-    __syncthreads();
-    // --- G_ij(L2, L1) (makegij_c.f:240-263)
-    if (offdiag) {
-      for (int e = lane; e < kkri * kkrj; e += 64) {
-        int lm2 = e % kkri, lm1 = e / kkri;
-        int l1 = (int)sqrt((double)lm1 + 0.5);
-        while (l1 * l1 > lm1) l1--;
-        while ((l1 + 1) * (l1 + 1) <= lm1) l1++;
-        int l2 = (int)sqrt((double)lm2 + 0.5);
-        while (l2 * l2 > lm2) l2--;
-        while ((l2 + 1) * (l2 + 1) <= lm2) l2++;
-        int m1 = lm1 - l1 * (l1 + 1), m2 = lm2 - l2 * (l2 + 1), m3 = m2 - m1;
-        int am3 = m3 < 0 ? -m3 : m3, dl12 = l1 - l2 < 0 ? l2 - l1 : l1 - l2;
+    wave_sync();
+    // --- G_ij(L2, L1) (makegij_c.f:240-263), stored 16 x 16 zero-padded
+    for (int e = lane; e < kMaxKkr * kMaxKkr; e += 64) {
+      const int lm2 = e % kMaxKkr, lm1 = e / kMaxKkr;
+      double2 gv = make_double2(0.0, 0.0);
+      if (lm2 < kkri && lm1 < kkrj) {
+        const int l1 = lofk[lm1], l2 = lofk[lm2];
+        const int m1 = lm1 - l1 * (l1 + 1), m2 = lm2 - l2 * (l2 + 1), m3 = m2 - m1;
+        const int am3 = m3 < 0 ? -m3 : m3, dl12 = l1 - l2 < 0 ? l2 - l1 : l1 - l2;
         int llow = am3 > dl12 ? am3 : dl12;
         if (prel0) llow = l1 + l2;
-        double2 s = make_double2(0.0, 0.0);
+        double2 sacc = make_double2(0.0, 0.0);
         for (int l3 = l1 + l2; l3 >= llow; l3 -= 2) {
-          double c = cg[l3 / 2 + (lmax_tab + 1) * (lm1 + kkr_tab * lm2)];
-          double2 dv = dl[w][l3 * (l3 + 1) + m3];
-          s.x += c * dv.x;
-          s.y += c * dv.y;
+          const double c = cg[l3 / 2 + (lmax_tab + 1) * (lm1 + kkr_tab * lm2)];
+          const double2 dv = dl[w][l3 * (l3 + 1) + m3];
+          sacc.x += c * dv.x;
+          sacc.y += c * dv.y;
         }
-        gs[w][lm2 + kkri * lm1] = cscale(cmul(il[lm2 + kkr_tab * lm1], s), pi4);
+        gv = cscale(cmul(il[lm2 + kkr_tab * lm1], sacc), pi4);
       }
+      gs[w][lm2 + kMaxKkr * lm1] = gv;
     }
-    __syncthreads();
-    // --- M block = -tpack_i * diag_spin(G) (calculateTauMatrix.cpp:155-183,240-242)
-    if (offdiag) {
-      const double2 *__restrict__ T = tmat + ldT * d.store[i] + d.t_off;
-      const int ldt = d.ldt, kkrsz = d.kkrsz;
-      for (int e = lane; e < kkri_ns * kkrj_ns; e += 64) {
-        int rr = e % kkri_ns, cc = e / kkri_ns;
-        int isr = rr / kkri, ii = rr % kkri;
-        int cs = cc / kkrj, jj = cc % kkrj;
-        const double2 *Trow = T + (isr * kkrsz + ii) + (long)ldt * (cs * kkrsz);
-        double2 acc = make_double2(0.0, 0.0);
-        for (int k = 0; k < kkri; k++) {
-          double2 tv = Trow[(long)ldt * k];
-          double2 gv = gs[w][k + kkri * jj];
-          acc.x += tv.x * gv.x - tv.y * gv.y;
-          acc.y += tv.x * gv.y + tv.y * gv.x;
+    wave_sync();
+    // --- M block = -tpack_i diag_spin(G) (calculateTauMatrix.cpp:240-242) on
+    // FP64 MFMA, 3M form, one 16 x 16 x 16 product per spin block pair
+    // (bi, bj): M_bi,bj = -t_bi,bj G.  Transposed product as in the LU update,
+    // so register r of lane (li, kq) is element (li, kq + 4r) of the block and
+    // each store is a run of 16 consecutive rows.
+    {
+      const int li = lane & 15, kq = lane >> 4;
+      for (int bi = 0; bi < nsc; bi++)
+        for (int bj = 0; bj < nsc; bj++) {
+          d4 q1 = {0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
+#pragma unroll
+          for (int k0 = 0; k0 < kMaxKkr; k0 += 4) {
+            const double2 g = gs[w][(k0 + kq) + kMaxKkr * li];
+            const double2 t = tn[bi * kMaxKkr + li][bj * kMaxKkr + k0 + kq];
+            q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.x, t.x, q1, 0, 0, 0);
+            q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.y, t.y, q2, 0, 0, 0);
+            q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.x + g.y, t.x + t.y, q3, 0, 0, 0);
+          }
+          if (li < kkri) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              const int jc = kq + 4 * r;
+              if (jc < kkrj) {
+                const double re = q1[r] - q2[r], im = q3[r] - q1[r] - q2[r];
+                const long idx = (offi + bi * kkri + li) + ld * (offj + bj * kkrj + jc);
+                M[idx] = make_double2(-re, -im);
+                if (W) W[idx] = make_float2((float)-re, (float)-im);
+              }
+            }
+          }
         }
-        M[(offi + rr) + ld * (offj + cc)] = make_double2(-acc.x, -acc.y);
-        if (W) W[(offi + rr) + ld * (offj + cc)] = make_float2((float)-acc.x, (float)-acc.y);
-      }
-    } else if (act) {
-      for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
-        int rr = e % kkri_ns, cc = e / kkri_ns;
-        M[(offi + rr) + ld * (offj + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
-        if (W) W[(offi + rr) + ld * (offj + cc)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
-      }
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
