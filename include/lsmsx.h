@@ -117,10 +117,51 @@ int lsmsx_set_workspace_limit(lsmsx_ctx *ctx, long bytes);
  * the context's stream; lsmsx_get_profile returns the accumulated time (ms)
  * and launch count of one kernel kind (0 assemble, 1 diagonal-tile LU,
  * 2 panel, 3 trailing update, 4 last tile, 5 refinement residual,
- * 6 refinement solve).  lsmsx_reset_profile clears them. */
+ * 6 refinement solve, 7 explicit inverse, 8 Lloyd/adjoint contraction).
+ * lsmsx_reset_profile clears them. */
 int lsmsx_set_profiling(lsmsx_ctx *ctx, int on);
 int lsmsx_reset_profile(lsmsx_ctx *ctx);
 int lsmsx_get_profile(lsmsx_ctx *ctx, int kind, double *ms, long *launches);
+
+/* Upload dt/dE for every entry of the t-matrix store, in the store's layout
+ * and dimensions (needed for the Lloyd term).  The reference has no such
+ * store: its t-matrices come from the single-site solver at each energy
+ * (SingleSite/SingleSiteScattering.cpp:6-54); the caller supplies their
+ * energy derivative. */
+int lsmsx_upload_dtmats(lsmsx_ctx *ctx, const double *dtmat, long ldTmatStore,
+                        int numStore, long blkSizeTmatStore, int groupSize);
+
+/* Full tau = M^-1 T (nrmat_ns x nrmat_ns, reference site order, leading
+ * dimension ldtau) with T the block-diagonal of the LIZ sites' t-matrices
+ * of energy slot iie, from an explicit FP64 inverse of the device-built M.
+ * Replaces solveTauFullzgetrf (linearSolvers_CPU.cpp:125-144, zgetrf +
+ * zgetrs with N right-hand sides; the reference always reads slot 0 and
+ * assumes identical lmax: pass iie = 0 to match it, reduced-lmax sites use
+ * the packing of the assembly) as called by Kubo/CurrentMatrix.cpp:184-190. */
+int lsmsx_tau_full(lsmsx_ctx *ctx, int atom_id, int ispin, int iie,
+                   const double *prel, int n_spin_cant, int n_spin_pola,
+                   double *tau, int ldtau, int *info);
+
+/* Adjoint route (FP64, explicit inverse of each M from its LU).  For nsolve
+ * (atom, spin, slot) solves with energies E (2*nsolve doubles, needed when
+ * lloyd != NULL) and prel = sqrt(E (1 + E/c^2)) (2*nsolve doubles, != 0):
+ *   tau00[b]  (kkrsz_ns^2 complex, may be NULL) as lsmsx_tau00_batched;
+ *   lloyd[b]  (1 complex, may be NULL) = Tr(M^-1 dM/dE) = d ln det M / dE,
+ *             dM/dE = -(dt/dE G + t dG/dE), dt/dE from lsmsx_upload_dtmats;
+ *   with tau00_bar (nsolve kkrsz_ns^2 complex cotangents, may be NULL):
+ *   tmat_grad (ldTmatStore x numStore complex, overwritten) = the vector-
+ *             Jacobian product sum_b J_b^H tau00_bar[b] of tau00 with respect
+ *             to every t-matrix store entry (conjugate convention of complex
+ *             reverse-mode differentiation: for tau00 = R t C, grad_t =
+ *             R^H bar C^H).
+ * kkrsz_ns must be 16 or 32.  No counterpart in the reference (it has no
+ * Lloyd term or derivative of tau); the tau00 it returns is that of
+ * lsmsx_tau00_batched. */
+int lsmsx_tau00_adjoint(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
+                        const int *ispins, const int *iies, const double *energies,
+                        const double *prel, int n_spin_cant, int n_spin_pola,
+                        double *tau00, double *lloyd, const double *tau00_bar,
+                        double *tmat_grad, int *info);
 
 #ifdef __cplusplus
 }

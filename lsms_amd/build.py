@@ -18,7 +18,7 @@ DRIVER = os.path.join(HERE, "_build", "host_mirror_driver")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kkr_assembly.hip", "tau_lu.hip", "ir.hip", "lsmsx_capi.hip"]
+HIP_SOURCES = ["kkr_assembly.hip", "tau_lu.hip", "ir.hip", "inverse.hip", "lsmsx_capi.hip"]
 CXX_SOURCES = ["tables.cpp"]
 
 HIP_FLAGS = [

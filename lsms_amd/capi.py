@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "liblsmsx.so")
 PREC_FP64 = 0
 PREC_MIXED_FP32 = 1
 (KIND_ASSEMBLE, KIND_DIAG, KIND_PANEL, KIND_UPDATE, KIND_FINISH, KIND_IR_RESIDUAL,
- KIND_IR_SOLVE) = range(7)
+ KIND_IR_SOLVE, KIND_INVERSE, KIND_ADJOINT) = range(9)
 
 EXPORTED = [
     "lsmsx_last_error", "lsmsx_ctx_create", "lsmsx_ctx_destroy", "lsmsx_set_tables",
@@ -21,6 +21,7 @@ EXPORTED = [
     "lsmsx_tau00", "lsmsx_tau00_batched", "lsmsx_solve_tau00_matrix",
     "lsmsx_set_workspace_limit", "lsmsx_set_profiling", "lsmsx_reset_profile",
     "lsmsx_get_profile", "lsmsx_set_refinement", "lsmsx_last_batch_stats",
+    "lsmsx_upload_dtmats", "lsmsx_tau_full", "lsmsx_tau00_adjoint",
 ]
 
 _lib = None
@@ -56,6 +57,11 @@ def load():
     L.lsmsx_get_profile.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
     L.lsmsx_set_refinement.argtypes = [_vp, C.c_int, C.c_double]
     L.lsmsx_last_batch_stats.argtypes = [_vp, C.c_int, _vp, _vp]
+    L.lsmsx_upload_dtmats.argtypes = [_vp, _vp, C.c_long, C.c_int, C.c_long, C.c_int]
+    L.lsmsx_tau_full.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
+                                 C.c_int, _vp]
+    L.lsmsx_tau00_adjoint.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
+                                      _vp, _vp, _vp, _vp, _vp]
     _lib = L
     return L
 
@@ -168,6 +174,45 @@ class Context:
         _ck(load().lsmsx_solve_tau00_matrix(self._h, n, k, _p(mm), n, _p(tt), _p(tau),
                                             C.byref(info)))
         return tau, info.value
+
+    def upload_dtmats(self, dtmat_store, blk, group):
+        ts = np.asfortranarray(dtmat_store, dtype=np.complex128)
+        _ck(load().lsmsx_upload_dtmats(self._h, _p(ts), ts.shape[0], ts.shape[1], blk, group))
+
+    def tau_full(self, atom_id, iie, prel, nrmat_ns, ispin=0):
+        """tau = M^-1 T (nrmat_ns x nrmat_ns, reference site order) and info."""
+        tau = np.zeros((nrmat_ns, nrmat_ns), dtype=np.complex128, order="F")
+        pr = np.array([complex(prel)], dtype=np.complex128)
+        info = C.c_int(0)
+        _ck(load().lsmsx_tau_full(self._h, atom_id, ispin, iie, _p(pr), self.nsc, self.nspola,
+                                  _p(tau), nrmat_ns, C.byref(info)))
+        return tau, info.value
+
+    def tau00_adjoint(self, atom_ids, iies, energies, kkrsz_ns, ispins=None, lloyd=True,
+                      tau00_bar=None, store_shape=None):
+        """Adjoint route: (tau00 (n, k, k), lloyd (n,) or None, t-store gradient or None, info).
+        tau00_bar: (n, k, k) cotangents (row-major blocks, like the returned tau00);
+        store_shape: shape of the t-matrix store (for the gradient)."""
+        n = len(atom_ids)
+        a = np.ascontiguousarray(atom_ids, dtype=np.int32)
+        e = np.ascontiguousarray(iies, dtype=np.int32)
+        s = np.zeros(n, dtype=np.int32) if ispins is None else np.ascontiguousarray(ispins, dtype=np.int32)
+        en = np.ascontiguousarray(energies, dtype=np.complex128)
+        from .configs import prel_of
+        pr = np.ascontiguousarray(prel_of(en), dtype=np.complex128)
+        tau = np.zeros((n, kkrsz_ns, kkrsz_ns), dtype=np.complex128)
+        ll = np.zeros(n, dtype=np.complex128) if lloyd else None
+        info = np.zeros(n, dtype=np.int32)
+        vb = grad = None
+        if tau00_bar is not None:
+            vb = np.ascontiguousarray(np.transpose(np.asarray(tau00_bar, dtype=np.complex128),
+                                                   (0, 2, 1)))
+            grad = np.zeros(store_shape, dtype=np.complex128, order="F")
+        _ck(load().lsmsx_tau00_adjoint(self._h, n, _p(a), _p(s), _p(e), _p(en), _p(pr), self.nsc,
+                                       self.nspola, _p(tau), _p(ll) if lloyd else None,
+                                       _p(vb) if vb is not None else None,
+                                       _p(grad) if grad is not None else None, _p(info)))
+        return np.ascontiguousarray(np.transpose(tau, (0, 2, 1))), ll, grad, info
 
     def set_refinement(self, max_sweeps=12, tol=1e-12):
         _ck(load().lsmsx_set_refinement(self._h, int(max_sweeps), float(tol)))

@@ -24,6 +24,12 @@ def prel_of(e):
     return np.sqrt(e * (1.0 + e / (C_PHOT * C_PHOT)))
 
 
+def dprel_of(e):
+    """d kappa / dE = (1 + 2E/c^2) / (2 kappa)."""
+    e = np.asarray(e, dtype=np.complex128)
+    return (1.0 + 2.0 * e / (C_PHOT * C_PHOT)) / (2.0 * prel_of(e))
+
+
 def contour_semicircle(ebot, etop, npts):
     """Gauss-Legendre semicircle contour, nodes ascending from ebot to etop."""
     x, w = np.polynomial.legendre.leggauss(npts)
@@ -127,6 +133,7 @@ class System:
     tmatStore: np.ndarray     # (lDim, num_store) complex, Fortran order
     blkSizeTmatStore: int
     deltas: np.ndarray        # (num_store, nspin, lmax+1) phase shifts
+    evecs: np.ndarray = None  # (num_store, 3) moment directions (spin-canted systems)
 
     @property
     def kkrsz_ns(self):
@@ -137,6 +144,26 @@ class System:
 
     def prel(self, ie):
         return complex(prel_of(self.energies[ie]))
+
+    def dtmat_store(self):
+        """dt/dE for every store entry.  The phase shifts do not depend on E,
+        so every t_l carries 1/kappa and dt/dE = -(kappa'/kappa) t."""
+        d = np.array(self.tmatStore, order="F", copy=True)
+        blk = self.blkSizeTmatStore
+        for ie, e in enumerate(self.energies):
+            d[ie * blk:(ie + 1) * blk, :] *= -dprel_of(e) / prel_of(e)
+        return d
+
+    def at_energies(self, energies, weights=None):
+        """The same atoms and t-matrix types with a new energy group."""
+        energies = np.asarray(energies, dtype=np.complex128)
+        store, blk = make_tmat_store(self.deltas, energies, self.maxlmax, self.n_spin_cant,
+                                     self.n_spin_pola, self.evecs)
+        w = np.zeros(len(energies), dtype=np.complex128) if weights is None else weights
+        return System(name=self.name, maxlmax=self.maxlmax, n_spin_cant=self.n_spin_cant,
+                      n_spin_pola=self.n_spin_pola, atoms=self.atoms, energies=energies,
+                      weights=np.asarray(w, dtype=np.complex128), tmatStore=store,
+                      blkSizeTmatStore=blk, deltas=self.deltas, evecs=self.evecs)
 
 
 def make_tmat_store(deltas, energies, lmax, n_spin_cant, n_spin_pola, evecs_by_store=None):
@@ -234,7 +261,7 @@ def make_system(name, kind, a, ncell, rcut, lmax, energies, weights, nspecies=2,
     return System(name=name, maxlmax=lmax, n_spin_cant=n_spin_cant, n_spin_pola=n_spin_pola,
                   atoms=atoms, energies=np.asarray(energies, dtype=np.complex128),
                   weights=np.asarray(weights, dtype=np.complex128), tmatStore=store,
-                  blkSizeTmatStore=blk, deltas=deltas)
+                  blkSizeTmatStore=blk, deltas=deltas, evecs=evecs)
 
 
 def config_c1(npts=31, seed=20261202):
@@ -266,3 +293,33 @@ def config_c3(seed=20261202, natoms=32):
     e, w = config_c3_energies()
     return make_system("c3_fcc135_mixed", "fcc", 6.82, (2, 2, 2), 13.0, 3, e, w, nspecies=2,
                        seed=seed, local_atoms=list(range(natoms)))
+
+
+def config_c5(npts=31, seed=20261202, natoms=8, rcut=13.3):
+    """Spin-canted bcc a=5.42 bohr, n_spin_cant = n_spin_pola = 2, LIZ r=13.3
+    (137 sites, N = 137*32 = 4384), lmax=3, 31-point semicircle contour.
+    2x2x2 conventional cells (16 sites); every site is its own t-matrix type
+    (store column = site) with a moment direction uniform on the sphere and
+    the spin-up/down phase shifts of its species."""
+    rng = np.random.default_rng(seed)
+    lmax = 3
+    e, w = contour_semicircle(-0.5, 0.7, npts)
+    pos, box = lattice_sites("bcc", 5.42, (2, 2, 2))
+    nat = len(pos)
+    species = rng.integers(0, 2, nat)
+    dspec = rng.uniform(-1.2, 1.2, size=(2, 2, lmax + 1))
+    v = rng.normal(size=(nat, 3))
+    evecs = v / np.linalg.norm(v, axis=1, keepdims=True)
+    deltas = dspec[species]
+    atoms = []
+    for i in range(natoms):
+        rel, idx = build_liz(pos, box, i, rcut)
+        lm = liz_lmax(rel, lmax)
+        atoms.append(Atom(numLIZ=len(rel), LIZPos=np.asfortranarray(rel.T),
+                          LIZlmax=lm.astype(np.int32), LIZStoreIdx=np.asarray(idx, dtype=np.int32),
+                          lmax=lmax, kkrsz=(lmax + 1) ** 2, nrmat=int(np.sum((lm + 1) ** 2)),
+                          evec=evecs[i]))
+    store, blk = make_tmat_store(deltas, e, lmax, 2, 2, evecs)
+    return System(name="c5_bcc137_canted", maxlmax=lmax, n_spin_cant=2, n_spin_pola=2,
+                  atoms=atoms, energies=e, weights=w, tmatStore=store, blkSizeTmatStore=blk,
+                  deltas=deltas, evecs=evecs)

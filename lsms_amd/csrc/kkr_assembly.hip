@@ -32,54 +32,6 @@ __device__ inline double2 ipow_c(int n) {
   }
 }
 
-// hfn_l for one l (makegij_c.f:139-163), prel == 0 -> static limit.
-__device__ double2 hankel_l(double2 prel, double r, int l) {
-  if (prel.x == 0.0 && prel.y == 0.0) {
-    double2 h = make_double2(1.0 / r, 0.0);
-    for (int k = 1; k <= l; k++) h = cscale(cmul(make_double2(0.0, 1.0), h), (2.0 * k - 1.0) / r);
-    return h;
-  }
-  double2 z = make_double2(prel.x * r, prel.y * r);
-  double2 pm = make_double2(0.0, -1.0);                      // p_0 = -i
-  double2 iz = cdiv(make_double2(0.0, 1.0), z);
-  double2 p = make_double2(-1.0 - iz.x, -iz.y);              // p_1 = -1 - i/z
-  double2 pl = (l == 0) ? pm : p;
-  for (int k = 1; k < l; k++) {
-    double2 t = cdiv(cscale(p, 2.0 * k + 1.0), z);
-    double2 pn = make_double2(t.x - pm.x, t.y - pm.y);
-    pm = p;
-    p = pn;
-    pl = p;
-  }
-  // e^{iz}/r
-  double ex = exp(-z.y) / r;
-  double s, c;
-  sincos(z.x, &s, &c);
-  double2 f = make_double2(ex * c, ex * s);
-  double2 h = cmul(cmul(pl, f), ipow_c(l + 1));
-  return make_double2(-h.x, -h.y);
-}
-
-// Pbar_{l,m} (m >= 0) by the recursions of Misc/associatedLegendreFunction.hpp:33-69.
-__device__ double plm_bar(int l, int m, double x) {
-  const double pi = 3.141592653589793238462643383279502884;
-  double y = sqrt(1.0 - x * x);
-  double pmm = sqrt(1.0 / (4.0 * pi));
-  for (int k = 1; k <= m; k++) pmm = -sqrt((2.0 * k + 1.0) / (2.0 * k)) * y * pmm;
-  if (l == m) return pmm;
-  double p1 = sqrt(2.0 * m + 3.0) * x * pmm;  // Pbar_{m+1,m}
-  if (l == m + 1) return p1;
-  double p0 = pmm;
-  for (int ll = m + 2; ll <= l; ll++) {
-    double a = sqrt((4.0 * ll * ll - 1.0) / (double)(ll * ll - m * m));
-    double b = sqrt(((ll - 1.0) * (ll - 1.0) - m * m) / (4.0 * (ll - 1.0) * (ll - 1.0) - 1.0));
-    double pn = a * (x * p1 - b * p0);
-    p0 = p1;
-    p1 = pn;
-  }
-  return p1;
-}
-
 // Wave-scope ordering of LDS traffic: every lane's earlier LDS writes are
 // visible to the wave's later LDS reads.
 __device__ __forceinline__ void wave_sync() {
@@ -92,49 +44,106 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int kLmaxD = 2 * kMaxLmax;                       // l <= lend of D
 constexpr int kNd = (kLmaxD + 1) * (kLmaxD + 1);           // 49 (l,m) of D
 
-// Assemble row-site block row i = blockIdx.x of solve blockIdx.y:
-// M(i,j) = -tpack_i diag_spin(G_ij) for j != i, identity for j == i.
-// The workgroup stages the packed t_i and the angular tables in LDS; each
-// wave then builds blocks (i, j), j = w, w+4, ..., independently.
-__global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ descs,
-                                                  const double2 *__restrict__ tmat, long ldT,
-                                                  const double *__restrict__ cgnt_g,
-                                                  const double2 *__restrict__ illp_g,
-                                                  int lmax_tab) {
-  const SolveDesc &d = descs[blockIdx.y];
-  const int i = blockIdx.x;
-  if (i >= d.numLIZ) return;
+// hfn_l = -i^(l+1) p_l(z) e^{iz}/r, z = prel r (makegij_c.f:139-163), with
+// p_0 = -i, p_1 = -1 - i/z, p_{k+1} = (2k+1) p_k / z - p_{k-1}.  With dh
+// given, also d hfn_l / d prel = -i^(l+1) (p_l' + i p_l) e^{iz}, p' from the
+// differentiated recursion p_0' = 0, p_1' = i/z^2,
+// p_{k+1}' = (2k+1)(p_k'/z - p_k/z^2) - p_{k-1}'.
+__device__ __forceinline__ double2 hankel_lane(double2 prel, double r, int l, double2 *dh) {
+  const double2 z = make_double2(prel.x * r, prel.y * r);
+  const double zn = 1.0 / (z.x * z.x + z.y * z.y);
+  const double2 iz = make_double2(z.x * zn, -z.y * zn);  // 1/z
+  const double2 iz2 = cmul(iz, iz);
+  double2 pm = make_double2(0.0, -1.0);
+  double2 p = make_double2(-1.0 + iz.y, -iz.x);          // -1 - i/z
+  double2 dpm = make_double2(0.0, 0.0);
+  double2 dp = make_double2(-iz2.y, iz2.x);              // i/z^2
+  if (l == 0) {
+    p = pm;
+    dp = dpm;
+  }
+  for (int k = 1; k < l; k++) {
+    const double2 t = cscale(cmul(p, iz), 2.0 * k + 1.0);
+    const double2 pn = make_double2(t.x - pm.x, t.y - pm.y);
+    if (dh) {
+      const double2 a = cmul(dp, iz), b = cmul(p, iz2);
+      const double2 dpn = make_double2((2.0 * k + 1.0) * (a.x - b.x) - dpm.x,
+                                       (2.0 * k + 1.0) * (a.y - b.y) - dpm.y);
+      dpm = dp;
+      dp = dpn;
+    }
+    pm = p;
+    p = pn;
+  }
+  const double ex = exp(-z.y);
+  double sn, cs;
+  sincos(z.x, &sn, &cs);
+  const double2 e = make_double2(ex * cs, ex * sn);      // e^{iz}
+  const double2 il1 = ipow_c(l + 1);
+  const double2 h = cmul(cmul(p, cscale(e, 1.0 / r)), il1);
+  if (dh) {
+    const double2 q = make_double2(dp.x - p.y, dp.y + p.x);  // p' + i p
+    const double2 g = cmul(cmul(q, e), il1);
+    *dh = make_double2(-g.x, -g.y);
+  }
+  return make_double2(-h.x, -h.y);
+}
 
-  __shared__ double cg[(kMaxLmax + 1) * kMaxKkr * kMaxKkr];
-  __shared__ double2 il[kMaxKkr * kMaxKkr];
-  __shared__ double2 tn[2 * kMaxKkr][2 * kMaxKkr + 1];   // t_i in 16-aligned spin blocks, [row][col]
-  __shared__ double pa[kLmaxD + 1][kLmaxD + 1], pb[kLmaxD + 1][kLmaxD + 1];
-  __shared__ double pmmf[kLmaxD + 1], pm1f[kLmaxD + 1];
-  __shared__ int lofk[kNd];
-  __shared__ double2 dl[4][kNd];
-  __shared__ double2 gs[4][kMaxKkr * kMaxKkr];
+// Angular part of D_{l,m} (makegij_c.f:170-214): Pbar_{l,|m|}(cos theta) by
+// the recursions of Misc/associatedLegendreFunction.hpp:33-69 (coefficient
+// tables in LDS), cos/sin(|m| phi) with zero azimuth on the z axis, and the
+// sign rule D_{l,-m} = (-1)^m h Pbar e^{+i|m|phi}, D_{l,m} = h Pbar e^{-i m phi}.
+struct AngTab {
+  const double (*pa)[kLmaxD + 1];
+  const double (*pb)[kLmaxD + 1];
+  const double *pmmf, *pm1f;
+};
+__device__ __forceinline__ double2 angular_lane(const AngTab &tb, int l, int m, double rx,
+                                                double ry, double rz, double r) {
+  const double y00 = 0.28209479177387814347;  // 1/sqrt(4 pi)
+  const int am = m < 0 ? -m : m;
+  const double x = rz / r, y = sqrt(1.0 - x * x);
+  double pmm = y00;
+  for (int k = 1; k <= am; k++) pmm = -tb.pmmf[k] * y * pmm;
+  double pv = pmm;
+  if (l > am) {
+    double p0 = pmm, p1 = tb.pm1f[am] * x * pmm;
+    for (int ll = am + 2; ll <= l; ll++) {
+      double pn = tb.pa[ll][am] * (x * p1 - tb.pb[ll][am] * p0);
+      p0 = p1;
+      p1 = pn;
+    }
+    pv = p1;
+  }
+  const double pmag = sqrt(rx * rx + ry * ry);
+  double c1 = 0.0, s1 = 0.0;
+  if (pmag >= 1.0e-6) {
+    c1 = rx / pmag;
+    s1 = ry / pmag;
+  }
+  double cm = 1.0, sm = 0.0;
+  for (int k = 1; k <= am; k++) {
+    double cn = cm * c1 - sm * s1;
+    double snn = sm * c1 + cm * s1;
+    cm = cn;
+    sm = snn;
+  }
+  if (m == 0) return make_double2(pv, 0.0);
+  if (m < 0) return cscale(make_double2(pv * cm, pv * sm), (am & 1) ? -1.0 : 1.0);
+  return make_double2(pv * cm, -pv * sm);
+}
 
+// Per-workgroup LDS set-up shared by the assembly kernels: Gaunt and i^(l-l')
+// tables, Pbar recursion coefficients and l(k).
+__device__ __forceinline__ void load_tables(const double *__restrict__ cgnt_g,
+                                            const double2 *__restrict__ illp_g, int lmax_tab,
+                                            double *cg, double2 *il, double (*pa)[kLmaxD + 1],
+                                            double (*pb)[kLmaxD + 1], double *pmmf, double *pm1f,
+                                            int *lofk) {
   const int tid = threadIdx.x;
   const int kkr_tab = (lmax_tab + 1) * (lmax_tab + 1);
-  const int nsc = d.n_spin_cant, kkrsz = d.kkrsz, ldt = d.ldt;
-  const int lmi = d.lmax[i], kkri = (lmi + 1) * (lmi + 1), kkri_ns = kkri * nsc;
   for (int t = tid; t < (lmax_tab + 1) * kkr_tab * kkr_tab; t += 256) cg[t] = cgnt_g[t];
   for (int t = tid; t < kkr_tab * kkr_tab; t += 256) il[t] = illp_g[t];
-  // t_i packed as the F77 route does (calculateTauMatrix.cpp:155-183: the
-  // leading kkri block of each spin block), each spin block placed at a
-  // 16-aligned offset and zero-padded for the MFMA product below.
-  {
-    const double2 *__restrict__ T = tmat + ldT * d.store[i] + d.t_off;
-    for (int t = tid; t < 4 * kMaxKkr * kMaxKkr; t += 256) {
-      int r = t % (2 * kMaxKkr), c = t / (2 * kMaxKkr);
-      int rs = r / kMaxKkr, ri = r % kMaxKkr, cs = c / kMaxKkr, ci = c % kMaxKkr;
-      double2 v = make_double2(0.0, 0.0);
-      if (rs < nsc && cs < nsc && ri < kkri && ci < kkri)
-        v = T[(rs * kkrsz + ri) + (long)ldt * (cs * kkrsz + ci)];
-      tn[r][c] = v;
-    }
-  }
-  // Pbar recursion coefficients (Misc/associatedLegendreFunction.hpp:33-69)
   if (tid < (kLmaxD + 1) * (kLmaxD + 1)) {
     int l = tid / (kLmaxD + 1), m = tid % (kLmaxD + 1);
     double a = 0.0, b = 0.0;
@@ -154,14 +163,87 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
     while ((l + 1) * (l + 1) <= tid) l++;
     lofk[tid] = l;
   }
+}
+
+// t of site i packed as the F77 route does (calculateTauMatrix.cpp:155-183:
+// the leading kkri block of each spin block), each spin block placed at a
+// 16-aligned offset and zero-padded for the MFMA products.
+__device__ __forceinline__ void load_tpack(const double2 *__restrict__ T, int nsc, int kkrsz,
+                                           int ldt, int kkri, double2 (*tn)[2 * kMaxKkr + 1]) {
+  for (int t = threadIdx.x; t < 4 * kMaxKkr * kMaxKkr; t += 256) {
+    int r = t % (2 * kMaxKkr), c = t / (2 * kMaxKkr);
+    int rs = r / kMaxKkr, ri = r % kMaxKkr, cs = c / kMaxKkr, ci = c % kMaxKkr;
+    double2 v = make_double2(0.0, 0.0);
+    if (rs < nsc && cs < nsc && ri < kkri && ci < kkri)
+      v = T[(rs * kkrsz + ri) + (long)ldt * (cs * kkrsz + ci)];
+    tn[r][c] = v;
+  }
+}
+
+// G(L2, L1) = 4pi i^(l2-l1) sum_l3 cgnt(l3/2, L1, L2) D_{l3, m2-m1}
+// (makegij_c.f:240-263) for the wave's current pair, 16 x 16 zero-padded,
+// G(lm2, lm1) at g[lm2 + 16 lm1].
+__device__ __forceinline__ void gaunt_sum(const double *cg, const double2 *il, const int *lofk,
+                                          int lmax_tab, int kkri, int kkrj, bool prel0,
+                                          const double2 *dlw, double2 *g) {
+  const int lane = threadIdx.x & 63;
+  const int kkr_tab = (lmax_tab + 1) * (lmax_tab + 1);
+  const double pi4 = 4.0 * 3.141592653589793238462643383279502884;
+  for (int e = lane; e < kMaxKkr * kMaxKkr; e += 64) {
+    const int lm2 = e % kMaxKkr, lm1 = e / kMaxKkr;
+    double2 gv = make_double2(0.0, 0.0);
+    if (lm2 < kkri && lm1 < kkrj) {
+      const int l1 = lofk[lm1], l2 = lofk[lm2];
+      const int m1 = lm1 - l1 * (l1 + 1), m2 = lm2 - l2 * (l2 + 1), m3 = m2 - m1;
+      const int am3 = m3 < 0 ? -m3 : m3, dl12 = l1 - l2 < 0 ? l2 - l1 : l1 - l2;
+      int llow = am3 > dl12 ? am3 : dl12;
+      if (prel0) llow = l1 + l2;
+      double2 sacc = make_double2(0.0, 0.0);
+      for (int l3 = l1 + l2; l3 >= llow; l3 -= 2) {
+        const double c = cg[l3 / 2 + (lmax_tab + 1) * (lm1 + kkr_tab * lm2)];
+        const double2 dv = dlw[l3 * (l3 + 1) + m3];
+        sacc.x += c * dv.x;
+        sacc.y += c * dv.y;
+      }
+      gv = cscale(cmul(il[lm2 + kkr_tab * lm1], sacc), pi4);
+    }
+    g[e] = gv;
+  }
+}
+
+// Assemble row-site block row i = blockIdx.x of solve blockIdx.y:
+// M(i,j) = -tpack_i diag_spin(G_ij) for j != i, identity for j == i.
+// The workgroup stages the packed t_i and the angular tables in LDS; each
+// wave then builds blocks (i, j), j = w, w+4, ..., independently.
+__global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ descs,
+                                                  const double2 *__restrict__ tmat, long ldT,
+                                                  const double *__restrict__ cgnt_g,
+                                                  const double2 *__restrict__ illp_g,
+                                                  int lmax_tab) {
+  const SolveDesc &d = descs[blockIdx.y];
+  const int i = blockIdx.x;
+  if (i >= d.numLIZ) return;
+
+  __shared__ double cg[(kMaxLmax + 1) * kMaxKkr * kMaxKkr];
+  __shared__ double2 il[kMaxKkr * kMaxKkr];
+  __shared__ double2 tn[2 * kMaxKkr][2 * kMaxKkr + 1];   // packed t_i, [row][col]
+  __shared__ double pa[kLmaxD + 1][kLmaxD + 1], pb[kLmaxD + 1][kLmaxD + 1];
+  __shared__ double pmmf[kLmaxD + 1], pm1f[kLmaxD + 1];
+  __shared__ int lofk[kNd];
+  __shared__ double2 dl[4][kNd];
+  __shared__ double2 gs[4][kMaxKkr * kMaxKkr];
+
+  const int nsc = d.n_spin_cant;
+  const int lmi = d.lmax[i], kkri = (lmi + 1) * (lmi + 1), kkri_ns = kkri * nsc;
+  load_tables(cgnt_g, illp_g, lmax_tab, cg, il, pa, pb, pmmf, pm1f, lofk);
+  load_tpack(tmat + ldT * d.store[i] + d.t_off, nsc, d.kkrsz, d.ldt, kkri, tn);
   __syncthreads();
 
-  const int w = tid >> 6, lane = tid & 63;
+  const AngTab tb{pa, pb, pmmf, pm1f};
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double xi = d.pos[3 * i], yi = d.pos[3 * i + 1], zi = d.pos[3 * i + 2];
   const double2 prel = make_double2(d.prel_re, d.prel_im);
   const bool prel0 = (prel.x == 0.0 && prel.y == 0.0);
-  const double pi4 = 4.0 * 3.141592653589793238462643383279502884;
-  const double y00 = 0.28209479177387814347;  // 1/sqrt(4 pi)
   double2 *__restrict__ M = d.M;
   // mixed precision: the solver factors an FP32 copy written alongside M
   float2 *__restrict__ W = (d.A != (void *)d.M) ? (float2 *)d.A : nullptr;
@@ -169,7 +251,7 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
   const int offi = d.site_off[i];
 
   for (int j = w; j < d.numLIZ; j += 4) {
-    const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1), kkrj_ns = kkrj * nsc;
+    const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1);
     const int offj = d.site_off[j];
     if (j == i) {
       for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
@@ -182,92 +264,21 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
     const double rx = xi - d.pos[3 * j], ry = yi - d.pos[3 * j + 1], rz = zi - d.pos[3 * j + 2];
     const double r = sqrt(rx * rx + ry * ry + rz * rz);
     const int lend = lmi + lmj;
-    // --- D_{l,m}, one (l,m) per lane (makegij_c.f:139-214)
+    // --- D_{l,m}, one (l,m) per lane
     if (lane < (lend + 1) * (lend + 1)) {
-      const int l = lofk[lane], m = lane - l * (l + 1), am = m < 0 ? -m : m;
+      const int l = lofk[lane], m = lane - l * (l + 1);
       double2 h;
       if (prel0) {  // static limit: hfn_l = i^l (2l-1)!! / r^(l+1)
         double v = 1.0 / r;
         for (int k = 1; k <= l; k++) v *= (2.0 * k - 1.0) / r;
         h = cmul(ipow_c(l), make_double2(v, 0.0));
       } else {
-        // p_0 = -i, p_1 = -1 - i/z, p_{k+1} = (2k+1) p_k / z - p_{k-1};
-        // hfn_l = -p_l e^{iz}/r i^{l+1}
-        const double2 z = make_double2(prel.x * r, prel.y * r);
-        const double zn = 1.0 / (z.x * z.x + z.y * z.y);
-        const double2 iz = make_double2(z.x * zn, -z.y * zn);  // 1/z
-        double2 pm = make_double2(0.0, -1.0);
-        double2 p = make_double2(-1.0 + iz.y, -iz.x);          // -1 - i/z
-        if (l == 0) p = pm;
-        for (int k = 1; k < l; k++) {
-          double2 t = cscale(cmul(p, iz), 2.0 * k + 1.0);
-          double2 pn = make_double2(t.x - pm.x, t.y - pm.y);
-          pm = p;
-          p = pn;
-        }
-        const double ex = exp(-z.y) / r;
-        double sn, cs;
-        sincos(z.x, &sn, &cs);
-        h = cmul(cmul(p, make_double2(ex * cs, ex * sn)), ipow_c(l + 1));
-        h = make_double2(-h.x, -h.y);
+        h = hankel_lane(prel, r, l, nullptr);
       }
-      // Pbar_{l,|m|}(cos theta)
-      const double x = rz / r, y = sqrt(1.0 - x * x);
-      double pmm = y00;
-      for (int k = 1; k <= am; k++) pmm = -pmmf[k] * y * pmm;
-      double pv = pmm;
-      if (l > am) {
-        double p0 = pmm, p1 = pm1f[am] * x * pmm;
-        for (int ll = am + 2; ll <= l; ll++) {
-          double pn = pa[ll][am] * (x * p1 - pb[ll][am] * p0);
-          p0 = p1;
-          p1 = pn;
-        }
-        pv = p1;
-      }
-      // cos/sin(|m| phi), zero azimuth on the z axis
-      const double pmag = sqrt(rx * rx + ry * ry);
-      double c1 = 0.0, s1 = 0.0;
-      if (pmag >= 1.0e-6) {
-        c1 = rx / pmag;
-        s1 = ry / pmag;
-      }
-      double cm = 1.0, sm = 0.0;
-      for (int k = 1; k <= am; k++) {
-        double cn = cm * c1 - sm * s1;
-        double snn = sm * c1 + cm * s1;
-        cm = cn;
-        sm = snn;
-      }
-      const double2 fac = make_double2(pv * cm, pv * sm);
-      double2 v;
-      if (m == 0) v = cscale(h, pv);
-      else if (m < 0) v = cscale(cmul(h, fac), (am & 1) ? -1.0 : 1.0);
-      else v = cmul(h, make_double2(fac.x, -fac.y));
-      dl[w][lane] = v;
+      dl[w][lane] = cmul(h, angular_lane(tb, l, m, rx, ry, rz, r));
     }
     wave_sync();
-    // --- G_ij(L2, L1) (makegij_c.f:240-263), stored 16 x 16 zero-padded
-    for (int e = lane; e < kMaxKkr * kMaxKkr; e += 64) {
-      const int lm2 = e % kMaxKkr, lm1 = e / kMaxKkr;
-      double2 gv = make_double2(0.0, 0.0);
-      if (lm2 < kkri && lm1 < kkrj) {
-        const int l1 = lofk[lm1], l2 = lofk[lm2];
-        const int m1 = lm1 - l1 * (l1 + 1), m2 = lm2 - l2 * (l2 + 1), m3 = m2 - m1;
-        const int am3 = m3 < 0 ? -m3 : m3, dl12 = l1 - l2 < 0 ? l2 - l1 : l1 - l2;
-        int llow = am3 > dl12 ? am3 : dl12;
-        if (prel0) llow = l1 + l2;
-        double2 sacc = make_double2(0.0, 0.0);
-        for (int l3 = l1 + l2; l3 >= llow; l3 -= 2) {
-          const double c = cg[l3 / 2 + (lmax_tab + 1) * (lm1 + kkr_tab * lm2)];
-          const double2 dv = dl[w][l3 * (l3 + 1) + m3];
-          sacc.x += c * dv.x;
-          sacc.y += c * dv.y;
-        }
-        gv = cscale(cmul(il[lm2 + kkr_tab * lm1], sacc), pi4);
-      }
-      gs[w][lm2 + kMaxKkr * lm1] = gv;
-    }
+    gaunt_sum(cg, il, lofk, lmax_tab, kkri, kkrj, prel0, dl[w], gs[w]);
     wave_sync();
     // --- M block = -tpack_i diag_spin(G) (calculateTauMatrix.cpp:240-242) on
     // FP64 MFMA, 3M form, one 16 x 16 x 16 product per spin block pair
@@ -302,6 +313,177 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
         }
     }
     wave_sync();
+  }
+}
+
+// Energy derivative of the assembly contracted with the explicit inverse, one
+// workgroup per row site i = blockIdx.x of solve blockIdx.y:
+//   lpart[i] = sum_j Tr(Minv_ji dM_ij/dE),
+//     dM_ij/dE = -(tdot_i diag_spin(G_ij) + t_i diag_spin(dG_ij/dprel) dprel/dE)
+//   gx(site i rows) = sum_j diag_spin(G_ij) X_j,   X = M^-1 [0; t0]  (d.X)
+// tdot = dt/dE from the derivative store dtmat (same layout as tmat); with
+// dtmat == nullptr only gx is formed.  The (i, i) blocks are constant (M_ii =
+// 1, G_ii = 0) and contribute nothing.
+__global__ __launch_bounds__(256) void k_lloyd(const SolveDesc *__restrict__ descs,
+                                               const double2 *__restrict__ tmat,
+                                               const double2 *__restrict__ dtmat, long ldT,
+                                               const double *__restrict__ cgnt_g,
+                                               const double2 *__restrict__ illp_g,
+                                               int lmax_tab) {
+  const SolveDesc &d = descs[blockIdx.y];
+  const int i = blockIdx.x;
+  if (i >= d.numLIZ) return;
+
+  __shared__ double cg[(kMaxLmax + 1) * kMaxKkr * kMaxKkr];
+  __shared__ double2 il[kMaxKkr * kMaxKkr];
+  __shared__ double2 tn[2 * kMaxKkr][2 * kMaxKkr + 1];
+  __shared__ double2 tdn[2 * kMaxKkr][2 * kMaxKkr + 1];
+  __shared__ double pa[kLmaxD + 1][kLmaxD + 1], pb[kLmaxD + 1][kLmaxD + 1];
+  __shared__ double pmmf[kLmaxD + 1], pm1f[kLmaxD + 1];
+  __shared__ int lofk[kNd];
+  __shared__ double2 dl[4][kNd], ddl[4][kNd];
+  __shared__ double2 gs[4][kMaxKkr * kMaxKkr], gds[4][kMaxKkr * kMaxKkr];
+  __shared__ double2 red[2 * kMaxKkr][2 * kMaxKkr + 1];
+  __shared__ double2 trs[4];
+
+  const int nsc = d.n_spin_cant, kc = d.kc, Ns = d.Ns;
+  const int lmi = d.lmax[i], kkri = (lmi + 1) * (lmi + 1), kkri_ns = kkri * nsc;
+  const bool want_tr = dtmat != nullptr;
+  load_tables(cgnt_g, illp_g, lmax_tab, cg, il, pa, pb, pmmf, pm1f, lofk);
+  load_tpack(tmat + ldT * d.store[i] + d.t_off, nsc, d.kkrsz, d.ldt, kkri, tn);
+  if (want_tr) load_tpack(dtmat + ldT * d.store[i] + d.t_off, nsc, d.kkrsz, d.ldt, kkri, tdn);
+  __syncthreads();
+
+  const AngTab tb{pa, pb, pmmf, pm1f};
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+  const double xi = d.pos[3 * i], yi = d.pos[3 * i + 1], zi = d.pos[3 * i + 2];
+  const double2 prel = make_double2(d.prel_re, d.prel_im);
+  const double2 dprel = make_double2(d.dprel_re, d.dprel_im);
+  const long ld = d.ld;
+  const int offi = d.site_off[i];
+  const double2 *__restrict__ Minv = d.Minv;
+  const double2 *__restrict__ X = d.X;
+  const int ncb = kc / 16;
+  double2 tr = make_double2(0.0, 0.0);
+  d4 gq[2][2][3];
+#pragma unroll
+  for (int bs = 0; bs < 2; bs++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+      for (int m = 0; m < 3; m++) gq[bs][cb][m] = d4{0.0, 0.0, 0.0, 0.0};
+
+  for (int j = w; j < d.numLIZ; j += 4) {
+    if (j == i) continue;
+    const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1);
+    const int offj = d.site_off[j];
+    const double rx = xi - d.pos[3 * j], ry = yi - d.pos[3 * j + 1], rz = zi - d.pos[3 * j + 2];
+    const double r = sqrt(rx * rx + ry * ry + rz * rz);
+    const int lend = lmi + lmj;
+    if (lane < (lend + 1) * (lend + 1)) {
+      const int l = lofk[lane], m = lane - l * (l + 1);
+      double2 dh;
+      const double2 h = hankel_lane(prel, r, l, &dh);
+      const double2 ang = angular_lane(tb, l, m, rx, ry, rz, r);
+      dl[w][lane] = cmul(h, ang);
+      ddl[w][lane] = cmul(dprel, cmul(dh, ang));
+    }
+    wave_sync();
+    gaunt_sum(cg, il, lofk, lmax_tab, kkri, kkrj, false, dl[w], gs[w]);
+    if (want_tr) gaunt_sum(cg, il, lofk, lmax_tab, kkri, kkrj, false, ddl[w], gds[w]);
+    wave_sync();
+    // trace part: D = -(tdot G + t G'), element (li, kq + 4r) of spin block
+    // (bi, bj), against Minv(site j row bj*kkrj + kq + 4r, site i col bi*kkri + li)
+    if (want_tr) {
+      for (int bi = 0; bi < nsc; bi++)
+        for (int bj = 0; bj < nsc; bj++) {
+          d4 q1 = {0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
+#pragma unroll
+          for (int k0 = 0; k0 < kMaxKkr; k0 += 4) {
+            const double2 g = gs[w][(k0 + kq) + kMaxKkr * li];
+            const double2 t = tdn[bi * kMaxKkr + li][bj * kMaxKkr + k0 + kq];
+            q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.x, t.x, q1, 0, 0, 0);
+            q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.y, t.y, q2, 0, 0, 0);
+            q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.x + g.y, t.x + t.y, q3, 0, 0, 0);
+          }
+#pragma unroll
+          for (int k0 = 0; k0 < kMaxKkr; k0 += 4) {
+            const double2 g = gds[w][(k0 + kq) + kMaxKkr * li];
+            const double2 t = tn[bi * kMaxKkr + li][bj * kMaxKkr + k0 + kq];
+            q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.x, t.x, q1, 0, 0, 0);
+            q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.y, t.y, q2, 0, 0, 0);
+            q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.x + g.y, t.x + t.y, q3, 0, 0, 0);
+          }
+          if (li < kkri) {
+#pragma unroll
+            for (int r4 = 0; r4 < 4; r4++) {
+              const int jc = kq + 4 * r4;
+              if (jc < kkrj) {
+                const double2 dv = make_double2(-(q1[r4] - q2[r4]), -(q3[r4] - q1[r4] - q2[r4]));
+                const double2 mv = Minv[(offj + bj * kkrj + jc) + ld * (offi + bi * kkri + li)];
+                tr.x += mv.x * dv.x - mv.y * dv.y;
+                tr.y += mv.x * dv.y + mv.y * dv.x;
+              }
+            }
+          }
+        }
+    }
+    // (G X)_i part: spin block bs rows li, column 16 cb + kq + 4r
+#pragma unroll
+    for (int bs = 0; bs < 2; bs++)
+#pragma unroll
+      for (int cb = 0; cb < 2; cb++) {
+        if (bs >= nsc || cb >= ncb) continue;
+#pragma unroll
+        for (int k0 = 0; k0 < kMaxKkr; k0 += 4) {
+          const int kk = k0 + kq;
+          double2 x = make_double2(0.0, 0.0);
+          if (kk < kkrj) x = X[(offj + bs * kkrj + kk) + (long)Ns * (16 * cb + li)];
+          const double2 g = gs[w][li + kMaxKkr * kk];
+          gq[bs][cb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, g.x, gq[bs][cb][0], 0, 0, 0);
+          gq[bs][cb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, g.y, gq[bs][cb][1], 0, 0, 0);
+          gq[bs][cb][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x + x.y, g.x + g.y,
+                                                               gq[bs][cb][2], 0, 0, 0);
+        }
+      }
+    wave_sync();
+  }
+  // reductions: trace over lanes and waves, (G X)_i over waves in wave order
+  for (int off = 32; off >= 1; off >>= 1) {
+    tr.x += __shfl_xor(tr.x, off, 64);
+    tr.y += __shfl_xor(tr.y, off, 64);
+  }
+  if (lane == 0) trs[w] = tr;
+  for (int ww = 0; ww < 4; ww++) {
+    if (w == ww) {
+#pragma unroll
+      for (int bs = 0; bs < 2; bs++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+          for (int r4 = 0; r4 < 4; r4++) {
+            if (bs >= nsc || cb >= ncb) continue;
+            const d4 *q = gq[bs][cb];
+            const double2 v = make_double2(q[0][r4] - q[1][r4], q[2][r4] - q[0][r4] - q[1][r4]);
+            double2 &o = red[bs * kMaxKkr + li][16 * cb + kq + 4 * r4];
+            if (ww == 0) o = v;
+            else o = make_double2(o.x + v.x, o.y + v.y);
+          }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int ww = 0; ww < 4; ww++) {
+      s.x += trs[ww].x;
+      s.y += trs[ww].y;
+    }
+    d.lpart[i] = s;
+  }
+  for (int e = threadIdx.x; e < kkri_ns * kc; e += 256) {
+    const int p = e % kkri_ns, col = e / kkri_ns;
+    const int bs = p / kkri, a = p % kkri;
+    d.gx[(offi + p) + (long)Ns * col] = red[bs * kMaxKkr + a][col];
   }
 }
 

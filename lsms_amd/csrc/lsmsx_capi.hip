@@ -36,6 +36,21 @@ __global__ void k_ir_residual(const SolveDesc *);
 __global__ void k_ir_solve(const SolveDesc *, double);
 __global__ void k_ir_extract(const SolveDesc *);
 __global__ void k_count_conv(const SolveDesc *, int, int *);
+__global__ void k_lloyd(const SolveDesc *, const double2 *, const double2 *, long, const double *,
+                        const double2 *, int);
+__global__ void k_inv_zero(const SolveDesc *);
+__global__ void k_inv_fwd_panel(const SolveDesc *, int);
+__global__ void k_inv_fwd_update(const SolveDesc *, int);
+__global__ void k_inv_bwd_panel(const SolveDesc *, int);
+__global__ void k_inv_bwd_update(const SolveDesc *, int);
+__global__ void k_inv_x(const SolveDesc *);
+__global__ void k_tau_full_out(const SolveDesc *, const double2 *, long, const int *, const int *,
+                               double2 *, long);
+__global__ void k_lloyd_reduce(const SolveDesc *, int, double2 *);
+__global__ void k_tau_grad(const SolveDesc *, const double2 *, double2 *, int);
+__global__ void k_grad_reduce(const double2 *, int, const int *, const long *, const int *,
+                              const int *, double2 *, long, int);
+__global__ void k_zero_c(double2 *, long);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout.
 __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
@@ -132,7 +147,7 @@ struct lsmsx_ctx {
   int lmax_tab = -1;
   DevBuf<double> cgnt;
   DevBuf<double2> illp;
-  DevBuf<double2> tmat;
+  DevBuf<double2> tmat, dtmat;
   long ldT = 0, blk = 0;
   int nstore = 0, group = 0;
   std::map<int, AtomDev> atoms;
@@ -142,6 +157,10 @@ struct lsmsx_ctx {
   DevBuf<int> perm, info, conv, iters, count;
   DevBuf<double> minpiv, dxn;
   DevBuf<SolveDesc> desc, desc2;
+  // explicit inverse / adjoint
+  DevBuf<double2> Minv, gx, lpart, lloyd, vbar, gblk, gstore, tfull;
+  DevBuf<int> keyi, colmap;
+  DevBuf<long> keyl;
   std::vector<int> last_iters;
   std::vector<double> last_minpiv;
   long ws_limit = 0;
@@ -149,8 +168,8 @@ struct lsmsx_ctx {
   double ir_tol = 1e-12;
   // profiling
   bool profiling = false;
-  double prof_ms[7] = {0, 0, 0, 0, 0, 0, 0};
-  long prof_n[7] = {0, 0, 0, 0, 0, 0, 0};
+  double prof_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  long prof_n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;
 };
@@ -496,6 +515,267 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
   return 0;
 }
 
+// LU of every tile (last one included, all diagonal-tile inverses kept) and
+// the in-place explicit inverse M^-1 in d.Minv, for nb solves in solver layout.
+int launch_inverse(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max) {
+  for (int s = 0; s < nt_max; s++) {
+    int nrest = nt_max - 1 - s;
+    {
+      ProfScope p(c, 1);
+      hipLaunchKernelGGL(k_lu_diag<double>, dim3(nb), dim3(256), 0, c->stream, dd, s);
+    }
+    if (nrest > 0) {
+      {
+        ProfScope p(c, 2);
+        hipLaunchKernelGGL(k_lu_panel<double>, dim3(2 * nrest, nb), dim3(256), 0, c->stream, dd, s);
+      }
+      {
+        ProfScope p(c, 3);
+        hipLaunchKernelGGL(k_lu_update<double>, dim3(nrest * nrest, nb), dim3(256), 0, c->stream,
+                           dd, s);
+      }
+    }
+  }
+  if (int r = check_launch("full LU")) return r;
+  ProfScope p(c, 7);
+  hipLaunchKernelGGL(k_inv_zero, dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
+  for (int s = 0; s < nt_max; s++) {
+    hipLaunchKernelGGL(k_inv_fwd_panel, dim3(s + 1, nb), dim3(256), 0, c->stream, dd, s);
+    int nrest = nt_max - 1 - s;
+    if (nrest > 0)
+      hipLaunchKernelGGL(k_inv_fwd_update, dim3(nrest * (s + 1), nb), dim3(256), 0, c->stream, dd,
+                         s);
+  }
+  for (int s = nt_max - 1; s >= 0; s--) {
+    hipLaunchKernelGGL(k_inv_bwd_panel, dim3(nt_max, nb), dim3(256), 0, c->stream, dd, s);
+    if (s > 0)
+      hipLaunchKernelGGL(k_inv_bwd_update, dim3(s * nt_max, nb), dim3(256), 0, c->stream, dd, s);
+  }
+  return check_launch("explicit inverse");
+}
+
+// d prel / dE for prel = sqrt(E (1 + E/c^2)), c = 2 * 137.035999206
+// (calculateTauMatrix.cpp:846): (1 + 2E/c^2) / (2 prel).
+void dprel_de(const double *E, const double *prel, double *out) {
+  const double cl = 2.0 * 137.035999206, ic2 = 1.0 / (cl * cl);
+  const double nr = 1.0 + 2.0 * E[0] * ic2, ni = 2.0 * E[1] * ic2;
+  const double dr = 2.0 * prel[0], di = 2.0 * prel[1];
+  const double s = 1.0 / (dr * dr + di * di);
+  out[0] = (nr * dr + ni * di) * s;
+  out[1] = (ni * dr - nr * di) * s;
+}
+
+// Adjoint route over a batch: tau00, Tr(M^-1 dM/dE) and, given the cotangent
+// of tau00, the t-store gradient (see lsmsx_tau00_adjoint).
+int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins, const int *iies,
+                const double *energies, const double *prel, int nsc, int nspola, double *tau_out,
+                double *lloyd_out, const double *tau_bar, double *tmat_grad, int *info_out) {
+  if (int r = validate_common(c, nsc, nspola)) return r;
+  if (nsolve <= 0) return 0;
+  if (lloyd_out && !c->dtmat.p) return fail(-3, "Lloyd term requested but no dt/dE store uploaded");
+  if (lloyd_out && !energies) return fail(-2, "Lloyd term needs the energies");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<long> bytes(nsolve);
+  int kc = -1;
+  for (int i = 0; i < nsolve; i++) {
+    auto it = c->atoms.find(atom_ids[i]);
+    if (it == c->atoms.end()) return fail(-5, "unknown atom id " + std::to_string(atom_ids[i]));
+    const AtomDev &a = it->second;
+    int kci = nsc * a.kkrsz;
+    if (kc < 0) kc = kci;
+    if (kci != kc) return fail(-6, "all atoms of a batch must share kkrsz");
+    if (kc != 16 && kc != 32) return fail(-6, "the adjoint route needs kkrsz_ns = 16 or 32");
+    if (a.kkrsz > (c->lmax_tab + 1) * (c->lmax_tab + 1))
+      return fail(-6, "atom lmax exceeds the lmax of the uploaded tables");
+    if (iies[i] < 0 || iies[i] >= c->group) return fail(-7, "energy slot out of range");
+    if (ispins[i] < 0 || ispins[i] >= nspola) return fail(-7, "spin index out of range");
+    if (prel[2 * i] == 0.0 && prel[2 * i + 1] == 0.0)
+      return fail(-7, "the adjoint route needs prel != 0");
+    for (int s = 0; s < a.numLIZ; s++)
+      if (a.store[s] < 0 || a.store[s] >= c->nstore) return fail(-7, "LIZStoreIdx out of range");
+    long Ns = roundup(nsc * a.nrmat, kTile), nt = Ns / kTile;
+    bytes[i] = 2 * Ns * Ns * 16 + nt * kTile * kTile * 16 + 2 * Ns * kc * 16 +
+               (tau_bar ? (long)a.numLIZ * kc * kc * 16 : 0);
+  }
+  long cap = c->ws_limit;
+  if (cap <= 0) {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    long held = (long)((c->M.n + c->Minv.n + c->inv.n + c->gblk.n) * sizeof(double2));
+    cap = (long)((double)(fr + held) * 0.7);
+  }
+  if (tau_bar) {
+    if (int r = c->gstore.ensure((size_t)c->ldT * c->nstore)) return r;
+    hipLaunchKernelGGL(k_zero_c, dim3(1024), dim3(256), 0, c->stream, c->gstore.p,
+                       (long)c->ldT * c->nstore);
+  }
+  int done = 0;
+  while (done < nsolve) {
+    long acc = 0;
+    int end = done;
+    while (end < nsolve && (end == done || acc + bytes[end] <= cap)) acc += bytes[end++];
+    int nb = end - done;
+    long mtot = 0, xtot = 0, ivtot = 0;
+    int maxliz = 1, nt_max = 1;
+    for (int b = 0; b < nb; b++) {
+      const AtomDev &a = c->atoms[atom_ids[done + b]];
+      long Ns = roundup(nsc * a.nrmat, kTile);
+      mtot += Ns * Ns;
+      xtot += Ns * kc;
+      ivtot += (Ns / kTile) * kTile * kTile;
+      maxliz = std::max(maxliz, a.numLIZ);
+      nt_max = std::max(nt_max, (int)(Ns / kTile));
+    }
+    if (int r = c->M.ensure(mtot)) return r;
+    if (int r = c->Minv.ensure(mtot)) return r;
+    if (int r = c->inv.ensure(ivtot)) return r;
+    if (int r = c->perm.ensure((size_t)nb * nt_max * kTile)) return r;
+    if (int r = c->info.ensure((size_t)nb)) return r;
+    if (int r = c->minpiv.ensure((size_t)nb)) return r;
+    if (int r = c->tau.ensure((size_t)nb * kc * kc)) return r;
+    if (int r = c->X.ensure(xtot)) return r;
+    if (int r = c->gx.ensure(xtot)) return r;
+    if (int r = c->lpart.ensure((size_t)nb * maxliz)) return r;
+    if (int r = c->lloyd.ensure((size_t)nb)) return r;
+    if (int r = c->desc.ensure((size_t)nb)) return r;
+    std::vector<SolveDesc> hd(nb);
+    long moff = 0, xoff = 0, ioff = 0;
+    for (int b = 0; b < nb; b++) {
+      int i = done + b;
+      AtomDev &a = c->atoms[atom_ids[i]];
+      SolveDesc &d = hd[b];
+      d = SolveDesc{};
+      int N = nsc * a.nrmat, Ns = roundup(N, kTile);
+      d.pos = a.d_pos;
+      d.lmax = a.d_lmax;
+      d.store = a.d_store;
+      d.site_off = a.d_off[1][nsc - 1];
+      d.numLIZ = a.numLIZ;
+      d.n_spin_cant = nsc;
+      d.kkrsz = a.kkrsz;
+      d.kc = kc;
+      d.prel_re = prel[2 * i];
+      d.prel_im = prel[2 * i + 1];
+      if (energies) {
+        double dp[2];
+        dprel_de(energies + 2 * i, prel + 2 * i, dp);
+        d.dprel_re = dp[0];
+        d.dprel_im = dp[1];
+      }
+      long jsm = (nspola != nsc) ? (long)kc * kc * ispins[i] : 0;
+      d.t_off = (long)iies[i] * c->blk + jsm;
+      d.ldt = kc;
+      d.M = c->M.p + moff;
+      d.A = d.M;
+      d.Minv = c->Minv.p + moff;
+      d.ld = Ns;
+      d.Ns = Ns;
+      d.pad = Ns - N;
+      d.nt = Ns / kTile;
+      d.full_lu = 1;
+      d.inv = c->inv.p + ioff;
+      d.inv_stride = kTile * kTile;
+      ioff += (long)d.nt * kTile * kTile;
+      d.perm = c->perm.p + (long)b * nt_max * kTile;
+      d.info = c->info.p + b;
+      d.minpiv = c->minpiv.p + b;
+      d.tau00 = c->tau.p + (long)b * kc * kc;
+      d.t0 = c->tmat.p + c->ldT * a.store[0] + d.t_off;
+      d.X = c->X.p + xoff;
+      d.gx = c->gx.p + xoff;
+      d.lpart = c->lpart.p + (long)b * maxliz;
+      xoff += (long)Ns * kc;
+      moff += (long)Ns * Ns;
+    }
+    HIPCHK(hipMemcpyAsync(c->desc.p, hd.data(), nb * sizeof(SolveDesc), hipMemcpyHostToDevice,
+                          c->stream));
+    hipLaunchKernelGGL(k_init_status, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->desc.p, nb);
+    {
+      ProfScope p(c, 0);
+      hipLaunchKernelGGL(k_pad_identity, dim3(nb), dim3(256), 0, c->stream, c->desc.p);
+      hipLaunchKernelGGL(k_assemble, dim3(maxliz, nb), dim3(256), 0, c->stream, c->desc.p,
+                         c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
+    }
+    if (int r = check_launch("k_assemble")) return r;
+    if (int r = launch_inverse(c, c->desc.p, nb, nt_max)) return r;
+    {
+      ProfScope p(c, 8);
+      hipLaunchKernelGGL(k_inv_x, dim3(nt_max, nb), dim3(256), 0, c->stream, c->desc.p);
+      hipLaunchKernelGGL(k_lloyd, dim3(maxliz, nb), dim3(256), 0, c->stream, c->desc.p, c->tmat.p,
+                         lloyd_out ? c->dtmat.p : nullptr, c->ldT, c->cgnt.p, c->illp.p,
+                         c->lmax_tab);
+      hipLaunchKernelGGL(k_lloyd_reduce, dim3((nb + 63) / 64), dim3(64), 0, c->stream, c->desc.p,
+                         nb, c->lloyd.p);
+    }
+    if (int r = check_launch("k_lloyd")) return r;
+    if (tau_bar) {
+      // gradient blocks of every (solve, site), then a keyed scatter-add
+      if (int r = c->vbar.ensure((size_t)nb * kc * kc)) return r;
+      if (int r = c->gblk.ensure((size_t)nb * maxliz * kc * kc)) return r;
+      HIPCHK(hipMemcpyAsync(c->vbar.p, tau_bar + (long)done * kc * kc * 2,
+                            (size_t)nb * kc * kc * sizeof(double2), hipMemcpyHostToDevice,
+                            c->stream));
+      hipLaunchKernelGGL(k_tau_grad, dim3(maxliz, nb), dim3(256), 0, c->stream, c->desc.p,
+                         c->vbar.p, c->gblk.p, maxliz);
+      std::map<std::pair<int, long>, std::vector<int>> keys;
+      for (int b = 0; b < nb; b++) {
+        const AtomDev &a = c->atoms[atom_ids[done + b]];
+        for (int s = 0; s < a.numLIZ; s++)
+          keys[{a.store[s], hd[b].t_off}].push_back(b * maxliz + s);
+      }
+      std::vector<int> kst, kptr(1, 0), kent;
+      std::vector<long> ktoff;
+      for (auto &kv : keys) {
+        kst.push_back(kv.first.first);
+        ktoff.push_back(kv.first.second);
+        kent.insert(kent.end(), kv.second.begin(), kv.second.end());
+        kptr.push_back((int)kent.size());
+      }
+      int nk = (int)kst.size();
+      if (int r = c->keyi.ensure((size_t)nk + nk + 1 + kent.size())) return r;
+      if (int r = c->keyl.ensure((size_t)nk)) return r;
+      std::vector<int> packed;
+      packed.insert(packed.end(), kst.begin(), kst.end());
+      packed.insert(packed.end(), kptr.begin(), kptr.end());
+      packed.insert(packed.end(), kent.begin(), kent.end());
+      HIPCHK(hipMemcpyAsync(c->keyi.p, packed.data(), packed.size() * sizeof(int),
+                            hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->keyl.p, ktoff.data(), nk * sizeof(long), hipMemcpyHostToDevice,
+                            c->stream));
+      hipLaunchKernelGGL(k_grad_reduce, dim3(nk), dim3(256), 0, c->stream, c->gblk.p, kc,
+                         c->keyi.p, c->keyl.p, c->keyi.p + nk, c->keyi.p + 2 * nk + 1, c->gstore.p,
+                         c->ldT, kc);
+      if (int r = check_launch("k_tau_grad")) return r;
+      // the host vectors above must outlive the copies
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (tau_out)
+      HIPCHK(hipMemcpyAsync(tau_out + (long)done * kc * kc * 2, c->tau.p,
+                            (size_t)nb * kc * kc * sizeof(double2), hipMemcpyDeviceToHost,
+                            c->stream));
+    if (lloyd_out)
+      HIPCHK(hipMemcpyAsync(lloyd_out + 2L * done, c->lloyd.p, nb * sizeof(double2),
+                            hipMemcpyDeviceToHost, c->stream));
+    std::vector<int> hinfo(nb);
+    std::vector<double> hmin(nb);
+    HIPCHK(hipMemcpyAsync(hinfo.data(), c->info.p, nb * sizeof(int), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(hmin.data(), c->minpiv.p, nb * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drain_profile(c);
+    for (int b = 0; b < nb; b++)
+      if (info_out) info_out[done + b] = hinfo[b];
+    done = end;
+  }
+  if (tau_bar && tmat_grad) {
+    HIPCHK(hipMemcpyAsync(tmat_grad, c->gstore.p, sizeof(double2) * c->ldT * c->nstore,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -554,6 +834,18 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->iters.release();
   c->count.release();
   c->dxn.release();
+  c->dtmat.release();
+  c->Minv.release();
+  c->gx.release();
+  c->lpart.release();
+  c->lloyd.release();
+  c->vbar.release();
+  c->gblk.release();
+  c->gstore.release();
+  c->tfull.release();
+  c->keyi.release();
+  c->colmap.release();
+  c->keyl.release();
   (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -776,7 +1068,7 @@ int lsmsx_set_profiling(lsmsx_ctx *c, int on) {
 
 int lsmsx_reset_profile(lsmsx_ctx *c) {
   if (!c) return fail(-1, "null context");
-  for (int i = 0; i < 7; i++) {
+  for (int i = 0; i < 9; i++) {
     c->prof_ms[i] = 0;
     c->prof_n[i] = 0;
   }
@@ -785,9 +1077,116 @@ int lsmsx_reset_profile(lsmsx_ctx *c) {
 
 int lsmsx_get_profile(lsmsx_ctx *c, int kind, double *ms, long *launches) {
   if (!c) return fail(-1, "null context");
-  if (kind < 0 || kind > 6) return fail(-2, "bad kernel kind");
+  if (kind < 0 || kind > 8) return fail(-2, "bad kernel kind");
   if (ms) *ms = c->prof_ms[kind];
   if (launches) *launches = c->prof_n[kind];
+  return 0;
+}
+
+int lsmsx_upload_dtmats(lsmsx_ctx *c, const double *dtmat, long ldT, int numStore, long blk,
+                        int groupSize) {
+  if (!c) return fail(-1, "null context");
+  if (!c->tmat.p) return fail(-3, "upload the t-matrix store first");
+  if (ldT != c->ldT || numStore != c->nstore || blk != c->blk || groupSize != c->group)
+    return fail(-2, "dt/dE store dimensions differ from the t-matrix store");
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->dtmat.ensure((size_t)ldT * numStore)) return r;
+  HIPCHK(hipMemcpy(c->dtmat.p, dtmat, sizeof(double2) * ldT * numStore, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int lsmsx_tau00_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins,
+                        const int *iies, const double *energies, const double *prel, int nsc,
+                        int nspola, double *tau00, double *lloyd, const double *tau00_bar,
+                        double *tmat_grad, int *info) {
+  if (tau00_bar && !tmat_grad) return fail(-2, "tau00_bar given without tmat_grad");
+  return run_adjoint(c, nsolve, atom_ids, ispins, iies, energies, prel, nsc, nspola, tau00, lloyd,
+                     tau00_bar, tmat_grad, info);
+}
+
+int lsmsx_tau_full(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *prel, int nsc,
+                   int nspola, double *tau, int ldtau, int *info) {
+  if (int r = validate_common(c, nsc, nspola)) return r;
+  auto it = c->atoms.find(atom_id);
+  if (it == c->atoms.end()) return fail(-5, "unknown atom id");
+  AtomDev &a = it->second;
+  if (iie < 0 || iie >= c->group) return fail(-7, "energy slot out of range");
+  if (ispin < 0 || ispin >= nspola) return fail(-7, "spin index out of range");
+  if (a.kkrsz > (c->lmax_tab + 1) * (c->lmax_tab + 1))
+    return fail(-6, "atom lmax exceeds the lmax of the uploaded tables");
+  for (int s = 0; s < a.numLIZ; s++)
+    if (a.store[s] < 0 || a.store[s] >= c->nstore) return fail(-7, "LIZStoreIdx out of range");
+  const int N = nsc * a.nrmat, Ns = roundup(N, kTile), nt = Ns / kTile, kc = nsc * a.kkrsz;
+  if (ldtau < N) return fail(-2, "ldtau < nrmat_ns");
+  if (kc > kTile) return fail(-6, "kkrsz_ns > 64 not supported");
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->M.ensure((size_t)Ns * Ns)) return r;
+  if (int r = c->Minv.ensure((size_t)Ns * Ns)) return r;
+  if (int r = c->inv.ensure((size_t)nt * kTile * kTile)) return r;
+  if (int r = c->perm.ensure((size_t)nt * kTile)) return r;
+  if (int r = c->info.ensure(1)) return r;
+  if (int r = c->minpiv.ensure(1)) return r;
+  if (int r = c->tfull.ensure((size_t)N * N)) return r;
+  if (int r = c->colmap.ensure((size_t)2 * Ns)) return r;
+  if (int r = c->desc.ensure(1)) return r;
+  // solver index -> (site, reference index)
+  std::vector<int> soff = site_offsets(a, nsc, true), roff = site_offsets(a, nsc, false);
+  std::vector<int> cmap(2 * Ns, -1);
+  for (int s = 0; s < a.numLIZ; s++) {
+    int w = nsc * (a.lmax[s] + 1) * (a.lmax[s] + 1);
+    for (int k = 0; k < w; k++) {
+      cmap[soff[s] + k] = s;
+      cmap[Ns + soff[s] + k] = roff[s] + k;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->colmap.p, cmap.data(), sizeof(int) * 2 * Ns, hipMemcpyHostToDevice,
+                        c->stream));
+  SolveDesc d{};
+  d.pos = a.d_pos;
+  d.lmax = a.d_lmax;
+  d.store = a.d_store;
+  d.site_off = a.d_off[1][nsc - 1];
+  d.numLIZ = a.numLIZ;
+  d.n_spin_cant = nsc;
+  d.kkrsz = a.kkrsz;
+  d.kc = kc;
+  d.prel_re = prel[0];
+  d.prel_im = prel[1];
+  d.t_off = (long)iie * c->blk + ((nspola != nsc) ? (long)kc * kc * ispin : 0);
+  d.ldt = kc;
+  d.M = c->M.p;
+  d.A = d.M;
+  d.Minv = c->Minv.p;
+  d.ld = Ns;
+  d.Ns = Ns;
+  d.pad = Ns - N;
+  d.nt = nt;
+  d.full_lu = 1;
+  d.inv = c->inv.p;
+  d.inv_stride = kTile * kTile;
+  d.perm = c->perm.p;
+  d.info = c->info.p;
+  d.minpiv = c->minpiv.p;
+  HIPCHK(hipMemcpyAsync(c->desc.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_init_status, dim3(1), dim3(64), 0, c->stream, c->desc.p, 1);
+  {
+    ProfScope p(c, 0);
+    hipLaunchKernelGGL(k_pad_identity, dim3(1), dim3(256), 0, c->stream, c->desc.p);
+    hipLaunchKernelGGL(k_assemble, dim3(a.numLIZ, 1), dim3(256), 0, c->stream, c->desc.p,
+                       c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
+  }
+  if (int r = check_launch("k_assemble")) return r;
+  if (int r = launch_inverse(c, c->desc.p, 1, nt)) return r;
+  hipLaunchKernelGGL(k_tau_full_out, dim3(Ns), dim3(256), 0, c->stream, c->desc.p, c->tmat.p,
+                     c->ldT, c->colmap.p, c->colmap.p + Ns, c->tfull.p, (long)N);
+  if (int r = check_launch("k_tau_full_out")) return r;
+  int hinfo = 0;
+  HIPCHK(hipMemcpy2DAsync(tau, sizeof(double2) * ldtau, c->tfull.p, sizeof(double2) * N,
+                          sizeof(double2) * N, N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&hinfo, c->info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drain_profile(c);
+  if (info) *info = hinfo;
   return 0;
 }
 

@@ -50,6 +50,12 @@ struct SolveDesc {
   int *conv;              // 1 once converged
   int *iters;             // refinement sweeps used
   double *dxn;            // ||dX_c||^2 of the previous sweep
+  // explicit inverse (full tau, Lloyd term, adjoint of tau00)
+  int full_lu;            // 1: the last tile is factored by k_lu_diag like every other tile
+  double2 *Minv;          // Ns x Ns (leading dim ld): M^-1 in the solver layout
+  double dprel_re, dprel_im;  // d prel / dE
+  double2 *gx;            // Ns x kc: the rows of site i hold (G X)_i, X = M^-1 [0; t0]
+  double2 *lpart;         // numLIZ per-row-site parts of Tr(M^-1 dM/dE)
 };
 
 }  // namespace lsmsx

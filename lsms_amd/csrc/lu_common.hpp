@@ -1,0 +1,176 @@
+// lu_common.hpp -- device helpers shared by the LU, refinement and inverse
+// kernels: complex types per real type, global-address-space pointers, the
+// 16x16x4 MFMA wrappers with their accumulator layout, and the 3M complex
+// wave product used by every trailing update.
+#pragma once
+#include "lsmsx_device.hpp"
+
+namespace lsmsx {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+struct Cplx;
+template <>
+struct Cplx<double> {
+  typedef double2 type;
+  typedef d4 acc;
+};
+template <>
+struct Cplx<float> {
+  typedef float2 type;
+  typedef f4 acc;
+};
+template <class T>
+using CxT = typename Cplx<T>::type;
+template <class T>
+using Acc4 = typename Cplx<T>::acc;
+
+// Pointers read from the solve descriptors are generic; casting them to the
+// global address space lets the compiler emit global (not flat) loads/stores.
+// (The host pass only type-checks kernel bodies, so it sees plain types.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define LSMSX_GLOBAL __attribute__((address_space(1)))
+#else
+#define LSMSX_GLOBAL
+#endif
+template <class T>
+using GCx = LSMSX_GLOBAL CxT<T>;
+
+__device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma4(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// Accumulator layout of the 16x16x4 MFMAs (measured on gfx950): register r
+// of lane l holds output row l/16 + 4r (f64) or 4(l/16) + r (f32), column l%16.
+template <class T>
+__device__ __forceinline__ int acc_row(int kq, int r);
+template <>
+__device__ __forceinline__ int acc_row<double>(int kq, int r) {
+  return kq + 4 * r;
+}
+template <>
+__device__ __forceinline__ int acc_row<float>(int kq, int r) {
+  return 4 * kq + r;
+}
+
+template <class V>
+__device__ inline auto cabs1(V z) -> decltype(z.x) {
+  return fabs(z.x) + fabs(z.y);
+}
+template <class V>
+__device__ inline V cmul2(V a, V b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <class V>
+__device__ inline V crecip(V b) {
+  auto s = 1 / (b.x * b.x + b.y * b.y);
+  V r;
+  r.x = b.x * s;
+  r.y = -b.y * s;
+  return r;
+}
+template <class V>
+__device__ inline V czero() {
+  V r;
+  r.x = 0;
+  r.y = 0;
+  return r;
+}
+
+// Wave-wide argmax of v (ties -> smallest index); all lanes get the result.
+template <class T>
+__device__ inline void wave_argmax(T &v, int &idx) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    T ov = __shfl_xor(v, off, 64);
+    int oi = __shfl_xor(idx, off, 64);
+    if (ov > v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Complex 32x32x64 wave product on MFMA 16x16x4, 3M form:
+//   P1 = Ar Br, P2 = Ai Bi, P3 = (Ar+Ai)(Br+Bi);  re = P1 - P2, im = P3 - P1 - P2
+// (three real MFMAs per complex 16x16x4 block instead of four).
+// The MFMA is applied to the transposed product, D'(j,i) = sum_k B(k,j) A(i,k),
+// so the accumulator of a lane covers consecutive rows i of C and every C and
+// A access is a run of 16 consecutive rows.  Operands are read straight from
+// L2/LDS into registers in MFMA layout (a-operand lane value B(k0+kq, j0+li),
+// b-operand lane value A(i0+li, k0+kq)), one k-step ahead; no LDS staging and
+// no barriers, so the waves of a CU run independently.
+template <class T>
+struct Acc3 {
+  Acc4<T> p[3][2][2];
+};
+
+template <class T, bool NEGA, class FA, class FB>
+__device__ __forceinline__ void wave_gemm3m(FA fa, FB fb, Acc3<T> &q) {
+  typedef CxT<T> V;
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+  V a[2], b[2], an[2], bn[2];
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    a[t] = fa(16 * t + li, kq);
+    b[t] = fb(kq, 16 * t + li);
+  }
+#pragma unroll 2
+  for (int k0 = 0; k0 < kTile; k0 += 4) {
+    if (k0 + 4 < kTile) {
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        an[t] = fa(16 * t + li, k0 + 4 + kq);
+        bn[t] = fb(k0 + 4 + kq, 16 * t + li);
+      }
+    }
+    T as[2], bs[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      if (NEGA) {
+        a[t].x = -a[t].x;
+        a[t].y = -a[t].y;
+      }
+      as[t] = a[t].x + a[t].y;
+      bs[t] = b[t].x + b[t].y;
+    }
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) q.p[0][bj][bi] = mfma4(b[bj].x, a[bi].x, q.p[0][bj][bi]);
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) q.p[1][bj][bi] = mfma4(b[bj].y, a[bi].y, q.p[1][bj][bi]);
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) q.p[2][bj][bi] = mfma4(bs[bj], as[bi], q.p[2][bj][bi]);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      a[t] = an[t];
+      b[t] = bn[t];
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void zero_acc(Acc3<T> &q) {
+#pragma unroll
+  for (int m = 0; m < 3; m++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) q.p[m][bj][bi][r] = T(0);
+}
+
+}  // namespace lsmsx
