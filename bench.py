@@ -103,6 +103,10 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed", "both"), default="both",
                     help="fp64: plain path only; mixed/both: also the mixed-precision path, "
                          "which is the headline when it meets the 1e-10 error bar")
+    ap.add_argument("--workload", choices=("c2", "c4", "c5"), default="c2",
+                    help="c2: the headline (fcc 135 sites, mixed vs FP64); c4: uneven 4096-site "
+                         "bcc alloy batch (graph + autotuned schedule vs eager); c5: spin-canted "
+                         "bcc 137 sites (N=4384), plain tau00 and the adjoint route")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
@@ -131,6 +135,11 @@ def main():
     if not os.path.exists(capi.LIB_PATH):
         import __graft_entry__
         __graft_entry__.build()
+
+    if args.workload == "c4":
+        return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
 
     s = CF.config_c2(npts=args.nenergies, natoms=args.natoms)
     ctx = capi.Context(0)
@@ -242,6 +251,111 @@ def main():
         "speedup_mixed_vs_fp64": round(mixed["solves_per_s"] / fp64_solves, 3) if mixed else None,
         "max_rel_err_vs_lapack": err,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def _timed_runs(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return out, (time.perf_counter() - t0) / steps
+
+
+def bench_c4(args):
+    """Uneven batch: 64 alloy atoms x 32 energies, KKR sizes from the LIZ
+    radii {11.5, 13.3} bohr with 10% vacancies; eager default schedule
+    against hipGraph replay with the autotuned schedule (FP64)."""
+    import numpy as np
+    from lsms_amd import capi
+    from lsms_amd import configs as CF
+    s = CF.config_c4()
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    na, ne = len(s.atoms), len(s.energies)
+    atoms = np.repeat(np.arange(na), ne)
+    iies = np.tile(np.arange(ne), na)
+    prels = np.array([s.prel(i) for i in iies])
+    ns = [s.nrmat_ns(a) for a in atoms]
+    flops = sum(solve_flops(n, 16) for n in ns)
+    run = lambda: ctx.tau00_batched(atoms, iies, prels, 16)
+    ctx.set_schedule(0)
+    (tau0, info0), t_eager = _timed_runs(run, args.steps, args.warmup)
+    best, ms = ctx.autotune(atoms, iies, prels)
+    ctx.set_graph(True)
+    (tau1, info1), t_tuned = _timed_runs(run, args.steps, args.warmup)
+    ctx.set_graph(False)
+    ctx.close()
+    assert np.all(info0 == 0) and np.all(info1 == 0)
+    out = {
+        "metric": "tau00 solves/sec, uneven batch (bcc alloy, 4096 sites, 10% vacancies)",
+        "value": round(len(atoms) / t_tuned, 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * t_tuned, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"bcc a=5.42, 16x16x8 cells, 10% vacancies, {na} atoms x {ne} "
+                               f"energies, LIZ r in {{11.5, 13.3}}, lmax=3",
+                   "solves_per_step": len(atoms), "N_min": int(min(ns)), "N_max": int(max(ns))},
+        "eager_default_schedule_solves_per_s": round(len(atoms) / t_eager, 3),
+        "graph_autotuned_solves_per_s": round(len(atoms) / t_tuned, 3),
+        "autotune": {"best_flags": best, "ms_by_flags": [round(x, 2) for x in ms],
+                     "flags": "1 right-aligned steps, 2 XCD swizzle"},
+        "bitwise_equal_to_eager": bool(np.array_equal(tau0, tau1)),
+        "path_roofline": {"achieved": round(flops / t_tuned / 1e12, 3), "unit": "TFLOP/s",
+                          "frac_of_fp64_peak": round(flops / t_tuned / 1e12 / FP64_PEAK_TFLOPS, 4)},
+    }
+    print(json.dumps(out))
+
+
+def bench_c5(args):
+    """Spin-canted bcc, 137-site LIZ (N = 4384, kkrsz_ns = 32), 8 atoms x 31
+    energies: the plain FP64 tau00 batch and the adjoint route (explicit
+    inverse, Lloyd term and the t-store gradient of a random cotangent)."""
+    import numpy as np
+    from lsms_amd import capi
+    from lsms_amd import configs as CF
+    s = CF.config_c5()
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    ctx.upload_dtmats(s.dtmat_store(), s.blkSizeTmatStore, len(s.energies))
+    na, ne = len(s.atoms), len(s.energies)
+    atoms = np.repeat(np.arange(na), ne)
+    iies = np.tile(np.arange(ne), na)
+    prels = np.array([s.prel(i) for i in iies])
+    N = s.nrmat_ns(0)
+    rng = np.random.default_rng(0)
+    V = rng.normal(size=(len(atoms), 32, 32)) + 1j * rng.normal(size=(len(atoms), 32, 32))
+    (tau, info), t_plain = _timed_runs(lambda: ctx.tau00_batched(atoms, iies, prels, 32),
+                                       args.steps, args.warmup)
+    adj = lambda: ctx.tau00_adjoint(atoms, iies, s.energies[iies], 32, lloyd=True, tau00_bar=V,
+                                    store_shape=s.tmatStore.shape)
+    (tau_a, ll, grad, info_a), t_adj = _timed_runs(adj, max(1, args.steps // 2), 1)
+    ctx.close()
+    assert np.all(info == 0) and np.all(info_a == 0)
+    err = max(float(np.linalg.norm(tau_a[b] - tau[b]) / np.linalg.norm(tau[b])) for b in range(len(atoms)))
+    fl = solve_flops(N, 32)
+    Ns = (N + 63) // 64 * 64
+    inv_fl = 8.0 * Ns ** 3 * (1.0 / 3 + 1.0 / 6 + 1.0 / 2)
+    out = {
+        "metric": "tau00 solves/sec, spin-canted bcc LIZ=137 (N=4384); adjoint route",
+        "value": round(len(atoms) / t_plain, 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * t_plain, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"bcc a=5.42, n_spin_cant=2, LIZ r=13.3 (137 sites, N={N}), "
+                               f"{na} atoms x {ne} energies", "solves_per_step": len(atoms)},
+        "path_roofline": {"achieved": round(fl * len(atoms) / t_plain / 1e12, 3), "unit": "TFLOP/s",
+                          "frac_of_fp64_peak": round(fl * len(atoms) / t_plain / 1e12 / FP64_PEAK_TFLOPS, 4)},
+        "adjoint": {"solves_per_s": round(len(atoms) / t_adj, 3),
+                    "ms_per_step": round(1e3 * t_adj, 3),
+                    "work": "assembly + LU + explicit inverse + Lloyd trace + t-store gradient",
+                    "inverse_tflops": round(inv_fl * len(atoms) / t_adj / 1e12, 3),
+                    "max_rel_diff_tau00_vs_plain": err},
     }
     print(json.dumps(out))
 

@@ -163,6 +163,34 @@ int lsmsx_tau00_adjoint(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
                         double *tau00, double *lloyd, const double *tau00_bar,
                         double *tmat_grad, int *info);
 
+/* Schedule of the batched factorization (no effect on results, which are
+ * bitwise identical under every schedule):
+ *   LSMSX_SCHED_RIGHT_ALIGN  solves of different size start their elimination
+ *                            late enough that all finish on the same step, so
+ *                            the small final steps of every solve run together;
+ *   LSMSX_SCHED_XCD_SWIZZLE  trailing-update workgroups are remapped so that
+ *                            each of the 8 XCDs works on whole solves and
+ *                            reads their panels into its own L2.
+ * No reference counterpart (the reference solves one matrix per OpenMP
+ * thread, calculateTauMatrix.cpp:891-900). */
+#define LSMSX_SCHED_RIGHT_ALIGN 1
+#define LSMSX_SCHED_XCD_SWIZZLE 2
+int lsmsx_set_schedule(lsmsx_ctx *ctx, int flags);
+int lsmsx_get_schedule(lsmsx_ctx *ctx, int *flags);
+
+/* hipGraph replay of the batched path: when on, the assembly + factorization
+ * launch sequence of each chunk shape is captured once and relaunched as one
+ * graph on later calls with the same shape (the refinement sweeps of the
+ * mixed path, which wait on convergence counts, stay outside the graph). */
+int lsmsx_set_graph(lsmsx_ctx *ctx, int on);
+
+/* Autotuner: times the given batch (arguments as lsmsx_tau00_batched) under
+ * each of the four schedules, keeps the fastest on the context and returns
+ * it in *best; ms[4] (may be NULL) receives the wall times. */
+int lsmsx_autotune(lsmsx_ctx *ctx, int nsolve, const int *atom_ids, const int *ispins,
+                   const int *iies, const double *prel, int n_spin_cant, int n_spin_pola,
+                   int precision, int *best, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

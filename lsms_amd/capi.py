@@ -21,8 +21,11 @@ EXPORTED = [
     "lsmsx_tau00", "lsmsx_tau00_batched", "lsmsx_solve_tau00_matrix",
     "lsmsx_set_workspace_limit", "lsmsx_set_profiling", "lsmsx_reset_profile",
     "lsmsx_get_profile", "lsmsx_set_refinement", "lsmsx_last_batch_stats",
-    "lsmsx_upload_dtmats", "lsmsx_tau_full", "lsmsx_tau00_adjoint",
+    "lsmsx_upload_dtmats", "lsmsx_tau_full", "lsmsx_tau00_adjoint", "lsmsx_set_schedule",
+    "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune",
 ]
+SCHED_RIGHT_ALIGN = 1
+SCHED_XCD_SWIZZLE = 2
 
 _lib = None
 _vp = C.c_void_p
@@ -60,6 +63,11 @@ def load():
     L.lsmsx_upload_dtmats.argtypes = [_vp, _vp, C.c_long, C.c_int, C.c_long, C.c_int]
     L.lsmsx_tau_full.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
                                  C.c_int, _vp]
+    L.lsmsx_set_schedule.argtypes = [_vp, C.c_int]
+    L.lsmsx_get_schedule.argtypes = [_vp, _vp]
+    L.lsmsx_set_graph.argtypes = [_vp, C.c_int]
+    L.lsmsx_autotune.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
+                                 _vp, _vp]
     L.lsmsx_tau00_adjoint.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                       _vp, _vp, _vp, _vp, _vp]
     _lib = L
@@ -213,6 +221,30 @@ class Context:
                                        _p(vb) if vb is not None else None,
                                        _p(grad) if grad is not None else None, _p(info)))
         return np.ascontiguousarray(np.transpose(tau, (0, 2, 1))), ll, grad, info
+
+    def set_schedule(self, flags):
+        _ck(load().lsmsx_set_schedule(self._h, int(flags)))
+
+    def schedule(self):
+        f = C.c_int(0)
+        _ck(load().lsmsx_get_schedule(self._h, C.byref(f)))
+        return f.value
+
+    def set_graph(self, on):
+        _ck(load().lsmsx_set_graph(self._h, 1 if on else 0))
+
+    def autotune(self, atom_ids, iies, prels, ispins=None, precision=PREC_FP64):
+        """Time the batch under the four schedules; returns (best flags, ms[4])."""
+        n = len(atom_ids)
+        a = np.ascontiguousarray(atom_ids, dtype=np.int32)
+        e = np.ascontiguousarray(iies, dtype=np.int32)
+        s = np.zeros(n, dtype=np.int32) if ispins is None else np.ascontiguousarray(ispins, dtype=np.int32)
+        pr = np.ascontiguousarray(prels, dtype=np.complex128)
+        best = C.c_int(0)
+        ms = np.zeros(4)
+        _ck(load().lsmsx_autotune(self._h, n, _p(a), _p(s), _p(e), _p(pr), self.nsc, self.nspola,
+                                  precision, C.byref(best), _p(ms)))
+        return best.value, ms
 
     def set_refinement(self, max_sweeps=12, tol=1e-12):
         _ck(load().lsmsx_set_refinement(self._h, int(max_sweeps), float(tol)))

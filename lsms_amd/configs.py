@@ -226,7 +226,8 @@ def local_to_global_t(tu, td, evec):
 
 def make_system(name, kind, a, ncell, rcut, lmax, energies, weights, nspecies=2,
                 seed=20261202, local_atoms=None, n_spin_cant=1, n_spin_pola=1,
-                vacancy=0.0, rcut_per_atom=None, rsteps=None, delta_range=1.2):
+                vacancy=0.0, rcut_per_atom=None, rsteps=None, delta_range=1.2,
+                _max_local=None):
     """Build a System from a lattice; species are drawn per site (seeded)."""
     rng = np.random.default_rng(seed)
     pos, box = lattice_sites(kind, a, ncell)
@@ -247,6 +248,8 @@ def make_system(name, kind, a, ncell, rcut, lmax, energies, weights, nspecies=2,
     if local_atoms is None:
         local_atoms = list(range(nat))
     local_atoms = [i for i in local_atoms if occupied[i]]
+    if _max_local is not None:
+        local_atoms = local_atoms[:_max_local]
     atoms = []
     for n, i in enumerate(local_atoms):
         rc = rcut if rcut_per_atom is None else rcut_per_atom[n % len(rcut_per_atom)]
@@ -323,3 +326,17 @@ def config_c5(npts=31, seed=20261202, natoms=8, rcut=13.3):
     return System(name="c5_bcc137_canted", maxlmax=lmax, n_spin_cant=2, n_spin_pola=2,
                   atoms=atoms, energies=e, weights=w, tmatStore=store, blkSizeTmatStore=blk,
                   deltas=deltas, evecs=evecs)
+
+
+def config_c4(npts=32, seed=20261202, natoms=64, rcuts=(11.5, 13.3)):
+    """Random binary bcc alloy, a=5.42 bohr, 16x16x8 conventional cells (4096
+    sites, periodic), 10% vacancies (seeded), lmax=3; the first `natoms`
+    occupied sites are local and each draws its LIZ radius from `rcuts`
+    (seeded), so the KKR sizes are uneven; 32-point semicircle contour."""
+    rng = np.random.default_rng(seed + 4)
+    r_per_atom = list(rng.choice(np.asarray(rcuts, dtype=float), size=natoms))
+    e, w = contour_semicircle(-0.5, 0.7, npts)
+    # draw enough local candidates that `natoms` occupied ones remain
+    return make_system("c4_bcc4096_alloy", "bcc", 5.42, (16, 16, 8), max(rcuts), 3, e, w,
+                       nspecies=2, seed=seed, local_atoms=list(range(2 * natoms)),
+                       vacancy=0.10, rcut_per_atom=r_per_atom, _max_local=natoms)

@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -28,7 +30,7 @@ __global__ void k_lu_diag(const SolveDesc *, int);
 template <class T>
 __global__ void k_lu_panel(const SolveDesc *, int);
 template <class T>
-__global__ void k_lu_update(const SolveDesc *, int);
+__global__ void k_lu_update(const SolveDesc *, int, int);
 template <class T>
 __global__ void k_lu_last(const SolveDesc *);
 __global__ void k_ir_init(const SolveDesc *);
@@ -166,6 +168,10 @@ struct lsmsx_ctx {
   long ws_limit = 0;
   int ir_max_sweeps = 12;
   double ir_tol = 1e-12;
+  // schedule (LSMSX_SCHED_* flags) and hipGraph replay of the batched path
+  int sched = 0;
+  bool use_graph = false;
+  std::map<std::vector<long>, hipGraphExec_t> graphs;
   // profiling
   bool profiling = false;
   double prof_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -266,7 +272,8 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max) {
     if (int r = check_launch("k_lu_panel")) return r;
     {
       ProfScope p(c, 3);
-      hipLaunchKernelGGL(k_lu_update<T>, dim3(nrest * nrest, nb), dim3(256), 0, c->stream, dd, s);
+      hipLaunchKernelGGL(k_lu_update<T>, dim3(nrest * nrest, nb), dim3(256), 0, c->stream, dd, s,
+                         (c->sched & LSMSX_SCHED_XCD_SWIZZLE) ? 1 : 0);
     }
     if (int r = check_launch("k_lu_update")) return r;
   }
@@ -423,6 +430,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       d.Ns = Ns;
       d.pad = Ns - N;
       d.nt = Ns / kTile;
+      d.step0 = (c->sched & LSMSX_SCHED_RIGHT_ALIGN) ? nt_max - d.nt : 0;
       d.perm = c->perm.p + (long)b * nt_max * kTile;
       d.info = c->info.p + b;
       d.minpiv = c->minpiv.p + b;
@@ -448,20 +456,54 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     }
     HIPCHK(hipMemcpyAsync(c->desc.p, hd.data(), nb * sizeof(SolveDesc), hipMemcpyHostToDevice,
                           c->stream));
-    hipLaunchKernelGGL(k_init_status, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->desc.p, nb);
-    if (int r = check_launch("k_init_status")) return r;
-    {
-      ProfScope p(c, 0);
-      hipLaunchKernelGGL(k_pad_identity, dim3(nb), dim3(256), 0, c->stream, c->desc.p);
-      hipLaunchKernelGGL(k_assemble, dim3(maxliz, nb), dim3(256), 0, c->stream, c->desc.p,
-                         c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
-    }
-    if (int r = check_launch("k_assemble")) return r;
-    std::vector<int> hiters(nb, 0);
-    if (!mixed) {
-      if (int r = launch_lu<double>(c, c->desc.p, nb, nt_max)) return r;
+    // assembly + factorization of the chunk: launched eagerly, or captured
+    // once per chunk shape into a hipGraph and replayed (the kernels read
+    // the freshly uploaded descriptors, so only the shape and the buffer
+    // addresses passed as arguments key the graph)
+    auto enqueue = [&]() -> int {
+      hipLaunchKernelGGL(k_init_status, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->desc.p,
+                         nb);
+      if (int r = check_launch("k_init_status")) return r;
+      {
+        ProfScope p(c, 0);
+        hipLaunchKernelGGL(k_pad_identity, dim3(nb), dim3(256), 0, c->stream, c->desc.p);
+        hipLaunchKernelGGL(k_assemble, dim3(maxliz, nb), dim3(256), 0, c->stream, c->desc.p,
+                           c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
+      }
+      if (int r = check_launch("k_assemble")) return r;
+      if (!mixed) return launch_lu<double>(c, c->desc.p, nb, nt_max);
+      return launch_lu<float>(c, c->desc.p, nb, nt_max);
+    };
+    if (c->use_graph) {
+      std::vector<long> key = {nb, nt_max, maxliz, mixed ? 1 : 0, c->sched, (long)c->desc.p,
+                               (long)c->tmat.p, c->ldT, (long)c->cgnt.p, (long)c->illp.p,
+                               c->lmax_tab};
+      auto it = c->graphs.find(key);
+      if (it == c->graphs.end()) {
+        const bool prof = c->profiling;
+        c->profiling = false;  // no event records inside the captured sequence
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        int r = enqueue();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->profiling = prof;
+        if (r) {
+          if (g) (void)hipGraphDestroy(g);
+          return r;
+        }
+        if (e != hipSuccess) return fail(-100 - (int)e, std::string("graph capture: ") + hipGetErrorString(e));
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(-100 - (int)e, std::string("graph instantiate: ") + hipGetErrorString(e));
+        it = c->graphs.emplace(key, ex).first;
+      }
+      HIPCHK(hipGraphLaunch(it->second, c->stream));
     } else {
-      if (int r = launch_lu<float>(c, c->desc.p, nb, nt_max)) return r;
+      if (int r = enqueue()) return r;
+    }
+    std::vector<int> hiters(nb, 0);
+    if (mixed) {
       int left = 0;
       if (int r = refine(c, c->desc.p, nb, nt_max, c->ir_max_sweeps, c->ir_tol, &left)) return r;
       std::vector<int> hconv(nb);
@@ -478,6 +520,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
         for (int b = 0; b < nb; b++)
           if (!hconv[b]) {
             SolveDesc d = hd[b];
+            d.step0 = 0;  // the fallback chunk has its own step count
             d.A = d.M;
             d.inv = c->inv.p + (long)fb.size() * kTile * kTile;
             d.inv_stride = 0;
@@ -532,7 +575,7 @@ int launch_inverse(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max) {
       {
         ProfScope p(c, 3);
         hipLaunchKernelGGL(k_lu_update<double>, dim3(nrest * nrest, nb), dim3(256), 0, c->stream,
-                           dd, s);
+                           dd, s, 0);
       }
     }
   }
@@ -834,6 +877,8 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->iters.release();
   c->count.release();
   c->dxn.release();
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
   c->dtmat.release();
   c->Minv.release();
   c->gx.release();
@@ -1188,6 +1233,62 @@ int lsmsx_tau_full(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *
   drain_profile(c);
   if (info) *info = hinfo;
   return 0;
+}
+
+int lsmsx_set_schedule(lsmsx_ctx *c, int flags) {
+  if (!c) return fail(-1, "null context");
+  if (flags & ~(LSMSX_SCHED_RIGHT_ALIGN | LSMSX_SCHED_XCD_SWIZZLE))
+    return fail(-2, "unknown schedule flags");
+  c->sched = flags;
+  return 0;
+}
+
+int lsmsx_get_schedule(lsmsx_ctx *c, int *flags) {
+  if (!c || !flags) return fail(-1, "null argument");
+  *flags = c->sched;
+  return 0;
+}
+
+int lsmsx_set_graph(lsmsx_ctx *c, int on) {
+  if (!c) return fail(-1, "null context");
+  c->use_graph = on != 0;
+  return 0;
+}
+
+int lsmsx_autotune(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins,
+                   const int *iies, const double *prel, int nsc, int nspola, int precision,
+                   int *best, double *ms) {
+  if (!c) return fail(-1, "null context");
+  if (nsolve <= 0) return fail(-2, "empty batch");
+  std::vector<double> tau((size_t)nsolve * 64 * 64 * 2);
+  std::vector<int> info(nsolve);
+  const bool prof = c->profiling;
+  c->profiling = false;
+  int rc = run_batched(c, nsolve, atom_ids, ispins, iies, prel, nsc, nspola, precision,
+                       tau.data(), info.data());  // warm-up: allocations, graphs of sched 0
+  double tbest = 1e300;
+  int sbest = c->sched;
+  for (int sch = 0; sch < 4 && rc == 0; sch++) {
+    c->sched = sch;
+    if (c->use_graph) {  // capture outside the timed run
+      rc = run_batched(c, nsolve, atom_ids, ispins, iies, prel, nsc, nspola, precision,
+                       tau.data(), info.data());
+      if (rc) break;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    rc = run_batched(c, nsolve, atom_ids, ispins, iies, prel, nsc, nspola, precision, tau.data(),
+                     info.data());
+    double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (ms) ms[sch] = dt;
+    if (dt < tbest) {
+      tbest = dt;
+      sbest = sch;
+    }
+  }
+  c->profiling = prof;
+  c->sched = sbest;
+  if (best) *best = sbest;
+  return rc;
 }
 
 }  // extern "C"

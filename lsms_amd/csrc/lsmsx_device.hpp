@@ -36,6 +36,7 @@ struct SolveDesc {
   int Ns;                 // matrix dimension in this layout
   int pad;                // identity rows/cols at the front
   int nt;                 // Ns / kTile (solver layout)
+  int step0;              // global step at which this solve's elimination starts
   // factorization state
   void *inv;              // per diagonal tile: L^-1 (strict lower) and U^-1 (upper), kTile^2
   long inv_stride;        // elements between the tiles of consecutive steps (0: one reused tile)

@@ -173,4 +173,22 @@ __device__ __forceinline__ void zero_acc(Acc3<T> &q) {
         for (int r = 0; r < 4; r++) q.p[m][bj][bi][r] = T(0);
 }
 
+// XCD-aware workgroup order.  Workgroups are dispatched round-robin over
+// the 8 XCDs by linear id, so consecutive tiles of one solve would land on
+// 8 different L2 caches.  The remap gives each XCD a contiguous range of
+// the (tile, solve) grid -- whole solves, in a batched launch -- so the L
+// and U panels a solve's tiles share are fetched into one L2.  A bijection
+// whenever the grid size is a multiple of 8 (otherwise the order is kept).
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ void xcd_remap(int &bx, int &by) {
+  const int gx = gridDim.x;
+  const long total = (long)gx * gridDim.y;
+  if (total % kNumXcd) return;
+  const long lin = bx + (long)gx * by;
+  const long per = total / kNumXcd;
+  const long l2 = (lin % kNumXcd) * per + lin / kNumXcd;
+  bx = (int)(l2 % gx);
+  by = (int)(l2 / gx);
+}
+
 }  // namespace lsmsx

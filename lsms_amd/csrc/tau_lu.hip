@@ -140,7 +140,8 @@ template <class T>
 __global__ __launch_bounds__(256) void k_lu_diag(const SolveDesc *__restrict__ descs, int step) {
   typedef CxT<T> V;
   const SolveDesc &d = descs[blockIdx.x];
-  if (step > d.nt - 1 || (step == d.nt - 1 && !d.full_lu)) return;
+  step -= d.step0;
+  if (step < 0 || step > d.nt - 1 || (step == d.nt - 1 && !d.full_lu)) return;
   if (d.conv && *d.conv) return;
   __shared__ V A[kTile][kTile + 1];
   __shared__ int piv[kTile], perm[kTile];
@@ -186,9 +187,10 @@ template <class T>
 __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ descs, int step) {
   typedef CxT<T> V;
   const SolveDesc &d = descs[blockIdx.y];
+  step -= d.step0;
   const int nrest = d.nt - 1 - step;
   const int x = blockIdx.x;
-  if (nrest <= 0 || x >= 2 * nrest) return;
+  if (step < 0 || nrest <= 0 || x >= 2 * nrest) return;
   if (d.conv && *d.conv) return;
   __shared__ V Ts[kTile][kTile + 1];  // Ts[col][row]
   const int tid = threadIdx.x, w = tid >> 6;
@@ -256,12 +258,16 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
 // start (q1 = Cre, q2 = 0, q3 = Cre + Cim, products accumulated with -A), so
 // C' = (q1 - q2, q3 - q1 - q2) and no load sits in the epilogue.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__ descs, int step) {
+__global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__ descs, int step,
+                                                   int xcd_swizzle) {
   typedef CxT<T> V;
-  const SolveDesc &d = descs[blockIdx.y];
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (xcd_swizzle) xcd_remap(bx, by);
+  const SolveDesc &d = descs[by];
+  step -= d.step0;
   const int nrest = d.nt - 1 - step;
-  const int x = blockIdx.x;
-  if (nrest <= 0 || x >= nrest * nrest) return;
+  const int x = bx;
+  if (step < 0 || nrest <= 0 || x >= nrest * nrest) return;
   if (d.conv && *d.conv) return;
   const int ti = step + 1 + x % nrest, tj = step + 1 + x / nrest;
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
@@ -401,8 +407,8 @@ template __global__ void k_lu_diag<double>(const SolveDesc *, int);
 template __global__ void k_lu_diag<float>(const SolveDesc *, int);
 template __global__ void k_lu_panel<double>(const SolveDesc *, int);
 template __global__ void k_lu_panel<float>(const SolveDesc *, int);
-template __global__ void k_lu_update<double>(const SolveDesc *, int);
-template __global__ void k_lu_update<float>(const SolveDesc *, int);
+template __global__ void k_lu_update<double>(const SolveDesc *, int, int);
+template __global__ void k_lu_update<float>(const SolveDesc *, int, int);
 template __global__ void k_lu_last<double>(const SolveDesc *);
 template __global__ void k_lu_last<float>(const SolveDesc *);
 
