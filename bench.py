@@ -32,13 +32,16 @@ def solve_flops(n, k):
 
 
 def update_kernel_flops(nt_list):
-    """Algorithmic flops of all trailing-update launches of one batch:
-    sum over steps s of (nt-1-s)^2 tiles of 8*64^3 per solve."""
+    """Algorithmic flops of all k_lu_update2 launches of one batch.  Steps are
+    eliminated in pairs (a, b = a + 1): after a cross update of tile row and
+    column b, k_lu_update2 applies both rank-64 panels to the (nt-2-a)^2
+    tiles beyond b, 2 * 8*64^3 flops per tile (8 flops per complex FMA)."""
     total = 0.0
     for nt in nt_list:
-        for s in range(nt - 1):
-            r = nt - 1 - s
-            total += r * r * 8.0 * 64 ** 3
+        for a in range(0, nt - 1, 2):
+            r = nt - 2 - a
+            if r > 0:
+                total += r * r * 2 * 8.0 * 64 ** 3
     return total
 
 
@@ -103,7 +106,7 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed", "both"), default="both",
                     help="fp64: plain path only; mixed/both: also the mixed-precision path, "
                          "which is the headline when it meets the 1e-10 error bar")
-    ap.add_argument("--workload", choices=("c2", "c4", "c5"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
                     help="c2: the headline (fcc 135 sites, mixed vs FP64); c4: uneven 4096-site "
                          "bcc alloy batch (graph + autotuned schedule vs eager); c5: spin-canted "
                          "bcc 137 sites (N=4384), plain tau00 and the adjoint route")
@@ -136,6 +139,8 @@ def main():
         import __graft_entry__
         __graft_entry__.build()
 
+    if args.workload == "c3":
+        return bench_c3(args)
     if args.workload == "c4":
         return bench_c4(args)
     if args.workload == "c5":
@@ -171,7 +176,8 @@ def main():
         ctx.tau00_batched(atoms, iies, prels, k, precision=precision)
         ctx.set_profiling(False)
         names = (("assemble", capi.KIND_ASSEMBLE), ("diag", capi.KIND_DIAG),
-                 ("panel", capi.KIND_PANEL), ("update", capi.KIND_UPDATE),
+                 ("panel", capi.KIND_PANEL), ("cross", capi.KIND_CROSS),
+                 ("update", capi.KIND_UPDATE),
                  ("last_tile", capi.KIND_FINISH), ("ir_residual", capi.KIND_IR_RESIDUAL),
                  ("ir_solve", capi.KIND_IR_SOLVE))
         return {name: ctx.profile(kind) for name, kind in names}
@@ -236,7 +242,7 @@ def main():
                          "mixed: complex FP32 LU on MFMA + FP64 iterative refinement",
         },
         "roofline": {
-            "bound": "mfma", "kernel": "k_lu_update<%s>" % ("double" if headline == "fp64" else "float"),
+            "bound": "mfma", "kernel": "k_lu_update2<%s>" % ("double" if headline == "fp64" else "float"),
             "achieved": round(achieved_tf, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved_tf / peak, 4),
             "traffic": profiled_traffic() if (headline, args.natoms, args.nenergies) == ("fp64", 32, 31) else None,
@@ -265,6 +271,67 @@ def _timed_runs(fn, steps, warmup):
         out = fn()
     torch.cuda.synchronize()
     return out, (time.perf_counter() - t0) / steps
+
+
+def bench_c3(args):
+    """48 energies (32 contour points + 16 at Im E = 0.001-0.01 Ry) for the
+    32 fcc-135 atoms: mixed vs FP64 throughput, the per-energy tau00 error of
+    the mixed path, the multiple-scattering IDOS error (unit radial
+    integrals) and the Lloyd IDOS of the FP64 adjoint route."""
+    import numpy as np
+    from lsms_amd import capi
+    from lsms_amd import configs as CF
+    from lsms_amd import contour
+    from oracle import oracle as O
+    s = CF.config_c3()
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    ctx.upload_dtmats(s.dtmat_store(), s.blkSizeTmatStore, len(s.energies))
+    na, ne = len(s.atoms), len(s.energies)
+    atoms = np.repeat(np.arange(na), ne)
+    iies = np.tile(np.arange(ne), na)
+    prels = np.array([s.prel(i) for i in iies])
+    (t64, i64), el64 = _timed_runs(lambda: ctx.tau00_batched(atoms, iies, prels, 16),
+                                   args.steps, args.warmup)
+    run32 = lambda: ctx.tau00_batched(atoms, iies, prels, 16, precision=capi.PREC_MIXED_FP32)
+    (t32, i32), el32 = _timed_runs(run32, args.steps, args.warmup)
+    iters, _ = ctx.last_stats(len(atoms))
+    on = np.nonzero(np.abs(s.weights) > 0)[0]
+    la, ll, _, ia = ctx.tau00_adjoint(np.repeat(np.arange(na), len(on)), np.tile(on, na),
+                                      np.tile(s.energies[on], na), 16)
+    ctx.close()
+    assert np.all(i64 == 0) and np.all(i32 == 0) and np.all(ia == 0)
+    err = np.array([np.linalg.norm(t32[n] - t64[n]) / np.linalg.norm(t64[n])
+                    for n in range(len(atoms))]).reshape(na, ne)
+    idos_err, idos_l = [], []
+    for a in range(na):
+        at = s.atoms[a]
+        t0 = np.array([O.t0(3, int(at.LIZStoreIdx[0]), s.tmatStore, s.blkSizeTmatStore, ie)
+                       for ie in range(ne)])
+        n64 = contour.idos_ms(t64[a * ne:(a + 1) * ne], t0, s.weights)
+        n32 = contour.idos_ms(t32[a * ne:(a + 1) * ne], t0, s.weights)
+        idos_err.append(abs(n32 - n64) / abs(n64))
+        idos_l.append(contour.idos_lloyd(ll[a * len(on):(a + 1) * len(on)], s.weights[on]))
+    out = {
+        "metric": "tau00 solves/sec at lmax=3 LIZ=135 with 16 near-axis energies: mixed vs FP64",
+        "value": round(len(atoms) / el32, 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el32, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c32-factor+f64-refine", "data": "synthetic",
+        "config": {"workload": f"fcc a=6.82, LIZ r=13.0 (135 sites, N=2160), {na} atoms x "
+                               f"{ne} energies (16 at Im E 0.001..0.01 Ry)",
+                   "solves_per_step": len(atoms)},
+        "fp64_solves_per_s": round(len(atoms) / el64, 3),
+        "speedup_mixed_vs_fp64": round(el64 / el32, 3),
+        "per_energy_max_rel_err": [float("%.3g" % x) for x in err.max(axis=0)],
+        "energies": [[round(float(e.real), 5), round(float(e.imag), 5)] for e in s.energies],
+        "refinement_sweeps": {"min": int(iters.min()), "max": int(iters.max()),
+                              "mean": round(float(iters.mean()), 3),
+                              "fp64_fallbacks": int(np.sum(iters < 0))},
+        "idos_ms_max_rel_err_mixed_vs_fp64": float("%.3g" % max(idos_err)),
+        "idos_lloyd_fp64": [round(x, 8) for x in idos_l],
+    }
+    print(json.dumps(out))
 
 
 def bench_c4(args):

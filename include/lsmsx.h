@@ -117,7 +117,8 @@ int lsmsx_set_workspace_limit(lsmsx_ctx *ctx, long bytes);
  * the context's stream; lsmsx_get_profile returns the accumulated time (ms)
  * and launch count of one kernel kind (0 assemble, 1 diagonal-tile LU,
  * 2 panel, 3 trailing update, 4 last tile, 5 refinement residual,
- * 6 refinement solve, 7 explicit inverse, 8 Lloyd/adjoint contraction).
+ * 6 refinement solve, 7 explicit inverse, 8 Lloyd/adjoint contraction,
+ * 9 cross update of the paired-step elimination).
  * lsmsx_reset_profile clears them. */
 int lsmsx_set_profiling(lsmsx_ctx *ctx, int on);
 int lsmsx_reset_profile(lsmsx_ctx *ctx);

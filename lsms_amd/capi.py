@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "liblsmsx.so")
 PREC_FP64 = 0
 PREC_MIXED_FP32 = 1
 (KIND_ASSEMBLE, KIND_DIAG, KIND_PANEL, KIND_UPDATE, KIND_FINISH, KIND_IR_RESIDUAL,
- KIND_IR_SOLVE, KIND_INVERSE, KIND_ADJOINT) = range(9)
+ KIND_IR_SOLVE, KIND_INVERSE, KIND_ADJOINT, KIND_CROSS) = range(10)
 
 EXPORTED = [
     "lsmsx_last_error", "lsmsx_ctx_create", "lsmsx_ctx_destroy", "lsmsx_set_tables",

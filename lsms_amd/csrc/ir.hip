@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256) void k_ir_init(const SolveDesc *__restrict__ d
     *d.conv = 0;
     *d.iters = 0;
     *d.dxn = 0.0;
+    *d.dxn_rho2 = 0.0;
   }
 }
 
@@ -106,11 +107,38 @@ struct Acc3f {
   f4 p[3][2];
 };
 
+// The k range is consumed in batches of 8 k-steps whose operand loads are
+// all issued before their MFMAs: one workgroup per solve leaves few waves per
+// SIMD, so latency is hidden by loads in flight, not by occupancy.
 template <class FA, class FB>
 __device__ __forceinline__ void sweep_gemm(FA fa, FB fb, int k_begin, int k_end, int nb,
                                            Acc3f &q) {
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
-  for (int k0 = k_begin; k0 < k_end; k0 += 4) {
+  constexpr int U = 8;
+  int kb = k_begin;
+  for (; kb + 4 * U <= k_end; kb += 4 * U) {
+    float2 a[U], b[U][2];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      a[u] = fa(li, kb + 4 * u + kq);
+      b[u][0] = fb(kb + 4 * u + kq, li);
+      if (nb > 1) b[u][1] = fb(kb + 4 * u + kq, 16 + li);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const float as = a[u].x + a[u].y;
+#pragma unroll
+      for (int jb = 0; jb < 2; jb++) {
+        if (jb < nb) {
+          const float2 bb = b[u][jb];
+          q.p[0][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb.x, a[u].x, q.p[0][jb], 0, 0, 0);
+          q.p[1][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb.y, a[u].y, q.p[1][jb], 0, 0, 0);
+          q.p[2][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bb.x + bb.y, as, q.p[2][jb], 0, 0, 0);
+        }
+      }
+    }
+  }
+  for (int k0 = kb; k0 < k_end; k0 += 4) {
     float2 a = fa(li, k0 + kq);
     float as = a.x + a.y;
 #pragma unroll
@@ -275,8 +303,11 @@ __global__ __launch_bounds__(256) void k_ir_solve(const SolveDesc *__restrict__ 
     const double prev = *d.dxn;  // previous ||dX_c||^2
     bool ok = false;
     if (k >= 1 && prev > 0.0) {
-      const double rho2 = dn2 / prev;
+      // the larger of the last two observed contractions, so that one
+      // unusually small ratio does not end the refinement early
+      const double rho2 = fmax(dn2 / prev, *d.dxn_rho2);
       ok = (rho2 < 1.0) && (rho2 * dn2 <= tol * tol * xn2) && (dn2 <= 1e-16 * xn2);
+      *d.dxn_rho2 = dn2 / prev;
     }
     if (dn2 <= tol * tol * xn2) ok = true;
     *d.iters = k + 1;

@@ -32,6 +32,10 @@ __global__ void k_lu_panel(const SolveDesc *, int);
 template <class T>
 __global__ void k_lu_update(const SolveDesc *, int, int);
 template <class T>
+__global__ void k_lu_cross(const SolveDesc *, int);
+template <class T>
+__global__ void k_lu_update2(const SolveDesc *, int, int);
+template <class T>
 __global__ void k_lu_last(const SolveDesc *);
 __global__ void k_ir_init(const SolveDesc *);
 __global__ void k_ir_residual(const SolveDesc *);
@@ -174,8 +178,8 @@ struct lsmsx_ctx {
   std::map<std::vector<long>, hipGraphExec_t> graphs;
   // profiling
   bool profiling = false;
-  double prof_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  long prof_n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double prof_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  long prof_n[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;
 };
@@ -255,27 +259,41 @@ std::vector<int> site_offsets(const AtomDev &a, int nsc, bool solver) {
   return off;
 }
 
-// Launch the blocked LU over a chunk of nb solves already in solver layout.
+// Launch the blocked LU over a chunk of nb solves already in solver layout:
+// steps taken in pairs (a, b = a + 1) -- panels of a, cross update of tile
+// row/column b, panels of b, then one K = 128 trailing update -- so the
+// trailing matrix is read and written once per two steps.
 template <class T>
 int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max) {
-  for (int s = 0; s < nt_max - 1; s++) {
-    int nrest = nt_max - 1 - s;
+  const int swz = (c->sched & LSMSX_SCHED_XCD_SWIZZLE) ? 1 : 0;
+  auto diag_panel = [&](int s) -> int {
+    const int nrest = nt_max - 1 - s;
+    if (nrest <= 0) return 0;
     {
       ProfScope p(c, 1);
       hipLaunchKernelGGL(k_lu_diag<T>, dim3(nb), dim3(256), 0, c->stream, dd, s);
     }
-    if (int r = check_launch("k_lu_diag")) return r;
     {
       ProfScope p(c, 2);
       hipLaunchKernelGGL(k_lu_panel<T>, dim3(2 * nrest, nb), dim3(256), 0, c->stream, dd, s);
     }
-    if (int r = check_launch("k_lu_panel")) return r;
-    {
-      ProfScope p(c, 3);
-      hipLaunchKernelGGL(k_lu_update<T>, dim3(nrest * nrest, nb), dim3(256), 0, c->stream, dd, s,
-                         (c->sched & LSMSX_SCHED_XCD_SWIZZLE) ? 1 : 0);
+    return check_launch("k_lu_diag/k_lu_panel");
+  };
+  for (int s = 0; s < nt_max - 1; s += 2) {
+    if (int r = diag_panel(s)) return r;
+    const int ncross = 2 * (nt_max - s - 1) - 1;
+    if (ncross > 0) {
+      ProfScope p(c, 9);
+      hipLaunchKernelGGL(k_lu_cross<T>, dim3(ncross, nb), dim3(256), 0, c->stream, dd, s);
     }
-    if (int r = check_launch("k_lu_update")) return r;
+    if (int r = diag_panel(s + 1)) return r;
+    const int nrest = nt_max - 2 - s;
+    if (nrest > 0) {
+      ProfScope p(c, 3);
+      hipLaunchKernelGGL(k_lu_update2<T>, dim3(nrest * nrest, nb), dim3(256), 0, c->stream, dd, s,
+                         swz);
+    }
+    if (int r = check_launch("k_lu_update2")) return r;
   }
   {
     ProfScope p(c, 4);
@@ -401,7 +419,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (int r = c->R.ensure(xtot)) return r;
       if (int r = c->conv.ensure((size_t)nb)) return r;
       if (int r = c->iters.ensure((size_t)nb)) return r;
-      if (int r = c->dxn.ensure((size_t)nb)) return r;
+      if (int r = c->dxn.ensure((size_t)2 * nb)) return r;
       if (int r = c->count.ensure(1)) return r;
     }
     std::vector<SolveDesc> hd(nb);
@@ -430,7 +448,11 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       d.Ns = Ns;
       d.pad = Ns - N;
       d.nt = Ns / kTile;
-      d.step0 = (c->sched & LSMSX_SCHED_RIGHT_ALIGN) ? nt_max - d.nt : 0;
+      // right alignment starts each solve late enough to end with the
+      // largest; the start is even so that every solve pairs its own steps
+      // (0,1), (2,3), ... in the K = 128 elimination, whatever its size: the
+      // arithmetic of a solve never depends on the schedule or the batch
+      d.step0 = (c->sched & LSMSX_SCHED_RIGHT_ALIGN) ? ((nt_max - d.nt) & ~1) : 0;
       d.perm = c->perm.p + (long)b * nt_max * kTile;
       d.info = c->info.p + b;
       d.minpiv = c->minpiv.p + b;
@@ -446,6 +468,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
         d.conv = c->conv.p + b;
         d.iters = c->iters.p + b;
         d.dxn = c->dxn.p + b;
+        d.dxn_rho2 = c->dxn.p + nb + b;
         xoff += (long)Ns * kc;
       } else {
         d.A = d.M;
@@ -1113,7 +1136,7 @@ int lsmsx_set_profiling(lsmsx_ctx *c, int on) {
 
 int lsmsx_reset_profile(lsmsx_ctx *c) {
   if (!c) return fail(-1, "null context");
-  for (int i = 0; i < 9; i++) {
+  for (int i = 0; i < 10; i++) {
     c->prof_ms[i] = 0;
     c->prof_n[i] = 0;
   }
@@ -1122,7 +1145,7 @@ int lsmsx_reset_profile(lsmsx_ctx *c) {
 
 int lsmsx_get_profile(lsmsx_ctx *c, int kind, double *ms, long *launches) {
   if (!c) return fail(-1, "null context");
-  if (kind < 0 || kind > 8) return fail(-2, "bad kernel kind");
+  if (kind < 0 || kind > 9) return fail(-2, "bad kernel kind");
   if (ms) *ms = c->prof_ms[kind];
   if (launches) *launches = c->prof_n[kind];
   return 0;

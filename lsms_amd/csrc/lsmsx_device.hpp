@@ -51,6 +51,7 @@ struct SolveDesc {
   int *conv;              // 1 once converged
   int *iters;             // refinement sweeps used
   double *dxn;            // ||dX_c||^2 of the previous sweep
+  double *dxn_rho2;       // the contraction (||dX_c|| ratio)^2 observed at the previous sweep
   // explicit inverse (full tau, Lloyd term, adjoint of tau00)
   int full_lu;            // 1: the last tile is factored by k_lu_diag like every other tile
   double2 *Minv;          // Ns x Ns (leading dim ld): M^-1 in the solver layout

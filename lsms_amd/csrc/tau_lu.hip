@@ -253,31 +253,19 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
       }
 }
 
-// Trailing update of step s: A_ij -= L_is U_sj for all i, j > s.  One 64x64
-// tile per workgroup, a 32x32 quarter per wave.  C enters the 3M sums at the
-// start (q1 = Cre, q2 = 0, q3 = Cre + Cim, products accumulated with -A), so
-// C' = (q1 - q2, q3 - q1 - q2) and no load sits in the epilogue.
+// C(ti, tj) -= sum_{u < ns} L(ti, s0+u) U(s0+u, tj): one 64x64 tile per
+// workgroup, a 32x32 quarter per wave.  C enters the 3M sums at the start
+// (q1 = Cre, q2 = 0, q3 = Cre + Cim, products accumulated with -A), so
+// C' = (q1 - q2, q3 - q1 - q2): the tile is read and written once however
+// many rank-64 panels are applied.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__ descs, int step,
-                                                   int xcd_swizzle) {
+__device__ __forceinline__ void update_tile(const SolveDesc &d, int ti, int tj, int s0, int ns) {
   typedef CxT<T> V;
-  int bx = blockIdx.x, by = blockIdx.y;
-  if (xcd_swizzle) xcd_remap(bx, by);
-  const SolveDesc &d = descs[by];
-  step -= d.step0;
-  const int nrest = d.nt - 1 - step;
-  const int x = bx;
-  if (step < 0 || nrest <= 0 || x >= nrest * nrest) return;
-  if (d.conv && *d.conv) return;
-  const int ti = step + 1 + x % nrest, tj = step + 1 + x / nrest;
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
   const int w = threadIdx.x >> 6;
   const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
-  const int c0 = step * kTile;
   GCx<T> *M = (GCx<T> *)d.A;
   const long ld = d.ld;
-  const GCx<T> *__restrict__ A = M + (long)(kTile * ti + wr) + ld * c0;
-  const GCx<T> *__restrict__ B = M + (long)c0 + ld * (kTile * tj + wc);
   GCx<T> *__restrict__ C = M + (long)(kTile * ti + wr) + ld * (kTile * tj + wc);
   Acc3<T> q;
 #pragma unroll
@@ -291,8 +279,13 @@ __global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__
         q.p[1][bj][bi][r] = T(0);
         q.p[2][bj][bi][r] = c.x + c.y;
       }
-  wave_gemm3m<T, true>([&](int i, int k) { return A[i + ld * k]; },
-                       [&](int k, int j) { return B[k + ld * j]; }, q);
+  for (int u = 0; u < ns; u++) {
+    const int c0 = (s0 + u) * kTile;
+    const GCx<T> *__restrict__ A = M + (long)(kTile * ti + wr) + ld * c0;
+    const GCx<T> *__restrict__ B = M + (long)c0 + ld * (kTile * tj + wc);
+    wave_gemm3m<T, true>([&](int i, int k) { return A[i + ld * k]; },
+                         [&](int k, int j) { return B[k + ld * j]; }, q);
+  }
 #pragma unroll
   for (int bj = 0; bj < 2; bj++)
 #pragma unroll
@@ -304,6 +297,55 @@ __global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__
         o.y = q.p[2][bj][bi][r] - q.p[0][bj][bi][r] - q.p[1][bj][bi][r];
         C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))] = o;
       }
+}
+
+// Trailing update of step s: A_ij -= L_is U_sj for all i, j > s.
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__ descs, int step,
+                                                   int xcd_swizzle) {
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (xcd_swizzle) xcd_remap(bx, by);
+  const SolveDesc &d = descs[by];
+  step -= d.step0;
+  const int nrest = d.nt - 1 - step;
+  const int x = bx;
+  if (step < 0 || nrest <= 0 || x >= nrest * nrest) return;
+  if (d.conv && *d.conv) return;
+  update_tile<T>(d, step + 1 + x % nrest, step + 1 + x / nrest, step, 1);
+}
+
+// Two-level elimination, steps a = s and b = s + 1 (global step s).
+// Cross update: after the panels of step a, bring tile column b (rows >= b)
+// and tile row b (columns > b) up to date with step a, so that step b can be
+// factored before the rest of the trailing matrix is touched.
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_cross(const SolveDesc *__restrict__ descs, int step) {
+  const SolveDesc &d = descs[blockIdx.y];
+  const int a = step - d.step0, b = a + 1;
+  if (a < 0 || a > d.nt - 2) return;
+  if (d.conv && *d.conv) return;
+  const int n1 = d.nt - b, x = blockIdx.x;
+  if (x >= 2 * n1 - 1) return;
+  if (x < n1) update_tile<T>(d, b + x, b, a, 1);
+  else update_tile<T>(d, b, b + 1 + (x - n1), a, 1);
+}
+
+// Trailing update of the step pair: A_ij -= L_ia U_aj + L_ib U_bj for all
+// i, j > b, one pass over C with K = 128 (K = 64 when step a does not exist
+// for this solve, i.e. its elimination starts at b).
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_update2(const SolveDesc *__restrict__ descs, int step,
+                                                    int xcd_swizzle) {
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (xcd_swizzle) xcd_remap(bx, by);
+  const SolveDesc &d = descs[by];
+  const int a = step - d.step0, b = a + 1;
+  if (b < 0 || b > d.nt - 2) return;
+  if (d.conv && *d.conv) return;
+  const int nrest = d.nt - 1 - b, x = bx;
+  if (x >= nrest * nrest) return;
+  const bool has_a = a >= 0;
+  update_tile<T>(d, b + 1 + x % nrest, b + 1 + x / nrest, has_a ? a : b, has_a ? 2 : 1);
 }
 
 // Last tile (step nt-1): factor it with the non-central rows (first nd =
@@ -409,6 +451,10 @@ template __global__ void k_lu_panel<double>(const SolveDesc *, int);
 template __global__ void k_lu_panel<float>(const SolveDesc *, int);
 template __global__ void k_lu_update<double>(const SolveDesc *, int, int);
 template __global__ void k_lu_update<float>(const SolveDesc *, int, int);
+template __global__ void k_lu_cross<double>(const SolveDesc *, int);
+template __global__ void k_lu_cross<float>(const SolveDesc *, int);
+template __global__ void k_lu_update2<double>(const SolveDesc *, int, int);
+template __global__ void k_lu_update2<float>(const SolveDesc *, int, int);
 template __global__ void k_lu_last<double>(const SolveDesc *);
 template __global__ void k_lu_last<float>(const SolveDesc *);
 

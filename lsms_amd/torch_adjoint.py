@@ -76,9 +76,4 @@ def lloyd(engine, system, atom_ids, iies, ispins=None):
     return ll, info
 
 
-def idos_lloyd(lloyd_values, weights):
-    """Lloyd integrated DOS  -(1/pi) Im sum_k w_k Tr(M^-1 dM/dE)(E_k)."""
-    return float(-np.imag(np.sum(np.asarray(weights) * np.asarray(lloyd_values))) / np.pi)
-
-
-__all__ = ["Tau00Function", "tau00", "lloyd", "idos_lloyd"]
+__all__ = ["Tau00Function", "tau00", "lloyd"]

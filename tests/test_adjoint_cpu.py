@@ -85,3 +85,21 @@ def test_c5_geometry():
     assert np.allclose(np.linalg.norm(s.evecs, axis=1), 1.0)
     # LIZ store index is the site itself, central first
     assert all(int(a.LIZStoreIdx[0]) == i for i, a in enumerate(s.atoms))
+
+
+def test_idos_formulas_on_closed_forms():
+    """The contour weights integrate analytic functions along the semicircle:
+    a constant tau00 - t = c I (k x k) gives -(1/pi) Im(c k (etop - ebot)),
+    and a Lloyd term f(E) = 1/(E - z0) with z0 below the axis gives
+    -(1/pi) Im ln((etop - z0)/(ebot - z0))."""
+    from lsms_amd import contour
+    e, w = CF.contour_semicircle(-0.5, 0.7, 31)
+    c = 0.3 - 0.2j
+    k = 16
+    tau = np.array([c * np.eye(k) + np.eye(k) for _ in e])
+    t0 = np.array([np.eye(k) for _ in e])
+    assert abs(contour.idos_ms(tau, t0, w) - (-np.imag(c * k * 1.2) / np.pi)) < 1e-12
+    z0 = 0.1 - 0.3j
+    f = 1.0 / (e - z0)
+    ref = -np.imag(np.log((0.7 - z0) / (-0.5 - z0))) / np.pi
+    assert abs(contour.idos_lloyd(f, w) - ref) < 1e-10

@@ -123,3 +123,61 @@ def test_mixed_spin_variants(nsc, nsp, ispin):
     assert np.all(info == 0)
     for n in range(3):
         assert rel(tau[n], lapack_tau00(s, atoms[n], iies[n], ispin)) < TAU_RTOL
+
+
+def t0_blocks(s, a, iies):
+    at = s.atoms[a]
+    return np.array([O.t0(s.maxlmax, int(at.LIZStoreIdx[0]), s.tmatStore, s.blkSizeTmatStore,
+                          int(ie)) for ie in iies])
+
+
+def test_c3_per_energy_error_and_idos():
+    """The 48-energy workload (32 contour points + 16 at Im E = 0.001-0.01 Ry):
+    per-energy mixed-vs-FP64 tau00 error within 1e-10, the FP64 path against
+    LAPACK at the nearest-axis energies, and the multiple-scattering IDOS
+    (unit radial integrals) of the mixed path within 1e-10 relative of FP64."""
+    from lsms_amd import contour
+    s = CF.config_c3(natoms=4)
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    na, ne = len(s.atoms), len(s.energies)
+    atoms = np.repeat(np.arange(na), ne)
+    iies = np.tile(np.arange(ne), na)
+    prels = np.array([s.prel(i) for i in iies])
+    t32, info32 = ctx.tau00_batched(atoms, iies, prels, 16, precision=MIXED)
+    t64, info64 = ctx.tau00_batched(atoms, iies, prels, 16)
+    ctx.close()
+    assert np.all(info32 == 0) and np.all(info64 == 0)
+    err = np.array([rel(t32[n], t64[n]) for n in range(len(atoms))]).reshape(na, ne)
+    per_e = err.max(axis=0)
+    assert per_e.max() < TAU_RTOL, per_e
+    near = np.argsort(np.imag(s.energies))[:2]  # Im E = 0.001, ...
+    for ie in near:
+        assert rel(t64[ie], lapack_tau00(s, 0, ie)) < TAU_RTOL
+    w = s.weights
+    for a in range(na):
+        sl_ = slice(a * ne, (a + 1) * ne)
+        t0 = t0_blocks(s, a, range(ne))
+        n64 = contour.idos_ms(t64[sl_], t0, w)
+        n32 = contour.idos_ms(t32[sl_], t0, w)
+        assert abs(n32 - n64) < 1e-10 * abs(n64), (a, n32, n64)
+
+
+def test_lloyd_idos_c1_against_oracle():
+    """Lloyd IDOS of both C1 atoms over the 31-point contour: GPU adjoint route
+    against the oracle's four-point-derivative traces."""
+    from lsms_amd import contour
+    from tests import adjoint_ref as R
+    s = CF.config_c1()
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    ctx.upload_dtmats(s.dtmat_store(), s.blkSizeTmatStore, len(s.energies))
+    ne = len(s.energies)
+    for a in range(len(s.atoms)):
+        _, ll, _, info = ctx.tau00_adjoint([a] * ne, np.arange(ne), s.energies, 16)
+        assert np.all(info == 0)
+        ref = [R.lloyd_trace(s, a, e) for e in s.energies]
+        n_gpu = contour.idos_lloyd(ll, s.weights)
+        n_ref = contour.idos_lloyd(ref, s.weights)
+        assert abs(n_gpu - n_ref) < 1e-9 * max(1.0, abs(n_ref)), (a, n_gpu, n_ref)
+    ctx.close()
