@@ -45,16 +45,17 @@ def update_kernel_flops(nt_list):
     return total
 
 
-PROFILE = os.path.join(ROOT, "profiles", "k_lu_update_fcc135_N2160.md")
+PROFILE = os.path.join(ROOT, "profiles", "k_lu_update2_fcc135_N2160.md")
 
 
-def profiled_traffic():
-    """HBM bytes per k_lu_update launch measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE (copy-calibrated) on this workload; the command and the
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (copy-calibrated) on this workload; the commands and the
     calibration are recorded in the profile summary."""
+    key = "traffic_bytes_per_launch[%s]:" % kernel
     try:
         for line in open(PROFILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return float(line.split(":")[1])
     except OSError:
         pass
@@ -245,7 +246,8 @@ def main():
             "bound": "mfma", "kernel": "k_lu_update2<%s>" % ("double" if headline == "fp64" else "float"),
             "achieved": round(achieved_tf, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved_tf / peak, 4),
-            "traffic": profiled_traffic() if (headline, args.natoms, args.nenergies) == ("fp64", 32, 31) else None,
+            "traffic": profiled_traffic("k_lu_update2<%s>" % ("double" if headline == "fp64" else "float"))
+                       if (args.natoms, args.nenergies) == (32, 31) else None,
             "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4), "launches": upd_n,
         },
         "path_roofline": {"achieved": round(path_tf, 3), "unit": "TFLOP/s",

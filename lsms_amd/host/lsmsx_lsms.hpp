@@ -226,4 +226,24 @@ void calculateAllTauMatricesMI355X(LSMS &lsms, Local &local, cplx energy, int ii
   }
 }
 
+// solveTauFullzgetrf replacement for the Kubo caller (Kubo/CurrentMatrix.cpp:184-190;
+// linearSolvers_CPU.cpp:125-144): the build + full solve pair becomes one call,
+// tau = M^-1 T (nrmat_ns x nrmat_ns, reference site order) from an explicit
+// FP64 inverse of the device-built M.  The reference reads t-matrix slot 0
+// (buildNRMatSizeTMatrix ignores its iie argument); iie defaults to 0.
+template <class LSMS, class Atom, class Mat>
+void calculateTauFullMI355X(LSMS &lsms, Atom &atom, int localAtomIndex, int ispin, cplx prel,
+                            Mat &tau, int iie = 0) {
+  ensureTables(lsms);
+  int n = lsms.n_spin_cant * atom.nrmat;
+  tau.resize(n, n);
+  double pr[2] = {prel.real(), prel.imag()};
+  int info = 0;
+  check(lsmsx_tau_full(runtime().ctx, localAtomIndex, ispin, iie, pr, lsms.n_spin_cant,
+                       lsms.n_spin_pola, reinterpret_cast<double *>(&tau(0, 0)), (int)tau.l_dim(),
+                       &info),
+        "lsmsx_tau_full");
+  if (info != 0) std::printf("lsmsx calculateTauFull: singular pivot info=%d\n", info);
+}
+
 }  // namespace lsmsx

@@ -11,7 +11,7 @@
 // output: complex tau00_l[groupSize][k*k][num_local*nspin] (calculateAllTauMatricesMI355X),
 //   then for atom 0, energy 0: complex M[n*n] (buildKKRMatrixMI355X),
 //   complex tau00[k*k] (solveTau00MI355X on that M), complex tau00_l[k*k]
-//   (calculateTauMatrixMI355X)
+//   (calculateTauMatrixMI355X), complex tau[n*n] (calculateTauFullMI355X)
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -92,6 +92,9 @@ int main(int argc, char **argv) {
   std::vector<Complex> tl(k * k);
   lsmsx::calculateTauMatrixMI355X(lsms, local, local.atom[0], 0, 0, prel, tl.data(), 0);
   std::fwrite(tl.data(), sizeof(Complex), (size_t)k * k, fo);
+  Matrix<Complex> tfull;
+  lsmsx::calculateTauFullMI355X(lsms, local.atom[0], 0, 0, prel, tfull, 0);
+  std::fwrite(&tfull(0, 0), sizeof(Complex), tfull.n_row() * tfull.n_col(), fo);
   std::fclose(fo);
   std::printf("host mirror driver ok: %d atoms x %d energies, k=%d\n", local.num_local, group, k);
   return 0;

@@ -99,4 +99,9 @@ def test_host_mirror_matches_oracle(name, s):
     pos += k * k
     assert np.linalg.norm(tau - ref_tau) <= 1e-10 * np.linalg.norm(ref_tau)
     tl = out[pos:pos + k * k]
+    pos += k * k
     assert np.linalg.norm(tl - ref_l) <= 1e-10 * np.linalg.norm(ref_l)
+    from tests import adjoint_ref as R
+    tf = out[pos:pos + n * n].reshape((n, n), order="F")
+    ref_f = R.tau_full(s, 0, 0)
+    assert np.linalg.norm(tf - ref_f) <= 1e-10 * np.linalg.norm(ref_f)
