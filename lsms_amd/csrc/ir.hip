@@ -29,11 +29,24 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef IR_GLOBAL double2 gd2;
 typedef IR_GLOBAL float2 gf2;
 
+// X = 0 and the first residual R = B - M*0 = B ([0; t0], t0 in the central
+// rows), so the first sweep needs no pass over M.
 __global__ __launch_bounds__(256) void k_ir_init(const SolveDesc *__restrict__ descs) {
   const SolveDesc &d = descs[blockIdx.x];
   gd2 *X = (gd2 *)d.X;
-  const long n = (long)d.Ns * d.kc;
-  for (long e = threadIdx.x; e < n; e += blockDim.x) X[e] = make_double2(0.0, 0.0);
+  gf2 *R = (gf2 *)d.R;
+  const int Ns = d.Ns, kc = d.kc;
+  const long n = (long)Ns * kc;
+  for (long e = threadIdx.x; e < n; e += blockDim.x) {
+    X[e] = make_double2(0.0, 0.0);
+    const int row = (int)(e % Ns), col = (int)(e / Ns);
+    float2 b = make_float2(0.f, 0.f);
+    if (row >= Ns - kc) {
+      const double2 t = d.t0[(row - (Ns - kc)) + (long)d.ldt * col];
+      b = make_float2((float)t.x, (float)t.y);
+    }
+    R[e] = b;
+  }
   if (threadIdx.x == 0) {
     *d.conv = 0;
     *d.iters = 0;

@@ -310,7 +310,7 @@ int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int max_sweeps
   if (int r = check_launch("k_ir_init")) return r;
   int done = 0;
   for (int it = 0; it < max_sweeps; it++) {
-    {
+    if (it > 0) {  // k_ir_init set the first residual, B
       ProfScope p(c, 5);
       hipLaunchKernelGGL(k_ir_residual, dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
     }
