@@ -2,10 +2,14 @@
 
 One step = every (local atom, contour energy) tau00 solve of the workload:
 KKR assembly M = 1 - t G(E) on the device plus the central-block inversion
-tau00 = [M^-1 [t0;0]]_00, complex FP64.  Workload: fcc a = 6.82 bohr, LIZ of
+tau00 = [M^-1 [t0;0]]_00.  Workload: fcc a = 6.82 bohr, LIZ of
 135 sites (N = 2160), lmax = 3, 32 local atoms x 31 semicircle energies
 (992 solves per step).  t-matrices, LIZ geometry and tables are resident on
-the device before timing; only the 16x16 tau00 blocks come back.
+the device before timing; only the 16x16 tau00 blocks come back.  For each
+path (plain FP64; mixed with FP32 trailing updates; mixed with split-FP16
+trailing updates) the library's autotuner first picks the batch schedule
+(untimed), then the timed steps replay the captured hipGraph; the headline
+is the fastest mixed path whose tau00 agree with the FP64 path to 1e-10.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -23,6 +27,7 @@ sys.path.insert(0, ROOT)
 METRIC = "tau00 solves/sec at lmax=3 LIZ=135: mixed+batched vs FP64, <=1e-10 rel err"
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix) peak
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (matrix) peak
+FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 (matrix) peak
 
 
 def solve_flops(n, k):
@@ -104,9 +109,11 @@ def main():
     ap.add_argument("--natoms", type=int, default=32)
     ap.add_argument("--nenergies", type=int, default=31)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", choices=("fp64", "mixed", "both"), default="both",
-                    help="fp64: plain path only; mixed/both: also the mixed-precision path, "
-                         "which is the headline when it meets the 1e-10 error bar")
+    ap.add_argument("--precision", choices=("fp64", "mixed", "f16", "all"), default="all",
+                    help="fp64: plain path only; mixed: also the mixed path with FP32 trailing "
+                         "updates; f16: also the mixed path with split-FP16 trailing updates; "
+                         "all: both mixed paths.  The fastest mixed path that meets the 1e-10 "
+                         "error bar against FP64 is the headline")
     ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
                     help="c2: the headline (fcc 135 sites, mixed vs FP64); c4: uneven 4096-site "
                          "bcc alloy batch (graph + autotuned schedule vs eager); c5: spin-canted "
@@ -158,8 +165,16 @@ def main():
     N = s.nrmat_ns(0)
     k = s.kkrsz_ns
 
+    schedules = {}
+
     def timed(precision):
-        for _ in range(args.warmup):
+        """The library's autotuner picks the schedule for this batch and
+        precision (untimed); the timed steps then replay the captured graph."""
+        ctx.set_graph(False)
+        best, ms = ctx.autotune(atoms, iies, prels, precision=precision)
+        schedules[precision] = {"flags": best, "ms_by_flags": [round(x, 1) for x in ms]}
+        ctx.set_graph(True)
+        for _ in range(args.warmup):  # the first call captures the graph
             ctx.tau00_batched(atoms, iies, prels, k, precision=precision)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -167,11 +182,13 @@ def main():
             tau, info = ctx.tau00_batched(atoms, iies, prels, k, precision=precision)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        ctx.set_graph(False)
         assert np.all(info == 0), "singular pivot reported"
         return tau, el
 
     def kernel_profile(precision):
-        """Per-kernel time of one extra step with HIP events on the library's stream."""
+        """Per-kernel time of one extra step with HIP events on the library's
+        stream (eager launches on one stream, so the kernels do not overlap)."""
         ctx.set_profiling(True)
         ctx.reset_profile()
         ctx.tau00_batched(atoms, iies, prels, k, precision=precision)
@@ -187,30 +204,41 @@ def main():
     fp64_solves = nsolve * args.steps / el64
     prof64 = kernel_profile(capi.PREC_FP64)
     headline = "fp64"
-    tau, el = tau64, el64
-    mixed = None
-    if args.precision in ("mixed", "both"):
-        tau32, el32 = timed(capi.PREC_MIXED_FP32)
+    tau, el, prof = tau64, el64, prof64
+    mixed = {}
+    modes = {"mixed": [("fp32_update", capi.PREC_MIXED_FP32)],
+             "f16": [("split_f16_update", capi.PREC_MIXED_F16)],
+             "all": [("fp32_update", capi.PREC_MIXED_FP32),
+                     ("split_f16_update", capi.PREC_MIXED_F16)]}.get(args.precision, [])
+    for name, mode in modes:
+        tm, elm = timed(mode)
         iters, _ = ctx.last_stats(nsolve)
-        prof32 = kernel_profile(capi.PREC_MIXED_FP32)
-        err32 = max(float(np.linalg.norm(tau32[n] - tau64[n]) / np.linalg.norm(tau64[n]))
-                    for n in range(nsolve))
-        mixed = {"solves_per_s": nsolve * args.steps / el32, "max_rel_err_vs_fp64": err32,
-                 "refinement_sweeps": {"min": int(iters.min()), "max": int(iters.max()),
-                                       "mean": round(float(iters.mean()), 3),
-                                       "fp64_fallbacks": int(np.sum(iters < 0))},
-                 "kernel_ms_per_step": {kk: round(v[0], 3) for kk, v in prof32.items()}}
-        if err32 <= 1e-10:
-            headline = "mixed"
-            tau, el = tau32, el32
+        profm = kernel_profile(mode)
+        errm = max(float(np.linalg.norm(tm[n] - tau64[n]) / np.linalg.norm(tau64[n]))
+                   for n in range(nsolve))
+        mixed[name] = {"solves_per_s": round(nsolve * args.steps / elm, 3),
+                       "max_rel_err_vs_fp64": errm,
+                       "refinement_sweeps": {"min": int(iters.min()), "max": int(iters.max()),
+                                             "mean": round(float(iters.mean()), 3),
+                                             "fp64_fallbacks": int(np.sum(iters < 0))},
+                       "kernel_ms_per_step": {kk: round(v[0], 3) for kk, v in profm.items()}}
+        if errm <= 1e-10 and elm < el:
+            headline = name
+            tau, el, prof = tm, elm, profm
     ms_per_step = 1e3 * el / args.steps
     solves_per_s = nsolve * args.steps / el
-    prof = prof64 if headline == "fp64" else prof32
     upd_ms, upd_n = prof["update"]
     nt = (N + 63) // 64
     upd_flops = update_kernel_flops([nt] * nsolve)
     achieved_tf = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
-    peak = FP64_PEAK_TFLOPS if headline == "fp64" else FP32_PEAK_TFLOPS
+    kernel, peak, dtype, prec = {
+        "fp64": ("k_lu_update2<double>", FP64_PEAK_TFLOPS, "f64", "FP64"),
+        "fp32_update": ("k_lu_update2<float>", FP32_PEAK_TFLOPS, "c32-factor+f64-refine",
+                        "mixed: complex FP32 LU on MFMA + FP64 iterative refinement"),
+        "split_f16_update": ("k_lu_update2_h", FP16_PEAK_TFLOPS, "f16x2-split-update+f32-factor+f64-refine",
+                             "mixed: complex FP32 LU with split-FP16 trailing updates on the FP16 "
+                             "matrix cores + FP64 iterative refinement"),
+    }[headline]
     path_tf = solve_flops(N, k) * solves_per_s / 1e12
 
     cpu = None
@@ -233,21 +261,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if headline == "fp64" else "c32-factor+f64-refine",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {
             "workload": f"fcc a=6.82 bohr, LIZ r=13.0 (135 sites, N={N}), lmax=3, "
                         f"{na} atoms x {ne} contour energies, batched",
             "solves_per_step": nsolve, "N": N, "kkrsz_ns": k,
-            "precision": "FP64" if headline == "fp64" else
-                         "mixed: complex FP32 LU on MFMA + FP64 iterative refinement",
+            "precision": prec,
         },
         "roofline": {
-            "bound": "mfma", "kernel": "k_lu_update2<%s>" % ("double" if headline == "fp64" else "float"),
+            "bound": "mfma", "kernel": kernel,
             "achieved": round(achieved_tf, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved_tf / peak, 4),
-            "traffic": profiled_traffic("k_lu_update2<%s>" % ("double" if headline == "fp64" else "float"))
-                       if (args.natoms, args.nenergies) == (32, 31) else None,
+            "traffic": profiled_traffic(kernel) if (args.natoms, args.nenergies) == (32, 31) else None,
             "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4), "launches": upd_n,
         },
         "path_roofline": {"achieved": round(path_tf, 3), "unit": "TFLOP/s",
@@ -255,8 +281,10 @@ def main():
                           "flops_per_solve": solve_flops(N, k)},
         "fp64_path": {"solves_per_s": round(fp64_solves, 3),
                       "kernel_ms_per_step": {kk: round(v[0], 3) for kk, v in prof64.items()}},
-        "mixed_path": mixed,
-        "speedup_mixed_vs_fp64": round(mixed["solves_per_s"] / fp64_solves, 3) if mixed else None,
+        "mixed_paths": mixed or None,
+        "schedules": {{0: "fp64", 1: "fp32_update", 2: "split_f16_update"}[p]: v
+                      for p, v in schedules.items()},
+        "speedup_mixed_vs_fp64": round(solves_per_s / fp64_solves, 3) if headline != "fp64" else None,
         "max_rel_err_vs_lapack": err,
         "cpu_baseline": cpu,
     }
@@ -295,42 +323,48 @@ def bench_c3(args):
     prels = np.array([s.prel(i) for i in iies])
     (t64, i64), el64 = _timed_runs(lambda: ctx.tau00_batched(atoms, iies, prels, 16),
                                    args.steps, args.warmup)
-    run32 = lambda: ctx.tau00_batched(atoms, iies, prels, 16, precision=capi.PREC_MIXED_FP32)
-    (t32, i32), el32 = _timed_runs(run32, args.steps, args.warmup)
-    iters, _ = ctx.last_stats(len(atoms))
+    t0s = [np.array([O.t0(3, int(s.atoms[a].LIZStoreIdx[0]), s.tmatStore, s.blkSizeTmatStore, ie)
+                     for ie in range(ne)]) for a in range(na)]
+    n64 = [contour.idos_ms(t64[a * ne:(a + 1) * ne], t0s[a], s.weights) for a in range(na)]
+    paths = {}
+    for name, mode in (("fp32_update", capi.PREC_MIXED_FP32),
+                       ("split_f16_update", capi.PREC_MIXED_F16)):
+        run = lambda: ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
+        (tm, im), elm = _timed_runs(run, args.steps, args.warmup)
+        iters, _ = ctx.last_stats(len(atoms))
+        assert np.all(im == 0)
+        err = np.array([np.linalg.norm(tm[n] - t64[n]) / np.linalg.norm(t64[n])
+                        for n in range(len(atoms))]).reshape(na, ne)
+        idos_err = max(abs(contour.idos_ms(tm[a * ne:(a + 1) * ne], t0s[a], s.weights) - n64[a])
+                       / abs(n64[a]) for a in range(na))
+        paths[name] = {
+            "solves_per_s": round(len(atoms) / elm, 3), "ms_per_step": round(1e3 * elm, 3),
+            "speedup_vs_fp64": round(el64 / elm, 3),
+            "per_energy_max_rel_err": [float("%.3g" % x) for x in err.max(axis=0)],
+            "refinement_sweeps": {"min": int(iters.min()), "max": int(iters.max()),
+                                  "mean": round(float(iters.mean()), 3),
+                                  "fp64_fallbacks": int(np.sum(iters < 0))},
+            "idos_ms_max_rel_err_vs_fp64": float("%.3g" % idos_err)}
     on = np.nonzero(np.abs(s.weights) > 0)[0]
     la, ll, _, ia = ctx.tau00_adjoint(np.repeat(np.arange(na), len(on)), np.tile(on, na),
                                       np.tile(s.energies[on], na), 16)
     ctx.close()
-    assert np.all(i64 == 0) and np.all(i32 == 0) and np.all(ia == 0)
-    err = np.array([np.linalg.norm(t32[n] - t64[n]) / np.linalg.norm(t64[n])
-                    for n in range(len(atoms))]).reshape(na, ne)
-    idos_err, idos_l = [], []
-    for a in range(na):
-        at = s.atoms[a]
-        t0 = np.array([O.t0(3, int(at.LIZStoreIdx[0]), s.tmatStore, s.blkSizeTmatStore, ie)
-                       for ie in range(ne)])
-        n64 = contour.idos_ms(t64[a * ne:(a + 1) * ne], t0, s.weights)
-        n32 = contour.idos_ms(t32[a * ne:(a + 1) * ne], t0, s.weights)
-        idos_err.append(abs(n32 - n64) / abs(n64))
-        idos_l.append(contour.idos_lloyd(ll[a * len(on):(a + 1) * len(on)], s.weights[on]))
+    assert np.all(i64 == 0) and np.all(ia == 0)
+    idos_l = [contour.idos_lloyd(ll[a * len(on):(a + 1) * len(on)], s.weights[on]) for a in range(na)]
+    best = max(paths, key=lambda k: paths[k]["solves_per_s"])
     out = {
         "metric": "tau00 solves/sec at lmax=3 LIZ=135 with 16 near-axis energies: mixed vs FP64",
-        "value": round(len(atoms) / el32, 3), "unit": "solves/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el32, 3),
+        "value": paths[best]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": paths[best]["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c32-factor+f64-refine", "data": "synthetic",
+        "dtype": "mixed (" + best + ") + f64 refinement", "data": "synthetic",
         "config": {"workload": f"fcc a=6.82, LIZ r=13.0 (135 sites, N=2160), {na} atoms x "
                                f"{ne} energies (16 at Im E 0.001..0.01 Ry)",
                    "solves_per_step": len(atoms)},
         "fp64_solves_per_s": round(len(atoms) / el64, 3),
-        "speedup_mixed_vs_fp64": round(el64 / el32, 3),
-        "per_energy_max_rel_err": [float("%.3g" % x) for x in err.max(axis=0)],
+        "mixed_paths": paths,
         "energies": [[round(float(e.real), 5), round(float(e.imag), 5)] for e in s.energies],
-        "refinement_sweeps": {"min": int(iters.min()), "max": int(iters.max()),
-                              "mean": round(float(iters.mean()), 3),
-                              "fp64_fallbacks": int(np.sum(iters < 0))},
-        "idos_ms_max_rel_err_mixed_vs_fp64": float("%.3g" % max(idos_err)),
+        "idos_ms_fp64": [round(x, 10) for x in n64],
         "idos_lloyd_fp64": [round(x, 8) for x in idos_l],
     }
     print(json.dumps(out))

@@ -26,6 +26,9 @@ typedef struct lsmsx_ctx lsmsx_ctx;
 #define LSMSX_PREC_MIXED_FP32 1 /* complex FP32 factors (FP32 MFMA) + FP64 iterative
                                    refinement against the FP64 matrix; solves whose
                                    refinement stalls are redone in FP64 */
+#define LSMSX_PREC_MIXED_F16 2  /* as LSMSX_PREC_MIXED_FP32, with the trailing updates of
+                                   the FP32 factorization on the FP16 matrix cores
+                                   (hi/lo split panels, FP32 accumulation) */
 
 /* linear-solver selector ids for the reference's 16-bit
  * lsms.global.linearSolver field (linearSolvers.hpp:18-27,196-199;
@@ -171,11 +174,15 @@ int lsmsx_tau00_adjoint(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
  *                            the small final steps of every solve run together;
  *   LSMSX_SCHED_XCD_SWIZZLE  trailing-update workgroups are remapped so that
  *                            each of the 8 XCDs works on whole solves and
- *                            reads their panels into its own L2.
+ *                            reads their panels into its own L2;
+ *   LSMSX_SCHED_TWO_STREAMS  the two halves of a chunk run on two streams (of
+ *                            default priority), so one half's diagonal-tile
+ *                            and panel steps overlap the other's updates.
  * No reference counterpart (the reference solves one matrix per OpenMP
  * thread, calculateTauMatrix.cpp:891-900). */
 #define LSMSX_SCHED_RIGHT_ALIGN 1
 #define LSMSX_SCHED_XCD_SWIZZLE 2
+#define LSMSX_SCHED_TWO_STREAMS 4
 int lsmsx_set_schedule(lsmsx_ctx *ctx, int flags);
 int lsmsx_get_schedule(lsmsx_ctx *ctx, int *flags);
 
@@ -186,8 +193,9 @@ int lsmsx_get_schedule(lsmsx_ctx *ctx, int *flags);
 int lsmsx_set_graph(lsmsx_ctx *ctx, int on);
 
 /* Autotuner: times the given batch (arguments as lsmsx_tau00_batched) under
- * each of the four schedules, keeps the fastest on the context and returns
- * it in *best; ms[4] (may be NULL) receives the wall times. */
+ * each of the eight schedules (flag combinations), keeps the fastest on the
+ * context and returns it in *best; ms[8] (may be NULL) receives the wall
+ * times. */
 int lsmsx_autotune(lsmsx_ctx *ctx, int nsolve, const int *atom_ids, const int *ispins,
                    const int *iies, const double *prel, int n_spin_cant, int n_spin_pola,
                    int precision, int *best, double *ms);

@@ -18,18 +18,22 @@ DRIVER = os.path.join(HERE, "_build", "host_mirror_driver")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kkr_assembly.hip", "tau_lu.hip", "ir.hip", "inverse.hip", "lsmsx_capi.hip"]
+HIP_SOURCES = ["kkr_assembly.hip", "tau_lu.hip", "ir.hip", "inverse.hip", "lsmsx_capi.hip",
+               "update_f16.hip"]
+# sources whose MFMA accumulators stay in AGPRs (no -amdgpu-mfma-vgpr-form)
+AGPR_SOURCES = {"update_f16.hip"}
 CXX_SOURCES = ["tables.cpp"]
 
 HIP_FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of every device code object
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # MFMA accumulators in ordinary VGPRs (unified register file): no
-    # accumulator copies between AGPRs and VGPRs around the K loops
-    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
+# MFMA accumulators in ordinary VGPRs (unified register file): no accumulator
+# copies between AGPRs and VGPRs around the K loops (all sources but
+# AGPR_SOURCES, whose accumulators are too many for the VGPRs)
+VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]
 
 
 def _run(cmd):
@@ -43,8 +47,8 @@ def build(verbose=False):
     os.makedirs(OUT, exist_ok=True)
     jobs = []
     for s in HIP_SOURCES:
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(SRC, s), "-o",
-                     os.path.join(OUT, s + ".o")])
+        flags = HIP_FLAGS + ([] if s in AGPR_SOURCES else VGPR_FORM)
+        jobs.append([HIPCC, *flags, "-c", os.path.join(SRC, s), "-o", os.path.join(OUT, s + ".o")])
     for s in CXX_SOURCES:
         jobs.append(["g++", "-O2", "-fPIC", "-std=c++17", "-c", os.path.join(SRC, s), "-o",
                      os.path.join(OUT, s + ".o")])

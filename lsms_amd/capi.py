@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "liblsmsx.so")
 
 PREC_FP64 = 0
 PREC_MIXED_FP32 = 1
+PREC_MIXED_F16 = 2
 (KIND_ASSEMBLE, KIND_DIAG, KIND_PANEL, KIND_UPDATE, KIND_FINISH, KIND_IR_RESIDUAL,
  KIND_IR_SOLVE, KIND_INVERSE, KIND_ADJOINT, KIND_CROSS) = range(10)
 
@@ -26,6 +27,7 @@ EXPORTED = [
 ]
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
+SCHED_TWO_STREAMS = 4
 
 _lib = None
 _vp = C.c_void_p
@@ -234,14 +236,14 @@ class Context:
         _ck(load().lsmsx_set_graph(self._h, 1 if on else 0))
 
     def autotune(self, atom_ids, iies, prels, ispins=None, precision=PREC_FP64):
-        """Time the batch under the four schedules; returns (best flags, ms[4])."""
+        """Time the batch under the eight schedules; returns (best flags, ms[8])."""
         n = len(atom_ids)
         a = np.ascontiguousarray(atom_ids, dtype=np.int32)
         e = np.ascontiguousarray(iies, dtype=np.int32)
         s = np.zeros(n, dtype=np.int32) if ispins is None else np.ascontiguousarray(ispins, dtype=np.int32)
         pr = np.ascontiguousarray(prels, dtype=np.complex128)
         best = C.c_int(0)
-        ms = np.zeros(4)
+        ms = np.zeros(8)
         _ck(load().lsmsx_autotune(self._h, n, _p(a), _p(s), _p(e), _p(pr), self.nsc, self.nspola,
                                   precision, C.byref(best), _p(ms)))
         return best.value, ms

@@ -57,6 +57,7 @@ __global__ void k_tau_grad(const SolveDesc *, const double2 *, double2 *, int);
 __global__ void k_grad_reduce(const double2 *, int, const int *, const long *, const int *,
                               const int *, double2 *, long, int);
 __global__ void k_zero_c(double2 *, long);
+__global__ void k_lu_update2_h(const SolveDesc *, int, int);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout.
 __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
@@ -150,6 +151,8 @@ int roundup(int x, int m) { return ((x + m - 1) / m) * m; }
 struct lsmsx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // second half of a chunk (LSMSX_SCHED_TWO_STREAMS)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int lmax_tab = -1;
   DevBuf<double> cgnt;
   DevBuf<double2> illp;
@@ -162,6 +165,8 @@ struct lsmsx_ctx {
   DevBuf<float2> W, R;
   DevBuf<int> perm, info, conv, iters, count;
   DevBuf<double> minpiv, dxn;
+  DevBuf<unsigned short> hp;  // split-FP16 panel planes
+  DevBuf<int> he;
   DevBuf<SolveDesc> desc, desc2;
   // explicit inverse / adjoint
   DevBuf<double2> Minv, gx, lpart, lloyd, vbar, gblk, gstore, tfull;
@@ -264,18 +269,20 @@ std::vector<int> site_offsets(const AtomDev &a, int nsc, bool solver) {
 // row/column b, panels of b, then one K = 128 trailing update -- so the
 // trailing matrix is read and written once per two steps.
 template <class T>
-int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max) {
+int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_f16 = false,
+              hipStream_t st = nullptr) {
+  if (!st) st = c->stream;
   const int swz = (c->sched & LSMSX_SCHED_XCD_SWIZZLE) ? 1 : 0;
   auto diag_panel = [&](int s) -> int {
     const int nrest = nt_max - 1 - s;
     if (nrest <= 0) return 0;
     {
       ProfScope p(c, 1);
-      hipLaunchKernelGGL(k_lu_diag<T>, dim3(nb), dim3(256), 0, c->stream, dd, s);
+      hipLaunchKernelGGL(k_lu_diag<T>, dim3(nb), dim3(256), 0, st, dd, s);
     }
     {
       ProfScope p(c, 2);
-      hipLaunchKernelGGL(k_lu_panel<T>, dim3(2 * nrest, nb), dim3(256), 0, c->stream, dd, s);
+      hipLaunchKernelGGL(k_lu_panel<T>, dim3(2 * nrest, nb), dim3(256), 0, st, dd, s);
     }
     return check_launch("k_lu_diag/k_lu_panel");
   };
@@ -284,20 +291,24 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max) {
     const int ncross = 2 * (nt_max - s - 1) - 1;
     if (ncross > 0) {
       ProfScope p(c, 9);
-      hipLaunchKernelGGL(k_lu_cross<T>, dim3(ncross, nb), dim3(256), 0, c->stream, dd, s);
+      hipLaunchKernelGGL(k_lu_cross<T>, dim3(ncross, nb), dim3(256), 0, st, dd, s);
     }
     if (int r = diag_panel(s + 1)) return r;
     const int nrest = nt_max - 2 - s;
     if (nrest > 0) {
       ProfScope p(c, 3);
-      hipLaunchKernelGGL(k_lu_update2<T>, dim3(nrest * nrest, nb), dim3(256), 0, c->stream, dd, s,
-                         swz);
+      if (split_f16)
+        hipLaunchKernelGGL(k_lu_update2_h, dim3((nrest * nrest + 1) / 2, nb), dim3(256), 0,
+                           st, dd, s, swz);
+      else
+        hipLaunchKernelGGL(k_lu_update2<T>, dim3(nrest * nrest, nb), dim3(256), 0, st, dd,
+                           s, swz);
     }
     if (int r = check_launch("k_lu_update2")) return r;
   }
   {
     ProfScope p(c, 4);
-    hipLaunchKernelGGL(k_lu_last<T>, dim3(nb), dim3(256), 0, c->stream, dd);
+    hipLaunchKernelGGL(k_lu_last<T>, dim3(nb), dim3(256), 0, st, dd);
   }
   return check_launch("k_lu_last");
 }
@@ -351,9 +362,11 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
                 const int *iies, const double *prel, int nsc, int nspola, int precision,
                 double *tau_out, int *info_out) {
   if (int r = validate_common(c, nsc, nspola)) return r;
-  if (precision != LSMSX_PREC_FP64 && precision != LSMSX_PREC_MIXED_FP32)
+  if (precision != LSMSX_PREC_FP64 && precision != LSMSX_PREC_MIXED_FP32 &&
+      precision != LSMSX_PREC_MIXED_F16)
     return fail(-8, "unsupported precision mode");
-  const bool mixed = precision == LSMSX_PREC_MIXED_FP32;
+  const bool mixed = precision != LSMSX_PREC_FP64;
+  const bool split = precision == LSMSX_PREC_MIXED_F16;
   if (nsolve <= 0) return 0;
   HIPCHK(hipSetDevice(c->device));
   c->last_iters.assign(nsolve, 0);
@@ -379,6 +392,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     long Ns = roundup(nsc * a.nrmat, kTile), nt = Ns / kTile;
     bytes[i] = Ns * Ns * 16;
     if (mixed) bytes[i] += Ns * Ns * 8 + nt * kTile * kTile * 8 + Ns * kc * 24;
+    if (split) bytes[i] += 4 * nt * (long)kPlaneHalves * 2 + 16 * nt;
     else bytes[i] += kTile * kTile * 16;
   }
   long cap = c->ws_limit;
@@ -395,7 +409,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     int end = done;
     while (end < nsolve && (end == done || acc + bytes[end] <= cap)) acc += bytes[end++];
     int nb = end - done;
-    long mtot = 0, xtot = 0, ivtot = 0;
+    long mtot = 0, xtot = 0, ivtot = 0, htot = 0;
     int maxliz = 1, nt_max = 1;
     for (int b = 0; b < nb; b++) {
       const AtomDev &a = c->atoms[atom_ids[done + b]];
@@ -403,6 +417,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       mtot += Ns * Ns;
       xtot += Ns * kc;
       ivtot += mixed ? (Ns / kTile) * kTile * kTile : kTile * kTile;
+      htot += 4 * (Ns / kTile);
       maxliz = std::max(maxliz, a.numLIZ);
       nt_max = std::max(nt_max, (int)(Ns / kTile));
     }
@@ -422,14 +437,23 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (int r = c->dxn.ensure((size_t)2 * nb)) return r;
       if (int r = c->count.ensure(1)) return r;
     }
+    if (split) {
+      if (int r = c->hp.ensure((size_t)htot * kPlaneHalves)) return r;
+      if (int r = c->he.ensure((size_t)htot)) return r;
+    }
     std::vector<SolveDesc> hd(nb);
-    long moff = 0, xoff = 0, ioff = 0;
+    long moff = 0, xoff = 0, ioff = 0, hoff = 0;
     for (int b = 0; b < nb; b++) {
       int i = done + b;
       AtomDev &a = c->atoms[atom_ids[i]];
       SolveDesc &d = hd[b];
       d = SolveDesc{};
       int N = nsc * a.nrmat, Ns = roundup(N, kTile);
+      if (split) {
+        d.hp = c->hp.p + hoff * kPlaneHalves;
+        d.he = c->he.p + hoff;
+        hoff += 4 * (Ns / kTile);
+      }
       d.pos = a.d_pos;
       d.lmax = a.d_lmax;
       d.store = a.d_store;
@@ -483,22 +507,41 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     // once per chunk shape into a hipGraph and replayed (the kernels read
     // the freshly uploaded descriptors, so only the shape and the buffer
     // addresses passed as arguments key the graph)
-    auto enqueue = [&]() -> int {
-      hipLaunchKernelGGL(k_init_status, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->desc.p,
-                         nb);
+    // solves [b0, b0 + n) of the chunk on stream st
+    auto enqueue_part = [&](hipStream_t st, int b0, int n) -> int {
+      const SolveDesc *dd = c->desc.p + b0;
+      // the step range is the chunk's: right-aligned starts are relative to it
+      const int ntm = nt_max;
+      int mliz = 1;
+      for (int b = b0; b < b0 + n; b++) mliz = std::max(mliz, hd[b].numLIZ);
+      hipLaunchKernelGGL(k_init_status, dim3((n + 255) / 256), dim3(256), 0, st, dd, n);
       if (int r = check_launch("k_init_status")) return r;
       {
         ProfScope p(c, 0);
-        hipLaunchKernelGGL(k_pad_identity, dim3(nb), dim3(256), 0, c->stream, c->desc.p);
-        hipLaunchKernelGGL(k_assemble, dim3(maxliz, nb), dim3(256), 0, c->stream, c->desc.p,
-                           c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
+        hipLaunchKernelGGL(k_pad_identity, dim3(n), dim3(256), 0, st, dd);
+        hipLaunchKernelGGL(k_assemble, dim3(mliz, n), dim3(256), 0, st, dd, c->tmat.p, c->ldT,
+                           c->cgnt.p, c->illp.p, c->lmax_tab);
       }
       if (int r = check_launch("k_assemble")) return r;
-      if (!mixed) return launch_lu<double>(c, c->desc.p, nb, nt_max);
-      return launch_lu<float>(c, c->desc.p, nb, nt_max);
+      if (!mixed) return launch_lu<double>(c, dd, n, ntm, false, st);
+      return launch_lu<float>(c, dd, n, ntm, split, st);
+    };
+    // two halves on two streams overlap one half's latency-bound steps
+    // (diagonal tiles, panels) with the other half's trailing updates
+    const bool two = (c->sched & LSMSX_SCHED_TWO_STREAMS) && nb >= 2 && !c->profiling;
+    auto enqueue = [&]() -> int {
+      if (!two) return enqueue_part(c->stream, 0, nb);
+      const int h = nb / 2;
+      HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+      HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+      if (int r = enqueue_part(c->stream, 0, h)) return r;
+      if (int r = enqueue_part(c->stream2, h, nb - h)) return r;
+      HIPCHK(hipEventRecord(c->ev_join, c->stream2));
+      HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+      return 0;
     };
     if (c->use_graph) {
-      std::vector<long> key = {nb, nt_max, maxliz, mixed ? 1 : 0, c->sched, (long)c->desc.p,
+      std::vector<long> key = {nb, nt_max, maxliz, precision, c->sched, (long)c->desc.p,
                                (long)c->tmat.p, c->ldT, (long)c->cgnt.p, (long)c->illp.p,
                                c->lmax_tab};
       auto it = c->graphs.find(key);
@@ -544,6 +587,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
           if (!hconv[b]) {
             SolveDesc d = hd[b];
             d.step0 = 0;  // the fallback chunk has its own step count
+            d.hp = nullptr;
+            d.he = nullptr;
             d.A = d.M;
             d.inv = c->inv.p + (long)fb.size() * kTile * kTile;
             d.inv_stride = 0;
@@ -858,7 +903,13 @@ int lsmsx_ctx_create(int device, lsmsx_ctx **out) {
   lsmsx_ctx *c = new lsmsx_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     delete c;
     return fail(-100 - (int)e, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
@@ -903,6 +954,8 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
   c->graphs.clear();
   c->dtmat.release();
+  c->hp.release();
+  c->he.release();
   c->Minv.release();
   c->gx.release();
   c->lpart.release();
@@ -914,6 +967,10 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->keyi.release();
   c->colmap.release();
   c->keyl.release();
+  (void)hipStreamSynchronize(c->stream2);
+  (void)hipEventDestroy(c->ev_fork);
+  (void)hipEventDestroy(c->ev_join);
+  (void)hipStreamDestroy(c->stream2);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1260,7 +1317,7 @@ int lsmsx_tau_full(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *
 
 int lsmsx_set_schedule(lsmsx_ctx *c, int flags) {
   if (!c) return fail(-1, "null context");
-  if (flags & ~(LSMSX_SCHED_RIGHT_ALIGN | LSMSX_SCHED_XCD_SWIZZLE))
+  if (flags & ~(LSMSX_SCHED_RIGHT_ALIGN | LSMSX_SCHED_XCD_SWIZZLE | LSMSX_SCHED_TWO_STREAMS))
     return fail(-2, "unknown schedule flags");
   c->sched = flags;
   return 0;
@@ -1291,7 +1348,7 @@ int lsmsx_autotune(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *isp
                        tau.data(), info.data());  // warm-up: allocations, graphs of sched 0
   double tbest = 1e300;
   int sbest = c->sched;
-  for (int sch = 0; sch < 4 && rc == 0; sch++) {
+  for (int sch = 0; sch < 8 && rc == 0; sch++) {
     c->sched = sch;
     if (c->use_graph) {  // capture outside the timed run
       rc = run_batched(c, nsolve, atom_ids, ispins, iies, prel, nsc, nspola, precision,

@@ -14,6 +14,7 @@ namespace lsmsx {
 constexpr int kTile = 64;     // LU tile / panel width
 constexpr int kMaxLmax = 3;   // tables and per-site blocks sized for lmax <= 3
 constexpr int kMaxKkr = (kMaxLmax + 1) * (kMaxLmax + 1);
+constexpr int kPlaneHalves = 4 * kTile * kTile;  // one tile's four FP16 planes
 
 // One solve (local atom, spin, energy slot) of a batch.
 struct SolveDesc {
@@ -52,6 +53,10 @@ struct SolveDesc {
   int *iters;             // refinement sweeps used
   double *dxn;            // ||dX_c||^2 of the previous sweep
   double *dxn_rho2;       // the contraction (||dX_c|| ratio)^2 observed at the previous sweep
+  // split-FP16 trailing updates (mixed path): the FP32 panels of the two
+  // steps of a pair as hi/lo FP16 planes with one power-of-2 scale per tile
+  void *hp;               // (parity 2) x (L, U) x nt tiles x 4 planes x 64 x 64 halves
+  int *he;                // (parity 2) x (L, U) x nt tile exponents
   // explicit inverse (full tau, Lloyd term, adjoint of tau00)
   int full_lu;            // 1: the last tile is factored by k_lu_diag like every other tile
   double2 *Minv;          // Ns x Ns (leading dim ld): M^-1 in the solver layout

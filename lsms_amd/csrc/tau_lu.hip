@@ -180,6 +180,57 @@ __global__ __launch_bounds__(256) void k_lu_diag(const SolveDesc *__restrict__ d
   }
 }
 
+// Split one FP32 panel tile (Ts[col][row], 256 threads) into hi/lo FP16
+// planes for the split-FP16 trailing update, with one power-of-2 scale e per
+// tile (max |x| 2^-e < 2^14, inside the FP16 range with room for the lo
+// part); x 2^-e = hi + lo to 22 bits.  Layout, k fastest:
+//   L tile (rows i, k = tile column):  planes -Re, -Re lo, Im, Im lo ([i][k]);
+//   U tile (k = tile row, columns j):  planes  Re,  Re lo, Im, Im lo ([j][k]).
+// The L planes carry the sign of the update C -= L U.
+__device__ void write_split_planes(float2 (*Ts)[kTile + 1], bool lower, _Float16 *out, int *eout) {
+  __shared__ float wmax[4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  __syncthreads();  // the tile was written to Ts by all four waves
+  float m = 0.f;
+  for (int e = tid; e < kTile * kTile; e += 256) {
+    const float2 v = Ts[e >> 6][e & 63];
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+  for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if (lane == 0) wmax[w] = m;
+  __syncthreads();
+  m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+  int ex = 0;
+  if (m > 0.f) {
+    frexpf(m, &ex);  // m < 2^ex
+    ex -= 14;
+  }
+  if (tid == 0) *eout = ex;
+  const float sc = ldexpf(1.f, -ex);
+  const float sgn = lower ? -1.f : 1.f;
+  typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+  for (int q = tid; q < kTile * 8; q += 256) {
+    const int rc = q >> 3, kg = q & 7;  // row i (L) or column j (U); k group
+    h8 rh, rl, ih, il;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int k = 8 * kg + u;
+      const float2 v = lower ? Ts[k][rc] : Ts[rc][k];
+      const float re = sgn * v.x * sc, im = v.y * sc;
+      const _Float16 a = (_Float16)re, b = (_Float16)im;
+      rh[u] = a;
+      rl[u] = (_Float16)(re - (float)a);
+      ih[u] = b;
+      il[u] = (_Float16)(im - (float)b);
+    }
+    h8 *o = (h8 *)(out + rc * kTile + 8 * kg);
+    o[0] = rh;
+    o[(kTile * kTile) / 8] = rl;
+    o[2 * (kTile * kTile) / 8] = ih;
+    o[3 * (kTile * kTile) / 8] = il;
+  }
+}
+
 // Panel of step s: x < nrest -> L_is = A_is U^-1 ; else U_sj = L^-1 P A_sj.
 // The operand tile is staged in LDS (rows gathered by the pivot permutation
 // for U_sj) before it is overwritten in place.
@@ -240,6 +291,8 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
                           [&](int k, int j) { return Ts[wc + j][k]; }, q);
   }
   GCx<T> *C = tile + wr + ld * wc;
+  const bool planes = sizeof(T) == 4 && d.hp != nullptr;
+  if (planes) __syncthreads();  // every wave is done reading Ts
 #pragma unroll
   for (int bj = 0; bj < 2; bj++)
 #pragma unroll
@@ -250,7 +303,17 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
         o.x = q.p[0][bj][bi][r] - q.p[1][bj][bi][r];
         o.y = q.p[2][bj][bi][r] - q.p[0][bj][bi][r] - q.p[1][bj][bi][r];
         C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))] = o;
+        if (planes) Ts[wc + 16 * bj + acc_row<T>(kq, r)][wr + 16 * bi + li] = o;
       }
+  if constexpr (sizeof(T) == 4) {
+    if (planes) {
+      // split-FP16 planes of this panel tile for k_lu_update2_h
+      const int parity = step & 1, side = lower ? 0 : 1;
+      const int t = step + 1 + (lower ? x : x - nrest);
+      write_split_planes(Ts, lower, (_Float16 *)d.hp + (long)((parity * 2 + side) * d.nt + t) * kPlaneHalves,
+                         d.he + (parity * 2 + side) * d.nt + t);
+    }
+  }
 }
 
 // C(ti, tj) -= sum_{u < ns} L(ti, s0+u) U(s0+u, tj): one 64x64 tile per

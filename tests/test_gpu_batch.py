@@ -44,14 +44,14 @@ def lapack_tau00(s, a, ie):
     return sla.solve(m, rhs)[:s.kkrsz_ns]
 
 
-@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED_FP32])
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED_FP32, capi.PREC_MIXED_F16])
 def test_schedules_bitwise_identical(c4, precision):
     s, ctx = c4
     atoms = pick(s, 6)
     iies = [3, 17, 31, 8, 25, 0]
     pr = [s.prel(i) for i in iies]
     outs = []
-    for flags in range(4):
+    for flags in range(8):
         ctx.set_schedule(flags)
         tau, info = ctx.tau00_batched(atoms, iies, pr, 16, precision=precision)
         assert np.all(info == 0)
@@ -68,7 +68,7 @@ def test_graph_replay_matches_eager(c4):
     s, ctx = c4
     atoms = pick(s, 8)
     rng = np.random.default_rng(7)
-    ctx.set_schedule(capi.SCHED_RIGHT_ALIGN | capi.SCHED_XCD_SWIZZLE)
+    ctx.set_schedule(capi.SCHED_RIGHT_ALIGN | capi.SCHED_TWO_STREAMS)
     for rep in range(3):
         iies = rng.integers(0, len(s.energies), len(atoms))
         pr = [s.prel(i) for i in iies]
@@ -95,7 +95,7 @@ def test_autotune_then_same_results(c4):
     ctx.set_schedule(0)
     before, _ = ctx.tau00_batched(atoms, iies, pr, 16)
     best, ms = ctx.autotune(atoms, iies, pr)
-    assert 0 <= best <= 3 and ctx.schedule() == best
+    assert 0 <= best <= 7 and ctx.schedule() == best
     assert np.all(ms > 0)
     after, _ = ctx.tau00_batched(atoms, iies, pr, 16)
     assert np.array_equal(before, after)
@@ -111,7 +111,7 @@ def test_full_alloy_batch_graph(c4):
     atoms = np.repeat(np.arange(na), ne)
     iies = np.tile(np.arange(ne), na)
     pr = [s.prel(i) for i in iies]
-    ctx.set_schedule(capi.SCHED_RIGHT_ALIGN | capi.SCHED_XCD_SWIZZLE)
+    ctx.set_schedule(capi.SCHED_RIGHT_ALIGN | capi.SCHED_TWO_STREAMS)
     ctx.set_graph(True)
     tau, info = ctx.tau00_batched(atoms, iies, pr, 16)
     tau2, info2 = ctx.tau00_batched(atoms, iies, pr, 16)  # replay

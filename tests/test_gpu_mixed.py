@@ -1,5 +1,7 @@
 """Mixed-precision tau00 (FP32 factors on MFMA + FP64 iterative refinement
-against the FP64 KKR matrix) -- needs a GPU.
+against the FP64 KKR matrix) -- needs a GPU.  Both mixed modes are covered:
+FP32 trailing updates, and split-FP16 trailing updates (hi/lo FP16 panels on
+the FP16 matrix cores, FP32 accumulation).
 
 Bar: tau00 within 1e-10 relative (Frobenius) of the FP64 oracle / LAPACK,
 including energies 0.001 Ry above the real axis; solves whose refinement
@@ -17,6 +19,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 MIXED = capi.PREC_MIXED_FP32
+MODES = [capi.PREC_MIXED_FP32, capi.PREC_MIXED_F16]
 TAU_RTOL = 1e-10
 
 
@@ -38,7 +41,8 @@ def lapack_tau00(s, a, ie, ispin=0):
     return x[:k]
 
 
-def test_mixed_c1_all_62_against_golden():
+@pytest.mark.parametrize("mode", MODES)
+def test_mixed_c1_all_62_against_golden(mode):
     s = CF.config_c1()
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "c1_golden.npz"),
                 allow_pickle=False)
@@ -48,7 +52,7 @@ def test_mixed_c1_all_62_against_golden():
     atoms = np.repeat(np.arange(na), ne)
     iies = np.tile(np.arange(ne), na)
     prels = np.array([s.prel(i) for i in iies])
-    tau, info = ctx.tau00_batched(atoms, iies, prels, 16, precision=MIXED)
+    tau, info = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
     iters, _ = ctx.last_stats(len(atoms))
     ctx.close()
     assert np.all(info == 0)
@@ -57,7 +61,8 @@ def test_mixed_c1_all_62_against_golden():
     assert worst < TAU_RTOL, worst
 
 
-def test_mixed_fcc135_near_real_axis():
+@pytest.mark.parametrize("mode", MODES)
+def test_mixed_fcc135_near_real_axis(mode):
     """N = 2160 at contour points and at Im E = 0.001-0.01 Ry (the C3 energies)."""
     s = CF.config_c3(natoms=2)
     ctx = capi.Context(0)
@@ -66,7 +71,7 @@ def test_mixed_fcc135_near_real_axis():
     atoms = np.array([a for a, _ in sel])
     iies = np.array([e for _, e in sel])
     prels = np.array([s.prel(i) for i in iies])
-    tau, info = ctx.tau00_batched(atoms, iies, prels, 16, precision=MIXED)
+    tau, info = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
     iters, _ = ctx.last_stats(len(sel))
     ctx.close()
     assert np.all(info == 0)
@@ -75,7 +80,8 @@ def test_mixed_fcc135_near_real_axis():
         assert err < TAU_RTOL, (a, ie, err, iters[n])
 
 
-def test_mixed_equals_plain_within_tolerance():
+@pytest.mark.parametrize("mode", MODES)
+def test_mixed_equals_plain_within_tolerance(mode):
     s = CF.config_c1()
     ctx = capi.Context(0)
     ctx.load_system(s)
@@ -83,13 +89,14 @@ def test_mixed_equals_plain_within_tolerance():
     iies = np.array([0, 7, 19, 30])
     prels = np.array([s.prel(i) for i in iies])
     t64, _ = ctx.tau00_batched(atoms, iies, prels, 16)
-    t32, _ = ctx.tau00_batched(atoms, iies, prels, 16, precision=MIXED)
+    t32, _ = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
     ctx.close()
     for n in range(4):
         assert rel(t32[n], t64[n]) < TAU_RTOL
 
 
-def test_mixed_fallback_to_fp64():
+@pytest.mark.parametrize("mode", MODES)
+def test_mixed_fallback_to_fp64(mode):
     """With one refinement sweep allowed nothing converges: every solve is
     redone by the plain FP64 path and still meets the bar."""
     s = CF.config_c1()
@@ -99,7 +106,7 @@ def test_mixed_fallback_to_fp64():
     atoms = np.array([0, 1, 1])
     iies = np.array([2, 9, 28])
     prels = np.array([s.prel(i) for i in iies])
-    tau, info = ctx.tau00_batched(atoms, iies, prels, 16, precision=MIXED)
+    tau, info = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
     iters, _ = ctx.last_stats(3)
     ctx.close()
     assert np.all(iters == -1)
@@ -107,8 +114,9 @@ def test_mixed_fallback_to_fp64():
         assert rel(tau[n], lapack_tau00(s, atoms[n], iies[n])) < TAU_RTOL
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("nsc,nsp,ispin", [(1, 2, 1), (2, 2, 0)])
-def test_mixed_spin_variants(nsc, nsp, ispin):
+def test_mixed_spin_variants(nsc, nsp, ispin, mode):
     e, w = CF.contour_semicircle(-0.5, 0.7, 4)
     s = CF.make_system("sp", "bcc", 5.42, (1, 1, 1), 8.0, 3, e, w, seed=12,
                        n_spin_cant=nsc, n_spin_pola=nsp)
@@ -118,7 +126,7 @@ def test_mixed_spin_variants(nsc, nsp, ispin):
     atoms = np.array([0, 1, 0])
     iies = np.array([0, 1, 3])
     prels = np.array([s.prel(i) for i in iies])
-    tau, info = ctx.tau00_batched(atoms, iies, prels, k, ispins=np.full(3, ispin), precision=MIXED)
+    tau, info = ctx.tau00_batched(atoms, iies, prels, k, ispins=np.full(3, ispin), precision=mode)
     ctx.close()
     assert np.all(info == 0)
     for n in range(3):
@@ -131,7 +139,8 @@ def t0_blocks(s, a, iies):
                           int(ie)) for ie in iies])
 
 
-def test_c3_per_energy_error_and_idos():
+@pytest.mark.parametrize("mode", MODES)
+def test_c3_per_energy_error_and_idos(mode):
     """The 48-energy workload (32 contour points + 16 at Im E = 0.001-0.01 Ry):
     per-energy mixed-vs-FP64 tau00 error within 1e-10, the FP64 path against
     LAPACK at the nearest-axis energies, and the multiple-scattering IDOS
@@ -144,7 +153,7 @@ def test_c3_per_energy_error_and_idos():
     atoms = np.repeat(np.arange(na), ne)
     iies = np.tile(np.arange(ne), na)
     prels = np.array([s.prel(i) for i in iies])
-    t32, info32 = ctx.tau00_batched(atoms, iies, prels, 16, precision=MIXED)
+    t32, info32 = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
     t64, info64 = ctx.tau00_batched(atoms, iies, prels, 16)
     ctx.close()
     assert np.all(info32 == 0) and np.all(info64 == 0)
