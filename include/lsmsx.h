@@ -29,6 +29,11 @@ typedef struct lsmsx_ctx lsmsx_ctx;
 #define LSMSX_PREC_MIXED_F16 2  /* as LSMSX_PREC_MIXED_FP32, with the trailing updates of
                                    the FP32 factorization on the FP16 matrix cores
                                    (hi/lo split panels, FP32 accumulation) */
+#define LSMSX_PREC_FP64_REFINED 3 /* FP64 factors of a copy of M + FP64 iterative
+                                   refinement against M: full FP64 accuracy when the
+                                   tile-local pivots are small.  Used automatically for
+                                   the plain path's guarded solves and the mixed path's
+                                   unconverged solves */
 
 /* linear-solver selector ids for the reference's 16-bit
  * lsms.global.linearSolver field (linearSolvers.hpp:18-27,196-199;
@@ -106,10 +111,19 @@ int lsmsx_solve_tau00_matrix(lsmsx_ctx *ctx, int n, int k, const double *m,
  * converge are redone in FP64. */
 int lsmsx_set_refinement(lsmsx_ctx *ctx, int max_sweeps, double tol);
 
+/* Pivot guard of the plain FP64 path: a solve whose smallest tile-local pivot
+ * (|u_kk|, pivoting inside 64x64 diagonal tiles) is below threshold is redone
+ * with LSMSX_PREC_FP64_REFINED (default 1e-2; 0 disables).  Small pivots mean
+ * element growth, which can cost tau00 digits on ill-conditioned matrices;
+ * the refinement against the untouched M recovers them.  No counterpart in the
+ * reference (LAPACK zgetrf pivots over whole columns). */
+int lsmsx_set_pivot_guard(lsmsx_ctx *ctx, double threshold);
+
 /* Per-solve statistics of the last batched / single call on this context:
- * iters[i] = refinement sweeps (0 on the plain FP64 path, -1 when a mixed
- * solve fell back to FP64), minpiv[i] = smallest |pivot| of the factorization.
- * Either pointer may be NULL. */
+ * iters[i] = refinement sweeps (0 on the plain FP64 path, > 0 for a refined
+ * solve including the plain path's guarded ones, -1 when a mixed solve was
+ * redone in FP64), minpiv[i] = smallest |pivot| of the factorization that
+ * produced the result.  Either pointer may be NULL. */
 int lsmsx_last_batch_stats(lsmsx_ctx *ctx, int nsolve, int *iters, double *minpiv);
 
 /* Workspace cap in bytes for the batched matrix pool (0 = automatic:

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "liblsmsx.so")
 PREC_FP64 = 0
 PREC_MIXED_FP32 = 1
 PREC_MIXED_F16 = 2
+PREC_FP64_REFINED = 3
 (KIND_ASSEMBLE, KIND_DIAG, KIND_PANEL, KIND_UPDATE, KIND_FINISH, KIND_IR_RESIDUAL,
  KIND_IR_SOLVE, KIND_INVERSE, KIND_ADJOINT, KIND_CROSS) = range(10)
 
@@ -23,7 +24,7 @@ EXPORTED = [
     "lsmsx_set_workspace_limit", "lsmsx_set_profiling", "lsmsx_reset_profile",
     "lsmsx_get_profile", "lsmsx_set_refinement", "lsmsx_last_batch_stats",
     "lsmsx_upload_dtmats", "lsmsx_tau_full", "lsmsx_tau00_adjoint", "lsmsx_set_schedule",
-    "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune",
+    "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune", "lsmsx_set_pivot_guard",
 ]
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
@@ -66,6 +67,7 @@ def load():
     L.lsmsx_tau_full.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
                                  C.c_int, _vp]
     L.lsmsx_set_schedule.argtypes = [_vp, C.c_int]
+    L.lsmsx_set_pivot_guard.argtypes = [_vp, C.c_double]
     L.lsmsx_get_schedule.argtypes = [_vp, _vp]
     L.lsmsx_set_graph.argtypes = [_vp, C.c_int]
     L.lsmsx_autotune.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
@@ -224,6 +226,9 @@ class Context:
                                        _p(grad) if grad is not None else None, _p(info)))
         return np.ascontiguousarray(np.transpose(tau, (0, 2, 1))), ll, grad, info
 
+    def set_pivot_guard(self, threshold):
+        _ck(load().lsmsx_set_pivot_guard(self._h, float(threshold)))
+
     def set_schedule(self, flags):
         _ck(load().lsmsx_set_schedule(self._h, int(flags)))
 
@@ -252,7 +257,8 @@ class Context:
         _ck(load().lsmsx_set_refinement(self._h, int(max_sweeps), float(tol)))
 
     def last_stats(self, n):
-        """(refinement sweeps per solve, -1 = FP64 fallback; smallest |pivot|)."""
+        """(refinement sweeps per solve: 0 plain, > 0 refined, -1 mixed solve redone in
+        FP64; smallest |pivot| of the factorization that produced each result)."""
         it = np.zeros(n, dtype=np.int32)
         mp = np.zeros(n, dtype=np.float64)
         _ck(load().lsmsx_last_batch_stats(self._h, n, _p(it), _p(mp)))

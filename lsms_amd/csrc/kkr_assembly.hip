@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
   const bool prel0 = (prel.x == 0.0 && prel.y == 0.0);
   double2 *__restrict__ M = d.M;
   // mixed precision: the solver factors an FP32 copy written alongside M
-  float2 *__restrict__ W = (d.A != (void *)d.M) ? (float2 *)d.A : nullptr;
+  float2 *__restrict__ W = d.w32;
   const long ld = d.ld;
   const int offi = d.site_off[i];
 
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_pad_identity(const SolveDesc *__restric
   const int pad = d.pad;
   if (pad == 0) return;
   double2 *M = d.M;
-  float2 *W = (d.A != (void *)d.M) ? (float2 *)d.A : nullptr;
+  float2 *W = d.w32;
   const long ld = d.ld;
   // columns 0..pad-1: full columns; columns pad..Ns-1: rows 0..pad-1
   for (long e = threadIdx.x; e < (long)pad * d.Ns; e += blockDim.x) {

@@ -37,9 +37,13 @@ template <class T>
 __global__ void k_lu_update2(const SolveDesc *, int, int);
 template <class T>
 __global__ void k_lu_last(const SolveDesc *);
+template <class T>
 __global__ void k_ir_init(const SolveDesc *);
+template <class T>
 __global__ void k_ir_residual(const SolveDesc *);
+template <class T>
 __global__ void k_ir_solve(const SolveDesc *, double);
+__global__ void k_copy_m(const SolveDesc *);
 __global__ void k_ir_extract(const SolveDesc *);
 __global__ void k_count_conv(const SolveDesc *, int, int *);
 __global__ void k_lloyd(const SolveDesc *, const double2 *, const double2 *, long, const double *,
@@ -161,7 +165,7 @@ struct lsmsx_ctx {
   int nstore = 0, group = 0;
   std::map<int, AtomDev> atoms;
   // batch pool
-  DevBuf<double2> M, inv, tau, tmpM, t0buf, X;
+  DevBuf<double2> M, inv, tau, tmpM, t0buf, X, A64, R64;
   DevBuf<float2> W, R;
   DevBuf<int> perm, info, conv, iters, count;
   DevBuf<double> minpiv, dxn;
@@ -177,6 +181,7 @@ struct lsmsx_ctx {
   long ws_limit = 0;
   int ir_max_sweeps = 12;
   double ir_tol = 1e-12;
+  double pivot_guard = 1e-2;  // plain FP64 solves with a smaller tile pivot are refined
   // schedule (LSMSX_SCHED_* flags) and hipGraph replay of the batched path
   int sched = 0;
   bool use_graph = false;
@@ -313,22 +318,24 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
   return check_launch("k_lu_last");
 }
 
-// Refinement sweeps of the mixed path on a chunk whose FP32 factors are in
-// place.  Returns the number of unconverged solves left in *left.
+// Refinement sweeps on a chunk whose factors (T = float: mixed path;
+// T = double: FP64-refined path) are in place.  Returns the number of
+// unconverged solves left in *left.
+template <class T>
 int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int max_sweeps, double tol,
            int *left) {
-  hipLaunchKernelGGL(k_ir_init, dim3(nb), dim3(256), 0, c->stream, dd);
+  hipLaunchKernelGGL(k_ir_init<T>, dim3(nb), dim3(256), 0, c->stream, dd);
   if (int r = check_launch("k_ir_init")) return r;
   int done = 0;
   for (int it = 0; it < max_sweeps; it++) {
     if (it > 0) {  // k_ir_init set the first residual, B
       ProfScope p(c, 5);
-      hipLaunchKernelGGL(k_ir_residual, dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
+      hipLaunchKernelGGL(k_ir_residual<T>, dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
     }
     if (int r = check_launch("k_ir_residual")) return r;
     {
       ProfScope p(c, 6);
-      hipLaunchKernelGGL(k_ir_solve, dim3(nb), dim3(256), 0, c->stream, dd, tol);
+      hipLaunchKernelGGL(k_ir_solve<T>, dim3(nb), dim3(256), 0, c->stream, dd, tol);
     }
     if (int r = check_launch("k_ir_solve")) return r;
     if (it >= 1) {  // the first correction never meets the tolerance
@@ -358,15 +365,25 @@ int validate_common(lsmsx_ctx *c, int nsc, int nspola) {
 }
 
 // Batched solve of nsolve (atom, spin, slot) problems; results to host.
+//   FP64            tile-LU in place, tau00 from the Schur complement; solves
+//                   whose smallest tile pivot is below the pivot guard are
+//                   redone with FP64_REFINED;
+//   MIXED_FP32/F16  FP32 factors of an FP32 copy + refinement against M;
+//                   unconverged solves are redone with FP64_REFINED;
+//   FP64_REFINED    FP64 factors of an FP64 copy + refinement against M;
+//                   unconverged solves fall back to the plain FP64 result.
 int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins,
                 const int *iies, const double *prel, int nsc, int nspola, int precision,
                 double *tau_out, int *info_out) {
   if (int r = validate_common(c, nsc, nspola)) return r;
   if (precision != LSMSX_PREC_FP64 && precision != LSMSX_PREC_MIXED_FP32 &&
-      precision != LSMSX_PREC_MIXED_F16)
+      precision != LSMSX_PREC_MIXED_F16 && precision != LSMSX_PREC_FP64_REFINED)
     return fail(-8, "unsupported precision mode");
-  const bool mixed = precision != LSMSX_PREC_FP64;
+  const bool plain = precision == LSMSX_PREC_FP64;
+  const bool refined64 = precision == LSMSX_PREC_FP64_REFINED;
   const bool split = precision == LSMSX_PREC_MIXED_F16;
+  const bool mixed = precision == LSMSX_PREC_MIXED_FP32 || split;
+  const bool refining = mixed || refined64;
   if (nsolve <= 0) return 0;
   HIPCHK(hipSetDevice(c->device));
   c->last_iters.assign(nsolve, 0);
@@ -382,7 +399,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     if (kc < 0) kc = kci;
     if (kci != kc) return fail(-6, "all atoms of a batch must share kkrsz");
     if (kc > kTile) return fail(-6, "kkrsz_ns > 64 not supported");
-    if (mixed && kc != 16 && kc != 32) return fail(-6, "mixed precision needs kkrsz_ns = 16 or 32");
+    if (refining && kc != 16 && kc != 32)
+      return fail(-6, "refined precision modes need kkrsz_ns = 16 or 32");
     if (a.kkrsz > (c->lmax_tab + 1) * (c->lmax_tab + 1))
       return fail(-6, "atom lmax exceeds the lmax of the uploaded tables");
     if (iies[i] < 0 || iies[i] >= c->group) return fail(-7, "energy slot out of range");
@@ -391,15 +409,16 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (a.store[s] < 0 || a.store[s] >= c->nstore) return fail(-7, "LIZStoreIdx out of range");
     long Ns = roundup(nsc * a.nrmat, kTile), nt = Ns / kTile;
     bytes[i] = Ns * Ns * 16;
+    if (plain) bytes[i] += kTile * kTile * 16;
     if (mixed) bytes[i] += Ns * Ns * 8 + nt * kTile * kTile * 8 + Ns * kc * 24;
+    if (refined64) bytes[i] += Ns * Ns * 16 + nt * kTile * kTile * 16 + Ns * kc * 32;
     if (split) bytes[i] += 4 * nt * (long)kPlaneHalves * 2 + 16 * nt;
-    else bytes[i] += kTile * kTile * 16;
   }
   long cap = c->ws_limit;
   if (cap <= 0) {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
-    long held = (long)(c->M.n * sizeof(double2) + c->W.n * sizeof(float2) + c->inv.n * sizeof(double2));
+    long held = (long)((c->M.n + c->inv.n + c->A64.n) * sizeof(double2) + c->W.n * sizeof(float2));
     cap = (long)((double)(fr + held) * 0.7);
   }
   int done = 0;
@@ -416,12 +435,13 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       long Ns = roundup(nsc * a.nrmat, kTile);
       mtot += Ns * Ns;
       xtot += Ns * kc;
-      ivtot += mixed ? (Ns / kTile) * kTile * kTile : kTile * kTile;
+      ivtot += refining ? (Ns / kTile) * kTile * kTile : kTile * kTile;
       htot += 4 * (Ns / kTile);
       maxliz = std::max(maxliz, a.numLIZ);
       nt_max = std::max(nt_max, (int)(Ns / kTile));
     }
     if (int r = c->M.ensure(mtot)) return r;
+    // inv holds double2 elements; FP32 inverses take half as many
     if (int r = c->inv.ensure(mixed ? (ivtot + 1) / 2 : ivtot)) return r;
     if (int r = c->perm.ensure((size_t)nb * nt_max * kTile)) return r;
     if (int r = c->info.ensure((size_t)nb)) return r;
@@ -430,8 +450,14 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     if (int r = c->desc.ensure((size_t)nb)) return r;
     if (mixed) {
       if (int r = c->W.ensure(mtot)) return r;
-      if (int r = c->X.ensure(xtot)) return r;
       if (int r = c->R.ensure(xtot)) return r;
+    }
+    if (refined64) {
+      if (int r = c->A64.ensure(mtot)) return r;
+      if (int r = c->R64.ensure(xtot)) return r;
+    }
+    if (refining) {
+      if (int r = c->X.ensure(xtot)) return r;
       if (int r = c->conv.ensure((size_t)nb)) return r;
       if (int r = c->iters.ensure((size_t)nb)) return r;
       if (int r = c->dxn.ensure((size_t)2 * nb)) return r;
@@ -482,23 +508,30 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       d.minpiv = c->minpiv.p + b;
       d.tau00 = c->tau.p + (long)b * kc * kc;
       d.t0 = c->tmat.p + c->ldT * a.store[0] + d.t_off;
-      if (mixed) {
-        d.A = c->W.p + moff;
-        d.inv = (float2 *)c->inv.p + ioff;
+      if (refining) {
         d.inv_stride = kTile * kTile;
-        ioff += (long)d.nt * kTile * kTile;
         d.X = c->X.p + xoff;
-        d.R = c->R.p + xoff;
         d.conv = c->conv.p + b;
         d.iters = c->iters.p + b;
         d.dxn = c->dxn.p + b;
         d.dxn_rho2 = c->dxn.p + nb + b;
-        xoff += (long)Ns * kc;
+      }
+      if (mixed) {
+        d.A = c->W.p + moff;
+        d.w32 = c->W.p + moff;
+        d.R = c->R.p + xoff;
+        d.inv = (float2 *)c->inv.p + ioff;
+      } else if (refined64) {
+        d.A = c->A64.p + moff;
+        d.R = c->R64.p + xoff;
+        d.inv = c->inv.p + ioff;
       } else {
         d.A = d.M;
         d.inv = c->inv.p + (long)b * kTile * kTile;
         d.inv_stride = 0;
       }
+      if (refining) ioff += (long)d.nt * kTile * kTile;
+      xoff += (long)Ns * kc;
       moff += (long)Ns * Ns;
     }
     HIPCHK(hipMemcpyAsync(c->desc.p, hd.data(), nb * sizeof(SolveDesc), hipMemcpyHostToDevice,
@@ -523,8 +556,12 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
                            c->cgnt.p, c->illp.p, c->lmax_tab);
       }
       if (int r = check_launch("k_assemble")) return r;
-      if (!mixed) return launch_lu<double>(c, dd, n, ntm, false, st);
-      return launch_lu<float>(c, dd, n, ntm, split, st);
+      if (refined64) {
+        hipLaunchKernelGGL(k_copy_m, dim3(ntm, n), dim3(256), 0, st, dd);
+        if (int r = check_launch("k_copy_m")) return r;
+      }
+      if (mixed) return launch_lu<float>(c, dd, n, ntm, split, st);
+      return launch_lu<double>(c, dd, n, ntm, false, st);
     };
     // two halves on two streams overlap one half's latency-bound steps
     // (diagonal tiles, panels) with the other half's trailing updates
@@ -569,41 +606,41 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (int r = enqueue()) return r;
     }
     std::vector<int> hiters(nb, 0);
-    if (mixed) {
+    if (refining) {
       int left = 0;
-      if (int r = refine(c, c->desc.p, nb, nt_max, c->ir_max_sweeps, c->ir_tol, &left)) return r;
+      int r = mixed ? refine<float>(c, c->desc.p, nb, nt_max, c->ir_max_sweeps, c->ir_tol, &left)
+                    : refine<double>(c, c->desc.p, nb, nt_max, c->ir_max_sweeps, c->ir_tol, &left);
+      if (r) return r;
       std::vector<int> hconv(nb);
       HIPCHK(hipMemcpyAsync(hiters.data(), c->iters.p, nb * sizeof(int), hipMemcpyDeviceToHost,
                             c->stream));
       HIPCHK(hipMemcpyAsync(hconv.data(), c->conv.p, nb * sizeof(int), hipMemcpyDeviceToHost,
                             c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
-      if (left > 0) {
-        // FP64 fallback for the solves whose refinement stalled: factor the
-        // untouched FP64 matrix with the plain path.
+      for (int b = 0; b < nb; b++)
+        if (!hconv[b]) hiters[b] = -1;
+      if (refined64 && left > 0) {
+        // last resort for a refinement that did not converge: the plain
+        // FP64 result from the untouched M
         std::vector<SolveDesc> fb;
-        std::vector<int> fidx;
         for (int b = 0; b < nb; b++)
           if (!hconv[b]) {
             SolveDesc d = hd[b];
             d.step0 = 0;  // the fallback chunk has its own step count
-            d.hp = nullptr;
-            d.he = nullptr;
             d.A = d.M;
+            d.X = nullptr;
+            d.conv = nullptr;
             d.inv = c->inv.p + (long)fb.size() * kTile * kTile;
             d.inv_stride = 0;
-            d.conv = nullptr;
             fb.push_back(d);
-            fidx.push_back(b);
-            hiters[b] = -1;
           }
         int nf = (int)fb.size();
         int fnt = 1;
         for (auto &d : fb) fnt = std::max(fnt, d.nt);
-        if (int r = c->desc2.ensure((size_t)nf)) return r;
+        if (int r2 = c->desc2.ensure((size_t)nf)) return r2;
         HIPCHK(hipMemcpyAsync(c->desc2.p, fb.data(), nf * sizeof(SolveDesc),
                               hipMemcpyHostToDevice, c->stream));
-        if (int r = launch_lu<double>(c, c->desc2.p, nf, fnt)) return r;
+        if (int r2 = launch_lu<double>(c, c->desc2.p, nf, fnt)) return r2;
       }
     }
     HIPCHK(hipMemcpyAsync(tau_out + (long)done * kc * kc * 2, c->tau.p,
@@ -622,6 +659,45 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       c->last_minpiv[done + b] = hmin[b];
     }
     done = end;
+  }
+  // Redo with FP64 factors + FP64 refinement: plain solves whose smallest
+  // tile-local pivot fell under the guard (element growth can cost digits),
+  // and mixed solves whose refinement did not converge.
+  std::vector<int> redo;
+  for (int i = 0; i < nsolve; i++) {
+    if (plain && c->pivot_guard > 0.0 && info_out[i] == 0 && c->last_minpiv[i] < c->pivot_guard)
+      redo.push_back(i);
+    if (mixed && c->last_iters[i] < 0) redo.push_back(i);
+  }
+  if (!redo.empty()) {
+    const int nr = (int)redo.size();
+    std::vector<int> ra(nr), rs(nr), re(nr), rinfo(nr);
+    std::vector<double> rp(2 * nr), rtau((size_t)nr * kc * kc * 2);
+    for (int j = 0; j < nr; j++) {
+      const int i = redo[j];
+      ra[j] = atom_ids[i];
+      rs[j] = ispins[i];
+      re[j] = iies[i];
+      rp[2 * j] = prel[2 * i];
+      rp[2 * j + 1] = prel[2 * i + 1];
+    }
+    std::vector<int> outer_iters = c->last_iters;
+    std::vector<double> outer_minpiv = c->last_minpiv;
+    if (int r = run_batched(c, nr, ra.data(), rs.data(), re.data(), rp.data(), nsc, nspola,
+                            LSMSX_PREC_FP64_REFINED, rtau.data(), rinfo.data()))
+      return r;
+    for (int j = 0; j < nr; j++) {
+      const int i = redo[j];
+      std::memcpy(tau_out + (long)i * kc * kc * 2, rtau.data() + (long)j * kc * kc * 2,
+                  sizeof(double) * 2 * kc * kc);
+      info_out[i] = rinfo[j];
+      // sweeps of the FP64 refinement (plain solves); -1 marks a mixed
+      // solve redone in FP64
+      outer_iters[i] = plain ? c->last_iters[j] : -1;
+      outer_minpiv[i] = c->last_minpiv[j];
+    }
+    c->last_iters = outer_iters;
+    c->last_minpiv = outer_minpiv;
   }
   return 0;
 }
@@ -945,6 +1021,8 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->desc.release();
   c->desc2.release();
   c->X.release();
+  c->A64.release();
+  c->R64.release();
   c->W.release();
   c->R.release();
   c->conv.release();
@@ -1369,6 +1447,13 @@ int lsmsx_autotune(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *isp
   c->sched = sbest;
   if (best) *best = sbest;
   return rc;
+}
+
+int lsmsx_set_pivot_guard(lsmsx_ctx *c, double threshold) {
+  if (!c) return fail(-1, "null context");
+  if (!(threshold >= 0.0)) return fail(-2, "pivot guard must be >= 0");
+  c->pivot_guard = threshold;
+  return 0;
 }
 
 }  // extern "C"

@@ -47,8 +47,9 @@ struct SolveDesc {
   double2 *tau00;         // kc x kc output
   const double2 *t0;      // central t-matrix (kc x kc, leading dim ldt)
   // iterative refinement (mixed precision)
-  double2 *X;             // Ns x kc solution, FP64
-  float2 *R;              // Ns x kc residual / correction, FP32
+  double2 *X;             // Ns x kc solution, FP64 (refined paths only)
+  void *R;                // Ns x kc residual / correction, in the factor precision
+  float2 *w32;            // FP32 copy of M written by the assembly (mixed path) or null
   int *conv;              // 1 once converged
   int *iters;             // refinement sweeps used
   double *dxn;            // ||dX_c||^2 of the previous sweep

@@ -414,9 +414,9 @@ __global__ __launch_bounds__(256) void k_lu_update2(const SolveDesc *__restrict_
 // Last tile (step nt-1): factor it with the non-central rows (first nd =
 // 64 - kc) pivoting among themselves and the central rows among themselves.
 // The trailing kc x kc block becomes LU(S), S the Schur complement.
-//   FP64 (plain path): solve S tau00 = t0 from the S factors.
-//   FP32 (mixed path): store the tile's factors, inverses and permutation
-//   like k_lu_diag, for the refinement solves.
+//   plain FP64 path: solve S tau00 = t0 from the S factors;
+//   refined paths (d.X set; FP32 mixed or FP64 refined): store the tile's
+//   factors, inverses and permutation like k_lu_diag, for the refinement.
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_last(const SolveDesc *__restrict__ descs) {
   typedef CxT<T> V;
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(256) void k_lu_last(const SolveDesc *__restrict__ d
     if (sinfo != 0 && *d.info == 0) *d.info = sinfo;
     if (sminpiv < *d.minpiv) *d.minpiv = sminpiv;
   }
-  if (sizeof(T) == sizeof(float)) {
+  if (d.X != nullptr) {  // refinement follows: keep the tile's factors
     for (int e = tid; e < kTile * kTile; e += 256) {
       int r = e & 63, c = e >> 6;
       M[(c0 + r) + ld * (c0 + c)] = A[c][r];
