@@ -28,12 +28,27 @@ METRIC = "tau00 solves/sec at lmax=3 LIZ=135: mixed+batched vs FP64, <=1e-10 rel
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix) peak
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (matrix) peak
 FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 (matrix) peak
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 
 
 def solve_flops(n, k):
     """Algorithmic work of one tau00 solve: complex LU + two triangular solves
     with k right-hand sides, F = 8/3 N^3 + 8 N^2 k real flops."""
     return (8.0 / 3.0) * n ** 3 + 8.0 * n * n * k
+
+
+def update_kernel_bytes(nt_list, c_bytes, panel_bytes):
+    """Algorithmic HBM bytes of all trailing-update launches of one batch: each
+    step pair reads and writes its (nt-2-a)^2 C tiles once and reads the two
+    L column and two U row panels of the pair once (bytes per complex element:
+    c_bytes for C, panel_bytes for the panel operands)."""
+    total = 0.0
+    for nt in nt_list:
+        for a in range(0, nt - 1, 2):
+            r = nt - 2 - a
+            if r > 0:
+                total += r * r * 2 * 64 * 64 * c_bytes + 2 * 2 * r * 64 * 64 * panel_bytes
+    return total
 
 
 def update_kernel_flops(nt_list):
@@ -50,20 +65,22 @@ def update_kernel_flops(nt_list):
     return total
 
 
-PROFILE = os.path.join(ROOT, "profiles", "k_lu_update2_fcc135_N2160.md")
+PROFILES = [os.path.join(ROOT, "profiles", f) for f in
+            ("k_lu_update2_fcc135_N2160.md", "k_lu_update2_h_fcc135_N2160.md")]
 
 
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
     WRITE_SIZE (copy-calibrated) on this workload; the commands and the
-    calibration are recorded in the profile summary."""
+    calibration are recorded in the profile summaries."""
     key = "traffic_bytes_per_launch[%s]:" % kernel
-    try:
-        for line in open(PROFILE):
-            if line.startswith(key):
-                return float(line.split(":")[1])
-    except OSError:
-        pass
+    for path in PROFILES:
+        try:
+            for line in open(path):
+                if line.startswith(key):
+                    return float(line.split(":")[1])
+        except OSError:
+            pass
     return None
 
 
@@ -114,6 +131,9 @@ def main():
                          "updates; f16: also the mixed path with split-FP16 trailing updates; "
                          "all: both mixed paths.  The fastest mixed path that meets the 1e-10 "
                          "error bar against FP64 is the headline")
+    ap.add_argument("--schedule", type=int, default=-1,
+                    help="schedule flags (LSMSX_SCHED_*) instead of the autotuner's choice; "
+                         "0 = one stream, as used for the rocprofv3 summaries")
     ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
                     help="c2: the headline (fcc 135 sites, mixed vs FP64); c4: uneven 4096-site "
                          "bcc alloy batch (graph + autotuned schedule vs eager); c5: spin-canted "
@@ -171,8 +191,12 @@ def main():
         """The library's autotuner picks the schedule for this batch and
         precision (untimed); the timed steps then replay the captured graph."""
         ctx.set_graph(False)
-        best, ms = ctx.autotune(atoms, iies, prels, precision=precision)
-        schedules[precision] = {"flags": best, "ms_by_flags": [round(x, 1) for x in ms]}
+        if args.schedule >= 0:
+            ctx.set_schedule(args.schedule)
+            schedules[precision] = {"flags": args.schedule, "ms_by_flags": None}
+        else:
+            best, ms = ctx.autotune(atoms, iies, prels, precision=precision)
+            schedules[precision] = {"flags": best, "ms_by_flags": [round(x, 1) for x in ms]}
         ctx.set_graph(True)
         for _ in range(args.warmup):  # the first call captures the graph
             ctx.tau00_batched(atoms, iies, prels, k, precision=precision)
@@ -229,16 +253,29 @@ def main():
     solves_per_s = nsolve * args.steps / el
     upd_ms, upd_n = prof["update"]
     nt = (N + 63) // 64
-    upd_flops = update_kernel_flops([nt] * nsolve)
-    achieved_tf = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
-    kernel, peak, dtype, prec = {
-        "fp64": ("k_lu_update2<double>", FP64_PEAK_TFLOPS, "f64", "FP64"),
-        "fp32_update": ("k_lu_update2<float>", FP32_PEAK_TFLOPS, "c32-factor+f64-refine",
+    # (kernel, MFMA peak, executed real flops per algorithmic flop, C and panel
+    # bytes per complex element, dtype, precision)
+    kernel, peak, flop_mult, cb, pb, dtype, prec = {
+        # 3M: 3 real products (6 flops) per complex multiply-add (8 flops)
+        "fp64": ("k_lu_update2<double>", FP64_PEAK_TFLOPS, 0.75, 16, 16, "f64", "FP64"),
+        "fp32_update": ("k_lu_update2<float>", FP32_PEAK_TFLOPS, 0.75, 8, 8,
+                        "c32-factor+f64-refine",
                         "mixed: complex FP32 LU on MFMA + FP64 iterative refinement"),
-        "split_f16_update": ("k_lu_update2_h", FP16_PEAK_TFLOPS, "f16x2-split-update+f32-factor+f64-refine",
+        # 4M x 3 split products: 12 real products (24 flops) per complex multiply-add
+        "split_f16_update": ("k_lu_update2_h", FP16_PEAK_TFLOPS, 3.0, 8, 8,
+                             "f16x2-split-update+f32-factor+f64-refine",
                              "mixed: complex FP32 LU with split-FP16 trailing updates on the FP16 "
                              "matrix cores + FP64 iterative refinement"),
     }[headline]
+    upd_flops = update_kernel_flops([nt] * nsolve)
+    upd_bytes = update_kernel_bytes([nt] * nsolve, cb, pb)
+    t_s = upd_ms * 1e-3 if upd_ms > 0 else float("nan")
+    achieved_tf = upd_flops / t_s / 1e12
+    achieved_gbs = upd_bytes / t_s / 1e9
+    # the bound is the roofline the kernel would hit first at peak rates
+    t_mfma = upd_flops * flop_mult / (peak * 1e12)
+    t_hbm = upd_bytes / (HBM_PEAK_GBS * 1e9)
+    bound = "mfma" if t_mfma >= t_hbm else "hbm"
     path_tf = solve_flops(N, k) * solves_per_s / 1e12
 
     cpu = None
@@ -270,9 +307,15 @@ def main():
             "precision": prec,
         },
         "roofline": {
-            "bound": "mfma", "kernel": kernel,
-            "achieved": round(achieved_tf, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved_tf / peak, 4),
+            "bound": bound, "kernel": kernel,
+            "achieved": round(achieved_tf, 3) if bound == "mfma" else round(achieved_gbs, 1),
+            "peak": peak if bound == "mfma" else HBM_PEAK_GBS,
+            "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
+            "frac": round(achieved_tf / peak, 4) if bound == "mfma" else round(achieved_gbs / HBM_PEAK_GBS, 4),
+            "algorithmic_flops_per_launch": upd_flops / max(upd_n, 1),
+            "algorithmic_bytes_per_launch": upd_bytes / max(upd_n, 1),
+            "mfma_frac_executed": round(achieved_tf * flop_mult / peak, 4),
+            "hbm_frac_algorithmic": round(achieved_gbs / HBM_PEAK_GBS, 4),
             "traffic": profiled_traffic(kernel) if (args.natoms, args.nenergies) == (32, 31) else None,
             "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4), "launches": upd_n,
         },

@@ -173,8 +173,9 @@ int lsmsx_tau_full(lsmsx_ctx *ctx, int atom_id, int ispin, int iie,
  *             reverse-mode differentiation: for tau00 = R t C, grad_t =
  *             R^H bar C^H).
  * kkrsz_ns must be 16 or 32.  No counterpart in the reference (it has no
- * Lloyd term or derivative of tau); the tau00 it returns is that of
- * lsmsx_tau00_batched. */
+ * Lloyd term or derivative of tau); the tau00 it returns agrees with
+ * lsmsx_tau00_batched (solves under the pivot guard take their tau00 from
+ * LSMSX_PREC_FP64_REFINED). */
 int lsmsx_tau00_adjoint(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
                         const int *ispins, const int *iies, const double *energies,
                         const double *prel, int n_spin_cant, int n_spin_pola,

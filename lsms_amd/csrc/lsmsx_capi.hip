@@ -796,6 +796,7 @@ int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     hipLaunchKernelGGL(k_zero_c, dim3(1024), dim3(256), 0, c->stream, c->gstore.p,
                        (long)c->ldT * c->nstore);
   }
+  std::vector<double> minpiv_all(nsolve, 0.0);
   int done = 0;
   while (done < nsolve) {
     long acc = 0;
@@ -951,9 +952,38 @@ int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     drain_profile(c);
-    for (int b = 0; b < nb; b++)
+    for (int b = 0; b < nb; b++) {
       if (info_out) info_out[done + b] = hinfo[b];
+      minpiv_all[done + b] = hmin[b];
+    }
     done = end;
+  }
+  // tau00 of solves with a small tile-local pivot: from the FP64-refined
+  // route, as on the plain path (the Lloyd term and the gradient keep the
+  // explicit inverse, well inside their tolerances)
+  if (tau_out && c->pivot_guard > 0.0) {
+    std::vector<int> redo;
+    for (int i = 0; i < nsolve; i++)
+      if (minpiv_all[i] < c->pivot_guard) redo.push_back(i);
+    if (!redo.empty()) {
+      const int nr = (int)redo.size();
+      std::vector<int> ra(nr), rs(nr), re(nr), rinfo(nr);
+      std::vector<double> rp(2 * nr), rtau((size_t)nr * kc * kc * 2);
+      for (int j = 0; j < nr; j++) {
+        const int i = redo[j];
+        ra[j] = atom_ids[i];
+        rs[j] = ispins[i];
+        re[j] = iies[i];
+        rp[2 * j] = prel[2 * i];
+        rp[2 * j + 1] = prel[2 * i + 1];
+      }
+      if (int r = run_batched(c, nr, ra.data(), rs.data(), re.data(), rp.data(), nsc, nspola,
+                              LSMSX_PREC_FP64_REFINED, rtau.data(), rinfo.data()))
+        return r;
+      for (int j = 0; j < nr; j++)
+        std::memcpy(tau_out + (long)redo[j] * kc * kc * 2, rtau.data() + (long)j * kc * kc * 2,
+                    sizeof(double) * 2 * kc * kc);
+    }
   }
   if (tau_bar && tmat_grad) {
     HIPCHK(hipMemcpyAsync(tmat_grad, c->gstore.p, sizeof(double2) * c->ldT * c->nstore,

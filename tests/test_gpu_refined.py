@@ -77,7 +77,8 @@ def test_refined_mode_spin_variants(nsc, nsp, ispin):
 
 def test_pivot_guard_on_small_tile_pivot():
     """The C5 solve with a small tile pivot, next to an ordinary one: the plain
-    path meets 1e-10 with the guard (and reports refinement sweeps for it)."""
+    path and the adjoint route's tau00 meet 1e-10 with the guard (and the
+    plain path reports refinement sweeps for the guarded solve)."""
     s = CF.config_c5()
     ctx = capi.Context(0)
     ctx.load_system(s)
@@ -87,6 +88,7 @@ def test_pivot_guard_on_small_tile_pivot():
     pr = [s.prel(i) for i in iies]
     tau, info = ctx.tau00_batched(atoms, iies, pr, 32)
     iters, minpiv = ctx.last_stats(2)
+    tau_adj, _, _, info_adj = ctx.tau00_adjoint(atoms, iies, s.energies[iies], 32, lloyd=False)
     ctx.set_pivot_guard(0.0)
     tau_unguarded, _ = ctx.tau00_batched(atoms, iies, pr, 32)
     _, minpiv_tile = ctx.last_stats(2)
@@ -95,5 +97,8 @@ def test_pivot_guard_on_small_tile_pivot():
     assert minpiv_tile[0] < 1e-2 <= minpiv_tile[1]
     assert iters[0] > 0 and iters[1] == 0
     assert np.array_equal(tau[1], tau_unguarded[1])
+    assert np.all(info_adj == 0)
     for b, (a, ie) in enumerate(cases):
-        assert rel(tau[b], lapack_refined_tau00(s, a, ie)) < 1e-10, (a, ie)
+        ref = lapack_refined_tau00(s, a, ie)
+        assert rel(tau[b], ref) < 1e-10, (a, ie)
+        assert rel(tau_adj[b], ref) < 1e-10, (a, ie)
