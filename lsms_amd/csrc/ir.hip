@@ -19,7 +19,8 @@
 // factorization: for tile row s,
 //   y_s = L_ss^-1 P_s (r_s - sum_{t<s} L_st y_t)      (L_st stored left of the diagonal tile)
 //   x_s = U_ss^-1 (y_s - sum_{t>s} U_st x_t)
-// with the diagonal-tile inverses and permutations kept by the factorization.
+// with the diagonal-tile inverses and permutations kept by the factorization;
+// the sums are taken right-looking, one tile column per launch (k_ir_step).
 #include "lu_common.hpp"
 
 namespace lsmsx {
@@ -58,44 +59,63 @@ __global__ __launch_bounds__(256) void k_ir_init(const SolveDesc *__restrict__ d
 }
 
 // R = B - M X for one 64-row tile (blockIdx.x) of one solve (blockIdx.y);
-// wave w handles rows 16w.. of the tile, all kc right-hand sides (16 or 32).
+// wave w handles rows 16w.. of the tile, all kc = 16 NB right-hand sides.
 // FP64 MFMA in the 3M, transposed form of the LU update:
 //   a-operand X(k0+kq, j0+li), b-operand M(i0+li, k0+kq).
-// The residual is stored in the correction type T.
-template <class T>
+// M streams from HBM in 64-column chunks, all 16 k-steps of a chunk in
+// flight per wave; the chunk's 64 x kc block of X is staged in LDS once per
+// workgroup (the four waves share it).  The residual is stored in the
+// correction type T.
+template <class T, int NB>
 __global__ __launch_bounds__(256) void k_ir_residual(const SolveDesc *__restrict__ descs) {
   typedef CxT<T> V;
   const SolveDesc &d = descs[blockIdx.y];
   if (*d.conv) return;
   const int tile = blockIdx.x;
   if (tile >= d.nt) return;
-  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
-  const int w = threadIdx.x >> 6;
+  __shared__ double2 xs[kTile][16 * NB + 1];
+  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, kq = lane >> 4;
+  const int w = tid >> 6;
   const int i0 = tile * kTile + 16 * w;
   const long ld = d.ld;
-  const int Ns = d.Ns, kc = d.kc, nb = kc / 16;
+  const int Ns = d.Ns, kc = d.kc;
   const gd2 *__restrict__ M = (const gd2 *)d.M + i0;
   const gd2 *__restrict__ X = (const gd2 *)d.X;
-  d4 q[3][2];
+  d4 q[3][NB];
   // accumulators start from B: rows >= Ns - kc hold t0
 #pragma unroll
-  for (int jb = 0; jb < 2; jb++)
+  for (int jb = 0; jb < NB; jb++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       int row = i0 + li, col = 16 * jb + kq + 4 * r;
       double2 b = make_double2(0.0, 0.0);
-      if (jb < nb && row >= Ns - kc) b = d.t0[(row - (Ns - kc)) + (long)d.ldt * col];
+      if (row >= Ns - kc) b = d.t0[(row - (Ns - kc)) + (long)d.ldt * col];
       q[0][jb][r] = b.x;
       q[1][jb][r] = 0.0;
       q[2][jb][r] = b.x + b.y;
     }
-  for (int k0 = 0; k0 < Ns; k0 += 4) {
-    double2 m = M[li + ld * (k0 + kq)];
-    double mr = -m.x, mi = -m.y, ms = mr + mi;
+  for (int k0 = 0; k0 < Ns; k0 += kTile) {
+    double2 m[kTile / 4], xv[4 * NB];
 #pragma unroll
-    for (int jb = 0; jb < 2; jb++) {
-      if (jb < nb) {
-        double2 x = X[(k0 + kq) + (long)Ns * (16 * jb + li)];
+    for (int u = 0; u < kTile / 4; u++) m[u] = M[li + ld * (k0 + 4 * u + kq)];
+#pragma unroll
+    for (int qq = 0; qq < 4 * NB; qq++) {
+      const int e = tid + 256 * qq;
+      xv[qq] = X[(k0 + (e & 63)) + (long)Ns * (e >> 6)];
+    }
+    __syncthreads();  // the previous chunk's readers are done
+#pragma unroll
+    for (int qq = 0; qq < 4 * NB; qq++) {
+      const int e = tid + 256 * qq;
+      xs[e & 63][e >> 6] = xv[qq];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kTile / 4; u++) {
+      const double mr = -m[u].x, mi = -m[u].y, ms = mr + mi;
+#pragma unroll
+      for (int jb = 0; jb < NB; jb++) {
+        const double2 x = xs[4 * u + kq][16 * jb + li];
         q[0][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, mr, q[0][jb], 0, 0, 0);
         q[1][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, mi, q[1][jb], 0, 0, 0);
         q[2][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x + x.y, ms, q[2][jb], 0, 0, 0);
@@ -104,15 +124,13 @@ __global__ __launch_bounds__(256) void k_ir_residual(const SolveDesc *__restrict
   }
   GCx<T> *R = (GCx<T> *)d.R;
 #pragma unroll
-  for (int jb = 0; jb < 2; jb++) {
-    if (jb < nb) {
+  for (int jb = 0; jb < NB; jb++) {
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        V o;
-        o.x = (T)(q[0][jb][r] - q[1][jb][r]);
-        o.y = (T)(q[2][jb][r] - q[0][jb][r] - q[1][jb][r]);
-        R[(i0 + li) + (long)Ns * (16 * jb + kq + 4 * r)] = o;
-      }
+    for (int r = 0; r < 4; r++) {
+      V o;
+      o.x = (T)(q[0][jb][r] - q[1][jb][r]);
+      o.y = (T)(q[2][jb][r] - q[0][jb][r] - q[1][jb][r]);
+      R[(i0 + li) + (long)Ns * (16 * jb + kq + 4 * r)] = o;
     }
   }
 }
@@ -130,12 +148,11 @@ struct AccIR {
 // The k range is consumed in batches of 8 k-steps whose operand loads are
 // all issued before their MFMAs: one workgroup per solve leaves few waves per
 // SIMD, so latency is hidden by loads in flight, not by occupancy.
-template <class T, class FA, class FB>
+template <class T, int U = 8, class FA, class FB>
 __device__ __forceinline__ void sweep_gemm(FA fa, FB fb, int k_begin, int k_end, int nb,
                                            AccIR<T> &q) {
   typedef CxT<T> V;
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
-  constexpr int U = 8;
   int kb = k_begin;
   for (; kb + 4 * U <= k_end; kb += 4 * U) {
     V a[U], b[U][2];
@@ -201,126 +218,159 @@ __device__ __forceinline__ void acc_zero(AccIR<T> &q) {
     }
 }
 
-// dX = (P^T L U)^-1 R in the correction type (in place in R), then X += dX
-// and the convergence test on the central rows.  One workgroup per solve, the
-// block rows of each step split over the 4 waves (16 rows each); the long
-// sums sum_t L_st y_t and sum_t U_st x_t run on MFMA streaming the stored
-// factors, the diagonal-tile inverses are applied from LDS.
-template <class T>
-__global__ __launch_bounds__(256) void k_ir_solve(const SolveDesc *__restrict__ descs, double tol) {
+// dX = (P^T L U)^-1 R in the correction type, in place in R, by right-looking
+// tile steps: one launch per step, one workgroup per (tile, solve), so a
+// sweep streams the stored factors with the whole chip instead of one
+// workgroup per solve.
+//   forward, step s = -1 .. nt-2, workgroup tile t = s+1+bx < nt:
+//     r_t -= L_ts y_s                       (s >= 0; y_s is final in rows s)
+//     t == s+1:  y_t = L_tt^-1 P_t r_t       (and, t == nt-1, x_t = U_tt^-1 y_t)
+//   backward, launch j = 0 .. nt-2, s = nt-1-j (x_s final), tile t = bx < s:
+//     y_t -= U_ts x_s;   t == s-1:  x_t = U_tt^-1 y_t
+// Solves of different sizes run the forward steps from the first tile and
+// the backward steps from their own last tile.  Wave w handles rows 16w.. of
+// tile t; the 64 x kc block of tile s is staged in LDS by the workgroup.
+template <class T, bool FWD>
+__global__ __launch_bounds__(256) void k_ir_step(const SolveDesc *__restrict__ descs, int step) {
   typedef CxT<T> V;
-  const SolveDesc &d = descs[blockIdx.x];
+  const SolveDesc &d = descs[blockIdx.y];
   if (*d.conv) return;
-  __shared__ V v[kTile][32 + 1];
-  __shared__ double red[3][256];
+  const int nt = d.nt;
+  const int s = FWD ? step : nt - 1 - step;
+  const int t = FWD ? s + 1 + (int)blockIdx.x : (int)blockIdx.x;
+  if (FWD ? t >= nt : t >= s) return;
+  __shared__ V buf[kTile][32 + 1];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, kq = lane >> 4;
-  const int kc = d.kc, nt = d.nt, Ns = d.Ns, nb = kc / 16;
-  const long ld = d.ld;
-  const GCx<T> *__restrict__ W = (const GCx<T> *)d.A;
+  const int kc = d.kc, Ns = d.Ns, nb = kc / 16;
+  const long ld = tile_ld(d);
   GCx<T> *R = (GCx<T> *)d.R;
   auto col_of = [&](int jb, int r) { return 16 * jb + acc_row<T>(kq, r); };
-  // ---- forward: y_s = L_ss^-1 P_s (r_s - sum_{t<s} L_st y_t)
-  for (int s = 0; s < nt; s++) {
-    const int rs = s * kTile, i0 = rs + 16 * w;
-    AccIR<T> q;
+  const int i0 = t * kTile + 16 * w, rs = s * kTile;
+  // factor tile (t, s), rows of this wave
+  const GCx<T> *__restrict__ Wt = tile_ptr<T>(d, t, s < 0 ? 0 : s) + 16 * w;
+  // every global load of the step is issued before the first wait: the
+  // factor tile (t, s) in MFMA layout, r_t, and the 64 x kc block of tile s
+  V a[kTile / 4], st[8];
+  if (s >= 0) {
 #pragma unroll
-    for (int jb = 0; jb < 2; jb++)
+    for (int u = 0; u < kTile / 4; u++) a[u] = Wt[li + ld * (4 * u + kq)];
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (jb < nb) acc_from<T>(q, jb, r, R[(i0 + li) + (long)Ns * col_of(jb, r)]);
-    sweep_gemm<T>([&](int i, int k) {
-                    V l = W[(i0 + i) + ld * k];
-                    l.x = -l.x;
-                    l.y = -l.y;
-                    return l;
-                  },
-                  [&](int k, int j) { return (V)R[k + (long)Ns * j]; }, 0, rs, nb, q);
-#pragma unroll
-    for (int jb = 0; jb < 2; jb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (jb < nb) v[16 * w + li][col_of(jb, r)] = acc_value<T>(q, jb, r);
-    __syncthreads();
-    const int *perm = d.perm + s * kTile;
-    const GCx<T> *inv = (const GCx<T> *)d.inv + s * d.inv_stride;
-    // y = L^-1 (P v): unit lower L^-1, rows 16w.. of this wave
-    acc_zero<T>(q);
-    sweep_gemm<T>([&](int i, int k) {
-                    const int ii = 16 * w + i;
-                    V l;
-                    l.x = ii == k ? T(1) : T(0);
-                    l.y = T(0);
-                    if (ii > k) l = inv[ii + kTile * k];
-                    return l;
-                  },
-                  [&](int k, int j) { return v[perm[k]][j]; }, 0, kTile, nb, q);
-#pragma unroll
-    for (int jb = 0; jb < 2; jb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (jb < nb) R[(i0 + li) + (long)Ns * col_of(jb, r)] = acc_value<T>(q, jb, r);
-    __threadfence_block();
-    __syncthreads();
+    for (int qq = 0; qq < 8; qq++) {
+      const int e = tid + 256 * qq;
+      if (qq < kc / 4) st[qq] = R[(rs + (e & 63)) + (long)Ns * (e >> 6)];
+    }
   }
-  // ---- backward: x_s = U_ss^-1 (y_s - sum_{t>s} U_st x_t)
-  for (int s = nt - 1; s >= 0; s--) {
-    const int rs = s * kTile, i0 = rs + 16 * w;
-    AccIR<T> q;
+  AccIR<T> q;
 #pragma unroll
-    for (int jb = 0; jb < 2; jb++)
+  for (int jb = 0; jb < 2; jb++)
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (jb < nb) acc_from<T>(q, jb, r, R[(i0 + li) + (long)Ns * col_of(jb, r)]);
-    sweep_gemm<T>([&](int i, int k) {
-                    V u = W[(i0 + i) + ld * k];
-                    u.x = -u.x;
-                    u.y = -u.y;
-                    return u;
-                  },
-                  [&](int k, int j) { return (V)R[k + (long)Ns * j]; }, rs + kTile, Ns, nb, q);
+    for (int r = 0; r < 4; r++)
+      if (jb < nb) acc_from<T>(q, jb, r, R[(i0 + li) + (long)Ns * col_of(jb, r)]);
+  if (s >= 0) {
 #pragma unroll
-    for (int jb = 0; jb < 2; jb++)
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (jb < nb) v[16 * w + li][col_of(jb, r)] = acc_value<T>(q, jb, r);
+    for (int qq = 0; qq < 8; qq++) {
+      const int e = tid + 256 * qq;
+      if (qq < kc / 4) buf[e & 63][e >> 6] = st[qq];
+    }
     __syncthreads();
-    const GCx<T> *inv = (const GCx<T> *)d.inv + s * d.inv_stride;
-    acc_zero<T>(q);
-    // x = U^-1 v: upper U^-1 incl. diagonal
-    sweep_gemm<T>([&](int i, int k) {
-                    const int ii = 16 * w + i;
-                    V u;
-                    u.x = T(0);
-                    u.y = T(0);
-                    if (k >= ii) u = inv[ii + kTile * k];
-                    return u;
-                  },
-                  [&](int k, int j) { return v[k][j]; }, 0, kTile, nb, q);
 #pragma unroll
-    for (int jb = 0; jb < 2; jb++)
+    for (int u = 0; u < kTile / 4; u++) {
+      V l = a[u];
+      l.x = -l.x;
+      l.y = -l.y;
+      const T ls = l.x + l.y;
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (jb < nb) R[(i0 + li) + (long)Ns * col_of(jb, r)] = acc_value<T>(q, jb, r);
-    __threadfence_block();
-    __syncthreads();
+      for (int jb = 0; jb < 2; jb++) {
+        if (jb < nb) {
+          const V bb = buf[4 * u + kq][16 * jb + li];
+          q.p[0][jb] = mfma4(bb.x, l.x, q.p[0][jb]);
+          q.p[1][jb] = mfma4(bb.y, l.y, q.p[1][jb]);
+          q.p[2][jb] = mfma4(bb.x + bb.y, ls, q.p[2][jb]);
+        }
+      }
+    }
   }
-  // ---- X += dX; convergence on the central rows.  The next error is about
-  // rho ||dX|| with rho = ||dX_k|| / ||dX_{k-1}|| the observed contraction;
-  // converge when that estimate (and ||dX|| itself, to 1e-8) is below tol.
+  const bool diag = FWD ? t == s + 1 : t == s - 1;
+  if (diag) {
+    const GCx<T> *inv = (const GCx<T> *)d.inv + t * d.inv_stride;
+    auto to_buf = [&]() {
+      __syncthreads();
+#pragma unroll
+      for (int jb = 0; jb < 2; jb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (jb < nb) buf[16 * w + li][col_of(jb, r)] = acc_value<T>(q, jb, r);
+      __syncthreads();
+      acc_zero<T>(q);
+    };
+    // upper U^-1 of the diagonal tile, diagonal included
+    auto apply_uinv = [&]() {
+      sweep_gemm<T, 4>([&](int i, int k) {
+                      const int ii = 16 * w + i;
+                      V u;
+                      u.x = T(0);
+                      u.y = T(0);
+                      if (k >= ii) u = inv[ii + kTile * k];
+                      return u;
+                    },
+                    [&](int k, int j) { return buf[k][j]; }, 0, kTile, nb, q);
+    };
+    to_buf();
+    if (FWD) {
+      // unit lower L^-1 applied to the permuted rows
+      const int *perm = d.perm + t * kTile;
+      sweep_gemm<T, 4>([&](int i, int k) {
+                      const int ii = 16 * w + i;
+                      V l;
+                      l.x = ii == k ? T(1) : T(0);
+                      l.y = T(0);
+                      if (ii > k) l = inv[ii + kTile * k];
+                      return l;
+                    },
+                    [&](int k, int j) { return buf[perm[k]][j]; }, 0, kTile, nb, q);
+      if (t == nt - 1) {
+        to_buf();
+        apply_uinv();
+      }
+    } else {
+      apply_uinv();
+    }
+  }
+#pragma unroll
+  for (int jb = 0; jb < 2; jb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (jb < nb) R[(i0 + li) + (long)Ns * col_of(jb, r)] = acc_value<T>(q, jb, r);
+}
+
+// X += dX for the rows of one tile (blockIdx.x) of one solve; the workgroup
+// of the last tile also sums ||dX_c||^2 and ||X_c||^2 over the central rows.
+template <class T>
+__global__ __launch_bounds__(256) void k_ir_accum(const SolveDesc *__restrict__ descs) {
+  typedef CxT<T> V;
+  const SolveDesc &d = descs[blockIdx.y];
+  if (*d.conv) return;
+  const int t = blockIdx.x;
+  if (t >= d.nt) return;
+  __shared__ double red[2][256];
+  const int tid = threadIdx.x, Ns = d.Ns, kc = d.kc;
   gd2 *X = (gd2 *)d.X;
+  const GCx<T> *R = (const GCx<T> *)d.R;
   double dn = 0.0, xn = 0.0;
-  for (long e = tid; e < (long)Ns * kc; e += 256) {
-    const int row = (int)(e % Ns);
-    const V dx = R[e];
-    double2 x = X[e];
+  for (int e = tid; e < kTile * kc; e += 256) {
+    const int row = t * kTile + (e & 63);
+    const long idx = row + (long)Ns * (e >> 6);
+    const V dx = R[idx];
+    double2 x = X[idx];
     x.x += dx.x;
     x.y += dx.y;
-    X[e] = x;
+    X[idx] = x;
     if (row >= Ns - kc) {
       dn += (double)dx.x * dx.x + (double)dx.y * dx.y;
       xn += x.x * x.x + x.y * x.y;
     }
   }
+  if (t != d.nt - 1) return;
   red[0][tid] = dn;
   red[1][tid] = xn;
   __syncthreads();
@@ -332,22 +382,35 @@ __global__ __launch_bounds__(256) void k_ir_solve(const SolveDesc *__restrict__ 
     __syncthreads();
   }
   if (tid == 0) {
-    const double dn2 = red[0][0], xn2 = red[1][0];
-    const int k = *d.iters;
-    const double prev = *d.dxn;  // previous ||dX_c||^2
-    bool ok = false;
-    if (k >= 1 && prev > 0.0) {
-      // the larger of the last two observed contractions, so that one
-      // unusually small ratio does not end the refinement early
-      const double rho2 = fmax(dn2 / prev, *d.dxn_rho2);
-      ok = (rho2 < 1.0) && (rho2 * dn2 <= tol * tol * xn2) && (dn2 <= 1e-16 * xn2);
-      *d.dxn_rho2 = dn2 / prev;
-    }
-    if (dn2 <= tol * tol * xn2) ok = true;
-    *d.iters = k + 1;
-    *d.dxn = dn2;
-    if (ok) *d.conv = 1;
+    d.nrm[0] = red[0][0];
+    d.nrm[1] = red[1][0];
   }
+}
+
+// Convergence of each unconverged solve after a sweep (one thread per solve).
+// The next error is about rho ||dX|| with rho = ||dX_k|| / ||dX_{k-1}|| the
+// observed contraction; converge when that estimate (and ||dX|| itself, to
+// 1e-8) is below tol, or when ||dX_c|| <= tol ||X_c|| outright.
+__global__ void k_ir_decide(const SolveDesc *__restrict__ descs, int n, double tol) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const SolveDesc &d = descs[b];
+  if (*d.conv) return;
+  const double dn2 = d.nrm[0], xn2 = d.nrm[1];
+  const int k = *d.iters;
+  const double prev = *d.dxn;  // previous ||dX_c||^2
+  bool ok = false;
+  if (k >= 1 && prev > 0.0) {
+    // the larger of the last two observed contractions, so that one
+    // unusually small ratio does not end the refinement early
+    const double rho2 = fmax(dn2 / prev, *d.dxn_rho2);
+    ok = (rho2 < 1.0) && (rho2 * dn2 <= tol * tol * xn2) && (dn2 <= 1e-16 * xn2);
+    *d.dxn_rho2 = dn2 / prev;
+  }
+  if (dn2 <= tol * tol * xn2) ok = true;
+  *d.iters = k + 1;
+  *d.dxn = dn2;
+  if (ok) *d.conv = 1;
 }
 
 // tau00 = central rows of X.
@@ -388,9 +451,15 @@ __global__ __launch_bounds__(256) void k_copy_m(const SolveDesc *__restrict__ de
 
 template __global__ void k_ir_init<float>(const SolveDesc *);
 template __global__ void k_ir_init<double>(const SolveDesc *);
-template __global__ void k_ir_residual<float>(const SolveDesc *);
-template __global__ void k_ir_residual<double>(const SolveDesc *);
-template __global__ void k_ir_solve<float>(const SolveDesc *, double);
-template __global__ void k_ir_solve<double>(const SolveDesc *, double);
+template __global__ void k_ir_residual<float, 1>(const SolveDesc *);
+template __global__ void k_ir_residual<float, 2>(const SolveDesc *);
+template __global__ void k_ir_residual<double, 1>(const SolveDesc *);
+template __global__ void k_ir_residual<double, 2>(const SolveDesc *);
+template __global__ void k_ir_step<float, true>(const SolveDesc *, int);
+template __global__ void k_ir_step<float, false>(const SolveDesc *, int);
+template __global__ void k_ir_step<double, true>(const SolveDesc *, int);
+template __global__ void k_ir_step<double, false>(const SolveDesc *, int);
+template __global__ void k_ir_accum<float>(const SolveDesc *);
+template __global__ void k_ir_accum<double>(const SolveDesc *);
 
 }  // namespace lsmsx

@@ -245,7 +245,8 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
   const double2 prel = make_double2(d.prel_re, d.prel_im);
   const bool prel0 = (prel.x == 0.0 && prel.y == 0.0);
   double2 *__restrict__ M = d.M;
-  // mixed precision: the solver factors an FP32 copy written alongside M
+  // mixed precision: the solver factors an FP32 copy written alongside M,
+  // tile-major (tiled_index)
   float2 *__restrict__ W = d.w32;
   const long ld = d.ld;
   const int offi = d.site_off[i];
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
       for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
         int rr = e % kkri_ns, cc = e / kkri_ns;
         M[(offi + rr) + ld * (offj + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
-        if (W) W[(offi + rr) + ld * (offj + cc)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
+        if (W) W[tiled_index(offi + rr, offj + cc, d.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
       }
       continue;
     }
@@ -306,7 +307,9 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
                 const double re = q1[r] - q2[r], im = q3[r] - q1[r] - q2[r];
                 const long idx = (offi + bi * kkri + li) + ld * (offj + bj * kkrj + jc);
                 M[idx] = make_double2(-re, -im);
-                if (W) W[idx] = make_float2((float)-re, (float)-im);
+                if (W)
+                  W[tiled_index(offi + bi * kkri + li, offj + bj * kkrj + jc, d.nt)] =
+                      make_float2((float)-re, (float)-im);
               }
             }
           }
@@ -499,12 +502,12 @@ __global__ __launch_bounds__(256) void k_pad_identity(const SolveDesc *__restric
   for (long e = threadIdx.x; e < (long)pad * d.Ns; e += blockDim.x) {
     int rr = (int)(e % d.Ns), cc = (int)(e / d.Ns);
     M[rr + ld * cc] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
-    if (W) W[rr + ld * cc] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
+    if (W) W[tiled_index(rr, cc, d.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
   }
   for (long e = threadIdx.x; e < (long)pad * (d.Ns - pad); e += blockDim.x) {
     int rr = (int)(e % pad), cc = pad + (int)(e / pad);
     M[rr + ld * cc] = make_double2(0.0, 0.0);
-    if (W) W[rr + ld * cc] = make_float2(0.f, 0.f);
+    if (W) W[tiled_index(rr, cc, d.nt)] = make_float2(0.f, 0.f);
   }
 }
 

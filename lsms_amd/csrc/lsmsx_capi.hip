@@ -39,10 +39,13 @@ template <class T>
 __global__ void k_lu_last(const SolveDesc *);
 template <class T>
 __global__ void k_ir_init(const SolveDesc *);
-template <class T>
+template <class T, int NB>
 __global__ void k_ir_residual(const SolveDesc *);
+template <class T, bool FWD>
+__global__ void k_ir_step(const SolveDesc *, int);
 template <class T>
-__global__ void k_ir_solve(const SolveDesc *, double);
+__global__ void k_ir_accum(const SolveDesc *);
+__global__ void k_ir_decide(const SolveDesc *, int, double);
 __global__ void k_copy_m(const SolveDesc *);
 __global__ void k_ir_extract(const SolveDesc *);
 __global__ void k_count_conv(const SolveDesc *, int, int *);
@@ -322,22 +325,32 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
 // T = double: FP64-refined path) are in place.  Returns the number of
 // unconverged solves left in *left.
 template <class T>
-int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int max_sweeps, double tol,
-           int *left) {
+int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int kc, int max_sweeps,
+           double tol, int *left) {
   hipLaunchKernelGGL(k_ir_init<T>, dim3(nb), dim3(256), 0, c->stream, dd);
   if (int r = check_launch("k_ir_init")) return r;
   int done = 0;
   for (int it = 0; it < max_sweeps; it++) {
     if (it > 0) {  // k_ir_init set the first residual, B
       ProfScope p(c, 5);
-      hipLaunchKernelGGL(k_ir_residual<T>, dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
+      if (kc == 32)
+        hipLaunchKernelGGL((k_ir_residual<T, 2>), dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
+      else
+        hipLaunchKernelGGL((k_ir_residual<T, 1>), dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
     }
     if (int r = check_launch("k_ir_residual")) return r;
     {
       ProfScope p(c, 6);
-      hipLaunchKernelGGL(k_ir_solve<T>, dim3(nb), dim3(256), 0, c->stream, dd, tol);
+      for (int s = -1; s <= nt_max - 2; s++)
+        hipLaunchKernelGGL((k_ir_step<T, true>), dim3(s < 0 ? 1 : nt_max - 1 - s, nb), dim3(256), 0,
+                           c->stream, dd, s);
+      for (int j = 0; j <= nt_max - 2; j++)
+        hipLaunchKernelGGL((k_ir_step<T, false>), dim3(nt_max - 1 - j, nb), dim3(256), 0, c->stream,
+                           dd, j);
+      hipLaunchKernelGGL(k_ir_accum<T>, dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
+      hipLaunchKernelGGL(k_ir_decide, dim3((nb + 255) / 256), dim3(256), 0, c->stream, dd, nb, tol);
     }
-    if (int r = check_launch("k_ir_solve")) return r;
+    if (int r = check_launch("k_ir_step")) return r;
     if (it >= 1) {  // the first correction never meets the tolerance
       hipLaunchKernelGGL(k_count_conv, dim3(1), dim3(256), 0, c->stream, dd, nb, c->count.p);
       HIPCHK(hipMemcpyAsync(&done, c->count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -460,7 +473,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (int r = c->X.ensure(xtot)) return r;
       if (int r = c->conv.ensure((size_t)nb)) return r;
       if (int r = c->iters.ensure((size_t)nb)) return r;
-      if (int r = c->dxn.ensure((size_t)2 * nb)) return r;
+      if (int r = c->dxn.ensure((size_t)4 * nb)) return r;
       if (int r = c->count.ensure(1)) return r;
     }
     if (split) {
@@ -515,9 +528,11 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
         d.iters = c->iters.p + b;
         d.dxn = c->dxn.p + b;
         d.dxn_rho2 = c->dxn.p + nb + b;
+        d.nrm = c->dxn.p + 2 * nb + 2 * b;
       }
       if (mixed) {
         d.A = c->W.p + moff;
+        d.tiled = 1;  // the assembly writes the FP32 copy tile-major
         d.w32 = c->W.p + moff;
         d.R = c->R.p + xoff;
         d.inv = (float2 *)c->inv.p + ioff;
@@ -608,8 +623,9 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     std::vector<int> hiters(nb, 0);
     if (refining) {
       int left = 0;
-      int r = mixed ? refine<float>(c, c->desc.p, nb, nt_max, c->ir_max_sweeps, c->ir_tol, &left)
-                    : refine<double>(c, c->desc.p, nb, nt_max, c->ir_max_sweeps, c->ir_tol, &left);
+      int r = mixed
+                  ? refine<float>(c, c->desc.p, nb, nt_max, kc, c->ir_max_sweeps, c->ir_tol, &left)
+                  : refine<double>(c, c->desc.p, nb, nt_max, kc, c->ir_max_sweeps, c->ir_tol, &left);
       if (r) return r;
       std::vector<int> hconv(nb);
       HIPCHK(hipMemcpyAsync(hiters.data(), c->iters.p, nb * sizeof(int), hipMemcpyDeviceToHost,
@@ -628,6 +644,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
             SolveDesc d = hd[b];
             d.step0 = 0;  // the fallback chunk has its own step count
             d.A = d.M;
+            d.tiled = 0;
             d.X = nullptr;
             d.conv = nullptr;
             d.inv = c->inv.p + (long)fb.size() * kTile * kTile;

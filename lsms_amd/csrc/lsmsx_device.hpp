@@ -16,6 +16,15 @@ constexpr int kMaxLmax = 3;   // tables and per-site blocks sized for lmax <= 3
 constexpr int kMaxKkr = (kMaxLmax + 1) * (kMaxLmax + 1);
 constexpr int kPlaneHalves = 4 * kTile * kTile;  // one tile's four FP16 planes
 
+// Element (r, c) of a tile-major matrix with nt tiles per tile column: every
+// 64 x 64 tile is one contiguous run (column-major inside), tiles ordered by
+// tile column, then tile row.  A tile column of complex FP32 values is then
+// one 512-byte run and a whole tile one 32 KiB run.
+static_assert(kTile == 64, "tiled_index assumes 64 x 64 tiles");
+__host__ __device__ inline long tiled_index(int r, int c, int nt) {
+  return ((long)(c >> 6) * nt + (r >> 6)) * (kTile * kTile) + (c & 63) * kTile + (r & 63);
+}
+
 // One solve (local atom, spin, energy slot) of a batch.
 struct SolveDesc {
   // LIZ of the atom (device pointers)
@@ -33,7 +42,8 @@ struct SolveDesc {
   // matrices
   double2 *M;             // FP64 KKR matrix (assembly target; residuals of the mixed path)
   void *A;                // matrix the LU kernels factor: M itself (FP64) or a float2 copy
-  int ld;                 // leading dimension of M and A
+  int ld;                 // leading dimension of M (and of A when not tiled)
+  int tiled;              // 1: A is tile-major (tiled_index), 0: column-major with ld
   int Ns;                 // matrix dimension in this layout
   int pad;                // identity rows/cols at the front
   int nt;                 // Ns / kTile (solver layout)
@@ -54,6 +64,7 @@ struct SolveDesc {
   int *iters;             // refinement sweeps used
   double *dxn;            // ||dX_c||^2 of the previous sweep
   double *dxn_rho2;       // the contraction (||dX_c|| ratio)^2 observed at the previous sweep
+  double *nrm;            // ||dX_c||^2, ||X_c||^2 of the current sweep
   // split-FP16 trailing updates (mixed path): the FP32 panels of the two
   // steps of a pair as hi/lo FP16 planes with one power-of-2 scale per tile
   void *hp;               // (parity 2) x (L, U) x nt tiles x 4 planes x 64 x 64 halves

@@ -84,6 +84,18 @@ __device__ inline V czero() {
   return r;
 }
 
+// Tile (ti, tj) of the matrix the LU kernels factor, and the stride between
+// the columns of a tile: column-major with leading dimension ld, or
+// tile-major (d.tiled, tiled_index) with stride 64.  Every LU, refinement and
+// update kernel addresses the factored matrix tile by tile through these.
+template <class T>
+__device__ __forceinline__ GCx<T> *tile_ptr(const SolveDesc &d, int ti, int tj) {
+  GCx<T> *A = (GCx<T> *)d.A;
+  return d.tiled ? A + ((long)tj * d.nt + ti) * (kTile * kTile)
+                 : A + (long)kTile * ti + (long)d.ld * (kTile * tj);
+}
+__device__ __forceinline__ long tile_ld(const SolveDesc &d) { return d.tiled ? kTile : d.ld; }
+
 // Wave-wide argmax of v (ties -> smallest index); all lanes get the result.
 template <class T>
 __device__ inline void wave_argmax(T &v, int &idx) {

@@ -149,11 +149,11 @@ __global__ __launch_bounds__(256) void k_lu_diag(const SolveDesc *__restrict__ d
   __shared__ double sminpiv;
   const int tid = threadIdx.x;
   const int c0 = step * kTile;
-  GCx<T> *M = (GCx<T> *)d.A;
-  const long ld = d.ld;
+  GCx<T> *Mt = tile_ptr<T>(d, step, step);
+  const long ld = tile_ld(d);
   for (int e = tid; e < kTile * kTile; e += 256) {
     int r = e & 63, c = e >> 6;
-    A[c][r] = M[(c0 + r) + ld * (c0 + c)];
+    A[c][r] = Mt[r + ld * c];
   }
   if (tid < kTile) perm[tid] = tid;
   if (tid == 0) {
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(const SolveDesc *__restrict__ d
   tile_lu_lds<T>(A, kTile, piv, perm, &sinfo, &sminpiv, c0);
   for (int e = tid; e < kTile * kTile; e += 256) {
     int r = e & 63, c = e >> 6;
-    M[(c0 + r) + ld * (c0 + c)] = A[c][r];
+    Mt[r + ld * c] = A[c][r];
   }
   if (tid < kTile) d.perm[step * kTile + tid] = perm[tid];
   if (tid == 0) {
@@ -247,13 +247,10 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
   const int tid = threadIdx.x, w = tid >> 6;
   const int lane = tid & 63, li = lane & 15, kq = lane >> 4;
   const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
-  const int c0 = step * kTile;
-  GCx<T> *M = (GCx<T> *)d.A;
-  const long ld = d.ld;
+  const long ld = tile_ld(d);
   const GCx<T> *__restrict__ inv = (const GCx<T> *)d.inv + step * d.inv_stride;
   const bool lower = x < nrest;
-  GCx<T> *tile = lower ? M + (long)(c0 + kTile * (1 + x)) + ld * c0
-                       : M + (long)c0 + ld * (c0 + kTile * (1 + x - nrest));
+  GCx<T> *tile = lower ? tile_ptr<T>(d, step + 1 + x, step) : tile_ptr<T>(d, step, step + 1 + x - nrest);
   if (lower) {
     for (int e = tid; e < kTile * kTile; e += 256) {
       int r = e & 63, c = e >> 6;
@@ -327,9 +324,8 @@ __device__ __forceinline__ void update_tile(const SolveDesc &d, int ti, int tj, 
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
   const int w = threadIdx.x >> 6;
   const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
-  GCx<T> *M = (GCx<T> *)d.A;
-  const long ld = d.ld;
-  GCx<T> *__restrict__ C = M + (long)(kTile * ti + wr) + ld * (kTile * tj + wc);
+  const long ld = tile_ld(d);
+  GCx<T> *__restrict__ C = tile_ptr<T>(d, ti, tj) + wr + ld * wc;
   Acc3<T> q;
 #pragma unroll
   for (int bj = 0; bj < 2; bj++)
@@ -343,9 +339,8 @@ __device__ __forceinline__ void update_tile(const SolveDesc &d, int ti, int tj, 
         q.p[2][bj][bi][r] = c.x + c.y;
       }
   for (int u = 0; u < ns; u++) {
-    const int c0 = (s0 + u) * kTile;
-    const GCx<T> *__restrict__ A = M + (long)(kTile * ti + wr) + ld * c0;
-    const GCx<T> *__restrict__ B = M + (long)c0 + ld * (kTile * tj + wc);
+    const GCx<T> *__restrict__ A = tile_ptr<T>(d, ti, s0 + u) + wr;
+    const GCx<T> *__restrict__ B = tile_ptr<T>(d, s0 + u, tj) + ld * wc;
     wave_gemm3m<T, true>([&](int i, int k) { return A[i + ld * k]; },
                          [&](int k, int j) { return B[k + ld * j]; }, q);
   }
@@ -431,11 +426,11 @@ __global__ __launch_bounds__(256) void k_lu_last(const SolveDesc *__restrict__ d
   const int step = d.nt - 1;
   const int c0 = step * kTile;
   const int kc = d.kc, nd = kTile - kc;
-  GCx<T> *M = (GCx<T> *)d.A;
-  const long ld = d.ld;
+  GCx<T> *Mt = tile_ptr<T>(d, step, step);
+  const long ld = tile_ld(d);
   for (int e = tid; e < kTile * kTile; e += 256) {
     int r = e & 63, c = e >> 6;
-    A[c][r] = M[(c0 + r) + ld * (c0 + c)];
+    A[c][r] = Mt[r + ld * c];
   }
   if (tid < kTile) perm[tid] = tid;
   if (tid == 0) {
@@ -451,7 +446,7 @@ __global__ __launch_bounds__(256) void k_lu_last(const SolveDesc *__restrict__ d
   if (d.X != nullptr) {  // refinement follows: keep the tile's factors
     for (int e = tid; e < kTile * kTile; e += 256) {
       int r = e & 63, c = e >> 6;
-      M[(c0 + r) + ld * (c0 + c)] = A[c][r];
+      Mt[r + ld * c] = A[c][r];
     }
     if (tid < kTile) d.perm[step * kTile + tid] = perm[tid];
     __syncthreads();

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_lu_update2_h(const SolveDesc *__restric
   const int ti = b + 1 + 2 * sr + wt, tj = b + 1 + sc;
   const bool valid = 2 * sr + wt < nrest;
   const int nt = d.nt;
-  const long ld = d.ld;
+  const long ld = tile_ld(d);
   const _Float16 *hp = (const _Float16 *)d.hp;
   const int ns = a >= 0 ? 2 : 1;
   const int s_first = a >= 0 ? a : b;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_lu_update2_h(const SolveDesc *__restric
     const int par = s & 1;
     return d.he[(par * 2 + 0) * nt + ti] + d.he[(par * 2 + 1) * nt + tj];
   };
-  GF2 *C = (GF2 *)d.A + (long)(kTile * ti) + ld * (kTile * tj + wc);
+  GF2 *C = tile_ptr<float>(d, valid ? ti : b + 1, tj) + ld * wc;
   f4 cr[4][2], ci[4][2];
   int e = valid ? tile_exp(s_first) : 0;
   if (valid) {
