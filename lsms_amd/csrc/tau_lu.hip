@@ -133,6 +133,21 @@ __device__ void tile_invert_lds(CxT<T> (*A)[kTile + 1]) {
   }
 }
 
+// Stage a 64 x 64 tile (column stride ld) in LDS, S[col][row] =
+// src[rowmap(row) + ld col]: the 16 loads of a thread are all issued before
+// its first LDS write, so staging costs one memory round trip, not sixteen.
+template <class T, class RM>
+__device__ __forceinline__ void stage_tile(CxT<T> (*S)[kTile + 1], const GCx<T> *src, long ld,
+                                           RM rowmap) {
+  const int tid = threadIdx.x, r = tid & 63, c0 = tid >> 6;
+  const int rr = rowmap(r);
+  CxT<T> v[kTile / 4];
+#pragma unroll
+  for (int q = 0; q < kTile / 4; q++) v[q] = src[rr + ld * (c0 + 4 * q)];
+#pragma unroll
+  for (int q = 0; q < kTile / 4; q++) S[c0 + 4 * q][r] = v[q];
+}
+
 // ---------------------------------------------------------------------------
 // Diagonal tile of step s: factor, write back L\U, store L^-1 (strict lower)
 // and U^-1 (upper incl. diagonal) packed in one tile, and the permutation.
@@ -151,10 +166,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(const SolveDesc *__restrict__ d
   const int c0 = step * kTile;
   GCx<T> *Mt = tile_ptr<T>(d, step, step);
   const long ld = tile_ld(d);
-  for (int e = tid; e < kTile * kTile; e += 256) {
-    int r = e & 63, c = e >> 6;
-    A[c][r] = Mt[r + ld * c];
-  }
+  stage_tile<T>(A, Mt, ld, [](int r) { return r; });
   if (tid < kTile) perm[tid] = tid;
   if (tid == 0) {
     sinfo = 0;
@@ -252,16 +264,10 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
   const bool lower = x < nrest;
   GCx<T> *tile = lower ? tile_ptr<T>(d, step + 1 + x, step) : tile_ptr<T>(d, step, step + 1 + x - nrest);
   if (lower) {
-    for (int e = tid; e < kTile * kTile; e += 256) {
-      int r = e & 63, c = e >> 6;
-      Ts[c][r] = tile[r + ld * c];
-    }
+    stage_tile<T>(Ts, tile, ld, [](int r) { return r; });
   } else {
     const int *perm = d.perm + step * kTile;
-    for (int e = tid; e < kTile * kTile; e += 256) {
-      int r = e & 63, c = e >> 6;
-      Ts[c][r] = tile[perm[r] + ld * c];
-    }
+    stage_tile<T>(Ts, tile, ld, [&](int r) { return perm[r]; });
   }
   __syncthreads();
   Acc3<T> q;
@@ -428,10 +434,7 @@ __global__ __launch_bounds__(256) void k_lu_last(const SolveDesc *__restrict__ d
   const int kc = d.kc, nd = kTile - kc;
   GCx<T> *Mt = tile_ptr<T>(d, step, step);
   const long ld = tile_ld(d);
-  for (int e = tid; e < kTile * kTile; e += 256) {
-    int r = e & 63, c = e >> 6;
-    A[c][r] = Mt[r + ld * c];
-  }
+  stage_tile<T>(A, Mt, ld, [](int r) { return r; });
   if (tid < kTile) perm[tid] = tid;
   if (tid == 0) {
     sinfo = 0;

@@ -16,7 +16,8 @@
 // of its four waves computes a 64 x 32 part of one tile.  Per 32-wide k
 // chunk the split planes of the two L row tiles and of the U column tile
 // are staged in LDS (48 KiB), so every plane element is fetched from L2 once
-// per workgroup and read by two waves; the four 16-byte k groups of a
+// per workgroup and read by two waves; the next chunk's loads are issued
+// before this chunk's MFMAs, so the L2 latency hides behind them; the four 16-byte k groups of a
 // 64-byte LDS row are XOR-swizzled by the row, so the 16 lanes of a 128-bit
 // read hit distinct banks.  Complex products in 4M form
 // (Cr += (-Ar) Br + Ai Bi, Ci += (-Ar) Bi + (-Ai) Br) with two FP32
@@ -82,6 +83,36 @@ __global__ __launch_bounds__(256) void k_lu_update2_h(const SolveDesc *__restric
     return d.he[(par * 2 + 0) * nt + ti] + d.he[(par * 2 + 1) * nt + tj];
   };
   GF2 *C = tile_ptr<float>(d, valid ? ti : b + 1, tj) + ld * wc;
+  // Chunk ch = (step u = ch / 2, k half ch % 2): 12 loads of 8 halves per
+  // thread (q < 8: the two L row tiles, q >= 8: the U column tile), all issued
+  // together; the next chunk's loads are in flight during this chunk's MFMAs.
+  // Row tiles past the end are clamped to a real tile (their waves skip the
+  // MFMAs).
+  auto chunk_src = [&](int ch, int q) -> GH8 * {
+    const int s = s_first + (ch >> 1), kh = ch & 1, par = s & 1;
+    const int e8 = tid + 256 * q;
+    if (q < 8) {
+      const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
+      const int t = min(b + 1 + 2 * sr + tt, nt - 1);
+      return (GH8 *)(hp + (long)((par * 2 + 0) * nt + t) * kPlaneHalves) +
+             ((plane * kTile + rr) * kTile + 32 * kh + 8 * grp) / 8;
+    }
+    const int f = e8 - 2048, plane = f >> 8, rr = (f >> 2) & 63, grp = f & 3;
+    return (GH8 *)(hp + (long)((par * 2 + 1) * nt + tj) * kPlaneHalves) +
+           ((plane * kTile + rr) * kTile + 32 * kh + 8 * grp) / 8;
+  };
+  auto chunk_dst = [&](int q) -> h8 * {
+    const int e8 = tid + 256 * q;
+    if (q < 8) {
+      const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
+      return &Ls[tt][plane][rr][swz_group(rr, grp)];
+    }
+    const int f = e8 - 2048, plane = f >> 8, rr = (f >> 2) & 63, grp = f & 3;
+    return &Us[plane][rr][swz_group(rr, grp)];
+  };
+  h8 pv[12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) pv[q] = *chunk_src(0, q);
   f4 cr[4][2], ci[4][2];
   int e = valid ? tile_exp(s_first) : 0;
   if (valid) {
@@ -97,10 +128,10 @@ __global__ __launch_bounds__(256) void k_lu_update2_h(const SolveDesc *__restric
           ci[bi][bj][r] = c.y * sc0;
         }
   }
-  for (int u = 0; u < ns; u++) {
-    const int s = s_first + u;
-    if (u > 0 && valid) {
-      const int e2 = tile_exp(s);
+  const int nch = 2 * ns;
+  for (int ch = 0; ch < nch; ch++) {
+    if (ch == 2 && valid) {  // second step of the pair: its tile exponents
+      const int e2 = tile_exp(s_first + 1);
       const float sc1 = ldexpf(1.f, e - e2);
 #pragma unroll
       for (int bi = 0; bi < 4; bi++)
@@ -111,61 +142,47 @@ __global__ __launch_bounds__(256) void k_lu_update2_h(const SolveDesc *__restric
         }
       e = e2;
     }
-    const int par = s & 1;
-    for (int kc = 0; kc < 2; kc++) {
-      __syncthreads();
-#pragma unroll 2
-      for (int q = 0; q < 12; q++) {
-        const int e8 = tid + 256 * q;  // 0..3071: L 0..2047, U 2048..3071
-        if (e8 < 2048) {
-          const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
-          const int t = b + 1 + 2 * sr + tt;
-          if (t < nt) {
-            GH8 *src = (GH8 *)(hp + (long)((par * 2 + 0) * nt + t) * kPlaneHalves);
-            Ls[tt][plane][rr][swz_group(rr, grp)] = src[((plane * kTile + rr) * kTile + 32 * kc + 8 * grp) / 8];
-          }
-        } else {
-          const int f = e8 - 2048;
-          const int plane = f >> 8, rr = (f >> 2) & 63, grp = f & 3;
-          GH8 *src = (GH8 *)(hp + (long)((par * 2 + 1) * nt + tj) * kPlaneHalves);
-          Us[plane][rr][swz_group(rr, grp)] = src[((plane * kTile + rr) * kTile + 32 * kc + 8 * grp) / 8];
-        }
+    __syncthreads();  // the previous chunk's readers are done
+#pragma unroll
+    for (int q = 0; q < 12; q++) *chunk_dst(q) = pv[q];
+    __syncthreads();
+    if (ch + 1 < nch) {
+#pragma unroll
+      for (int q = 0; q < 12; q++) pv[q] = *chunk_src(ch + 1, q);
+    }
+    if (valid) {
+      h8 ur_h[2], ur_l[2], ui_h[2], ui_l[2];
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++) {
+        const int col = wc + 16 * bj + l16, gg = swz_group(col, g);
+        ur_h[bj] = Us[0][col][gg];
+        ur_l[bj] = Us[1][col][gg];
+        ui_h[bj] = Us[2][col][gg];
+        ui_l[bj] = Us[3][col][gg];
       }
-      __syncthreads();
-      if (valid) {
-        h8 ur_h[2], ur_l[2], ui_h[2], ui_l[2];
+#pragma unroll
+      for (int bi = 0; bi < 4; bi++) {
+        const int row = 16 * bi + l16, gg = swz_group(row, g);
+        const h8 nar_h = Ls[wt][0][row][gg], nar_l = Ls[wt][1][row][gg];
+        const h8 ai_h = Ls[wt][2][row][gg], ai_l = Ls[wt][3][row][gg];
+        const h8 nai_h = neg8(ai_h), nai_l = neg8(ai_l);
 #pragma unroll
         for (int bj = 0; bj < 2; bj++) {
-          const int col = wc + 16 * bj + l16, gg = swz_group(col, g);
-          ur_h[bj] = Us[0][col][gg];
-          ur_l[bj] = Us[1][col][gg];
-          ui_h[bj] = Us[2][col][gg];
-          ui_l[bj] = Us[3][col][gg];
-        }
-#pragma unroll
-        for (int bi = 0; bi < 4; bi++) {
-          const int row = 16 * bi + l16, gg = swz_group(row, g);
-          const h8 nar_h = Ls[wt][0][row][gg], nar_l = Ls[wt][1][row][gg];
-          const h8 ai_h = Ls[wt][2][row][gg], ai_l = Ls[wt][3][row][gg];
-          const h8 nai_h = neg8(ai_h), nai_l = neg8(ai_l);
-#pragma unroll
-          for (int bj = 0; bj < 2; bj++) {
-            f4 r = cr[bi][bj], m = ci[bi][bj];
-            r = mfma_h(ur_h[bj], nar_h, r);
-            r = mfma_h(ur_l[bj], nar_h, r);
-            r = mfma_h(ur_h[bj], nar_l, r);
-            r = mfma_h(ui_h[bj], ai_h, r);
-            r = mfma_h(ui_l[bj], ai_h, r);
-            r = mfma_h(ui_h[bj], ai_l, r);
-            m = mfma_h(ui_h[bj], nar_h, m);
-            m = mfma_h(ui_l[bj], nar_h, m);
-            m = mfma_h(ui_h[bj], nar_l, m);
-            m = mfma_h(ur_h[bj], nai_h, m);
-            m = mfma_h(ur_l[bj], nai_h, m);
-            m = mfma_h(ur_h[bj], nai_l, m);
-            cr[bi][bj] = r;
-            ci[bi][bj] = m;
-          }
+          f4 r = cr[bi][bj], m = ci[bi][bj];
+          r = mfma_h(ur_h[bj], nar_h, r);
+          r = mfma_h(ur_l[bj], nar_h, r);
+          r = mfma_h(ur_h[bj], nar_l, r);
+          r = mfma_h(ui_h[bj], ai_h, r);
+          r = mfma_h(ui_l[bj], ai_h, r);
+          r = mfma_h(ui_h[bj], ai_l, r);
+          m = mfma_h(ui_h[bj], nar_h, m);
+          m = mfma_h(ui_l[bj], nar_h, m);
+          m = mfma_h(ui_h[bj], nar_l, m);
+          m = mfma_h(ur_h[bj], nai_h, m);
+          m = mfma_h(ur_l[bj], nai_h, m);
+          m = mfma_h(ur_h[bj], nai_l, m);
+          cr[bi][bj] = r;
+          ci[bi][bj] = m;
         }
       }
     }
