@@ -62,18 +62,21 @@ __global__ __launch_bounds__(256) void k_ir_init(const SolveDesc *__restrict__ d
 // wave w handles rows 16w.. of the tile, all kc = 16 NB right-hand sides.
 // FP64 MFMA in the 3M, transposed form of the LU update:
 //   a-operand X(k0+kq, j0+li), b-operand M(i0+li, k0+kq).
-// M streams from HBM in 64-column chunks, all 16 k-steps of a chunk in
-// flight per wave; the chunk's 64 x kc block of X is staged in LDS once per
-// workgroup (the four waves share it).  The residual is stored in the
-// correction type T.
+// M streams from HBM in 32-column chunks through two register buffers: the
+// loads of chunk c+1 are in flight during the MFMAs of chunk c.  The chunk's
+// 32 x kc block of X is staged in LDS once per workgroup (the four waves
+// share it), alternating between two buffers so that one barrier per chunk
+// suffices.  The residual is stored in the correction type T.
 template <class T, int NB>
 __global__ __launch_bounds__(256) void k_ir_residual(const SolveDesc *__restrict__ descs) {
   typedef CxT<T> V;
+  constexpr int KC = 32;               // k per chunk
+  constexpr int XQ = KC * 16 * NB / 256;  // X values per thread per chunk
   const SolveDesc &d = descs[blockIdx.y];
   if (*d.conv) return;
   const int tile = blockIdx.x;
   if (tile >= d.nt) return;
-  __shared__ double2 xs[kTile][16 * NB + 1];
+  __shared__ double2 xs[2][KC][16 * NB + 1];
   const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, kq = lane >> 4;
   const int w = tid >> 6;
   const int i0 = tile * kTile + 16 * w;
@@ -94,33 +97,42 @@ __global__ __launch_bounds__(256) void k_ir_residual(const SolveDesc *__restrict
       q[1][jb][r] = 0.0;
       q[2][jb][r] = b.x + b.y;
     }
-  for (int k0 = 0; k0 < Ns; k0 += kTile) {
-    double2 m[kTile / 4], xv[4 * NB];
+  auto load = [&](int k0, double2 (&m)[KC / 4], double2 (&xv)[XQ]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int u = 0; u < kTile / 4; u++) m[u] = M[li + ld * (k0 + 4 * u + kq)];
+    for (int u = 0; u < KC / 4; u++) m[u] = ldg<double>(M + li + ld * (k0 + 4 * u + kq));
 #pragma unroll
-    for (int qq = 0; qq < 4 * NB; qq++) {
+    for (int qq = 0; qq < XQ; qq++) {
       const int e = tid + 256 * qq;
-      xv[qq] = X[(k0 + (e & 63)) + (long)Ns * (e >> 6)];
+      xv[qq] = ldg<double>(X + (k0 + e % KC) + (long)Ns * (e / KC));
     }
-    __syncthreads();  // the previous chunk's readers are done
+  };
+  auto compute = [&](int buf, const double2 (&m)[KC / 4], const double2 (&xv)[XQ])
+                     __attribute__((always_inline)) {
 #pragma unroll
-    for (int qq = 0; qq < 4 * NB; qq++) {
+    for (int qq = 0; qq < XQ; qq++) {
       const int e = tid + 256 * qq;
-      xs[e & 63][e >> 6] = xv[qq];
+      xs[buf][e % KC][e / KC] = xv[qq];
     }
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < kTile / 4; u++) {
+    for (int u = 0; u < KC / 4; u++) {
       const double mr = -m[u].x, mi = -m[u].y, ms = mr + mi;
 #pragma unroll
       for (int jb = 0; jb < NB; jb++) {
-        const double2 x = xs[4 * u + kq][16 * jb + li];
+        const double2 x = xs[buf][4 * u + kq][16 * jb + li];
         q[0][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, mr, q[0][jb], 0, 0, 0);
         q[1][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, mi, q[1][jb], 0, 0, 0);
         q[2][jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x + x.y, ms, q[2][jb], 0, 0, 0);
       }
     }
+  };
+  double2 ma[KC / 4], mb[KC / 4], xa[XQ], xb[XQ];
+  load(0, ma, xa);
+  for (int k0 = 0; k0 < Ns; k0 += 2 * KC) {  // Ns is a multiple of 64
+    load(k0 + KC, mb, xb);
+    compute(0, ma, xa);
+    if (k0 + 2 * KC < Ns) load(k0 + 2 * KC, ma, xa);
+    compute(1, mb, xb);
   }
   GCx<T> *R = (GCx<T> *)d.R;
 #pragma unroll
@@ -253,11 +265,11 @@ __global__ __launch_bounds__(256) void k_ir_step(const SolveDesc *__restrict__ d
   V a[kTile / 4], st[8];
   if (s >= 0) {
 #pragma unroll
-    for (int u = 0; u < kTile / 4; u++) a[u] = Wt[li + ld * (4 * u + kq)];
+    for (int u = 0; u < kTile / 4; u++) a[u] = ldg<T>(Wt + li + ld * (4 * u + kq));
 #pragma unroll
     for (int qq = 0; qq < 8; qq++) {
       const int e = tid + 256 * qq;
-      if (qq < kc / 4) st[qq] = R[(rs + (e & 63)) + (long)Ns * (e >> 6)];
+      if (qq < kc / 4) st[qq] = ldg<T>(R + (rs + (e & 63)) + (long)Ns * (e >> 6));
     }
   }
   AccIR<T> q;

@@ -38,6 +38,18 @@ using Acc4 = typename Cplx<T>::acc;
 template <class T>
 using GCx = LSMSX_GLOBAL CxT<T>;
 
+// Load of one complex value from global memory, field by field.  A struct
+// copy out of a global-address-space pointer into an array element becomes a
+// memcpy that keeps the array in scratch memory; two scalar loads (merged
+// into one wide load) keep it in registers.
+template <class T>
+__device__ __forceinline__ CxT<T> ldg(const GCx<T> *p) {
+  CxT<T> v;
+  v.x = p->x;
+  v.y = p->y;
+  return v;
+}
+
 __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }

@@ -143,7 +143,7 @@ __device__ __forceinline__ void stage_tile(CxT<T> (*S)[kTile + 1], const GCx<T> 
   const int rr = rowmap(r);
   CxT<T> v[kTile / 4];
 #pragma unroll
-  for (int q = 0; q < kTile / 4; q++) v[q] = src[rr + ld * (c0 + 4 * q)];
+  for (int q = 0; q < kTile / 4; q++) v[q] = ldg<T>(src + rr + ld * (c0 + 4 * q));
 #pragma unroll
   for (int q = 0; q < kTile / 4; q++) S[c0 + 4 * q][r] = v[q];
 }
