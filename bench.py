@@ -37,36 +37,43 @@ def solve_flops(n, k):
     return (8.0 / 3.0) * n ** 3 + 8.0 * n * n * k
 
 
-def update_kernel_bytes(nt_list, c_bytes, panel_bytes):
+def _groups(nt, group):
+    """(steps in the group, trailing tiles per dimension) of every trailing
+    update of one solve: groups of `group` elimination steps g .. g+ns-1 over
+    steps 0 .. nt-2 (step nt-1 is the last tile), each followed by one update
+    of the (nt-1-last)^2 tiles beyond the group."""
+    for g in range(0, nt - 1, group):
+        ns = min(group, nt - 1 - g)
+        r = nt - 1 - (g + ns - 1)
+        if r > 0:
+            yield ns, r
+
+
+def update_kernel_bytes(nt_list, c_bytes, panel_bytes, group):
     """Algorithmic HBM bytes of all trailing-update launches of one batch: each
-    step pair reads and writes its (nt-2-a)^2 C tiles once and reads the two
-    L column and two U row panels of the pair once (bytes per complex element:
-    c_bytes for C, panel_bytes for the panel operands)."""
+    update reads and writes its r^2 C tiles once and reads the ns L column and
+    ns U row panels of its group once (bytes per complex element: c_bytes for
+    C, panel_bytes for the panel operands)."""
     total = 0.0
     for nt in nt_list:
-        for a in range(0, nt - 1, 2):
-            r = nt - 2 - a
-            if r > 0:
-                total += r * r * 2 * 64 * 64 * c_bytes + 2 * 2 * r * 64 * 64 * panel_bytes
+        for ns, r in _groups(nt, group):
+            total += r * r * 2 * 64 * 64 * c_bytes + 2 * ns * r * 64 * 64 * panel_bytes
     return total
 
 
-def update_kernel_flops(nt_list):
-    """Algorithmic flops of all k_lu_update2 launches of one batch.  Steps are
-    eliminated in pairs (a, b = a + 1): after a cross update of tile row and
-    column b, k_lu_update2 applies both rank-64 panels to the (nt-2-a)^2
-    tiles beyond b, 2 * 8*64^3 flops per tile (8 flops per complex FMA)."""
+def update_kernel_flops(nt_list, group):
+    """Algorithmic flops of all trailing-update launches of one batch: the ns
+    rank-64 panels of a group applied to the r^2 tiles beyond it, ns * 8*64^3
+    flops per tile (8 flops per complex FMA)."""
     total = 0.0
     for nt in nt_list:
-        for a in range(0, nt - 1, 2):
-            r = nt - 2 - a
-            if r > 0:
-                total += r * r * 2 * 8.0 * 64 ** 3
+        for ns, r in _groups(nt, group):
+            total += r * r * ns * 8.0 * 64 ** 3
     return total
 
 
 PROFILES = [os.path.join(ROOT, "profiles", f) for f in
-            ("k_lu_update2_fcc135_N2160.md", "k_lu_update2_h_fcc135_N2160.md")]
+            ("k_lu_trail_fcc135_N2160.md", "k_lu_trail_h_fcc135_N2160.md")]
 
 
 def profiled_traffic(kernel):
@@ -255,20 +262,21 @@ def main():
     nt = (N + 63) // 64
     # (kernel, MFMA peak, executed real flops per algorithmic flop, C and panel
     # bytes per complex element, dtype, precision)
-    kernel, peak, flop_mult, cb, pb, dtype, prec = {
+    # (kernel, elimination steps per trailing update)
+    kernel, peak, flop_mult, cb, pb, dtype, prec, group = {
         # 3M: 3 real products (6 flops) per complex multiply-add (8 flops)
-        "fp64": ("k_lu_update2<double>", FP64_PEAK_TFLOPS, 0.75, 16, 16, "f64", "FP64"),
-        "fp32_update": ("k_lu_update2<float>", FP32_PEAK_TFLOPS, 0.75, 8, 8,
+        "fp64": ("k_lu_trail<double>", FP64_PEAK_TFLOPS, 0.75, 16, 16, "f64", "FP64", 2),
+        "fp32_update": ("k_lu_trail<float>", FP32_PEAK_TFLOPS, 0.75, 8, 8,
                         "c32-factor+f64-refine",
-                        "mixed: complex FP32 LU on MFMA + FP64 iterative refinement"),
+                        "mixed: complex FP32 LU on MFMA + FP64 iterative refinement", 2),
         # 4M x 3 split products: 12 real products (24 flops) per complex multiply-add
-        "split_f16_update": ("k_lu_update2_h", FP16_PEAK_TFLOPS, 3.0, 8, 8,
+        "split_f16_update": ("k_lu_trail_h", FP16_PEAK_TFLOPS, 3.0, 8, 8,
                              "f16x2-split-update+f32-factor+f64-refine",
                              "mixed: complex FP32 LU with split-FP16 trailing updates on the FP16 "
-                             "matrix cores + FP64 iterative refinement"),
+                             "matrix cores + FP64 iterative refinement", 4),
     }[headline]
-    upd_flops = update_kernel_flops([nt] * nsolve)
-    upd_bytes = update_kernel_bytes([nt] * nsolve, cb, pb)
+    upd_flops = update_kernel_flops([nt] * nsolve, group)
+    upd_bytes = update_kernel_bytes([nt] * nsolve, cb, pb, group)
     t_s = upd_ms * 1e-3 if upd_ms > 0 else float("nan")
     achieved_tf = upd_flops / t_s / 1e12
     achieved_gbs = upd_bytes / t_s / 1e9

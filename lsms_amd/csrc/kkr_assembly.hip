@@ -211,10 +211,46 @@ __device__ __forceinline__ void gaunt_sum(const double *cg, const double2 *il, c
   }
 }
 
+// The four entries G(lm2 = 4u + kq, lm1 = li), u = 0..3, of the wave's
+// current pair that lane (li, kq) feeds to the t G product on the MFMA
+// (a-operand layout), by the Gaunt sum of gaunt_sum, in registers.
+__device__ __forceinline__ void gaunt_lane(const double *cg, const double2 *il, const int *lofk,
+                                           int lmax_tab, int kkri, int kkrj, bool prel0,
+                                           const double2 *dlw, double2 (&g)[4]) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+  const int kkr_tab = (lmax_tab + 1) * (lmax_tab + 1);
+  const double pi4 = 4.0 * 3.141592653589793238462643383279502884;
+  const int lm1 = li;
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int lm2 = 4 * u + kq;
+    double2 gv = make_double2(0.0, 0.0);
+    if (lm2 < kkri && lm1 < kkrj) {
+      const int l1 = lofk[lm1], l2 = lofk[lm2];
+      const int m1 = lm1 - l1 * (l1 + 1), m2 = lm2 - l2 * (l2 + 1), m3 = m2 - m1;
+      const int am3 = m3 < 0 ? -m3 : m3, dl12 = l1 - l2 < 0 ? l2 - l1 : l1 - l2;
+      int llow = am3 > dl12 ? am3 : dl12;
+      if (prel0) llow = l1 + l2;
+      double2 sacc = make_double2(0.0, 0.0);
+      for (int l3 = l1 + l2; l3 >= llow; l3 -= 2) {
+        const double c = cg[l3 / 2 + (lmax_tab + 1) * (lm1 + kkr_tab * lm2)];
+        const double2 dv = dlw[l3 * (l3 + 1) + m3];
+        sacc.x += c * dv.x;
+        sacc.y += c * dv.y;
+      }
+      gv = cscale(cmul(il[lm2 + kkr_tab * lm1], sacc), pi4);
+    }
+    g[u] = gv;
+  }
+}
+
 // Assemble row-site block row i = blockIdx.x of solve blockIdx.y:
 // M(i,j) = -tpack_i diag_spin(G_ij) for j != i, identity for j == i.
-// The workgroup stages the packed t_i and the angular tables in LDS; each
-// wave then builds blocks (i, j), j = w, w+4, ..., independently.
+// The workgroup stages the angular and Gaunt tables in LDS (16 KiB, so that
+// many workgroups share a CU and hide the latency of the special functions);
+// each wave keeps its MFMA operands of t_i in registers and builds blocks
+// (i, j), j = w, w+4, ..., independently.  NSC = n_spin_cant.
+template <int NSC>
 __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ descs,
                                                   const double2 *__restrict__ tmat, long ldT,
                                                   const double *__restrict__ cgnt_g,
@@ -226,21 +262,36 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
 
   __shared__ double cg[(kMaxLmax + 1) * kMaxKkr * kMaxKkr];
   __shared__ double2 il[kMaxKkr * kMaxKkr];
-  __shared__ double2 tn[2 * kMaxKkr][2 * kMaxKkr + 1];   // packed t_i, [row][col]
   __shared__ double pa[kLmaxD + 1][kLmaxD + 1], pb[kLmaxD + 1][kLmaxD + 1];
   __shared__ double pmmf[kLmaxD + 1], pm1f[kLmaxD + 1];
   __shared__ int lofk[kNd];
   __shared__ double2 dl[4][kNd];
-  __shared__ double2 gs[4][kMaxKkr * kMaxKkr];
 
-  const int nsc = d.n_spin_cant;
-  const int lmi = d.lmax[i], kkri = (lmi + 1) * (lmi + 1), kkri_ns = kkri * nsc;
+  const int lmi = d.lmax[i], kkri = (lmi + 1) * (lmi + 1), kkri_ns = kkri * NSC;
   load_tables(cgnt_g, illp_g, lmax_tab, cg, il, pa, pb, pmmf, pm1f, lofk);
-  load_tpack(tmat + ldT * d.store[i] + d.t_off, nsc, d.kkrsz, d.ldt, kkri, tn);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, kq = lane >> 4;
+  // t_i packed as the F77 route does (calculateTauMatrix.cpp:155-183: the
+  // leading kkri block of each spin block), in the b-operand layout:
+  // tv[bi][bj][u] = t(bi kkrsz + li, bj kkrsz + 4u + kq), zero outside kkri
+  double2 tv[NSC][NSC][4];
+  {
+    const double2 *T = tmat + ldT * d.store[i] + d.t_off;
+#pragma unroll
+    for (int bi = 0; bi < NSC; bi++)
+#pragma unroll
+      for (int bj = 0; bj < NSC; bj++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int ci = 4 * u + kq;
+          tv[bi][bj][u] = (li < kkri && ci < kkri)
+                              ? T[(bi * d.kkrsz + li) + (long)d.ldt * (bj * d.kkrsz + ci)]
+                              : make_double2(0.0, 0.0);
+        }
+  }
   __syncthreads();
 
   const AngTab tb{pa, pb, pmmf, pm1f};
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double xi = d.pos[3 * i], yi = d.pos[3 * i + 1], zi = d.pos[3 * i + 2];
   const double2 prel = make_double2(d.prel_re, d.prel_im);
   const bool prel0 = (prel.x == 0.0 && prel.y == 0.0);
@@ -279,45 +330,48 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
       dl[w][lane] = cmul(h, angular_lane(tb, l, m, rx, ry, rz, r));
     }
     wave_sync();
-    gaunt_sum(cg, il, lofk, lmax_tab, kkri, kkrj, prel0, dl[w], gs[w]);
-    wave_sync();
+    double2 g[4];
+    gaunt_lane(cg, il, lofk, lmax_tab, kkri, kkrj, prel0, dl[w], g);
     // --- M block = -tpack_i diag_spin(G) (calculateTauMatrix.cpp:240-242) on
     // FP64 MFMA, 3M form, one 16 x 16 x 16 product per spin block pair
     // (bi, bj): M_bi,bj = -t_bi,bj G.  Transposed product as in the LU update,
     // so register r of lane (li, kq) is element (li, kq + 4r) of the block and
     // each store is a run of 16 consecutive rows.
-    {
-      const int li = lane & 15, kq = lane >> 4;
-      for (int bi = 0; bi < nsc; bi++)
-        for (int bj = 0; bj < nsc; bj++) {
-          d4 q1 = {0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
 #pragma unroll
-          for (int k0 = 0; k0 < kMaxKkr; k0 += 4) {
-            const double2 g = gs[w][(k0 + kq) + kMaxKkr * li];
-            const double2 t = tn[bi * kMaxKkr + li][bj * kMaxKkr + k0 + kq];
-            q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.x, t.x, q1, 0, 0, 0);
-            q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.y, t.y, q2, 0, 0, 0);
-            q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(g.x + g.y, t.x + t.y, q3, 0, 0, 0);
-          }
-          if (li < kkri) {
+    for (int bi = 0; bi < NSC; bi++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-              const int jc = kq + 4 * r;
-              if (jc < kkrj) {
-                const double re = q1[r] - q2[r], im = q3[r] - q1[r] - q2[r];
-                const long idx = (offi + bi * kkri + li) + ld * (offj + bj * kkrj + jc);
-                M[idx] = make_double2(-re, -im);
-                if (W)
-                  W[tiled_index(offi + bi * kkri + li, offj + bj * kkrj + jc, d.nt)] =
-                      make_float2((float)-re, (float)-im);
-              }
+      for (int bj = 0; bj < NSC; bj++) {
+        d4 q1 = {0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const double2 gg = g[u], t = tv[bi][bj][u];
+          q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(gg.x, t.x, q1, 0, 0, 0);
+          q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(gg.y, t.y, q2, 0, 0, 0);
+          q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(gg.x + gg.y, t.x + t.y, q3, 0, 0, 0);
+        }
+        if (li < kkri) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int jc = kq + 4 * r;
+            if (jc < kkrj) {
+              const double re = q1[r] - q2[r], im = q3[r] - q1[r] - q2[r];
+              const long idx = (offi + bi * kkri + li) + ld * (offj + bj * kkrj + jc);
+              M[idx] = make_double2(-re, -im);
+              if (W)
+                W[tiled_index(offi + bi * kkri + li, offj + bj * kkrj + jc, d.nt)] =
+                    make_float2((float)-re, (float)-im);
             }
           }
         }
-    }
-    wave_sync();
+      }
+    wave_sync();  // dl[w] is rewritten for the next pair
   }
 }
+
+template __global__ void k_assemble<1>(const SolveDesc *, const double2 *, long, const double *,
+                                       const double2 *, int);
+template __global__ void k_assemble<2>(const SolveDesc *, const double2 *, long, const double *,
+                                       const double2 *, int);
 
 // Energy derivative of the assembly contracted with the explicit inverse, one
 // workgroup per row site i = blockIdx.x of solve blockIdx.y:

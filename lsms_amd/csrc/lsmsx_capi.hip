@@ -22,6 +22,7 @@
 #include "lsmsx_device.hpp"
 
 namespace lsmsx {
+template <int NSC>
 __global__ void k_assemble(const SolveDesc *, const double2 *, long, const double *,
                            const double2 *, int);
 __global__ void k_pad_identity(const SolveDesc *);
@@ -32,9 +33,9 @@ __global__ void k_lu_panel(const SolveDesc *, int);
 template <class T>
 __global__ void k_lu_update(const SolveDesc *, int, int);
 template <class T>
-__global__ void k_lu_cross(const SolveDesc *, int);
+__global__ void k_lu_cross(const SolveDesc *, int, int);
 template <class T>
-__global__ void k_lu_update2(const SolveDesc *, int, int);
+__global__ void k_lu_trail(const SolveDesc *, int, int);
 template <class T>
 __global__ void k_lu_last(const SolveDesc *);
 template <class T>
@@ -64,7 +65,7 @@ __global__ void k_tau_grad(const SolveDesc *, const double2 *, double2 *, int);
 __global__ void k_grad_reduce(const double2 *, int, const int *, const long *, const int *,
                               const int *, double2 *, long, int);
 __global__ void k_zero_c(double2 *, long);
-__global__ void k_lu_update2_h(const SolveDesc *, int, int);
+__global__ void k_lu_trail_h(const SolveDesc *, int, int);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout.
 __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
@@ -229,6 +230,16 @@ struct ProfScope {
   }
 };
 
+// KKR assembly of a batch (descriptors dd) with n_spin_cant = nsc.
+void launch_assemble(lsmsx_ctx *c, int nsc, dim3 grid, hipStream_t st, const SolveDesc *dd) {
+  if (nsc == 2)
+    hipLaunchKernelGGL(k_assemble<2>, grid, dim3(256), 0, st, dd, c->tmat.p, c->ldT, c->cgnt.p,
+                       c->illp.p, c->lmax_tab);
+  else
+    hipLaunchKernelGGL(k_assemble<1>, grid, dim3(256), 0, st, dd, c->tmat.p, c->ldT, c->cgnt.p,
+                       c->illp.p, c->lmax_tab);
+}
+
 int drain_profile(lsmsx_ctx *c) {
   for (auto &p : c->pending) {
     float ms = 0.f;
@@ -272,15 +283,23 @@ std::vector<int> site_offsets(const AtomDev &a, int nsc, bool solver) {
   return off;
 }
 
-// Launch the blocked LU over a chunk of nb solves already in solver layout:
-// steps taken in pairs (a, b = a + 1) -- panels of a, cross update of tile
-// row/column b, panels of b, then one K = 128 trailing update -- so the
-// trailing matrix is read and written once per two steps.
+// Steps per trailing update: four with the split-FP16 update (its C traffic
+// halves against pairs, while the cross updates of the group's later steps
+// stay small), two otherwise.
+inline int lu_group(bool split_f16) { return split_f16 ? 4 : 2; }
+
+// Launch the blocked LU over a chunk of nb solves already in solver layout
+// (each descriptor's group = lu_group(split_f16)): steps taken in groups
+// g .. g+G-1 -- panels of g, then for each later step t of the group a cross
+// update of tile row/column t with the group's earlier steps and the panels
+// of t -- then one K = 64 G trailing update, so the trailing matrix is read
+// and written once per group.
 template <class T>
 int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_f16 = false,
               hipStream_t st = nullptr) {
   if (!st) st = c->stream;
   const int swz = (c->sched & LSMSX_SCHED_XCD_SWIZZLE) ? 1 : 0;
+  const int G = lu_group(split_f16);
   auto diag_panel = [&](int s) -> int {
     const int nrest = nt_max - 1 - s;
     if (nrest <= 0) return 0;
@@ -294,25 +313,30 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
     }
     return check_launch("k_lu_diag/k_lu_panel");
   };
-  for (int s = 0; s < nt_max - 1; s += 2) {
+  for (int s = 0; s < nt_max - 1; s += G) {
     if (int r = diag_panel(s)) return r;
-    const int ncross = 2 * (nt_max - s - 1) - 1;
-    if (ncross > 0) {
-      ProfScope p(c, 9);
-      hipLaunchKernelGGL(k_lu_cross<T>, dim3(ncross, nb), dim3(256), 0, st, dd, s);
+    int last = s;  // last step of the group for the largest solve
+    for (int u = 1; u < G && s + u < nt_max - 1; u++) {
+      {
+        ProfScope p(c, 9);
+        hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * (nt_max - s - u) - 1, nb), dim3(256), 0, st,
+                           dd, s, u);
+      }
+      if (int r = check_launch("k_lu_cross")) return r;
+      if (int r = diag_panel(s + u)) return r;
+      last = s + u;
     }
-    if (int r = diag_panel(s + 1)) return r;
-    const int nrest = nt_max - 2 - s;
+    const int nrest = nt_max - 1 - last;
     if (nrest > 0) {
       ProfScope p(c, 3);
       if (split_f16)
-        hipLaunchKernelGGL(k_lu_update2_h, dim3(((nrest + 1) / 2) * nrest, nb), dim3(256), 0,
-                           st, dd, s, swz);
+        hipLaunchKernelGGL(k_lu_trail_h, dim3(((nrest + 1) / 2) * nrest, nb), dim3(256), 0, st,
+                           dd, s, swz);
       else
-        hipLaunchKernelGGL(k_lu_update2<T>, dim3(nrest * nrest, nb), dim3(256), 0, st, dd,
-                           s, swz);
+        hipLaunchKernelGGL(k_lu_trail<T>, dim3(nrest * nrest, nb), dim3(256), 0, st, dd, s,
+                           swz);
     }
-    if (int r = check_launch("k_lu_update2")) return r;
+    if (int r = check_launch("k_lu_trail")) return r;
   }
   {
     ProfScope p(c, 4);
@@ -425,7 +449,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     if (plain) bytes[i] += kTile * kTile * 16;
     if (mixed) bytes[i] += Ns * Ns * 8 + nt * kTile * kTile * 8 + Ns * kc * 24;
     if (refined64) bytes[i] += Ns * Ns * 16 + nt * kTile * kTile * 16 + Ns * kc * 32;
-    if (split) bytes[i] += 4 * nt * (long)kPlaneHalves * 2 + 16 * nt;
+    if (split) bytes[i] += 2 * lu_group(true) * nt * ((long)kPlaneHalves * 2 + 4);
   }
   long cap = c->ws_limit;
   if (cap <= 0) {
@@ -449,7 +473,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       mtot += Ns * Ns;
       xtot += Ns * kc;
       ivtot += refining ? (Ns / kTile) * kTile * kTile : kTile * kTile;
-      htot += 4 * (Ns / kTile);
+      htot += 2 * lu_group(split) * (Ns / kTile);
       maxliz = std::max(maxliz, a.numLIZ);
       nt_max = std::max(nt_max, (int)(Ns / kTile));
     }
@@ -491,7 +515,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (split) {
         d.hp = c->hp.p + hoff * kPlaneHalves;
         d.he = c->he.p + hoff;
-        hoff += 4 * (Ns / kTile);
+        hoff += 2 * lu_group(split) * (Ns / kTile);
       }
       d.pos = a.d_pos;
       d.lmax = a.d_lmax;
@@ -512,10 +536,12 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       d.pad = Ns - N;
       d.nt = Ns / kTile;
       // right alignment starts each solve late enough to end with the
-      // largest; the start is even so that every solve pairs its own steps
-      // (0,1), (2,3), ... in the K = 128 elimination, whatever its size: the
-      // arithmetic of a solve never depends on the schedule or the batch
-      d.step0 = (c->sched & LSMSX_SCHED_RIGHT_ALIGN) ? ((nt_max - d.nt) & ~1) : 0;
+      // largest; the start is a multiple of the group size so that every
+      // solve groups its own steps (0..G-1), (G..2G-1), ... whatever its
+      // size: the arithmetic of a solve never depends on the schedule or the
+      // batch
+      d.group = lu_group(split);
+      d.step0 = (c->sched & LSMSX_SCHED_RIGHT_ALIGN) ? ((nt_max - d.nt) & ~(d.group - 1)) : 0;
       d.perm = c->perm.p + (long)b * nt_max * kTile;
       d.info = c->info.p + b;
       d.minpiv = c->minpiv.p + b;
@@ -567,8 +593,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       {
         ProfScope p(c, 0);
         hipLaunchKernelGGL(k_pad_identity, dim3(n), dim3(256), 0, st, dd);
-        hipLaunchKernelGGL(k_assemble, dim3(mliz, n), dim3(256), 0, st, dd, c->tmat.p, c->ldT,
-                           c->cgnt.p, c->illp.p, c->lmax_tab);
+        launch_assemble(c, nsc, dim3(mliz, n), st, dd);
       }
       if (int r = check_launch("k_assemble")) return r;
       if (refined64) {
@@ -877,6 +902,7 @@ int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       d.Ns = Ns;
       d.pad = Ns - N;
       d.nt = Ns / kTile;
+      d.group = lu_group(false);
       d.full_lu = 1;
       d.inv = c->inv.p + ioff;
       d.inv_stride = kTile * kTile;
@@ -898,8 +924,7 @@ int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     {
       ProfScope p(c, 0);
       hipLaunchKernelGGL(k_pad_identity, dim3(nb), dim3(256), 0, c->stream, c->desc.p);
-      hipLaunchKernelGGL(k_assemble, dim3(maxliz, nb), dim3(256), 0, c->stream, c->desc.p,
-                         c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
+      launch_assemble(c, nsc, dim3(maxliz, nb), c->stream, c->desc.p);
     }
     if (int r = check_launch("k_assemble")) return r;
     if (int r = launch_inverse(c, c->desc.p, nb, nt_max)) return r;
@@ -1211,8 +1236,7 @@ int lsmsx_build_kkr(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double 
   HIPCHK(hipMemcpyAsync(c->desc.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
   {
     ProfScope p(c, 0);
-    hipLaunchKernelGGL(k_assemble, dim3(a.numLIZ, 1), dim3(256), 0, c->stream, c->desc.p,
-                       c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
+    launch_assemble(c, nsc, dim3(a.numLIZ, 1), c->stream, c->desc.p);
   }
   if (int r = check_launch("k_assemble")) return r;
   HIPCHK(hipMemcpy2DAsync(m, sizeof(double2) * ldm, c->M.p, sizeof(double2) * N,
@@ -1264,6 +1288,7 @@ int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ld
   d.Ns = Ns;
   d.pad = Ns - n;
   d.nt = Ns / kTile;
+  d.group = lu_group(false);
   d.inv = c->inv.p;
   d.perm = c->perm.p;
   d.info = c->info.p;
@@ -1411,6 +1436,7 @@ int lsmsx_tau_full(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *
   d.Ns = Ns;
   d.pad = Ns - N;
   d.nt = nt;
+  d.group = lu_group(false);
   d.full_lu = 1;
   d.inv = c->inv.p;
   d.inv_stride = kTile * kTile;
@@ -1422,8 +1448,7 @@ int lsmsx_tau_full(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *
   {
     ProfScope p(c, 0);
     hipLaunchKernelGGL(k_pad_identity, dim3(1), dim3(256), 0, c->stream, c->desc.p);
-    hipLaunchKernelGGL(k_assemble, dim3(a.numLIZ, 1), dim3(256), 0, c->stream, c->desc.p,
-                       c->tmat.p, c->ldT, c->cgnt.p, c->illp.p, c->lmax_tab);
+    launch_assemble(c, nsc, dim3(a.numLIZ, 1), c->stream, c->desc.p);
   }
   if (int r = check_launch("k_assemble")) return r;
   if (int r = launch_inverse(c, c->desc.p, 1, nt)) return r;

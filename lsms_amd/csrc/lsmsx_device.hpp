@@ -48,6 +48,7 @@ struct SolveDesc {
   int pad;                // identity rows/cols at the front
   int nt;                 // Ns / kTile (solver layout)
   int step0;              // global step at which this solve's elimination starts
+  int group;              // elimination steps per trailing update (2, or 4 with split planes)
   // factorization state
   void *inv;              // per diagonal tile: L^-1 (strict lower) and U^-1 (upper), kTile^2
   long inv_stride;        // elements between the tiles of consecutive steps (0: one reused tile)
@@ -65,10 +66,10 @@ struct SolveDesc {
   double *dxn;            // ||dX_c||^2 of the previous sweep
   double *dxn_rho2;       // the contraction (||dX_c|| ratio)^2 observed at the previous sweep
   double *nrm;            // ||dX_c||^2, ||X_c||^2 of the current sweep
-  // split-FP16 trailing updates (mixed path): the FP32 panels of the two
-  // steps of a pair as hi/lo FP16 planes with one power-of-2 scale per tile
-  void *hp;               // (parity 2) x (L, U) x nt tiles x 4 planes x 64 x 64 halves
-  int *he;                // (parity 2) x (L, U) x nt tile exponents
+  // split-FP16 trailing updates (mixed path): the FP32 panels of the steps
+  // of a group as hi/lo FP16 planes with one power-of-2 scale per tile
+  void *hp;               // (step % group) x (L, U) x nt tiles x 4 planes x 64 x 64 halves
+  int *he;                // (step % group) x (L, U) x nt tile exponents
   // explicit inverse (full tau, Lloyd term, adjoint of tau00)
   int full_lu;            // 1: the last tile is factored by k_lu_diag like every other tile
   double2 *Minv;          // Ns x Ns (leading dim ld): M^-1 in the solver layout

@@ -129,59 +129,95 @@ __device__ inline void wave_argmax(T &v, int &idx) {
 // so the accumulator of a lane covers consecutive rows i of C and every C and
 // A access is a run of 16 consecutive rows.  Operands are read straight from
 // L2/LDS into registers in MFMA layout (a-operand lane value B(k0+kq, j0+li),
-// b-operand lane value A(i0+li, k0+kq)), one k-step ahead; no LDS staging and
-// no barriers, so the waves of a CU run independently.
+// b-operand lane value A(i0+li, k0+kq)), loaded ahead of the MFMAs; no LDS
+// staging and no barriers, so the waves of a CU run independently.
 template <class T>
 struct Acc3 {
   Acc4<T> p[3][2][2];
 };
 
-template <class T, bool NEGA, class FA, class FB>
+// One k-step (12 MFMAs) of the 3M product from operands already in registers.
+template <class T, bool NEGA>
+__device__ __forceinline__ void gemm3m_step(CxT<T> (&av)[2], CxT<T> (&bv)[2], Acc3<T> &q) {
+  T as[2], bs[2];
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    if (NEGA) {
+      av[t].x = -av[t].x;
+      av[t].y = -av[t].y;
+    }
+    as[t] = av[t].x + av[t].y;
+    bs[t] = bv[t].x + bv[t].y;
+  }
+#pragma unroll
+  for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) q.p[0][bj][bi] = mfma4(bv[bj].x, av[bi].x, q.p[0][bj][bi]);
+#pragma unroll
+  for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) q.p[1][bj][bi] = mfma4(bv[bj].y, av[bi].y, q.p[1][bj][bi]);
+#pragma unroll
+  for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) q.p[2][bj][bi] = mfma4(bs[bj], as[bi], q.p[2][bj][bi]);
+}
+
+// Operand loads run PF k-steps ahead of the MFMAs: one step (768 MFMA
+// cycles) covers an L2 round trip in FP64; FP32 steps (384 cycles) with both
+// operands in L2 load three steps ahead (callers whose occupancy matters more
+// pass PF = 1).
+template <class T, bool NEGA, int PF = (sizeof(T) == 4 ? 3 : 1), class FA, class FB>
 __device__ __forceinline__ void wave_gemm3m(FA fa, FB fb, Acc3<T> &q) {
   typedef CxT<T> V;
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
-  V a[2], b[2], an[2], bn[2];
+  if constexpr (PF == 1) {
+    V a[2], b[2], an[2], bn[2];
 #pragma unroll
-  for (int t = 0; t < 2; t++) {
-    a[t] = fa(16 * t + li, kq);
-    b[t] = fb(kq, 16 * t + li);
-  }
+    for (int t = 0; t < 2; t++) {
+      a[t] = fa(16 * t + li, kq);
+      b[t] = fb(kq, 16 * t + li);
+    }
 #pragma unroll 2
-  for (int k0 = 0; k0 < kTile; k0 += 4) {
-    if (k0 + 4 < kTile) {
+    for (int k0 = 0; k0 < kTile; k0 += 4) {
+      if (k0 + 4 < kTile) {
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          an[t] = fa(16 * t + li, k0 + 4 + kq);
+          bn[t] = fb(k0 + 4 + kq, 16 * t + li);
+        }
+      }
+      gemm3m_step<T, NEGA>(a, b, q);
 #pragma unroll
       for (int t = 0; t < 2; t++) {
-        an[t] = fa(16 * t + li, k0 + 4 + kq);
-        bn[t] = fb(k0 + 4 + kq, 16 * t + li);
+        a[t] = an[t];
+        b[t] = bn[t];
       }
     }
-    T as[2], bs[2];
+    return;
+  }
+  constexpr int NS = kTile / 4;
+  V a[PF + 1][2], b[PF + 1][2];
+#pragma unroll
+  for (int p = 0; p < PF; p++)
 #pragma unroll
     for (int t = 0; t < 2; t++) {
-      if (NEGA) {
-        a[t].x = -a[t].x;
-        a[t].y = -a[t].y;
+      a[p][t] = fa(16 * t + li, 4 * p + kq);
+      b[p][t] = fb(4 * p + kq, 16 * t + li);
+    }
+#pragma unroll
+  for (int ks = 0; ks < NS; ks++) {
+    if (ks + PF < NS) {
+      const int p = (ks + PF) % (PF + 1);
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        a[p][t] = fa(16 * t + li, 4 * (ks + PF) + kq);
+        b[p][t] = fb(4 * (ks + PF) + kq, 16 * t + li);
       }
-      as[t] = a[t].x + a[t].y;
-      bs[t] = b[t].x + b[t].y;
     }
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++) q.p[0][bj][bi] = mfma4(b[bj].x, a[bi].x, q.p[0][bj][bi]);
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++) q.p[1][bj][bi] = mfma4(b[bj].y, a[bi].y, q.p[1][bj][bi]);
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++) q.p[2][bj][bi] = mfma4(bs[bj], as[bi], q.p[2][bj][bi]);
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-      a[t] = an[t];
-      b[t] = bn[t];
-    }
+    const int c = ks % (PF + 1);
+    V av[2] = {a[c][0], a[c][1]}, bv[2] = {b[c][0], b[c][1]};
+    gemm3m_step<T, NEGA>(av, bv, q);
   }
 }
 

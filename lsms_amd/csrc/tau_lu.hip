@@ -13,7 +13,8 @@
 //   k_lu_diag    factor A_ss = P^T L U in LDS; store L^-1, U^-1 and P
 //   k_lu_panel   L_is = A_is U^-1 (tiles below), U_sj = L^-1 P A_sj (right)
 //   k_lu_update  A_ij -= L_is U_sj                     <- MFMA, dominant
-// then k_lu_last factors the last tile with the pivots of its non-central
+// with the steps taken in groups (k_lu_cross, k_lu_trail: the trailing
+// matrix is read and written once per group); then k_lu_last factors the last tile with the pivots of its non-central
 // rows and of the central rows kept apart.  In FP64 it also solves
 // S tau00 = t0 with the Schur complement S (the plain path); in FP32 it
 // stores the last tile's factors for the refinement solves (mixed path).
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
   zero_acc(q);
   if (lower) {
     // A(i,k) = Ts(wr+i, k); B(k,j) = U^-1(k, wc+j), upper incl. diagonal
-    wave_gemm3m<T, false>([&](int i, int k) { return Ts[k][wr + i]; },
+    wave_gemm3m<T, false, 1>([&](int i, int k) { return Ts[k][wr + i]; },
                           [&](int k, int j) {
                             int jj = wc + j;
                             V v = czero<V>();
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
                           q);
   } else {
     // A(i,k) = L^-1(wr+i, k), unit lower; B(k,j) = (P A_sj)(k, wc+j)
-    wave_gemm3m<T, false>([&](int i, int k) {
+    wave_gemm3m<T, false, 1>([&](int i, int k) {
                             int ii = wr + i;
                             V v = czero<V>();
                             if (ii == k) v.x = T(1);
@@ -310,8 +311,8 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
       }
   if constexpr (sizeof(T) == 4) {
     if (planes) {
-      // split-FP16 planes of this panel tile for k_lu_update2_h
-      const int parity = step & 1, side = lower ? 0 : 1;
+      // split-FP16 planes of this panel tile for k_lu_trail_h
+      const int parity = step % d.group, side = lower ? 0 : 1;
       const int t = step + 1 + (lower ? x : x - nrest);
       write_split_planes(Ts, lower, (_Float16 *)d.hp + (long)((parity * 2 + side) * d.nt + t) * kPlaneHalves,
                          d.he + (parity * 2 + side) * d.nt + t);
@@ -378,38 +379,49 @@ __global__ __launch_bounds__(256) void k_lu_update(const SolveDesc *__restrict__
   update_tile<T>(d, step + 1 + x % nrest, step + 1 + x / nrest, step, 1);
 }
 
-// Two-level elimination, steps a = s and b = s + 1 (global step s).
-// Cross update: after the panels of step a, bring tile column b (rows >= b)
-// and tile row b (columns > b) up to date with step a, so that step b can be
-// factored before the rest of the trailing matrix is touched.
-template <class T>
-__global__ __launch_bounds__(256) void k_lu_cross(const SolveDesc *__restrict__ descs, int step) {
-  const SolveDesc &d = descs[blockIdx.y];
-  const int a = step - d.step0, b = a + 1;
-  if (a < 0 || a > d.nt - 2) return;
-  if (d.conv && *d.conv) return;
-  const int n1 = d.nt - b, x = blockIdx.x;
-  if (x >= 2 * n1 - 1) return;
-  if (x < n1) update_tile<T>(d, b + x, b, a, 1);
-  else update_tile<T>(d, b, b + 1 + (x - n1), a, 1);
+// Grouped elimination: the steps g .. g+ns-1 of a group (ns = min(group,
+// nt-1-g); step nt-1 is k_lu_last) are factored one after another, each
+// after a cross update that brings its tile column (rows >= t) and tile row
+// (columns > t) up to date with the group's earlier steps; then one trailing
+// update applies all ns steps to the tiles beyond the group.
+// Number of steps of the group starting at solve step g.
+__device__ __forceinline__ int group_steps(const SolveDesc &d, int g) {
+  return min(d.group, d.nt - 1 - g);
 }
 
-// Trailing update of the step pair: A_ij -= L_ia U_aj + L_ib U_bj for all
-// i, j > b, one pass over C with K = 128 (K = 64 when step a does not exist
-// for this solve, i.e. its elimination starts at b).
+// Cross update for step t = g + u of the group starting at global step s:
+// tile column t (rows >= t) and tile row t (columns > t) -= the products of
+// steps g .. t-1.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_update2(const SolveDesc *__restrict__ descs, int step,
-                                                    int xcd_swizzle) {
+__global__ __launch_bounds__(256) void k_lu_cross(const SolveDesc *__restrict__ descs, int step,
+                                                  int u) {
+  const SolveDesc &d = descs[blockIdx.y];
+  const int g = step - d.step0, t = g + u;
+  if (g < 0 || u >= group_steps(d, g)) return;
+  if (d.conv && *d.conv) return;
+  const int n1 = d.nt - t, x = blockIdx.x;
+  if (x >= 2 * n1 - 1) return;
+  if (x < n1) update_tile<T>(d, t + x, t, g, u);
+  else update_tile<T>(d, t, t + 1 + (x - n1), g, u);
+}
+
+// Trailing update of the group starting at global step s: A_ij -= sum over
+// the group's steps k of L_ik U_kj for all i, j beyond the group, one pass
+// over C with K = 64 ns.
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_trail(const SolveDesc *__restrict__ descs, int step,
+                                                  int xcd_swizzle) {
   int bx = blockIdx.x, by = blockIdx.y;
   if (xcd_swizzle) xcd_remap(bx, by);
   const SolveDesc &d = descs[by];
-  const int a = step - d.step0, b = a + 1;
-  if (b < 0 || b > d.nt - 2) return;
+  const int g = step - d.step0;
+  if (g < 0) return;
+  const int ns = group_steps(d, g);
+  if (ns <= 0) return;
   if (d.conv && *d.conv) return;
-  const int nrest = d.nt - 1 - b, x = bx;
+  const int last = g + ns - 1, nrest = d.nt - 1 - last, x = bx;
   if (x >= nrest * nrest) return;
-  const bool has_a = a >= 0;
-  update_tile<T>(d, b + 1 + x % nrest, b + 1 + x / nrest, has_a ? a : b, has_a ? 2 : 1);
+  update_tile<T>(d, last + 1 + x % nrest, last + 1 + x / nrest, g, ns);
 }
 
 // Last tile (step nt-1): factor it with the non-central rows (first nd =
@@ -512,10 +524,10 @@ template __global__ void k_lu_panel<double>(const SolveDesc *, int);
 template __global__ void k_lu_panel<float>(const SolveDesc *, int);
 template __global__ void k_lu_update<double>(const SolveDesc *, int, int);
 template __global__ void k_lu_update<float>(const SolveDesc *, int, int);
-template __global__ void k_lu_cross<double>(const SolveDesc *, int);
-template __global__ void k_lu_cross<float>(const SolveDesc *, int);
-template __global__ void k_lu_update2<double>(const SolveDesc *, int, int);
-template __global__ void k_lu_update2<float>(const SolveDesc *, int, int);
+template __global__ void k_lu_cross<double>(const SolveDesc *, int, int);
+template __global__ void k_lu_cross<float>(const SolveDesc *, int, int);
+template __global__ void k_lu_trail<double>(const SolveDesc *, int, int);
+template __global__ void k_lu_trail<float>(const SolveDesc *, int, int);
 template __global__ void k_lu_last<double>(const SolveDesc *);
 template __global__ void k_lu_last<float>(const SolveDesc *);
 

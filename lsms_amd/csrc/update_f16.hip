@@ -1,9 +1,10 @@
 // update_f16.hip -- split-FP16 trailing update of the mixed-precision LU.
 //
 // The mixed path factors an FP32 copy of the KKR matrix (tau_lu.hip, T =
-// float) and refines against FP64.  Its dominant kernel is the paired-step
-// trailing update C -= L_a U_a + L_b U_b.  Here the products run on the FP16
-// matrix cores (v_mfma_f32_16x16x32_f16, FP32 accumulation): every FP32
+// float) and refines against FP64.  Its dominant kernel is the trailing
+// update of a group of four elimination steps, C -= sum_k L_k U_k (K = 256:
+// C is read and written once per four steps).  Here the products run on the
+// FP16 matrix cores (v_mfma_f32_16x16x32_f16, FP32 accumulation): every FP32
 // panel entry was split by the panel kernel into x 2^-e = hi + lo (FP16 each,
 // one power-of-2 scale e per 64x64 tile), and each real product is
 //   x y ~ (hx hy + hx ly + lx hy) 2^(ex + ey)
@@ -17,16 +18,16 @@
 // chunk the split planes of the two L row tiles and of the U column tile
 // are staged in LDS (48 KiB), so every plane element is fetched from L2 once
 // per workgroup and read by two waves; the next chunk's loads are issued
-// before this chunk's MFMAs, so the L2 latency hides behind them; the four 16-byte k groups of a
-// 64-byte LDS row are XOR-swizzled by the row, so the 16 lanes of a 128-bit
-// read hit distinct banks.  Complex products in 4M form
+// before this chunk's MFMAs, so the L2 latency hides behind them.  The four
+// 16-byte k groups of a 64-byte LDS row are XOR-swizzled by the row, so the
+// 16 lanes of a 128-bit read hit distinct banks.  Complex products in 4M form
 // (Cr += (-Ar) Br + Ai Bi, Ci += (-Ar) Bi + (-Ai) Br) with two FP32
 // accumulator sets.  The MFMA computes D = C^T = U^T L^T: the a-operand is
 // a U plane (lane l: column l%16, k = 8(l/16)..+7), the b-operand an L plane
 // (lane l: row l%16, same k), and accumulator register r of lane l is
 // C(row l%16, column 4(l/16) + r) (layout measured on gfx950), so C moves
 // in 128 B runs.  C is scaled by the tile exponents on entry, between the
-// two steps and on exit (exact powers of two).
+// steps and on exit (exact powers of two).
 //
 // Built without -amdgpu-mfma-vgpr-form: the 64 accumulators per lane live
 // in AGPRs.
@@ -54,18 +55,25 @@ __device__ __forceinline__ h8 neg8(h8 v) {
 // four groups of a 64-byte row by bits 2-3 of the row index.
 __device__ __forceinline__ int swz_group(int r, int g) { return g ^ ((r >> 2) & 3); }
 
-// Paired-step trailing update (see k_lu_update2 in tau_lu.hip for the tile
-// ranges) on the split-FP16 planes of steps a and b.
-__global__ __launch_bounds__(256) void k_lu_update2_h(const SolveDesc *__restrict__ descs, int step,
-                                                      int xcd_swizzle) {
+// Steps per group on this path (the descriptors' group; a power of two).
+constexpr int kGroupH = 4;
+
+// Trailing update of the step group starting at global step `step` (see
+// k_lu_trail in tau_lu.hip for the tile ranges) on the split-FP16 planes of
+// the group's steps.
+__global__ __launch_bounds__(256) void k_lu_trail_h(const SolveDesc *__restrict__ descs, int step,
+                                                    int xcd_swizzle) {
   __shared__ h8 Ls[2][4][kTile][4];
   __shared__ h8 Us[4][kTile][4];
   int bx = blockIdx.x, by = blockIdx.y;
   if (xcd_swizzle) xcd_remap(bx, by);
   const SolveDesc &d = descs[by];
-  const int a = step - d.step0, b = a + 1;
-  if (b < 0 || b > d.nt - 2) return;
+  const int g0 = step - d.step0;
+  if (g0 < 0) return;
+  const int ns = min(kGroupH, d.nt - 1 - g0);
+  if (ns <= 0) return;
   if (d.conv && *d.conv) return;
+  const int b = g0 + ns - 1;  // last step of the group
   const int nrest = d.nt - 1 - b, nsr = (nrest + 1) / 2;
   if (bx >= nsr * nrest) return;
   const int sr = bx % nsr, sc = bx / nsr;
@@ -76,10 +84,9 @@ __global__ __launch_bounds__(256) void k_lu_update2_h(const SolveDesc *__restric
   const int nt = d.nt;
   const long ld = tile_ld(d);
   const _Float16 *hp = (const _Float16 *)d.hp;
-  const int ns = a >= 0 ? 2 : 1;
-  const int s_first = a >= 0 ? a : b;
+  const int s_first = g0;
   auto tile_exp = [&](int s) {
-    const int par = s & 1;
+    const int par = s & (kGroupH - 1);
     return d.he[(par * 2 + 0) * nt + ti] + d.he[(par * 2 + 1) * nt + tj];
   };
   GF2 *C = tile_ptr<float>(d, valid ? ti : b + 1, tj) + ld * wc;
@@ -89,7 +96,7 @@ __global__ __launch_bounds__(256) void k_lu_update2_h(const SolveDesc *__restric
   // Row tiles past the end are clamped to a real tile (their waves skip the
   // MFMAs).
   auto chunk_src = [&](int ch, int q) -> GH8 * {
-    const int s = s_first + (ch >> 1), kh = ch & 1, par = s & 1;
+    const int s = s_first + (ch >> 1), kh = ch & 1, par = s & (kGroupH - 1);
     const int e8 = tid + 256 * q;
     if (q < 8) {
       const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
@@ -129,9 +136,10 @@ __global__ __launch_bounds__(256) void k_lu_update2_h(const SolveDesc *__restric
         }
   }
   const int nch = 2 * ns;
+#pragma unroll 1
   for (int ch = 0; ch < nch; ch++) {
-    if (ch == 2 && valid) {  // second step of the pair: its tile exponents
-      const int e2 = tile_exp(s_first + 1);
+    if (ch > 0 && !(ch & 1) && valid) {  // next step of the group: its tile exponents
+      const int e2 = tile_exp(s_first + (ch >> 1));
       const float sc1 = ldexpf(1.f, e - e2);
 #pragma unroll
       for (int bi = 0; bi < 4; bi++)
