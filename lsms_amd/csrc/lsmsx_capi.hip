@@ -66,6 +66,7 @@ __global__ void k_grad_reduce(const double2 *, int, const int *, const long *, c
                               const int *, double2 *, long, int);
 __global__ void k_zero_c(double2 *, long);
 __global__ void k_lu_trail_h(const SolveDesc *, int, int);
+__global__ void k_lu_cross_h(const SolveDesc *, int, int);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout.
 __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
@@ -319,8 +320,12 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
     for (int u = 1; u < G && s + u < nt_max - 1; u++) {
       {
         ProfScope p(c, 9);
-        hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * (nt_max - s - u) - 1, nb), dim3(256), 0, st,
-                           dd, s, u);
+        const int n1 = nt_max - s - u;
+        if (split_f16)
+          hipLaunchKernelGGL(k_lu_cross_h, dim3((n1 + 1) / 2 + n1 - 1, nb), dim3(256), 0, st, dd,
+                             s, u);
+        else
+          hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * n1 - 1, nb), dim3(256), 0, st, dd, s, u);
       }
       if (int r = check_launch("k_lu_cross")) return r;
       if (int r = diag_panel(s + u)) return r;

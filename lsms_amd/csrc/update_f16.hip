@@ -58,38 +58,24 @@ __device__ __forceinline__ int swz_group(int r, int g) { return g ^ ((r >> 2) & 
 // Steps per group on this path (the descriptors' group; a power of two).
 constexpr int kGroupH = 4;
 
-// Trailing update of the step group starting at global step `step` (see
-// k_lu_trail in tau_lu.hip for the tile ranges) on the split-FP16 planes of
-// the group's steps.
-__global__ __launch_bounds__(256) void k_lu_trail_h(const SolveDesc *__restrict__ descs, int step,
-                                                    int xcd_swizzle) {
-  __shared__ h8 Ls[2][4][kTile][4];
-  __shared__ h8 Us[4][kTile][4];
-  int bx = blockIdx.x, by = blockIdx.y;
-  if (xcd_swizzle) xcd_remap(bx, by);
-  const SolveDesc &d = descs[by];
-  const int g0 = step - d.step0;
-  if (g0 < 0) return;
-  const int ns = min(kGroupH, d.nt - 1 - g0);
-  if (ns <= 0) return;
-  if (d.conv && *d.conv) return;
-  const int b = g0 + ns - 1;  // last step of the group
-  const int nrest = d.nt - 1 - b, nsr = (nrest + 1) / 2;
-  if (bx >= nsr * nrest) return;
-  const int sr = bx % nsr, sc = bx / nsr;
+// C tiles (r0 + wt, tj), wt < nrow <= 2 (a 128 x 64 region; row tiles past
+// nrow are skipped), -= sum over the steps s_first .. s_first+ns-1 of
+// L(., s) U(s, tj), on the split-FP16 planes of those steps.
+__device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, int tj,
+                                         int s_first, int ns, h8 (&Ls)[2][4][kTile][4],
+                                         h8 (&Us)[4][kTile][4]) {
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
   const int wt = w >> 1, wc = 32 * (w & 1);
-  const int ti = b + 1 + 2 * sr + wt, tj = b + 1 + sc;
-  const bool valid = 2 * sr + wt < nrest;
+  const int ti = r0 + wt;
+  const bool valid = wt < nrow;
   const int nt = d.nt;
   const long ld = tile_ld(d);
   const _Float16 *hp = (const _Float16 *)d.hp;
-  const int s_first = g0;
   auto tile_exp = [&](int s) {
     const int par = s & (kGroupH - 1);
     return d.he[(par * 2 + 0) * nt + ti] + d.he[(par * 2 + 1) * nt + tj];
   };
-  GF2 *C = tile_ptr<float>(d, valid ? ti : b + 1, tj) + ld * wc;
+  GF2 *C = tile_ptr<float>(d, valid ? ti : r0, tj) + ld * wc;
   // Chunk ch = (step u = ch / 2, k half ch % 2): 12 loads of 8 halves per
   // thread (q < 8: the two L row tiles, q >= 8: the U column tile), all issued
   // together; the next chunk's loads are in flight during this chunk's MFMAs.
@@ -100,7 +86,7 @@ __global__ __launch_bounds__(256) void k_lu_trail_h(const SolveDesc *__restrict_
     const int e8 = tid + 256 * q;
     if (q < 8) {
       const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
-      const int t = min(b + 1 + 2 * sr + tt, nt - 1);
+      const int t = min(r0 + tt, nt - 1);
       return (GH8 *)(hp + (long)((par * 2 + 0) * nt + t) * kPlaneHalves) +
              ((plane * kTile + rr) * kTile + 32 * kh + 8 * grp) / 8;
     }
@@ -206,6 +192,46 @@ __global__ __launch_bounds__(256) void k_lu_trail_h(const SolveDesc *__restrict_
           C[(16 * bi + l16) + ld * (16 * bj + 4 * g + r)] =
               make_float2(cr[bi][bj][r] * sc2, ci[bi][bj][r] * sc2);
   }
+}
+
+// Trailing update of the step group starting at global step `step` (see
+// k_lu_trail in tau_lu.hip for the tile ranges) on the split-FP16 planes of
+// the group's steps.
+__global__ __launch_bounds__(256) void k_lu_trail_h(const SolveDesc *__restrict__ descs, int step,
+                                                    int xcd_swizzle) {
+  __shared__ h8 Ls[2][4][kTile][4];
+  __shared__ h8 Us[4][kTile][4];
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (xcd_swizzle) xcd_remap(bx, by);
+  const SolveDesc &d = descs[by];
+  const int g0 = step - d.step0;
+  if (g0 < 0) return;
+  const int ns = min(kGroupH, d.nt - 1 - g0);
+  if (ns <= 0) return;
+  if (d.conv && *d.conv) return;
+  const int b = g0 + ns - 1;  // last step of the group
+  const int nrest = d.nt - 1 - b, nsr = (nrest + 1) / 2;
+  if (bx >= nsr * nrest) return;
+  const int sr = bx % nsr, sc = bx / nsr;
+  region_h(d, b + 1 + 2 * sr, min(2, nrest - 2 * sr), b + 1 + sc, g0, ns, Ls, Us);
+}
+
+// Cross update for step t = g + u of the group starting at global step
+// `step` (see k_lu_cross in tau_lu.hip) on the split planes of steps
+// g .. t-1: tile column t (rows >= t, two row tiles per workgroup), then
+// tile row t (columns > t, one tile per workgroup).
+__global__ __launch_bounds__(256) void k_lu_cross_h(const SolveDesc *__restrict__ descs, int step,
+                                                    int u) {
+  __shared__ h8 Ls[2][4][kTile][4];
+  __shared__ h8 Us[4][kTile][4];
+  const SolveDesc &d = descs[blockIdx.y];
+  const int g0 = step - d.step0, t = g0 + u;
+  if (g0 < 0 || u >= min(kGroupH, d.nt - 1 - g0)) return;
+  if (d.conv && *d.conv) return;
+  const int n1 = d.nt - t, nc = (n1 + 1) / 2, x = blockIdx.x;
+  if (x >= nc + n1 - 1) return;
+  if (x < nc) region_h(d, t + 2 * x, min(2, n1 - 2 * x), t, g0, u, Ls, Us);
+  else region_h(d, t, 1, t + 1 + (x - nc), g0, u, Ls, Us);
 }
 
 }  // namespace lsmsx
