@@ -212,7 +212,14 @@ __global__ __launch_bounds__(256) void k_lu_trail_h(const SolveDesc *__restrict_
   const int b = g0 + ns - 1;  // last step of the group
   const int nrest = d.nt - 1 - b, nsr = (nrest + 1) / 2;
   if (bx >= nsr * nrest) return;
-  const int sr = bx % nsr, sc = bx / nsr;
+  // Regions in super-blocks of kBlkR row pairs x kBlkC columns, block by
+  // block (columns of blocks within a block row): the workgroups resident on
+  // an XCD at one time share the planes of about 16 row tiles and 8 column
+  // tiles, which stay in its L2, instead of sweeping all row tiles per column.
+  constexpr int kBlkR = 8, kBlkC = 8;
+  const int br = bx / (kBlkR * nrest), hb = min(kBlkR, nsr - br * kBlkR);
+  const int off = bx - br * kBlkR * nrest, bc = off / (hb * kBlkC), off2 = off - bc * hb * kBlkC;
+  const int sr = br * kBlkR + off2 % hb, sc = bc * kBlkC + off2 / hb;
   region_h(d, b + 1 + 2 * sr, min(2, nrest - 2 * sr), b + 1 + sc, g0, ns, Ls, Us);
 }
 
