@@ -335,8 +335,8 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
     if (nrest > 0) {
       ProfScope p(c, 3);
       if (split_f16)
-        hipLaunchKernelGGL(k_lu_trail_h, dim3(((nrest + 1) / 2) * nrest, nb), dim3(256), 0, st,
-                           dd, s, swz);
+        hipLaunchKernelGGL(k_lu_trail_h, dim3(((nrest + 1) / 2) * ((nrest + 1) / 2), nb),
+                           dim3(512), 0, st, dd, s, swz);
       else
         hipLaunchKernelGGL(k_lu_trail<T>, dim3(nrest * nrest, nb), dim3(256), 0, st, dd, s,
                            swz);

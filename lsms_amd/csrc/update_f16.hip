@@ -37,6 +37,9 @@ namespace lsmsx {
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef GCx<float> GF2;
+// C moves with non-temporal loads and stores: it is touched once per group,
+// and keeping it out of L2 leaves room for the split planes, which are re-read.
+typedef float fv2 __attribute__((ext_vector_type(2)));
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef __attribute__((address_space(1))) const h8 GH8;
 #else
@@ -58,16 +61,27 @@ __device__ __forceinline__ int swz_group(int r, int g) { return g ^ ((r >> 2) & 
 // Steps per group on this path (the descriptors' group; a power of two).
 constexpr int kGroupH = 4;
 
-// C tiles (r0 + wt, tj), wt < nrow <= 2 (a 128 x 64 region; row tiles past
-// nrow are skipped), -= sum over the steps s_first .. s_first+ns-1 of
-// L(., s) U(s, tj), on the split-FP16 planes of those steps.
-__device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, int tj,
-                                         int s_first, int ns, h8 (&Ls)[2][4][kTile][4],
-                                         h8 (&Us)[4][kTile][4]) {
+// C tiles (r0 + wt, c0 + ct), wt < nrow <= 2, ct < ncol <= NC (a 128 x 64 NC
+// region of 256 NC threads; tiles past nrow / ncol are skipped), -= sum over
+// the steps s_first .. s_first+ns-1 of L(., s) U(s, .), on the split-FP16
+// planes of those steps.  Waves: wt = row tile, ct = column tile, wc = the
+// 32-column half.  Per 32-wide k chunk the planes of the two L row tiles and
+// the NC U column tiles are staged in LDS, (2 + NC) x 16 KiB, so a chunk's
+// staging serves 2 x NC output tiles.  DB = 1: one LDS stage, the next
+// chunk's loads in flight during this chunk's MFMAs, two barriers per chunk.
+// DB = 2: two LDS stages and two register stages, loads issued two chunks
+// ahead, one barrier per chunk.
+template <int NC, int DB>
+__device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, int c0, int ncol,
+                                         int s_first, int ns, h8 (&Ls)[DB][2][4][kTile][4],
+                                         h8 (&Us)[DB][NC][4][kTile][4]) {
+  constexpr int NT = 256 * NC;                 // threads
+  constexpr int NQ = (2 + NC) * 1024 / NT;     // 8-half loads per thread per chunk
+  constexpr int NQL = 2048 / NT;               // of which L planes
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
-  const int wt = w >> 1, wc = 32 * (w & 1);
-  const int ti = r0 + wt;
-  const bool valid = wt < nrow;
+  const int wt = w / (2 * NC), ct = (w >> 1) % NC, wc = 32 * (w & 1);
+  const int ti = r0 + wt, tj = c0 + ct;
+  const bool valid = wt < nrow && ct < ncol;
   const int nt = d.nt;
   const long ld = tile_ld(d);
   const _Float16 *hp = (const _Float16 *)d.hp;
@@ -75,56 +89,39 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
     const int par = s & (kGroupH - 1);
     return d.he[(par * 2 + 0) * nt + ti] + d.he[(par * 2 + 1) * nt + tj];
   };
-  GF2 *C = tile_ptr<float>(d, valid ? ti : r0, tj) + ld * wc;
-  // Chunk ch = (step u = ch / 2, k half ch % 2): 12 loads of 8 halves per
-  // thread (q < 8: the two L row tiles, q >= 8: the U column tile), all issued
-  // together; the next chunk's loads are in flight during this chunk's MFMAs.
-  // Row tiles past the end are clamped to a real tile (their waves skip the
-  // MFMAs).
+  GF2 *C = tile_ptr<float>(d, valid ? ti : r0, valid ? tj : c0) + ld * wc;
+  // Chunk ch = (step u = ch / 2, k half ch % 2): NQ loads of 8 halves per
+  // thread (q < NQL: the two L row tiles, then the U column tiles), all
+  // issued together.  Tiles past the end are clamped to a real tile (their
+  // waves skip the MFMAs).
   auto chunk_src = [&](int ch, int q) -> GH8 * {
     const int s = s_first + (ch >> 1), kh = ch & 1, par = s & (kGroupH - 1);
-    const int e8 = tid + 256 * q;
-    if (q < 8) {
+    const int e8 = tid + NT * q;
+    if (q < NQL) {
       const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
       const int t = min(r0 + tt, nt - 1);
       return (GH8 *)(hp + (long)((par * 2 + 0) * nt + t) * kPlaneHalves) +
              ((plane * kTile + rr) * kTile + 32 * kh + 8 * grp) / 8;
     }
-    const int f = e8 - 2048, plane = f >> 8, rr = (f >> 2) & 63, grp = f & 3;
-    return (GH8 *)(hp + (long)((par * 2 + 1) * nt + tj) * kPlaneHalves) +
+    const int f = e8 - 2048, cu = f >> 10, plane = (f >> 8) & 3, rr = (f >> 2) & 63, grp = f & 3;
+    const int t = min(c0 + cu, nt - 1);
+    return (GH8 *)(hp + (long)((par * 2 + 1) * nt + t) * kPlaneHalves) +
            ((plane * kTile + rr) * kTile + 32 * kh + 8 * grp) / 8;
   };
-  auto chunk_dst = [&](int q) -> h8 * {
-    const int e8 = tid + 256 * q;
-    if (q < 8) {
+  auto chunk_dst = [&](int buf, int q) -> h8 * {
+    const int e8 = tid + NT * q;
+    if (q < NQL) {
       const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
-      return &Ls[tt][plane][rr][swz_group(rr, grp)];
+      return &Ls[buf][tt][plane][rr][swz_group(rr, grp)];
     }
-    const int f = e8 - 2048, plane = f >> 8, rr = (f >> 2) & 63, grp = f & 3;
-    return &Us[plane][rr][swz_group(rr, grp)];
+    const int f = e8 - 2048, cu = f >> 10, plane = (f >> 8) & 3, rr = (f >> 2) & 63, grp = f & 3;
+    return &Us[buf][cu][plane][rr][swz_group(rr, grp)];
   };
-  h8 pv[12];
-#pragma unroll
-  for (int q = 0; q < 12; q++) pv[q] = *chunk_src(0, q);
   f4 cr[4][2], ci[4][2];
   int e = valid ? tile_exp(s_first) : 0;
-  if (valid) {
-    const float sc0 = ldexpf(1.f, -e);
-#pragma unroll
-    for (int bi = 0; bi < 4; bi++)
-#pragma unroll
-      for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const float2 c = C[(16 * bi + l16) + ld * (16 * bj + 4 * g + r)];
-          cr[bi][bj][r] = c.x * sc0;
-          ci[bi][bj][r] = c.y * sc0;
-        }
-  }
-  const int nch = 2 * ns;
-#pragma unroll 1
-  for (int ch = 0; ch < nch; ch++) {
-    if (ch > 0 && !(ch & 1) && valid) {  // next step of the group: its tile exponents
+  // rescale C to the tile exponents of the step chunk ch starts
+  auto rescale = [&](int ch) __attribute__((always_inline)) {
+    if (ch > 0 && !(ch & 1) && valid) {
       const int e2 = tile_exp(s_first + (ch >> 1));
       const float sc1 = ldexpf(1.f, e - e2);
 #pragma unroll
@@ -136,49 +133,114 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
         }
       e = e2;
     }
-    __syncthreads();  // the previous chunk's readers are done
+  };
+  auto compute = [&](int buf) __attribute__((always_inline)) {
+    if (!valid) return;
+    h8 ur_h[2], ur_l[2], ui_h[2], ui_l[2];
 #pragma unroll
-    for (int q = 0; q < 12; q++) *chunk_dst(q) = pv[q];
-    __syncthreads();
-    if (ch + 1 < nch) {
-#pragma unroll
-      for (int q = 0; q < 12; q++) pv[q] = *chunk_src(ch + 1, q);
+    for (int bj = 0; bj < 2; bj++) {
+      const int col = wc + 16 * bj + l16, gg = swz_group(col, g);
+      ur_h[bj] = Us[buf][ct][0][col][gg];
+      ur_l[bj] = Us[buf][ct][1][col][gg];
+      ui_h[bj] = Us[buf][ct][2][col][gg];
+      ui_l[bj] = Us[buf][ct][3][col][gg];
     }
-    if (valid) {
-      h8 ur_h[2], ur_l[2], ui_h[2], ui_l[2];
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++) {
+      const int row = 16 * bi + l16, gg = swz_group(row, g);
+      const h8 nar_h = Ls[buf][wt][0][row][gg], nar_l = Ls[buf][wt][1][row][gg];
+      const h8 ai_h = Ls[buf][wt][2][row][gg], ai_l = Ls[buf][wt][3][row][gg];
+      const h8 nai_h = neg8(ai_h), nai_l = neg8(ai_l);
 #pragma unroll
       for (int bj = 0; bj < 2; bj++) {
-        const int col = wc + 16 * bj + l16, gg = swz_group(col, g);
-        ur_h[bj] = Us[0][col][gg];
-        ur_l[bj] = Us[1][col][gg];
-        ui_h[bj] = Us[2][col][gg];
-        ui_l[bj] = Us[3][col][gg];
+        f4 r = cr[bi][bj], m = ci[bi][bj];
+        r = mfma_h(ur_h[bj], nar_h, r);
+        r = mfma_h(ur_l[bj], nar_h, r);
+        r = mfma_h(ur_h[bj], nar_l, r);
+        r = mfma_h(ui_h[bj], ai_h, r);
+        r = mfma_h(ui_l[bj], ai_h, r);
+        r = mfma_h(ui_h[bj], ai_l, r);
+        m = mfma_h(ui_h[bj], nar_h, m);
+        m = mfma_h(ui_l[bj], nar_h, m);
+        m = mfma_h(ui_h[bj], nar_l, m);
+        m = mfma_h(ur_h[bj], nai_h, m);
+        m = mfma_h(ur_l[bj], nai_h, m);
+        m = mfma_h(ur_h[bj], nai_l, m);
+        cr[bi][bj] = r;
+        ci[bi][bj] = m;
       }
+    }
+  };
+  auto load_c = [&]() __attribute__((always_inline)) {
+    if (!valid) return;
+    const float sc0 = ldexpf(1.f, -e);
 #pragma unroll
-      for (int bi = 0; bi < 4; bi++) {
-        const int row = 16 * bi + l16, gg = swz_group(row, g);
-        const h8 nar_h = Ls[wt][0][row][gg], nar_l = Ls[wt][1][row][gg];
-        const h8 ai_h = Ls[wt][2][row][gg], ai_l = Ls[wt][3][row][gg];
-        const h8 nai_h = neg8(ai_h), nai_l = neg8(ai_l);
+    for (int bi = 0; bi < 4; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 2; bj++) {
-          f4 r = cr[bi][bj], m = ci[bi][bj];
-          r = mfma_h(ur_h[bj], nar_h, r);
-          r = mfma_h(ur_l[bj], nar_h, r);
-          r = mfma_h(ur_h[bj], nar_l, r);
-          r = mfma_h(ui_h[bj], ai_h, r);
-          r = mfma_h(ui_l[bj], ai_h, r);
-          r = mfma_h(ui_h[bj], ai_l, r);
-          m = mfma_h(ui_h[bj], nar_h, m);
-          m = mfma_h(ui_l[bj], nar_h, m);
-          m = mfma_h(ui_h[bj], nar_l, m);
-          m = mfma_h(ur_h[bj], nai_h, m);
-          m = mfma_h(ur_l[bj], nai_h, m);
-          m = mfma_h(ur_h[bj], nai_l, m);
-          cr[bi][bj] = r;
-          ci[bi][bj] = m;
+      for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const fv2 c = __builtin_nontemporal_load((const fv2 *)(C + (16 * bi + l16) + ld * (16 * bj + 4 * g + r)));
+          cr[bi][bj][r] = c.x * sc0;
+          ci[bi][bj][r] = c.y * sc0;
+        }
+  };
+  const int nch = 2 * ns;
+  if constexpr (DB == 1) {
+    h8 pv[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) pv[q] = *chunk_src(0, q);
+    load_c();
+#pragma unroll 1
+    for (int ch = 0; ch < nch; ch++) {
+      rescale(ch);
+      __syncthreads();  // the previous chunk's readers are done
+#pragma unroll
+      for (int q = 0; q < NQ; q++) *chunk_dst(0, q) = pv[q];
+      __syncthreads();
+      if (ch + 1 < nch) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) pv[q] = *chunk_src(ch + 1, q);
+      }
+      compute(0);
+    }
+  } else {
+    // chunk c is staged in LDS buffer c & 1 and loaded into pa (even c) or pb
+    // (odd c); nch is even
+    h8 pa[NQ], pb[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) pa[q] = *chunk_src(0, q);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) pb[q] = *chunk_src(1, q);
+    load_c();
+#pragma unroll
+    for (int q = 0; q < NQ; q++) *chunk_dst(0, q) = pa[q];
+    __syncthreads();
+    if (nch > 2) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) pa[q] = *chunk_src(2, q);
+    }
+#pragma unroll 1
+    for (int ch = 0; ch < nch; ch += 2) {
+      rescale(ch);
+      compute(0);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) *chunk_dst(1, q) = pb[q];
+      if (ch + 3 < nch) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) pb[q] = *chunk_src(ch + 3, q);
+      }
+      __syncthreads();  // buffer 1 staged; every wave is done with buffer 0
+      compute(1);
+      if (ch + 2 < nch) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) *chunk_dst(0, q) = pa[q];
+        if (ch + 4 < nch) {
+#pragma unroll
+          for (int q = 0; q < NQ; q++) pa[q] = *chunk_src(ch + 4, q);
         }
       }
+      __syncthreads();  // buffer 0 staged; every wave is done with buffer 1
     }
   }
   if (valid) {
@@ -189,18 +251,19 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
       for (int bj = 0; bj < 2; bj++)
 #pragma unroll
         for (int r = 0; r < 4; r++)
-          C[(16 * bi + l16) + ld * (16 * bj + 4 * g + r)] =
-              make_float2(cr[bi][bj][r] * sc2, ci[bi][bj][r] * sc2);
+          __builtin_nontemporal_store(fv2{cr[bi][bj][r] * sc2, ci[bi][bj][r] * sc2},
+                                      (fv2 *)(C + (16 * bi + l16) + ld * (16 * bj + 4 * g + r)));
   }
 }
 
 // Trailing update of the step group starting at global step `step` (see
 // k_lu_trail in tau_lu.hip for the tile ranges) on the split-FP16 planes of
-// the group's steps.
-__global__ __launch_bounds__(256) void k_lu_trail_h(const SolveDesc *__restrict__ descs, int step,
+// the group's steps: 128 x 128 regions (two row tiles x two column tiles) of
+// 512 threads.
+__global__ __launch_bounds__(512) void k_lu_trail_h(const SolveDesc *__restrict__ descs, int step,
                                                     int xcd_swizzle) {
-  __shared__ h8 Ls[2][4][kTile][4];
-  __shared__ h8 Us[4][kTile][4];
+  __shared__ h8 Ls[2][2][4][kTile][4];
+  __shared__ h8 Us[2][2][4][kTile][4];
   int bx = blockIdx.x, by = blockIdx.y;
   if (xcd_swizzle) xcd_remap(bx, by);
   const SolveDesc &d = descs[by];
@@ -210,35 +273,37 @@ __global__ __launch_bounds__(256) void k_lu_trail_h(const SolveDesc *__restrict_
   if (ns <= 0) return;
   if (d.conv && *d.conv) return;
   const int b = g0 + ns - 1;  // last step of the group
-  const int nrest = d.nt - 1 - b, nsr = (nrest + 1) / 2;
-  if (bx >= nsr * nrest) return;
-  // Regions in super-blocks of kBlkR row pairs x kBlkC columns, block by
-  // block (columns of blocks within a block row): the workgroups resident on
-  // an XCD at one time share the planes of about 16 row tiles and 8 column
-  // tiles, which stay in its L2, instead of sweeping all row tiles per column.
-  constexpr int kBlkR = 8, kBlkC = 8;
-  const int br = bx / (kBlkR * nrest), hb = min(kBlkR, nsr - br * kBlkR);
-  const int off = bx - br * kBlkR * nrest, bc = off / (hb * kBlkC), off2 = off - bc * hb * kBlkC;
-  const int sr = br * kBlkR + off2 % hb, sc = bc * kBlkC + off2 / hb;
-  region_h(d, b + 1 + 2 * sr, min(2, nrest - 2 * sr), b + 1 + sc, g0, ns, Ls, Us);
+  const int nrest = d.nt - 1 - b, np = (nrest + 1) / 2;
+  if (bx >= np * np) return;
+  // Regions in super-blocks of kBlk x kBlk tile pairs, block by block
+  // (columns of blocks within a block row): the workgroups resident on an
+  // XCD at one time share the planes of about 16 row and 16 column tiles,
+  // which stay in its L2.
+  constexpr int kBlk = 8;
+  const int br = bx / (kBlk * np), hb = min(kBlk, np - br * kBlk);
+  const int off = bx - br * kBlk * np, bc = off / (hb * kBlk), off2 = off - bc * hb * kBlk;
+  const int sr = br * kBlk + off2 % hb, sc = bc * kBlk + off2 / hb;
+  region_h<2, 2>(d, b + 1 + 2 * sr, min(2, nrest - 2 * sr), b + 1 + 2 * sc,
+                 min(2, nrest - 2 * sc), g0, ns, Ls, Us);
 }
 
 // Cross update for step t = g + u of the group starting at global step
 // `step` (see k_lu_cross in tau_lu.hip) on the split planes of steps
-// g .. t-1: tile column t (rows >= t, two row tiles per workgroup), then
-// tile row t (columns > t, one tile per workgroup).
+// g .. t-1, in 128 x 64 regions of 256 threads: tile column t (rows >= t,
+// two row tiles per workgroup), then tile row t (columns > t, one tile per
+// workgroup).
 __global__ __launch_bounds__(256) void k_lu_cross_h(const SolveDesc *__restrict__ descs, int step,
                                                     int u) {
-  __shared__ h8 Ls[2][4][kTile][4];
-  __shared__ h8 Us[4][kTile][4];
+  __shared__ h8 Ls[1][2][4][kTile][4];
+  __shared__ h8 Us[1][1][4][kTile][4];
   const SolveDesc &d = descs[blockIdx.y];
   const int g0 = step - d.step0, t = g0 + u;
   if (g0 < 0 || u >= min(kGroupH, d.nt - 1 - g0)) return;
   if (d.conv && *d.conv) return;
   const int n1 = d.nt - t, nc = (n1 + 1) / 2, x = blockIdx.x;
   if (x >= nc + n1 - 1) return;
-  if (x < nc) region_h(d, t + 2 * x, min(2, n1 - 2 * x), t, g0, u, Ls, Us);
-  else region_h(d, t, 1, t + 1 + (x - nc), g0, u, Ls, Us);
+  if (x < nc) region_h<1, 1>(d, t + 2 * x, min(2, n1 - 2 * x), t, 1, g0, u, Ls, Us);
+  else region_h<1, 1>(d, t, 1, t + 1 + (x - nc), 1, g0, u, Ls, Us);
 }
 
 }  // namespace lsmsx
