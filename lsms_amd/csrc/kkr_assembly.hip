@@ -213,10 +213,12 @@ __device__ __forceinline__ void gaunt_sum(const double *cg, const double2 *il, c
 
 // The four entries G(lm2 = 4u + kq, lm1 = li), u = 0..3, of the wave's
 // current pair that lane (li, kq) feeds to the t G product on the MFMA
-// (a-operand layout), by the Gaunt sum of gaunt_sum, in registers.
+// (a-operand layout), by the Gaunt sum of gaunt_sum, in registers, over the
+// whole 16 x 16 range (the entries outside a block's sites meet zero t
+// entries or are not stored).  dlw holds all 49 D_{l,m}.
 __device__ __forceinline__ void gaunt_lane(const double *cg, const double2 *il, const int *lofk,
-                                           int lmax_tab, int kkri, int kkrj, bool prel0,
-                                           const double2 *dlw, double2 (&g)[4]) {
+                                           int lmax_tab, bool prel0, const double2 *dlw,
+                                           double2 (&g)[4]) {
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
   const int kkr_tab = (lmax_tab + 1) * (lmax_tab + 1);
   const double pi4 = 4.0 * 3.141592653589793238462643383279502884;
@@ -225,7 +227,7 @@ __device__ __forceinline__ void gaunt_lane(const double *cg, const double2 *il, 
   for (int u = 0; u < 4; u++) {
     const int lm2 = 4 * u + kq;
     double2 gv = make_double2(0.0, 0.0);
-    if (lm2 < kkri && lm1 < kkrj) {
+    if (lm2 < kkr_tab && lm1 < kkr_tab) {
       const int l1 = lofk[lm1], l2 = lofk[lm2];
       const int m1 = lm1 - l1 * (l1 + 1), m2 = lm2 - l2 * (l2 + 1), m3 = m2 - m1;
       const int am3 = m3 < 0 ? -m3 : m3, dl12 = l1 - l2 < 0 ? l2 - l1 : l1 - l2;
@@ -244,12 +246,78 @@ __device__ __forceinline__ void gaunt_lane(const double *cg, const double2 *il, 
   }
 }
 
-// Assemble row-site block row i = blockIdx.x of solve blockIdx.y:
-// M(i,j) = -tpack_i diag_spin(G_ij) for j != i, identity for j == i.
+// t of site s packed as the F77 route does (calculateTauMatrix.cpp:155-183:
+// the leading kkr block of each spin block) in the b-operand layout:
+// tv[bi][bj][u] = t(bi kkrsz + li, bj kkrsz + 4u + kq), zero outside kkr.
+template <int NSC>
+__device__ __forceinline__ void load_t_operand(const SolveDesc &d, const double2 *__restrict__ tmat,
+                                               long ldT, int s, int kkr, double2 (&tv)[NSC][NSC][4]) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+  const double2 *T = tmat + ldT * d.store[s] + d.t_off;
+#pragma unroll
+  for (int bi = 0; bi < NSC; bi++)
+#pragma unroll
+    for (int bj = 0; bj < NSC; bj++)
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int ci = 4 * u + kq;
+        tv[bi][bj][u] = (li < kkr && ci < kkr)
+                            ? T[(bi * d.kkrsz + li) + (long)d.ldt * (bj * d.kkrsz + ci)]
+                            : make_double2(0.0, 0.0);
+      }
+}
+
+// Block (r, c) = -tpack_r diag_spin(G) (calculateTauMatrix.cpp:240-242) on
+// FP64 MFMA, 3M form, one 16 x 16 x 16 product per spin block pair (bi, bj),
+// stored to M (and the tile-major FP32 copy W).  Transposed product as in
+// the LU update, so register q of lane (li, kq) is element (li, kq + 4q) of
+// the block and each store is a run of 16 consecutive rows.
+template <int NSC>
+__device__ __forceinline__ void store_block(const SolveDesc &d, const double2 (&g)[4],
+                                            const double2 (&tv)[NSC][NSC][4], int offr, int kkrr,
+                                            int offc, int kkrc) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+  double2 *__restrict__ M = d.M;
+  float2 *__restrict__ W = d.w32;
+  const long ld = d.ld;
+#pragma unroll
+  for (int bi = 0; bi < NSC; bi++)
+#pragma unroll
+    for (int bj = 0; bj < NSC; bj++) {
+      d4 q1 = {0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const double2 gg = g[u], t = tv[bi][bj][u];
+        q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(gg.x, t.x, q1, 0, 0, 0);
+        q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(gg.y, t.y, q2, 0, 0, 0);
+        q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(gg.x + gg.y, t.x + t.y, q3, 0, 0, 0);
+      }
+      if (li < kkrr) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int jc = kq + 4 * q;
+          if (jc < kkrc) {
+            const double re = q1[q] - q2[q], im = q3[q] - q1[q] - q2[q];
+            const int row = offr + bi * kkrr + li, col = offc + bj * kkrc + jc;
+            M[row + ld * col] = make_double2(-re, -im);
+            if (W) W[tiled_index(row, col, d.nt)] = make_float2((float)-re, (float)-im);
+          }
+        }
+      }
+    }
+}
+
+// Assemble for row site i = blockIdx.x of solve blockIdx.y the identity
+// block (i, i) and, for the pairs j = i + k (mod n), k = 1 .. n/2 (the
+// middle pair of an even n once), both blocks
+//   M(i,j) = -tpack_i diag_spin(G_ij),  M(j,i) = -tpack_j diag_spin(G_ji),
+// from one evaluation of D and G: D_{l,m}(-R) = (-1)^l D_{l,m}(R) and the
+// Gaunt coefficients vanish unless l1 + l2 + l3 is even, so
+// G_ji(L2, L1) = (-1)^(l1+l2) G_ij(L2, L1) entry by entry (exact sign flips).
 // The workgroup stages the angular and Gaunt tables in LDS (16 KiB, so that
 // many workgroups share a CU and hide the latency of the special functions);
-// each wave keeps its MFMA operands of t_i in registers and builds blocks
-// (i, j), j = w, w+4, ..., independently.  NSC = n_spin_cant.
+// each wave keeps its MFMA operands of t_i in registers and builds the pairs
+// k = w+1, w+5, ... independently.  NSC = n_spin_cant.
 template <int NSC>
 __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ descs,
                                                   const double2 *__restrict__ tmat, long ldT,
@@ -258,7 +326,8 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
                                                   int lmax_tab) {
   const SolveDesc &d = descs[blockIdx.y];
   const int i = blockIdx.x;
-  if (i >= d.numLIZ) return;
+  const int n = d.numLIZ;
+  if (i >= n) return;
 
   __shared__ double cg[(kMaxLmax + 1) * kMaxKkr * kMaxKkr];
   __shared__ double2 il[kMaxKkr * kMaxKkr];
@@ -271,58 +340,44 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
   load_tables(cgnt_g, illp_g, lmax_tab, cg, il, pa, pb, pmmf, pm1f, lofk);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, kq = lane >> 4;
-  // t_i packed as the F77 route does (calculateTauMatrix.cpp:155-183: the
-  // leading kkri block of each spin block), in the b-operand layout:
-  // tv[bi][bj][u] = t(bi kkrsz + li, bj kkrsz + 4u + kq), zero outside kkri
-  double2 tv[NSC][NSC][4];
-  {
-    const double2 *T = tmat + ldT * d.store[i] + d.t_off;
-#pragma unroll
-    for (int bi = 0; bi < NSC; bi++)
-#pragma unroll
-      for (int bj = 0; bj < NSC; bj++)
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int ci = 4 * u + kq;
-          tv[bi][bj][u] = (li < kkri && ci < kkri)
-                              ? T[(bi * d.kkrsz + li) + (long)d.ldt * (bj * d.kkrsz + ci)]
-                              : make_double2(0.0, 0.0);
-        }
-  }
+  double2 tvi[NSC][NSC][4];
+  load_t_operand<NSC>(d, tmat, ldT, i, kkri, tvi);
   __syncthreads();
 
   const AngTab tb{pa, pb, pmmf, pm1f};
   const double xi = d.pos[3 * i], yi = d.pos[3 * i + 1], zi = d.pos[3 * i + 2];
   const double2 prel = make_double2(d.prel_re, d.prel_im);
   const bool prel0 = (prel.x == 0.0 && prel.y == 0.0);
-  double2 *__restrict__ M = d.M;
-  // mixed precision: the solver factors an FP32 copy written alongside M,
-  // tile-major (tiled_index)
-  float2 *__restrict__ W = d.w32;
-  const long ld = d.ld;
   const int offi = d.site_off[i];
+  if (w == 0) {
+    for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
+      int rr = e % kkri_ns, cc = e / kkri_ns;
+      d.M[(offi + rr) + (long)d.ld * (offi + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
+      if (d.w32)
+        d.w32[tiled_index(offi + rr, offi + cc, d.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
+    }
+  }
+  // (-1)^(l1+l2) of the lane's four G entries
+  double sg[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) sg[u] = ((lofk[li] + lofk[4 * u + kq]) & 1) ? -1.0 : 1.0;
 
-  for (int j = w; j < d.numLIZ; j += 4) {
+  for (int k = 1 + w; 2 * k <= n; k += 4) {
+    const int j = (i + k) % n;
+    if (2 * k == n && j < i) continue;  // the middle pair of an even n, once
     const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1);
     const int offj = d.site_off[j];
-    if (j == i) {
-      for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
-        int rr = e % kkri_ns, cc = e / kkri_ns;
-        M[(offi + rr) + ld * (offj + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
-        if (W) W[tiled_index(offi + rr, offj + cc, d.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
-      }
-      continue;
-    }
+    double2 tvj[NSC][NSC][4];
+    load_t_operand<NSC>(d, tmat, ldT, j, kkrj, tvj);
     const double rx = xi - d.pos[3 * j], ry = yi - d.pos[3 * j + 1], rz = zi - d.pos[3 * j + 2];
     const double r = sqrt(rx * rx + ry * ry + rz * rz);
-    const int lend = lmi + lmj;
-    // --- D_{l,m}, one (l,m) per lane
-    if (lane < (lend + 1) * (lend + 1)) {
+    // --- D_{l,m}(R_ij), one (l,m) per lane, all l <= 6
+    if (lane < kNd) {
       const int l = lofk[lane], m = lane - l * (l + 1);
       double2 h;
       if (prel0) {  // static limit: hfn_l = i^l (2l-1)!! / r^(l+1)
         double v = 1.0 / r;
-        for (int k = 1; k <= l; k++) v *= (2.0 * k - 1.0) / r;
+        for (int kk = 1; kk <= l; kk++) v *= (2.0 * kk - 1.0) / r;
         h = cmul(ipow_c(l), make_double2(v, 0.0));
       } else {
         h = hankel_lane(prel, r, l, nullptr);
@@ -331,39 +386,11 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
     }
     wave_sync();
     double2 g[4];
-    gaunt_lane(cg, il, lofk, lmax_tab, kkri, kkrj, prel0, dl[w], g);
-    // --- M block = -tpack_i diag_spin(G) (calculateTauMatrix.cpp:240-242) on
-    // FP64 MFMA, 3M form, one 16 x 16 x 16 product per spin block pair
-    // (bi, bj): M_bi,bj = -t_bi,bj G.  Transposed product as in the LU update,
-    // so register r of lane (li, kq) is element (li, kq + 4r) of the block and
-    // each store is a run of 16 consecutive rows.
+    gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[w], g);
+    store_block<NSC>(d, g, tvi, offi, kkri, offj, kkrj);
 #pragma unroll
-    for (int bi = 0; bi < NSC; bi++)
-#pragma unroll
-      for (int bj = 0; bj < NSC; bj++) {
-        d4 q1 = {0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const double2 gg = g[u], t = tv[bi][bj][u];
-          q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(gg.x, t.x, q1, 0, 0, 0);
-          q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(gg.y, t.y, q2, 0, 0, 0);
-          q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(gg.x + gg.y, t.x + t.y, q3, 0, 0, 0);
-        }
-        if (li < kkri) {
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int jc = kq + 4 * r;
-            if (jc < kkrj) {
-              const double re = q1[r] - q2[r], im = q3[r] - q1[r] - q2[r];
-              const long idx = (offi + bi * kkri + li) + ld * (offj + bj * kkrj + jc);
-              M[idx] = make_double2(-re, -im);
-              if (W)
-                W[tiled_index(offi + bi * kkri + li, offj + bj * kkrj + jc, d.nt)] =
-                    make_float2((float)-re, (float)-im);
-            }
-          }
-        }
-      }
+    for (int u = 0; u < 4; u++) g[u] = make_double2(sg[u] * g[u].x, sg[u] * g[u].y);
+    store_block<NSC>(d, g, tvj, offj, kkrj, offi, kkri);
     wave_sync();  // dl[w] is rewritten for the next pair
   }
 }
