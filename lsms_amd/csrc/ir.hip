@@ -240,15 +240,20 @@ __device__ __forceinline__ void acc_zero(AccIR<T> &q) {
 //   backward, launch j = 0 .. nt-2, s = nt-1-j (x_s final), tile t = bx < s:
 //     y_t -= U_ts x_s;   t == s-1:  x_t = U_tt^-1 y_t
 // Solves of different sizes run the forward steps from the first tile and
-// the backward steps from their own last tile.  Wave w handles rows 16w.. of
-// tile t; the 64 x kc block of tile s is staged in LDS by the workgroup.
+// the backward steps from their own last tile.  The first sweep starts from
+// R = B, zero outside the last tile; with tile-local pivoting its forward
+// steps leave every earlier tile at zero, so it runs as one launch
+// (step = kIrFirstSweep) that only finishes the last tile of each solve.
+// Wave w handles rows 16w.. of tile t; the 64 x kc block of tile s is
+// staged in LDS by the workgroup.
 template <class T, bool FWD>
 __global__ __launch_bounds__(256) void k_ir_step(const SolveDesc *__restrict__ descs, int step) {
   typedef CxT<T> V;
   const SolveDesc &d = descs[blockIdx.y];
   if (*d.conv) return;
   const int nt = d.nt;
-  const int s = FWD ? step : nt - 1 - step;
+  const bool first = FWD && step == kIrFirstSweep;
+  const int s = FWD ? (first ? nt - 2 : step) : nt - 1 - step;
   const int t = FWD ? s + 1 + (int)blockIdx.x : (int)blockIdx.x;
   if (FWD ? t >= nt : t >= s) return;
   __shared__ V buf[kTile][32 + 1];
@@ -263,7 +268,7 @@ __global__ __launch_bounds__(256) void k_ir_step(const SolveDesc *__restrict__ d
   // every global load of the step is issued before the first wait: the
   // factor tile (t, s) in MFMA layout, r_t, and the 64 x kc block of tile s
   V a[kTile / 4], st[8];
-  if (s >= 0) {
+  if (s >= 0 && !first) {
 #pragma unroll
     for (int u = 0; u < kTile / 4; u++) a[u] = ldg<T>(Wt + li + ld * (4 * u + kq));
 #pragma unroll
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(256) void k_ir_step(const SolveDesc *__restrict__ d
 #pragma unroll
     for (int r = 0; r < 4; r++)
       if (jb < nb) acc_from<T>(q, jb, r, R[(i0 + li) + (long)Ns * col_of(jb, r)]);
-  if (s >= 0) {
+  if (s >= 0 && !first) {
 #pragma unroll
     for (int qq = 0; qq < 8; qq++) {
       const int e = tid + 256 * qq;

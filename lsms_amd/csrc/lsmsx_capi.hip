@@ -370,9 +370,14 @@ int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int kc, int ma
     if (int r = check_launch("k_ir_residual")) return r;
     {
       ProfScope p(c, 6);
-      for (int s = -1; s <= nt_max - 2; s++)
-        hipLaunchKernelGGL((k_ir_step<T, true>), dim3(s < 0 ? 1 : nt_max - 1 - s, nb), dim3(256), 0,
-                           c->stream, dd, s);
+      if (it == 0) {  // R = B: only the last tile of each solve (kIrFirstSweep)
+        hipLaunchKernelGGL((k_ir_step<T, true>), dim3(1, nb), dim3(256), 0, c->stream, dd,
+                           kIrFirstSweep);
+      } else {
+        for (int s = -1; s <= nt_max - 2; s++)
+          hipLaunchKernelGGL((k_ir_step<T, true>), dim3(s < 0 ? 1 : nt_max - 1 - s, nb), dim3(256),
+                             0, c->stream, dd, s);
+      }
       for (int j = 0; j <= nt_max - 2; j++)
         hipLaunchKernelGGL((k_ir_step<T, false>), dim3(nt_max - 1 - j, nb), dim3(256), 0, c->stream,
                            dd, j);

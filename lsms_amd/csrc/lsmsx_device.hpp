@@ -15,6 +15,7 @@ constexpr int kTile = 64;     // LU tile / panel width
 constexpr int kMaxLmax = 3;   // tables and per-site blocks sized for lmax <= 3
 constexpr int kMaxKkr = (kMaxLmax + 1) * (kMaxLmax + 1);
 constexpr int kPlaneHalves = 4 * kTile * kTile;  // one tile's four FP16 planes
+constexpr int kIrFirstSweep = -1000;  // k_ir_step's step value for the first forward sweep
 
 // Element (r, c) of a tile-major matrix with nt tiles per tile column: every
 // 64 x 64 tile is one contiguous run (column-major inside), tiles ordered by
