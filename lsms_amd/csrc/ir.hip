@@ -360,15 +360,19 @@ __global__ __launch_bounds__(256) void k_ir_step(const SolveDesc *__restrict__ d
       if (jb < nb) R[(i0 + li) + (long)Ns * col_of(jb, r)] = acc_value<T>(q, jb, r);
 }
 
-// X += dX for the rows of one tile (blockIdx.x) of one solve; the workgroup
-// of the last tile also sums ||dX_c||^2 and ||X_c||^2 over the central rows.
+// X += dX for the rows of one tile of one solve.  central = 1: the last tile
+// (which holds the central rows and is final once the forward steps end),
+// with the sums ||dX_c||^2 and ||X_c||^2 over the central rows; central = 0:
+// tile blockIdx.x < nt-1, after the backward steps.  The convergence test runs
+// between the two, so a solve that converges skips the backward half of its
+// last sweep (the other rows of X are not used again).
 template <class T>
-__global__ __launch_bounds__(256) void k_ir_accum(const SolveDesc *__restrict__ descs) {
+__global__ __launch_bounds__(256) void k_ir_accum(const SolveDesc *__restrict__ descs, int central) {
   typedef CxT<T> V;
   const SolveDesc &d = descs[blockIdx.y];
   if (*d.conv) return;
-  const int t = blockIdx.x;
-  if (t >= d.nt) return;
+  const int t = central ? d.nt - 1 : (int)blockIdx.x;
+  if (!central && t >= d.nt - 1) return;
   __shared__ double red[2][256];
   const int tid = threadIdx.x, Ns = d.Ns, kc = d.kc;
   gd2 *X = (gd2 *)d.X;
@@ -387,7 +391,7 @@ __global__ __launch_bounds__(256) void k_ir_accum(const SolveDesc *__restrict__ 
       xn += x.x * x.x + x.y * x.y;
     }
   }
-  if (t != d.nt - 1) return;
+  if (!central) return;
   red[0][tid] = dn;
   red[1][tid] = xn;
   __syncthreads();
@@ -476,7 +480,7 @@ template __global__ void k_ir_step<float, true>(const SolveDesc *, int);
 template __global__ void k_ir_step<float, false>(const SolveDesc *, int);
 template __global__ void k_ir_step<double, true>(const SolveDesc *, int);
 template __global__ void k_ir_step<double, false>(const SolveDesc *, int);
-template __global__ void k_ir_accum<float>(const SolveDesc *);
-template __global__ void k_ir_accum<double>(const SolveDesc *);
+template __global__ void k_ir_accum<float>(const SolveDesc *, int);
+template __global__ void k_ir_accum<double>(const SolveDesc *, int);
 
 }  // namespace lsmsx

@@ -45,7 +45,7 @@ __global__ void k_ir_residual(const SolveDesc *);
 template <class T, bool FWD>
 __global__ void k_ir_step(const SolveDesc *, int);
 template <class T>
-__global__ void k_ir_accum(const SolveDesc *);
+__global__ void k_ir_accum(const SolveDesc *, int);
 __global__ void k_ir_decide(const SolveDesc *, int, double);
 __global__ void k_copy_m(const SolveDesc *);
 __global__ void k_ir_extract(const SolveDesc *);
@@ -378,11 +378,15 @@ int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int kc, int ma
           hipLaunchKernelGGL((k_ir_step<T, true>), dim3(s < 0 ? 1 : nt_max - 1 - s, nb), dim3(256),
                              0, c->stream, dd, s);
       }
+      // the central rows are final after the forward steps: test convergence
+      // before the backward steps, which converged solves then skip
+      hipLaunchKernelGGL(k_ir_accum<T>, dim3(1, nb), dim3(256), 0, c->stream, dd, 1);
+      hipLaunchKernelGGL(k_ir_decide, dim3((nb + 255) / 256), dim3(256), 0, c->stream, dd, nb, tol);
       for (int j = 0; j <= nt_max - 2; j++)
         hipLaunchKernelGGL((k_ir_step<T, false>), dim3(nt_max - 1 - j, nb), dim3(256), 0, c->stream,
                            dd, j);
-      hipLaunchKernelGGL(k_ir_accum<T>, dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
-      hipLaunchKernelGGL(k_ir_decide, dim3((nb + 255) / 256), dim3(256), 0, c->stream, dd, nb, tol);
+      if (nt_max > 1)
+        hipLaunchKernelGGL(k_ir_accum<T>, dim3(nt_max - 1, nb), dim3(256), 0, c->stream, dd, 0);
     }
     if (int r = check_launch("k_ir_step")) return r;
     if (it >= 1) {  // the first correction never meets the tolerance
