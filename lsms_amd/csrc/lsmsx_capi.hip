@@ -161,7 +161,7 @@ struct lsmsx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // second half of a chunk (LSMSX_SCHED_TWO_STREAMS)
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_asm = nullptr, ev_join = nullptr;  // second-half start and join
   int lmax_tab = -1;
   DevBuf<double> cgnt;
   DevBuf<double2> illp;
@@ -595,8 +595,9 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     // once per chunk shape into a hipGraph and replayed (the kernels read
     // the freshly uploaded descriptors, so only the shape and the buffer
     // addresses passed as arguments key the graph)
-    // solves [b0, b0 + n) of the chunk on stream st
-    auto enqueue_part = [&](hipStream_t st, int b0, int n) -> int {
+    // solves [b0, b0 + n) of the chunk on stream st; after_asm, if set, is
+    // recorded once the assembly is enqueued
+    auto enqueue_part = [&](hipStream_t st, int b0, int n, hipEvent_t after_asm) -> int {
       const SolveDesc *dd = c->desc.p + b0;
       // the step range is the chunk's: right-aligned starts are relative to it
       const int ntm = nt_max;
@@ -610,6 +611,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
         launch_assemble(c, nsc, dim3(mliz, n), st, dd);
       }
       if (int r = check_launch("k_assemble")) return r;
+      if (after_asm) HIPCHK(hipEventRecord(after_asm, st));
       if (refined64) {
         hipLaunchKernelGGL(k_copy_m, dim3(ntm, n), dim3(256), 0, st, dd);
         if (int r = check_launch("k_copy_m")) return r;
@@ -617,16 +619,17 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (mixed) return launch_lu<float>(c, dd, n, ntm, split, st);
       return launch_lu<double>(c, dd, n, ntm, false, st);
     };
-    // two halves on two streams overlap one half's latency-bound steps
-    // (diagonal tiles, panels) with the other half's trailing updates
+    // two halves on two streams, the second started once the first half's
+    // assembly is done: its assembly (FP64 VALU work) runs beside the first
+    // half's factorization (MFMA and HBM work), and one half's latency-bound
+    // steps (diagonal tiles, panels) overlap the other half's trailing updates
     const bool two = (c->sched & LSMSX_SCHED_TWO_STREAMS) && nb >= 2 && !c->profiling;
     auto enqueue = [&]() -> int {
-      if (!two) return enqueue_part(c->stream, 0, nb);
+      if (!two) return enqueue_part(c->stream, 0, nb, nullptr);
       const int h = nb / 2;
-      HIPCHK(hipEventRecord(c->ev_fork, c->stream));
-      HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-      if (int r = enqueue_part(c->stream, 0, h)) return r;
-      if (int r = enqueue_part(c->stream2, h, nb - h)) return r;
+      if (int r = enqueue_part(c->stream, 0, h, c->ev_asm)) return r;
+      HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_asm, 0));
+      if (int r = enqueue_part(c->stream2, h, nb - h, nullptr)) return r;
       HIPCHK(hipEventRecord(c->ev_join, c->stream2));
       HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
       return 0;
@@ -1066,12 +1069,12 @@ int lsmsx_ctx_create(int device, lsmsx_ctx **out) {
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_asm, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
-    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_asm) (void)hipEventDestroy(c->ev_asm);
     delete c;
     return fail(-100 - (int)e, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
@@ -1132,8 +1135,8 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->colmap.release();
   c->keyl.release();
   (void)hipStreamSynchronize(c->stream2);
-  (void)hipEventDestroy(c->ev_fork);
   (void)hipEventDestroy(c->ev_join);
+  (void)hipEventDestroy(c->ev_asm);
   (void)hipStreamDestroy(c->stream2);
   (void)hipStreamDestroy(c->stream);
   delete c;
