@@ -412,12 +412,9 @@ __global__ __launch_bounds__(256) void k_ir_accum(const SolveDesc *__restrict__ 
 // The next error is about rho ||dX|| with rho = ||dX_k|| / ||dX_{k-1}|| the
 // observed contraction; converge when that estimate (and ||dX|| itself, to
 // 1e-8) is below tol, or when ||dX_c|| <= tol ||X_c|| outright.
-__global__ void k_ir_decide(const SolveDesc *__restrict__ descs, int n, double tol) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
-  const SolveDesc &d = descs[b];
-  if (*d.conv) return;
-  const double dn2 = d.nrm[0], xn2 = d.nrm[1];
+// The convergence test of one solve on ||dX_c||^2 = dn2 and ||X_c||^2 = xn2
+// of the sweep just finished (one thread).
+__device__ inline void ir_decide_one(const SolveDesc &d, double dn2, double xn2, double tol) {
   const int k = *d.iters;
   const double prev = *d.dxn;  // previous ||dX_c||^2
   bool ok = false;
@@ -432,6 +429,307 @@ __global__ void k_ir_decide(const SolveDesc *__restrict__ descs, int n, double t
   *d.iters = k + 1;
   *d.dxn = dn2;
   if (ok) *d.conv = 1;
+}
+
+__global__ void k_ir_decide(const SolveDesc *__restrict__ descs, int n, double tol) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const SolveDesc &d = descs[b];
+  if (*d.conv) return;
+  ir_decide_one(d, d.nrm[0], d.nrm[1], tol);
+}
+
+// ---------------------------------------------------------------------------
+// One refinement sweep per launch, one workgroup per solve (kc = 16, complex
+// FP32 factors, tile-major): dX = (P^T L U)^-1 R, the convergence test on the
+// central rows between the forward and the backward halves, and X += dX.  It
+// does the work of k_ir_step, k_ir_accum and k_ir_decide for one sweep.
+//
+// Left-looking by tile row, with the tile rows dealt round-robin to the
+// waves of the workgroup: wave w takes t = w, w+8, ... (forward, ascending)
+// and computes
+//   y_t = L_tt^-1 P_t (r_t - sum_{s<t} L_ts y_s)
+// with the 64 x 16 block in MFMA accumulators (4M form), then publishes y_t
+// (in place of r_t in R) by a workgroup-scope flag in LDS; the waves that
+// need y_s wait on its flag.  The backward half runs the same way from the
+// last tile down (x_t = U_tt^-1 (y_t - sum_{s>t} U_ts x_s)).  Every wave
+// waits only for tiles below its own (above, backward) and takes its tiles
+// in order, so the waits always end.  The diagonal-tile inverse is applied
+// as one more 64 x 64 operand, its input staged in the wave's LDS slot.
+// All operand tiles (L_ts, U_ts, the inverses) stream from HBM through a
+// ring of kSweepD k-steps per wave that runs on across tiles; the published
+// y_s / x_s come from L2, one item ahead when its flag is already set.
+// HBM traffic per sweep: the factors once, R and X; no launch per tile
+// column.  The first sweep (R = B, zero outside the last tile, so y_t = 0
+// for t < nt-1 under tile-local pivoting) runs only the last tile forward.
+constexpr int kSweepD = 4;  // k-steps (4 columns of a 64 x 64 operand) in flight per wave
+
+template <bool FIRST>
+__global__ __launch_bounds__(kSweepWaves * 64) void k_ir_sweep(const SolveDesc *__restrict__ descs,
+                                                               double tol) {
+  typedef float2 V;
+  const SolveDesc &d = descs[blockIdx.x];
+  if (*d.conv) return;
+  const int nt = d.nt, Ns = d.Ns;
+  const int tid = threadIdx.x, lane = tid & 63, li = lane & 15, kq = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  __shared__ V slot[kSweepWaves][kTile][17];  // a wave's block, input of its diagonal step
+  __shared__ int pslot[kSweepWaves][kTile];
+  __shared__ int ready[kSweepMaxTiles], done[kSweepMaxTiles];
+  __shared__ int conv_flag;
+  for (int i = tid; i < kSweepMaxTiles; i += kSweepWaves * 64) {
+    ready[i] = 0;
+    done[i] = 0;
+  }
+  __syncthreads();
+  GCx<float> *R = (GCx<float> *)d.R;
+  gd2 *X = (gd2 *)d.X;
+  const GCx<float> *__restrict__ F = (const GCx<float> *)d.A;  // tile-major factors
+  const GCx<float> *__restrict__ Inv = (const GCx<float> *)d.inv;
+  const LSMSX_GLOBAL int *Perm = (const LSMSX_GLOBAL int *)d.perm;
+  const long istr = d.inv_stride;
+  const int loff = li + kTile * kq;
+
+  f4 cr[4], ci[4];  // the current tile row's 64 x 16 block: (16 rb + li, 4 kq + r)
+  V ycur[16], ynext[16];  // y_s / x_s operand: (4 kk + kq, li)
+
+  // ---- operand stream.  Items are 64 x 64 column-major operands, consumed
+  // 4 columns (one k-step) at a time.  Forward items of tile t: s < t: L_ts,
+  // s == t: the inverse (lower part), s == t+1: the inverse again (upper
+  // part; last tile only).  Backward items of tile t: s > t: U_ts, s == t:
+  // the inverse (upper part).
+  auto item_base = [&](bool fwd, int t, int s) __attribute__((always_inline)) -> const GCx<float> * {
+    const bool inv = fwd ? s >= t : s == t;
+    return inv ? Inv + t * istr : F + ((long)s * nt + t) * (kTile * kTile);
+  };
+  auto next_item = [&](bool fwd, int &t, int &s) __attribute__((always_inline)) -> bool {
+    if (fwd) {
+      if (s < t || (s == t && t == nt - 1)) {
+        s++;
+        return true;
+      }
+      if (FIRST) return false;
+      t += kSweepWaves;
+      s = 0;
+      return t < nt;
+    }
+    if (s > t) {
+      s--;
+      return true;
+    }
+    t -= kSweepWaves;
+    s = nt - 1;
+    return t >= 0;
+  };
+  V rg[kSweepD][4];
+  const GCx<float> *cb = F;
+  int pt = 0, ps = 0, pk = 0;
+  bool pok = false;
+  // load the cursor's k-step into a ring slot and advance; the loads are
+  // unconditional (an exhausted cursor re-reads its last k-step, never
+  // consumed) so that the wait counts stay static
+  auto load_slot = [&](bool fwd, int u) __attribute__((always_inline)) {
+    const GCx<float> *src = cb + kTile * 4 * pk + loff;
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) rg[u][rb] = ldg<float>(src + 16 * rb);
+    if (pok && ++pk == 16) {
+      pok = next_item(fwd, pt, ps);
+      if (pok) {
+        pk = 0;
+        cb = item_base(fwd, pt, ps);
+      } else {
+        pk = 15;
+      }
+    }
+  };
+  auto prime = [&](bool fwd, int t0, int s0, bool ok) __attribute__((always_inline)) {
+    pt = t0;
+    ps = s0;
+    pk = 0;
+    pok = ok;
+    cb = ok ? item_base(fwd, t0, s0) : F;
+#pragma unroll
+    for (int u = 0; u < kSweepD; u++) load_slot(fwd, u);
+  };
+  // block -= operand * ycur
+  auto consume_update = [&](bool fwd) __attribute__((always_inline)) {
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      const int u = kk % kSweepD;
+      const V yv = ycur[kk];
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) {
+        const V l = rg[u][rb];
+        cr[rb] = mfma4(yv.x, -l.x, cr[rb]);
+        cr[rb] = mfma4(yv.y, l.y, cr[rb]);
+        ci[rb] = mfma4(yv.y, -l.x, ci[rb]);
+        ci[rb] = mfma4(yv.x, -l.y, ci[rb]);
+      }
+      load_slot(fwd, u);
+    }
+  };
+  // block = operand * In, In = the wave's slot (rows permuted by pslot for
+  // the unit lower L^-1; as stored for the upper U^-1, diagonal included)
+  auto consume_diag = [&](bool fwd, bool upper) __attribute__((always_inline)) {
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+      cr[rb] = f4{0.f, 0.f, 0.f, 0.f};
+      ci[rb] = cr[rb];
+    }
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      const int u = kk % kSweepD, k = 4 * kk + kq;
+      const V in = upper ? slot[w][k][li] : slot[w][pslot[w][k]][li];
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) {
+        const int i = 16 * rb + li;
+        V op = rg[u][rb];
+        if (upper ? k < i : k >= i) {
+          op.x = (!upper && k == i) ? 1.f : 0.f;
+          op.y = 0.f;
+        }
+        cr[rb] = mfma4(in.x, op.x, cr[rb]);
+        cr[rb] = mfma4(in.y, -op.y, cr[rb]);
+        ci[rb] = mfma4(in.y, op.x, ci[rb]);
+        ci[rb] = mfma4(in.x, op.y, ci[rb]);
+      }
+      load_slot(fwd, u);
+    }
+  };
+  auto block_to_slot = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) slot[w][16 * rb + li][4 * kq + r] = make_float2(cr[rb][r], ci[rb][r]);
+  };
+  auto load_block = [&](int t) __attribute__((always_inline)) {
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const V v = ldg<float>(R + (t * kTile + 16 * rb + li) + (long)Ns * (4 * kq + r));
+        cr[rb][r] = v.x;
+        ci[rb][r] = v.y;
+      }
+  };
+  auto store_block = [&](int t) __attribute__((always_inline)) {
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        R[(t * kTile + 16 * rb + li) + (long)Ns * (4 * kq + r)] = make_float2(cr[rb][r], ci[rb][r]);
+  };
+  // X += the block of tile t; returns the central-row sums (rb = 3 of the
+  // last tile: rows Ns-16 .. Ns-1)
+  auto add_to_x = [&](int t, double &dn, double &xn) __attribute__((always_inline)) {
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long idx = (t * kTile + 16 * rb + li) + (long)Ns * (4 * kq + r);
+        const float dr = cr[rb][r], di = ci[rb][r];
+        double2 x = X[idx];
+        x.x += dr;
+        x.y += di;
+        X[idx] = x;
+        if (rb == 3) {
+          dn += (double)dr * dr + (double)di * di;
+          xn += x.x * x.x + x.y * x.y;
+        }
+      }
+  };
+  auto wait_flag = [&](int *flag) __attribute__((always_inline)) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+      __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  };
+  auto flag_set = [&](int *flag) __attribute__((always_inline)) {
+    return __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+  };
+  auto publish = [&](int *flag) __attribute__((always_inline)) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  auto load_y = [&](int s) __attribute__((always_inline)) {
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) ynext[kk] = ldg<float>(R + (s * kTile + 4 * kk + kq) + (long)Ns * li);
+  };
+  // the update items of a tile: steps s = first, first + dir, ..., last
+  // (inclusive), operand y_s / x_s published under flag[s]
+  auto run_updates = [&](bool fwd, int first, int last, int *flag) __attribute__((always_inline)) {
+    const int dir = fwd ? 1 : -1;
+    bool have = false;
+    for (int s = first; fwd ? s <= last : s >= last; s += dir) {
+      if (!have) {
+        wait_flag(flag + s);
+        load_y(s);
+      }
+#pragma unroll
+      for (int kk = 0; kk < 16; kk++) ycur[kk] = ynext[kk];
+      have = false;
+      const int sn = s + dir;
+      if ((fwd ? sn <= last : sn >= last) && flag_set(flag + sn)) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        load_y(sn);
+        have = true;
+      }
+      consume_update(fwd);
+    }
+  };
+
+  // ---- forward half
+  const int tl = nt - 1, wl = tl % kSweepWaves;
+  {
+    const int t0 = FIRST ? tl : w;
+    const bool mine = FIRST ? w == wl : w < nt;
+    prime(true, t0, FIRST ? tl : 0, mine);
+    if (mine) {
+      for (int t = t0; t < nt; t += kSweepWaves) {
+        load_block(t);
+        if (!FIRST && t > 0) run_updates(true, 0, t - 1, ready);
+        block_to_slot();
+        if (lane < kTile) pslot[w][lane] = Perm[t * kTile + lane];
+        consume_diag(true, false);
+        if (t == tl) {
+          block_to_slot();
+          consume_diag(true, true);
+          store_block(t);
+          double dn = 0.0, xn = 0.0;
+          add_to_x(t, dn, xn);
+          for (int off = 32; off >= 1; off >>= 1) {
+            dn += __shfl_xor(dn, off, 64);
+            xn += __shfl_xor(xn, off, 64);
+          }
+          if (lane == 0) {
+            ir_decide_one(d, dn, xn, tol);
+            conv_flag = *d.conv;
+          }
+          publish(done + t);
+        } else {
+          store_block(t);
+          publish(ready + t);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (conv_flag) return;
+
+  // ---- backward half: this wave's tiles below the last, descending
+  {
+    const int t0 = w <= nt - 2 ? w + kSweepWaves * ((nt - 2 - w) / kSweepWaves) : -1;
+    prime(false, t0 < 0 ? 0 : t0, tl, t0 >= 0);
+    for (int t = t0; t >= 0; t -= kSweepWaves) {
+      load_block(t);
+      run_updates(false, tl, t + 1, done);
+      block_to_slot();
+      consume_diag(false, true);
+      store_block(t);
+      double dn = 0.0, xn = 0.0;
+      add_to_x(t, dn, xn);
+      publish(done + t);
+    }
+  }
 }
 
 // tau00 = central rows of X.
@@ -482,5 +780,7 @@ template __global__ void k_ir_step<double, true>(const SolveDesc *, int);
 template __global__ void k_ir_step<double, false>(const SolveDesc *, int);
 template __global__ void k_ir_accum<float>(const SolveDesc *, int);
 template __global__ void k_ir_accum<double>(const SolveDesc *, int);
+template __global__ void k_ir_sweep<true>(const SolveDesc *, double);
+template __global__ void k_ir_sweep<false>(const SolveDesc *, double);
 
 }  // namespace lsmsx

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <tuple>
+#include <type_traits>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -47,6 +48,8 @@ __global__ void k_ir_step(const SolveDesc *, int);
 template <class T>
 __global__ void k_ir_accum(const SolveDesc *, int);
 __global__ void k_ir_decide(const SolveDesc *, int, double);
+template <bool FIRST>
+__global__ void k_ir_sweep(const SolveDesc *, double);
 __global__ void k_copy_m(const SolveDesc *);
 __global__ void k_ir_extract(const SolveDesc *);
 __global__ void k_count_conv(const SolveDesc *, int, int *);
@@ -358,6 +361,9 @@ int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int kc, int ma
            double tol, int *left) {
   hipLaunchKernelGGL(k_ir_init<T>, dim3(nb), dim3(256), 0, c->stream, dd);
   if (int r = check_launch("k_ir_init")) return r;
+  // mixed path, 16 right-hand sides: the whole sweep in one launch per chunk
+  // (k_ir_sweep); otherwise one launch per tile column (k_ir_step)
+  const bool one_launch = std::is_same<T, float>::value && kc == 16 && nt_max <= kSweepMaxTiles;
   int done = 0;
   for (int it = 0; it < max_sweeps; it++) {
     if (it > 0) {  // k_ir_init set the first residual, B
@@ -368,7 +374,13 @@ int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int kc, int ma
         hipLaunchKernelGGL((k_ir_residual<T, 1>), dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
     }
     if (int r = check_launch("k_ir_residual")) return r;
-    {
+    if (one_launch) {
+      ProfScope p(c, 6);
+      if (it == 0)
+        hipLaunchKernelGGL(k_ir_sweep<true>, dim3(nb), dim3(kSweepWaves * 64), 0, c->stream, dd, tol);
+      else
+        hipLaunchKernelGGL(k_ir_sweep<false>, dim3(nb), dim3(kSweepWaves * 64), 0, c->stream, dd, tol);
+    } else {
       ProfScope p(c, 6);
       if (it == 0) {  // R = B: only the last tile of each solve (kIrFirstSweep)
         hipLaunchKernelGGL((k_ir_step<T, true>), dim3(1, nb), dim3(256), 0, c->stream, dd,

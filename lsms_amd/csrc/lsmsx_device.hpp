@@ -16,6 +16,10 @@ constexpr int kMaxLmax = 3;   // tables and per-site blocks sized for lmax <= 3
 constexpr int kMaxKkr = (kMaxLmax + 1) * (kMaxLmax + 1);
 constexpr int kPlaneHalves = 4 * kTile * kTile;  // one tile's four FP16 planes
 constexpr int kIrFirstSweep = -1000;  // k_ir_step's step value for the first forward sweep
+// k_ir_sweep (one launch per refinement sweep, kc = 16): waves per workgroup
+// and the largest number of 64-row tiles per solve it takes
+constexpr int kSweepWaves = 8;
+constexpr int kSweepMaxTiles = 128;
 
 // Element (r, c) of a tile-major matrix with nt tiles per tile column: every
 // 64 x 64 tile is one contiguous run (column-major inside), tiles ordered by
