@@ -115,6 +115,29 @@ def test_mixed_fallback_to_fp64(mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_mixed_tile_counts_one_to_seven(mode):
+    """Solves of 1, 2, 5 and 7 tiles (N = 16, 112, 304, 432) in one batch:
+    the one-launch refinement sweep deals tile rows to eight waves, so this
+    covers waves with no tile, one tile, and rows split unevenly."""
+    e, w = CF.contour_semicircle(-0.5, 0.7, 4)
+    s = CF.make_system("tiles", "sc", 5.0, (2, 2, 2), 0, 3, e, w, seed=3,
+                       local_atoms=[0, 1, 2, 3], rcut_per_atom=[3.0, 5.5, 7.5, 9.0])
+    assert [s.nrmat_ns(i) for i in range(4)] == [16, 112, 304, 432]
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    atoms = np.array([0, 1, 2, 3, 3, 0])
+    iies = np.array([0, 1, 2, 3, 0, 3])
+    prels = np.array([s.prel(i) for i in iies])
+    tau, info = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
+    iters, _ = ctx.last_stats(len(atoms))
+    ctx.close()
+    assert np.all(info == 0)
+    assert np.all(iters >= 2), iters
+    for n in range(len(atoms)):
+        assert rel(tau[n], lapack_tau00(s, atoms[n], iies[n])) < TAU_RTOL
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("nsc,nsp,ispin", [(1, 2, 1), (2, 2, 0)])
 def test_mixed_spin_variants(nsc, nsp, ispin, mode):
     e, w = CF.contour_semicircle(-0.5, 0.7, 4)
