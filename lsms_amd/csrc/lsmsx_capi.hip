@@ -12,7 +12,6 @@
 #include <chrono>
 #include <cmath>
 #include <tuple>
-#include <type_traits>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -353,17 +352,15 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
   return check_launch("k_lu_last");
 }
 
-// Refinement sweeps on a chunk whose factors (T = float: mixed path;
-// T = double: FP64-refined path) are in place.  Returns the number of
+// Refinement sweeps, one launch per tile column (k_ir_step), on a chunk whose
+// factors (T = float: mixed path with 32 right-hand sides; T = double:
+// FP64-refined path) are in place.  Returns the number of
 // unconverged solves left in *left.
 template <class T>
 int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int kc, int max_sweeps,
            double tol, int *left) {
   hipLaunchKernelGGL(k_ir_init<T>, dim3(nb), dim3(256), 0, c->stream, dd);
   if (int r = check_launch("k_ir_init")) return r;
-  // mixed path, 16 right-hand sides: the whole sweep in one launch per chunk
-  // (k_ir_sweep); otherwise one launch per tile column (k_ir_step)
-  const bool one_launch = std::is_same<T, float>::value && kc == 16 && nt_max <= kSweepMaxTiles;
   int done = 0;
   for (int it = 0; it < max_sweeps; it++) {
     if (it > 0) {  // k_ir_init set the first residual, B
@@ -374,13 +371,7 @@ int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int kc, int ma
         hipLaunchKernelGGL((k_ir_residual<T, 1>), dim3(nt_max, nb), dim3(256), 0, c->stream, dd);
     }
     if (int r = check_launch("k_ir_residual")) return r;
-    if (one_launch) {
-      ProfScope p(c, 6);
-      if (it == 0)
-        hipLaunchKernelGGL(k_ir_sweep<true>, dim3(nb), dim3(kSweepWaves * 64), 0, c->stream, dd, tol);
-      else
-        hipLaunchKernelGGL(k_ir_sweep<false>, dim3(nb), dim3(kSweepWaves * 64), 0, c->stream, dd, tol);
-    } else {
+    {
       ProfScope p(c, 6);
       if (it == 0) {  // R = B: only the last tile of each solve (kIrFirstSweep)
         hipLaunchKernelGGL((k_ir_step<T, true>), dim3(1, nb), dim3(256), 0, c->stream, dd,
@@ -415,6 +406,29 @@ int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int kc, int ma
   HIPCHK(hipStreamSynchronize(c->stream));
   *left = nb - done;
   return 0;
+}
+
+// Refinement of a chunk part on stream st with no host round trip (mixed
+// path, kc = 16, k_ir_sweep): max_sweeps sweeps are enqueued up front; the
+// residual and sweep kernels return at once for solves already converged,
+// so a solve's arithmetic and sweep count are those of refine<float>.  The
+// part's refinement can then run beside another part's factorization.
+int refine_async(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int max_sweeps,
+                 double tol, hipStream_t st) {
+  hipLaunchKernelGGL(k_ir_init<float>, dim3(nb), dim3(256), 0, st, dd);
+  for (int it = 0; it < max_sweeps; it++) {
+    if (it > 0) {
+      ProfScope p(c, 5);
+      hipLaunchKernelGGL((k_ir_residual<float, 1>), dim3(nt_max, nb), dim3(256), 0, st, dd);
+    }
+    ProfScope p(c, 6);
+    if (it == 0)
+      hipLaunchKernelGGL(k_ir_sweep<true>, dim3(nb), dim3(kSweepWaves * 64), 0, st, dd, tol);
+    else
+      hipLaunchKernelGGL(k_ir_sweep<false>, dim3(nb), dim3(kSweepWaves * 64), 0, st, dd, tol);
+  }
+  hipLaunchKernelGGL(k_ir_extract, dim3(nb), dim3(256), 0, st, dd);
+  return check_launch("refine_async");
 }
 
 int validate_common(lsmsx_ctx *c, int nsc, int nspola) {
@@ -609,6 +623,9 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     // addresses passed as arguments key the graph)
     // solves [b0, b0 + n) of the chunk on stream st; after_asm, if set, is
     // recorded once the assembly is enqueued
+    // the mixed path with 16 right-hand sides refines each part on its
+    // stream right after its factorization (refine_async)
+    const bool ir_inline = mixed && kc == 16 && nt_max <= kSweepMaxTiles;
     auto enqueue_part = [&](hipStream_t st, int b0, int n, hipEvent_t after_asm) -> int {
       const SolveDesc *dd = c->desc.p + b0;
       // the step range is the chunk's: right-aligned starts are relative to it
@@ -628,8 +645,10 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
         hipLaunchKernelGGL(k_copy_m, dim3(ntm, n), dim3(256), 0, st, dd);
         if (int r = check_launch("k_copy_m")) return r;
       }
-      if (mixed) return launch_lu<float>(c, dd, n, ntm, split, st);
-      return launch_lu<double>(c, dd, n, ntm, false, st);
+      if (!mixed) return launch_lu<double>(c, dd, n, ntm, false, st);
+      if (int r = launch_lu<float>(c, dd, n, ntm, split, st)) return r;
+      if (ir_inline) return refine_async(c, dd, n, ntm, c->ir_max_sweeps, c->ir_tol, st);
+      return 0;
     };
     // two halves on two streams, the second started once the first half's
     // assembly is done: its assembly (FP64 VALU work) runs beside the first
@@ -647,9 +666,11 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       return 0;
     };
     if (c->use_graph) {
+      long tol_bits = 0;
+      std::memcpy(&tol_bits, &c->ir_tol, sizeof(tol_bits));
       std::vector<long> key = {nb, nt_max, maxliz, precision, c->sched, (long)c->desc.p,
                                (long)c->tmat.p, c->ldT, (long)c->cgnt.p, (long)c->illp.p,
-                               c->lmax_tab};
+                               c->lmax_tab, c->ir_max_sweeps, tol_bits};
       auto it = c->graphs.find(key);
       if (it == c->graphs.end()) {
         const bool prof = c->profiling;
@@ -677,9 +698,11 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     std::vector<int> hiters(nb, 0);
     if (refining) {
       int left = 0;
-      int r = mixed
-                  ? refine<float>(c, c->desc.p, nb, nt_max, kc, c->ir_max_sweeps, c->ir_tol, &left)
-                  : refine<double>(c, c->desc.p, nb, nt_max, kc, c->ir_max_sweeps, c->ir_tol, &left);
+      int r = ir_inline ? 0
+              : mixed   ? refine<float>(c, c->desc.p, nb, nt_max, kc, c->ir_max_sweeps, c->ir_tol,
+                                        &left)
+                        : refine<double>(c, c->desc.p, nb, nt_max, kc, c->ir_max_sweeps,
+                                         c->ir_tol, &left);
       if (r) return r;
       std::vector<int> hconv(nb);
       HIPCHK(hipMemcpyAsync(hiters.data(), c->iters.p, nb * sizeof(int), hipMemcpyDeviceToHost,
