@@ -273,7 +273,7 @@ def main():
         "split_f16_update": ("k_lu_trail_h", FP16_PEAK_TFLOPS, 3.0, 8, 8,
                              "f16x2-split-update+f32-factor+f64-refine",
                              "mixed: complex FP32 LU with split-FP16 trailing updates on the FP16 "
-                             "matrix cores + FP64 iterative refinement", 4),
+                             "matrix cores + FP64 iterative refinement", 8),
     }[headline]
     upd_flops = update_kernel_flops([nt] * nsolve, group)
     upd_bytes = update_kernel_bytes([nt] * nsolve, cb, pb, group)

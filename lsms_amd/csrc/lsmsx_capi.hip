@@ -69,6 +69,7 @@ __global__ void k_grad_reduce(const double2 *, int, const int *, const long *, c
 __global__ void k_zero_c(double2 *, long);
 __global__ void k_lu_trail_h(const SolveDesc *, int, int);
 __global__ void k_lu_cross_h(const SolveDesc *, int, int);
+__global__ void k_lu_strip_h(const SolveDesc *, int);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout.
 __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
@@ -286,10 +287,11 @@ std::vector<int> site_offsets(const AtomDev &a, int nsc, bool solver) {
   return off;
 }
 
-// Steps per trailing update: four with the split-FP16 update (its C traffic
-// halves against pairs, while the cross updates of the group's later steps
-// stay small), two otherwise.
-inline int lu_group(bool split_f16) { return split_f16 ? 4 : 2; }
+// Steps per trailing update: eight with the split-FP16 update (taken as two
+// halves of four, joined by a strip update: the trailing matrix is read and
+// written once per eight steps while the cross updates stay at most three
+// steps deep), two otherwise.
+inline int lu_group(bool split_f16) { return split_f16 ? 8 : 2; }
 
 // Launch the blocked LU over a chunk of nb solves already in solver layout
 // (each descriptor's group = lu_group(split_f16)): steps taken in groups
@@ -316,22 +318,36 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
     }
     return check_launch("k_lu_diag/k_lu_panel");
   };
+  // split path: the steps of a group in halves of H; the crosses of a half
+  // apply that half's earlier steps, and the strip update the first half to
+  // the second half's columns and rows
+  const int H = split_f16 ? G / 2 : G;
   for (int s = 0; s < nt_max - 1; s += G) {
-    if (int r = diag_panel(s)) return r;
     int last = s;  // last step of the group for the largest solve
-    for (int u = 1; u < G && s + u < nt_max - 1; u++) {
-      {
+    for (int s1 = s; s1 < s + G && s1 < nt_max - 1; s1 += H) {
+      if (s1 > s) {
         ProfScope p(c, 9);
-        const int n1 = nt_max - s - u;
-        if (split_f16)
-          hipLaunchKernelGGL(k_lu_cross_h, dim3((n1 + 1) / 2 + n1 - 1, nb), dim3(256), 0, st, dd,
-                             s, u);
-        else
-          hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * n1 - 1, nb), dim3(256), 0, st, dd, s, u);
+        const int b = std::min(s + G - 1, nt_max - 2);
+        const int w2 = (b - s1 + 2) / 2, nr2 = (nt_max - s1 + 1) / 2, nc2 = (nt_max - b) / 2;
+        hipLaunchKernelGGL(k_lu_strip_h, dim3(nr2 * w2 + w2 * nc2, nb), dim3(512), 0, st, dd, s);
+        if (int r = check_launch("k_lu_strip_h")) return r;
       }
-      if (int r = check_launch("k_lu_cross")) return r;
-      if (int r = diag_panel(s + u)) return r;
-      last = s + u;
+      if (int r = diag_panel(s1)) return r;
+      last = s1;
+      for (int u = 1; u < H && s1 + u < nt_max - 1; u++) {
+        {
+          ProfScope p(c, 9);
+          const int n1 = nt_max - s1 - u;
+          if (split_f16)
+            hipLaunchKernelGGL(k_lu_cross_h, dim3((n1 + 1) / 2 + n1 - 1, nb), dim3(256), 0, st,
+                               dd, s1, u);
+          else
+            hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * n1 - 1, nb), dim3(256), 0, st, dd, s1, u);
+        }
+        if (int r = check_launch("k_lu_cross")) return r;
+        if (int r = diag_panel(s1 + u)) return r;
+        last = s1 + u;
+      }
     }
     const int nrest = nt_max - 1 - last;
     if (nrest > 0) {

@@ -2,8 +2,8 @@
 //
 // The mixed path factors an FP32 copy of the KKR matrix (tau_lu.hip, T =
 // float) and refines against FP64.  Its dominant kernel is the trailing
-// update of a group of four elimination steps, C -= sum_k L_k U_k (K = 256:
-// C is read and written once per four steps).  Here the products run on the
+// update of a group of eight elimination steps, C -= sum_k L_k U_k (K = 512:
+// C is read and written once per eight steps).  Here the products run on the
 // FP16 matrix cores (v_mfma_f32_16x16x32_f16, FP32 accumulation): every FP32
 // panel entry was split by the panel kernel into x 2^-e = hi + lo (FP16 each,
 // one power-of-2 scale e per 64x64 tile), and each real product is
@@ -58,8 +58,12 @@ __device__ __forceinline__ h8 neg8(h8 v) {
 // four groups of a 64-byte row by bits 2-3 of the row index.
 __device__ __forceinline__ int swz_group(int r, int g) { return g ^ ((r >> 2) & 3); }
 
-// Steps per group on this path (the descriptors' group; a power of two).
-constexpr int kGroupH = 4;
+// Steps per trailing update on this path (the descriptors' group; a power of
+// two), taken as two halves of kHalfH: the crosses of each half apply at
+// most kHalfH - 1 steps, and k_lu_strip_h applies the first half to the
+// second half's tile columns and rows before they are factored.
+constexpr int kGroupH = 8;
+constexpr int kHalfH = kGroupH / 2;
 
 // C tiles (r0 + wt, c0 + ct), wt < nrow <= 2, ct < ncol <= NC (a 128 x 64 NC
 // region of 256 NC threads; tiles past nrow / ncol are skipped), -= sum over
@@ -287,6 +291,38 @@ __global__ __launch_bounds__(512) void k_lu_trail_h(const SolveDesc *__restrict_
                  min(2, nrest - 2 * sc), g0, ns, Ls, Us);
 }
 
+// Strip update of a group of kGroupH steps g .. g+7 (local g = step - step0):
+// the first half's steps g .. g+3 applied to the second half's tile columns
+// h .. b (rows >= h) and tile rows h .. b (columns > b), h = g + 4, b its last
+// step, in 128 x 128 regions (K = 256), so that the second half starts from
+// columns and rows updated by the whole first half, like a group of its own.
+// The trailing update of the group then applies all of g .. b to the tiles
+// beyond b.  Nothing to do when the group has no second half.
+__global__ __launch_bounds__(512) void k_lu_strip_h(const SolveDesc *__restrict__ descs, int step) {
+  __shared__ h8 Ls[2][2][4][kTile][4];
+  __shared__ h8 Us[2][2][4][kTile][4];
+  const SolveDesc &d = descs[blockIdx.y];
+  const int g0 = step - d.step0, nt = d.nt;
+  if (g0 < 0) return;
+  if (d.conv && *d.conv) return;
+  const int h = g0 + kHalfH;
+  if (h > nt - 2) return;  // no second half
+  const int b = min(g0 + kGroupH - 1, nt - 2);
+  const int w2 = (b - h + 2) / 2;            // column (row) pairs of the strip
+  const int nr2 = (nt - h + 1) / 2;          // row pairs of its column part
+  const int nc2 = (nt - 1 - b + 1) / 2;      // column pairs of its row part
+  const int x = blockIdx.x;
+  if (x < nr2 * w2) {
+    const int rp = x / w2, cp = x % w2;
+    const int r0 = h + 2 * rp, c0 = h + 2 * cp;
+    region_h<2, 2>(d, r0, min(2, nt - r0), c0, min(2, b + 1 - c0), g0, kHalfH, Ls, Us);
+  } else if (x < nr2 * w2 + w2 * nc2) {
+    const int y = x - nr2 * w2, rp = y / nc2, cp = y % nc2;
+    const int r0 = h + 2 * rp, c0 = b + 1 + 2 * cp;
+    region_h<2, 2>(d, r0, min(2, b + 1 - r0), c0, min(2, nt - c0), g0, kHalfH, Ls, Us);
+  }
+}
+
 // Cross update for step t = g + u of the group starting at global step
 // `step` (see k_lu_cross in tau_lu.hip) on the split planes of steps
 // g .. t-1, in 128 x 64 regions of 256 threads: tile column t (rows >= t,
@@ -298,7 +334,7 @@ __global__ __launch_bounds__(256) void k_lu_cross_h(const SolveDesc *__restrict_
   __shared__ h8 Us[1][1][4][kTile][4];
   const SolveDesc &d = descs[blockIdx.y];
   const int g0 = step - d.step0, t = g0 + u;
-  if (g0 < 0 || u >= min(kGroupH, d.nt - 1 - g0)) return;
+  if (g0 < 0 || u >= min(kHalfH, d.nt - 1 - g0)) return;
   if (d.conv && *d.conv) return;
   const int n1 = d.nt - t, nc = (n1 + 1) / 2, x = blockIdx.x;
   if (x >= nc + n1 - 1) return;
