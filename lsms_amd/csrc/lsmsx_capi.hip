@@ -14,6 +14,7 @@
 #include <tuple>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -69,7 +70,7 @@ __global__ void k_grad_reduce(const double2 *, int, const int *, const long *, c
 __global__ void k_zero_c(double2 *, long);
 __global__ void k_lu_trail_h(const SolveDesc *, int, int);
 __global__ void k_lu_cross_h(const SolveDesc *, int, int);
-__global__ void k_lu_strip_h(const SolveDesc *, int);
+__global__ void k_lu_strip_h(const SolveDesc *, int, int);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout.
 __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
@@ -318,37 +319,44 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
     }
     return check_launch("k_lu_diag/k_lu_panel");
   };
-  // split path: the steps of a group in halves of H; the crosses of a half
-  // apply that half's earlier steps, and the strip update the first half to
-  // the second half's columns and rows
-  const int H = split_f16 ? G / 2 : G;
-  for (int s = 0; s < nt_max - 1; s += G) {
-    int last = s;  // last step of the group for the largest solve
-    for (int s1 = s; s1 < s + G && s1 < nt_max - 1; s1 += H) {
-      if (s1 > s) {
-        ProfScope p(c, 9);
-        const int b = std::min(s + G - 1, nt_max - 2);
-        const int w2 = (b - s1 + 2) / 2, nr2 = (nt_max - s1 + 1) / 2, nc2 = (nt_max - b) / 2;
-        hipLaunchKernelGGL(k_lu_strip_h, dim3(nr2 * w2 + w2 * nc2, nb), dim3(512), 0, st, dd, s);
-        if (int r = check_launch("k_lu_strip_h")) return r;
-      }
-      if (int r = diag_panel(s1)) return r;
-      last = s1;
-      for (int u = 1; u < H && s1 + u < nt_max - 1; u++) {
+  // the steps s .. s+n-1 of a group: n <= H, panels and crosses step by
+  // step; otherwise the first half, the strip update of the second half's
+  // columns and rows with the first half, then the second half
+  const int H = split_f16 ? 4 : G;
+  std::function<int(int, int)> block = [&](int s, int n) -> int {
+    if (s >= nt_max - 1) return 0;
+    if (n <= H) {
+      if (int r = diag_panel(s)) return r;
+      for (int u = 1; u < n && s + u < nt_max - 1; u++) {
         {
           ProfScope p(c, 9);
-          const int n1 = nt_max - s1 - u;
+          const int n1 = nt_max - s - u;
           if (split_f16)
             hipLaunchKernelGGL(k_lu_cross_h, dim3((n1 + 1) / 2 + n1 - 1, nb), dim3(256), 0, st,
-                               dd, s1, u);
+                               dd, s, u);
           else
-            hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * n1 - 1, nb), dim3(256), 0, st, dd, s1, u);
+            hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * n1 - 1, nb), dim3(256), 0, st, dd, s, u);
         }
         if (int r = check_launch("k_lu_cross")) return r;
-        if (int r = diag_panel(s1 + u)) return r;
-        last = s1 + u;
+        if (int r = diag_panel(s + u)) return r;
       }
+      return 0;
     }
+    const int h = n / 2;
+    if (int r = block(s, h)) return r;
+    if (s + h >= nt_max - 1) return 0;
+    {
+      ProfScope p(c, 9);
+      const int hs = s + h, b = std::min(hs + h - 1, nt_max - 2);
+      const int w2 = (b - hs + 2) / 2, nr2 = (nt_max - hs + 1) / 2, nc2 = (nt_max - b) / 2;
+      hipLaunchKernelGGL(k_lu_strip_h, dim3(nr2 * w2 + w2 * nc2, nb), dim3(512), 0, st, dd, s, h);
+      if (int r = check_launch("k_lu_strip_h")) return r;
+    }
+    return block(s + h, h);
+  };
+  for (int s = 0; s < nt_max - 1; s += G) {
+    if (int r = block(s, G)) return r;
+    const int last = std::min(s + G - 1, nt_max - 2);  // the group's last step, largest solve
     const int nrest = nt_max - 1 - last;
     if (nrest > 0) {
       ProfScope p(c, 3);

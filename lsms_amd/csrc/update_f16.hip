@@ -59,11 +59,12 @@ __device__ __forceinline__ h8 neg8(h8 v) {
 __device__ __forceinline__ int swz_group(int r, int g) { return g ^ ((r >> 2) & 3); }
 
 // Steps per trailing update on this path (the descriptors' group; a power of
-// two), taken as two halves of kHalfH: the crosses of each half apply at
-// most kHalfH - 1 steps, and k_lu_strip_h applies the first half to the
-// second half's tile columns and rows before they are factored.
+// two), taken by recursive halving down to blocks of kCrossH steps: the
+// crosses within a block apply at most kCrossH - 1 steps, and k_lu_strip_h
+// applies each first half to its second half's tile columns and rows before
+// they are factored.
 constexpr int kGroupH = 8;
-constexpr int kHalfH = kGroupH / 2;
+constexpr int kCrossH = 4;
 
 // C tiles (r0 + wt, c0 + ct), wt < nrow <= 2, ct < ncol <= NC (a 128 x 64 NC
 // region of 256 NC threads; tiles past nrow / ncol are skipped), -= sum over
@@ -291,23 +292,25 @@ __global__ __launch_bounds__(512) void k_lu_trail_h(const SolveDesc *__restrict_
                  min(2, nrest - 2 * sc), g0, ns, Ls, Us);
 }
 
-// Strip update of a group of kGroupH steps g .. g+7 (local g = step - step0):
-// the first half's steps g .. g+3 applied to the second half's tile columns
-// h .. b (rows >= h) and tile rows h .. b (columns > b), h = g + 4, b its last
-// step, in 128 x 128 regions (K = 256), so that the second half starts from
-// columns and rows updated by the whole first half, like a group of its own.
-// The trailing update of the group then applies all of g .. b to the tiles
-// beyond b.  Nothing to do when the group has no second half.
-__global__ __launch_bounds__(512) void k_lu_strip_h(const SolveDesc *__restrict__ descs, int step) {
+// Strip update between the halves of a block of 2 ns steps g .. g+2ns-1
+// (local g = step - step0): the first half's steps g .. g+ns-1 applied to the
+// second half's tile columns h .. b (rows >= h) and tile rows h .. b
+// (columns > b), h = g + ns, b its last step, in 128 x 128 regions
+// (K = 64 ns), so that the second half starts from columns and rows updated
+// by the whole first half, like a block of its own.  The group's trailing
+// update then applies all of its steps to the tiles beyond the group.
+// Nothing to do when the block has no second half.
+__global__ __launch_bounds__(512) void k_lu_strip_h(const SolveDesc *__restrict__ descs, int step,
+                                                    int ns) {
   __shared__ h8 Ls[2][2][4][kTile][4];
   __shared__ h8 Us[2][2][4][kTile][4];
   const SolveDesc &d = descs[blockIdx.y];
   const int g0 = step - d.step0, nt = d.nt;
   if (g0 < 0) return;
   if (d.conv && *d.conv) return;
-  const int h = g0 + kHalfH;
+  const int h = g0 + ns;
   if (h > nt - 2) return;  // no second half
-  const int b = min(g0 + kGroupH - 1, nt - 2);
+  const int b = min(h + ns - 1, nt - 2);
   const int w2 = (b - h + 2) / 2;            // column (row) pairs of the strip
   const int nr2 = (nt - h + 1) / 2;          // row pairs of its column part
   const int nc2 = (nt - 1 - b + 1) / 2;      // column pairs of its row part
@@ -315,11 +318,11 @@ __global__ __launch_bounds__(512) void k_lu_strip_h(const SolveDesc *__restrict_
   if (x < nr2 * w2) {
     const int rp = x / w2, cp = x % w2;
     const int r0 = h + 2 * rp, c0 = h + 2 * cp;
-    region_h<2, 2>(d, r0, min(2, nt - r0), c0, min(2, b + 1 - c0), g0, kHalfH, Ls, Us);
+    region_h<2, 2>(d, r0, min(2, nt - r0), c0, min(2, b + 1 - c0), g0, ns, Ls, Us);
   } else if (x < nr2 * w2 + w2 * nc2) {
     const int y = x - nr2 * w2, rp = y / nc2, cp = y % nc2;
     const int r0 = h + 2 * rp, c0 = b + 1 + 2 * cp;
-    region_h<2, 2>(d, r0, min(2, b + 1 - r0), c0, min(2, nt - c0), g0, kHalfH, Ls, Us);
+    region_h<2, 2>(d, r0, min(2, b + 1 - r0), c0, min(2, nt - c0), g0, ns, Ls, Us);
   }
 }
 
@@ -334,7 +337,7 @@ __global__ __launch_bounds__(256) void k_lu_cross_h(const SolveDesc *__restrict_
   __shared__ h8 Us[1][1][4][kTile][4];
   const SolveDesc &d = descs[blockIdx.y];
   const int g0 = step - d.step0, t = g0 + u;
-  if (g0 < 0 || u >= min(kHalfH, d.nt - 1 - g0)) return;
+  if (g0 < 0 || u >= min(kCrossH, d.nt - 1 - g0)) return;
   if (d.conv && *d.conv) return;
   const int n1 = d.nt - t, nc = (n1 + 1) / 2, x = blockIdx.x;
   if (x >= nc + n1 - 1) return;
