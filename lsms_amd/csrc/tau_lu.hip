@@ -66,69 +66,98 @@ __device__ void tile_lu_lds(CxT<T> (*A)[kTile + 1], int nd, int *piv, int *perm,
     const bool nz = (a.x != T(0) || a.y != T(0));
     if (nz && tid > j && tid < 64) A[j][tid] = cmul2(A[j][tid], crecip(a));
     __syncthreads();
-    if (nz) {
-      for (int e = tid; e < 64 * 64; e += 256) {
-        int r = e & 63, c = e >> 6;
-        if (r > j && c > j) {
-          V l = A[j][r], u = A[c][j];
-          A[c][r].x -= l.x * u.x - l.y * u.y;
-          A[c][r].y -= l.x * u.y + l.y * u.x;
-        }
+    // rank-1 update of the trailing block: thread (row r = tid % 64, column
+    // class tid / 64) takes columns c > j of its class, with l = A(r, j) read
+    // once
+    if (nz && (tid & 63) > j) {
+      const int r = tid & 63;
+      const V l = A[j][r];
+      int c = tid >> 6;
+      if (c <= j) c += ((j - c) / 4 + 1) * 4;
+      for (; c < 64; c += 4) {
+        const V u = A[c][j];
+        A[c][r].x -= l.x * u.x - l.y * u.y;
+        A[c][r].y -= l.x * u.y + l.y * u.x;
       }
     }
     __syncthreads();
   }
 }
 
-// In-place inversion of the factored tile: unit-lower L (threads 0..63 own
-// rows) and upper U (threads 64..127 own rows).  L^-1: for j = 63..0,
+// In-place inversion of the factored tile: unit-lower L (waves 0 and 2 own
+// rows) and upper U (waves 1 and 3 own rows).  L^-1: for j = 63..0,
 // X[r][j] = -(L[r][j] + sum_{j<k<r} X[r][k] L[k][j]) for r > j.  U^-1: for
 // j = 0..63, X[j][j] = 1/U[j][j], X[r][j] = -X[j][j] sum_{r<=k<j} X[r][k] U[k][j].
+// Each sum is split between the two waves of its part by the parity of k
+// (two accumulators per thread), so the dependent LDS loads of a row are a
+// quarter as long; the second wave's half goes through LDS.
 template <class T>
 __device__ void tile_invert_lds(CxT<T> (*A)[kTile + 1]) {
   typedef CxT<T> V;
-  const int tid = threadIdx.x;
+  __shared__ V part[2][kTile];
+  const int tid = threadIdx.x, r = tid & 63, w = tid >> 6;
+  const bool lpart = (w & 1) == 0;  // waves 0, 2: L^-1; waves 1, 3: U^-1
+  const int half = w >> 1;          // the parity of k this wave sums
+  auto madd = [](V &acc, V x, V y) {
+    acc.x += x.x * y.x - x.y * y.y;
+    acc.y += x.x * y.y + x.y * y.x;
+  };
   for (int jj = 0; jj < kTile; jj++) {
-    V nl = czero<V>(), nu = czero<V>();
     const int jl = kTile - 1 - jj;
     const int ju = jj;
-    if (tid < 64) {
-      int r = tid;
+    V s0 = czero<V>(), s1 = czero<V>();
+    if (lpart) {
       if (r > jl) {
-        V s = A[jl][r];
-        for (int k = jl + 1; k < r; k++) {
-          V x = A[k][r], l = A[jl][k];
-          s.x += x.x * l.x - x.y * l.y;
-          s.y += x.x * l.y + x.y * l.x;
+        int k = jl + 1 + half;
+        for (; k + 2 < r; k += 4) {
+          madd(s0, A[k][r], A[jl][k]);
+          madd(s1, A[k + 2][r], A[jl][k + 2]);
         }
-        nl.x = -s.x;
-        nl.y = -s.y;
+        if (k < r) madd(s0, A[k][r], A[jl][k]);
       }
-    } else if (tid < 128) {
-      int r = tid - 64;
-      V ujj = A[ju][ju];
-      bool nz = (ujj.x != T(0) || ujj.y != T(0));
-      V xjj = nz ? crecip(ujj) : czero<V>();
-      if (r < ju) {
-        V s = czero<V>();
-        for (int k = r; k < ju; k++) {
-          V x = A[k][r], u = A[ju][k];
-          s.x += x.x * u.x - x.y * u.y;
-          s.y += x.x * u.y + x.y * u.x;
+    } else if (r < ju) {
+      int k = r + half;
+      for (; k + 2 < ju; k += 4) {
+        madd(s0, A[k][r], A[ju][k]);
+        madd(s1, A[k + 2][r], A[ju][k + 2]);
+      }
+      if (k < ju) madd(s0, A[k][r], A[ju][k]);
+    }
+    s0.x += s1.x;
+    s0.y += s1.y;
+    if (half == 1) part[lpart ? 0 : 1][r] = s0;
+    __syncthreads();
+    V nv = czero<V>();
+    if (half == 0) {
+      const V o = part[lpart ? 0 : 1][r];
+      s0.x += o.x;
+      s0.y += o.y;
+      if (lpart) {
+        if (r > jl) {
+          const V a = A[jl][r];
+          nv.x = -(a.x + s0.x);
+          nv.y = -(a.y + s0.y);
         }
-        V v = cmul2(s, xjj);
-        nu.x = -v.x;
-        nu.y = -v.y;
-      } else if (r == ju) {
-        nu = xjj;
+      } else {
+        const V ujj = A[ju][ju];
+        const bool nz = (ujj.x != T(0) || ujj.y != T(0));
+        const V xjj = nz ? crecip(ujj) : czero<V>();
+        if (r < ju) {
+          const V v = cmul2(s0, xjj);
+          nv.x = -v.x;
+          nv.y = -v.y;
+        } else if (r == ju) {
+          nv = xjj;
+        }
       }
     }
-    __syncthreads();
-    if (tid < 64) {
-      if (tid > jl) A[jl][tid] = nl;
-    } else if (tid < 128) {
-      int r = tid - 64;
-      if (r <= ju) A[ju][r] = nu;
+    __syncthreads();  // every read of this step is done
+    if (half == 0) {
+      if (lpart) {
+        if (r > jl) A[jl][r] = nv;
+      } else if (r <= ju) {
+        A[ju][r] = nv;
+      }
     }
     __syncthreads();
   }
