@@ -35,15 +35,15 @@ __device__ void tile_lu_lds(CxT<T> (*A)[kTile + 1], int nd, int *piv, int *perm,
                             double *sminpiv, int col_base) {
   typedef CxT<T> V;
   const int tid = threadIdx.x;
-  __shared__ int s_p;
   for (int j = 0; j < kTile; j++) {
     const int hi = (j < nd) ? nd : kTile;
     if (tid < 64) {
+      // wave 0: pivot search, row swap (lane = column) and column scaling;
+      // wave-scope ordering of its LDS traffic is enough between them
       T v = (tid >= j && tid < hi) ? cabs1(A[j][tid]) : T(-1);
       int idx = tid;
       wave_argmax(v, idx);
       if (tid == 0) {
-        s_p = idx;
         piv[j] = idx;
         int t = perm[j];
         perm[j] = perm[idx];
@@ -53,19 +53,21 @@ __device__ void tile_lu_lds(CxT<T> (*A)[kTile + 1], int nd, int *piv, int *perm,
         if (a < *sminpiv) *sminpiv = a;
         if (v == T(0) && *sinfo == 0) *sinfo = col_base + j + 1;
       }
-    }
-    __syncthreads();
-    const int p = s_p;
-    if (p != j && tid < 64) {
-      V t = A[tid][j];
-      A[tid][j] = A[tid][p];
-      A[tid][p] = t;
+      const int p = idx;
+      if (p != j) {
+        V t = A[tid][j];
+        A[tid][j] = A[tid][p];
+        A[tid][p] = t;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const V a = A[j][j];
+      if ((a.x != T(0) || a.y != T(0)) && tid > j) A[j][tid] = cmul2(A[j][tid], crecip(a));
     }
     __syncthreads();
     const V a = A[j][j];
     const bool nz = (a.x != T(0) || a.y != T(0));
-    if (nz && tid > j && tid < 64) A[j][tid] = cmul2(A[j][tid], crecip(a));
-    __syncthreads();
     // rank-1 update of the trailing block: thread (row r = tid % 64, column
     // class tid / 64) takes columns c > j of its class, with l = A(r, j) read
     // once
@@ -74,10 +76,23 @@ __device__ void tile_lu_lds(CxT<T> (*A)[kTile + 1], int nd, int *piv, int *perm,
       const V l = A[j][r];
       int c = tid >> 6;
       if (c <= j) c += ((j - c) / 4 + 1) * 4;
-      for (; c < 64; c += 4) {
-        const V u = A[c][j];
-        A[c][r].x -= l.x * u.x - l.y * u.y;
-        A[c][r].y -= l.x * u.y + l.y * u.x;
+      // four columns at a time, all loads before the stores, so the LDS
+      // round trips overlap
+      for (; c < 64; c += 16) {
+        V u[4], x[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (c + 4 * q < 64) {
+            u[q] = A[c + 4 * q][j];
+            x[q] = A[c + 4 * q][r];
+          }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (c + 4 * q < 64) {
+            x[q].x -= l.x * u[q].x - l.y * u[q].y;
+            x[q].y -= l.x * u[q].y + l.y * u[q].x;
+            A[c + 4 * q][r] = x[q];
+          }
       }
     }
     __syncthreads();
