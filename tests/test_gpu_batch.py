@@ -87,6 +87,43 @@ def test_graph_replay_matches_eager(c4):
     ctx.set_schedule(0)
 
 
+def test_mixed_graph_replay_matches_eager(c4):
+    """Mixed split-FP16 path, whose captured graph also holds the refinement
+    sweeps: replays (with eager work between them) equal the eager result
+    bitwise, and a change of the refinement limits is honoured by the graph
+    (one sweep allowed: every solve falls back to the FP64 redo, as eager)."""
+    s, ctx = c4
+    atoms = pick(s, 8)
+    iies = [1, 9, 17, 25, 4, 12, 20, 28]
+    pr = [s.prel(i) for i in iies]
+    mixed = capi.PREC_MIXED_F16
+    ctx.set_schedule(capi.SCHED_RIGHT_ALIGN | capi.SCHED_TWO_STREAMS)
+    ctx.set_graph(False)
+    eager, info = ctx.tau00_batched(atoms, iies, pr, 16, precision=mixed)
+    it_eager, _ = ctx.last_stats(len(atoms))
+    assert np.all(info == 0) and np.all(it_eager >= 2)
+    ctx.set_graph(True)
+    g1, _ = ctx.tau00_batched(atoms, iies, pr, 16, precision=mixed)
+    ctx.set_graph(False)
+    ctx.tau00_batched(atoms[:3], iies[:3], pr[:3], 16, precision=mixed)
+    ctx.set_graph(True)
+    g2, _ = ctx.tau00_batched(atoms, iies, pr, 16, precision=mixed)
+    it_graph, _ = ctx.last_stats(len(atoms))
+    assert np.array_equal(g1, eager) and np.array_equal(g2, eager)
+    assert np.array_equal(it_graph, it_eager)
+    ctx.set_refinement(max_sweeps=1, tol=1e-12)
+    g3, info3 = ctx.tau00_batched(atoms, iies, pr, 16, precision=mixed)
+    it3, _ = ctx.last_stats(len(atoms))
+    ctx.set_graph(False)
+    e3, _ = ctx.tau00_batched(atoms, iies, pr, 16, precision=mixed)
+    ctx.set_refinement()
+    ctx.set_schedule(0)
+    assert np.all(info3 == 0) and np.all(it3 == -1)
+    assert np.array_equal(g3, e3)
+    for b in (0, len(atoms) - 1):
+        assert rel(eager[b], lapack_tau00(s, atoms[b], iies[b])) < 1e-10
+
+
 def test_autotune_then_same_results(c4):
     s, ctx = c4
     atoms = pick(s, 12)
