@@ -92,8 +92,11 @@ int lsmsx_tau00(lsmsx_ctx *ctx, int atom_id, int ispin, int iie,
 
 /* Batched form: nsolve independent (atom, spin, energy slot) solves in one
  * grouped launch sequence.  prel holds 2*nsolve doubles; tau00 receives
- * nsolve blocks of kkrsz_ns^2 complex; info nsolve ints.  Replaces the
- * OpenMP loop of calculateAllTauMatrices (calculateTauMatrix.cpp:841-945). */
+ * nsolve blocks of kkrsz_ns^2 complex; info nsolve ints (0 ok, > 0 the
+ * first singular pivot, -2 an internal wait bound of the mixed path's
+ * refinement sweep was hit: never expected, the solve's tau00 is then not
+ * valid).  Replaces the OpenMP loop of calculateAllTauMatrices
+ * (calculateTauMatrix.cpp:841-945). */
 int lsmsx_tau00_batched(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
                         const int *ispins, const int *iies, const double *prel,
                         int n_spin_cant, int n_spin_pola, int precision,

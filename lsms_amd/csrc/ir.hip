@@ -638,9 +638,18 @@ __global__ __launch_bounds__(kSweepWaves * 64) void k_ir_sweep(const SolveDesc *
         }
       }
   };
+  // Every wait ends (see above); the spin bound (about half a second) only
+  // guards the GPU against a broken invariant: the solve then reports
+  // kInfoSweepStall and the wave goes on, so the kernel still exits.
   auto wait_flag = [&](int *flag) __attribute__((always_inline)) {
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+    unsigned spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
+      if (++spins > kSweepSpinLimit) {
+        if (lane == 0) *d.info = kInfoSweepStall;
+        break;
+      }
       __builtin_amdgcn_s_sleep(1);
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   };
   auto flag_set = [&](int *flag) __attribute__((always_inline)) {

@@ -20,6 +20,8 @@ constexpr int kIrFirstSweep = -1000;  // k_ir_step's step value for the first fo
 // and the largest number of 64-row tiles per solve it takes
 constexpr int kSweepWaves = 8;
 constexpr int kSweepMaxTiles = 128;
+constexpr unsigned kSweepSpinLimit = 1u << 24;  // bound of one k_ir_sweep flag wait
+constexpr int kInfoSweepStall = -2;             // per-solve info when that bound is hit
 
 // Element (r, c) of a tile-major matrix with nt tiles per tile column: every
 // 64 x 64 tile is one contiguous run (column-major inside), tiles ordered by
