@@ -332,7 +332,7 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
           ProfScope p(c, 9);
           const int n1 = nt_max - s - u;
           if (split_f16)
-            hipLaunchKernelGGL(k_lu_cross_h, dim3((n1 + 1) / 2 + n1 - 1, nb), dim3(256), 0, st,
+            hipLaunchKernelGGL(k_lu_cross_h, dim3((n1 + 1) / 2 + n1 / 2, nb), dim3(256), 0, st,
                                dd, s, u);
           else
             hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * n1 - 1, nb), dim3(256), 0, st, dd, s, u);

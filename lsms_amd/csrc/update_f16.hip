@@ -66,23 +66,23 @@ __device__ __forceinline__ int swz_group(int r, int g) { return g ^ ((r >> 2) & 
 constexpr int kGroupH = 8;
 constexpr int kCrossH = 4;
 
-// C tiles (r0 + wt, c0 + ct), wt < nrow <= 2, ct < ncol <= NC (a 128 x 64 NC
-// region of 256 NC threads; tiles past nrow / ncol are skipped), -= sum over
-// the steps s_first .. s_first+ns-1 of L(., s) U(s, .), on the split-FP16
-// planes of those steps.  Waves: wt = row tile, ct = column tile, wc = the
-// 32-column half.  Per 32-wide k chunk the planes of the two L row tiles and
-// the NC U column tiles are staged in LDS, (2 + NC) x 16 KiB, so a chunk's
-// staging serves 2 x NC output tiles.  DB = 1: one LDS stage, the next
+// C tiles (r0 + wt, c0 + ct), wt < nrow <= RT, ct < ncol <= NC (a 64 RT x
+// 64 NC region of 128 RT NC threads; tiles past nrow / ncol are skipped),
+// -= sum over the steps s_first .. s_first+ns-1 of L(., s) U(s, .), on the
+// split-FP16 planes of those steps.  Waves: wt = row tile, ct = column tile,
+// wc = the 32-column half.  Per 32-wide k chunk the planes of the RT L row
+// tiles and the NC U column tiles are staged in LDS, (RT + NC) x 16 KiB, so a
+// chunk's staging serves RT x NC output tiles.  DB = 1: one LDS stage, the next
 // chunk's loads in flight during this chunk's MFMAs, two barriers per chunk.
 // DB = 2: two LDS stages and two register stages, loads issued two chunks
 // ahead, one barrier per chunk.
-template <int NC, int DB>
+template <int RT, int NC, int DB>
 __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, int c0, int ncol,
-                                         int s_first, int ns, h8 (&Ls)[DB][2][4][kTile][4],
+                                         int s_first, int ns, h8 (&Ls)[DB][RT][4][kTile][4],
                                          h8 (&Us)[DB][NC][4][kTile][4]) {
-  constexpr int NT = 256 * NC;                 // threads
-  constexpr int NQ = (2 + NC) * 1024 / NT;     // 8-half loads per thread per chunk
-  constexpr int NQL = 2048 / NT;               // of which L planes
+  constexpr int NT = 128 * RT * NC;            // threads
+  constexpr int NQ = (RT + NC) * 1024 / NT;    // 8-half loads per thread per chunk
+  constexpr int NQL = RT * 1024 / NT;          // of which L planes
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
   const int wt = w / (2 * NC), ct = (w >> 1) % NC, wc = 32 * (w & 1);
   const int ti = r0 + wt, tj = c0 + ct;
@@ -108,7 +108,7 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
       return (GH8 *)(hp + (long)((par * 2 + 0) * nt + t) * kPlaneHalves) +
              ((plane * kTile + rr) * kTile + 32 * kh + 8 * grp) / 8;
     }
-    const int f = e8 - 2048, cu = f >> 10, plane = (f >> 8) & 3, rr = (f >> 2) & 63, grp = f & 3;
+    const int f = e8 - RT * 1024, cu = f >> 10, plane = (f >> 8) & 3, rr = (f >> 2) & 63, grp = f & 3;
     const int t = min(c0 + cu, nt - 1);
     return (GH8 *)(hp + (long)((par * 2 + 1) * nt + t) * kPlaneHalves) +
            ((plane * kTile + rr) * kTile + 32 * kh + 8 * grp) / 8;
@@ -119,7 +119,7 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
       const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
       return &Ls[buf][tt][plane][rr][swz_group(rr, grp)];
     }
-    const int f = e8 - 2048, cu = f >> 10, plane = (f >> 8) & 3, rr = (f >> 2) & 63, grp = f & 3;
+    const int f = e8 - RT * 1024, cu = f >> 10, plane = (f >> 8) & 3, rr = (f >> 2) & 63, grp = f & 3;
     return &Us[buf][cu][plane][rr][swz_group(rr, grp)];
   };
   f4 cr[4][2], ci[4][2];
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(512) void k_lu_trail_h(const SolveDesc *__restrict_
   const int br = bx / (kBlk * np), hb = min(kBlk, np - br * kBlk);
   const int off = bx - br * kBlk * np, bc = off / (hb * kBlk), off2 = off - bc * hb * kBlk;
   const int sr = br * kBlk + off2 % hb, sc = bc * kBlk + off2 / hb;
-  region_h<2, 2>(d, b + 1 + 2 * sr, min(2, nrest - 2 * sr), b + 1 + 2 * sc,
+  region_h<2, 2, 2>(d, b + 1 + 2 * sr, min(2, nrest - 2 * sr), b + 1 + 2 * sc,
                  min(2, nrest - 2 * sc), g0, ns, Ls, Us);
 }
 
@@ -318,31 +318,38 @@ __global__ __launch_bounds__(512) void k_lu_strip_h(const SolveDesc *__restrict_
   if (x < nr2 * w2) {
     const int rp = x / w2, cp = x % w2;
     const int r0 = h + 2 * rp, c0 = h + 2 * cp;
-    region_h<2, 2>(d, r0, min(2, nt - r0), c0, min(2, b + 1 - c0), g0, ns, Ls, Us);
+    region_h<2, 2, 2>(d, r0, min(2, nt - r0), c0, min(2, b + 1 - c0), g0, ns, Ls, Us);
   } else if (x < nr2 * w2 + w2 * nc2) {
     const int y = x - nr2 * w2, rp = y / nc2, cp = y % nc2;
     const int r0 = h + 2 * rp, c0 = b + 1 + 2 * cp;
-    region_h<2, 2>(d, r0, min(2, b + 1 - r0), c0, min(2, nt - c0), g0, ns, Ls, Us);
+    region_h<2, 2, 2>(d, r0, min(2, b + 1 - r0), c0, min(2, nt - c0), g0, ns, Ls, Us);
   }
 }
 
-// Cross update for step t = g + u of the group starting at global step
+// Cross update for step t = g + u of the block starting at global step
 // `step` (see k_lu_cross in tau_lu.hip) on the split planes of steps
-// g .. t-1, in 128 x 64 regions of 256 threads: tile column t (rows >= t,
-// two row tiles per workgroup), then tile row t (columns > t, one tile per
-// workgroup).
+// g .. t-1, 256 threads per workgroup: tile column t (rows >= t) in 128 x 64
+// regions (two row tiles), then tile row t (columns > t) in 64 x 128 regions
+// (two column tiles, sharing the row's L planes), so every wave has a tile.
 __global__ __launch_bounds__(256) void k_lu_cross_h(const SolveDesc *__restrict__ descs, int step,
                                                     int u) {
-  __shared__ h8 Ls[1][2][4][kTile][4];
-  __shared__ h8 Us[1][1][4][kTile][4];
+  // three tiles' planes: (L, L, U) for the column part, (L, U, U) for the row part
+  __shared__ h8 stage[3][4][kTile][4];
   const SolveDesc &d = descs[blockIdx.y];
   const int g0 = step - d.step0, t = g0 + u;
   if (g0 < 0 || u >= min(kCrossH, d.nt - 1 - g0)) return;
   if (d.conv && *d.conv) return;
-  const int n1 = d.nt - t, nc = (n1 + 1) / 2, x = blockIdx.x;
-  if (x >= nc + n1 - 1) return;
-  if (x < nc) region_h<1, 1>(d, t + 2 * x, min(2, n1 - 2 * x), t, 1, g0, u, Ls, Us);
-  else region_h<1, 1>(d, t, 1, t + 1 + (x - nc), 1, g0, u, Ls, Us);
+  const int n1 = d.nt - t, nc = (n1 + 1) / 2, nr = n1 / 2, x = blockIdx.x;
+  if (x < nc) {
+    auto &Ls = *reinterpret_cast<h8(*)[1][2][4][kTile][4]>(&stage[0]);
+    auto &Us = *reinterpret_cast<h8(*)[1][1][4][kTile][4]>(&stage[2]);
+    region_h<2, 1, 1>(d, t + 2 * x, min(2, n1 - 2 * x), t, 1, g0, u, Ls, Us);
+  } else if (x < nc + nr) {
+    auto &Ls = *reinterpret_cast<h8(*)[1][1][4][kTile][4]>(&stage[0]);
+    auto &Us = *reinterpret_cast<h8(*)[1][2][4][kTile][4]>(&stage[1]);
+    const int c0 = t + 1 + 2 * (x - nc);
+    region_h<1, 2, 1>(d, t, 1, c0, min(2, d.nt - c0), g0, u, Ls, Us);
+  }
 }
 
 }  // namespace lsmsx
