@@ -115,6 +115,33 @@ def test_mixed_fallback_to_fp64(mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_mixed_chunked_equals_unchunked(mode):
+    """A batch split into chunks of one solve (one stream, no halves) and of a
+    few solves gives the unchunked results and sweep counts bitwise; an empty
+    batch returns at once."""
+    s = CF.config_c1()
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    atoms = np.array([0, 1, 0, 1, 0, 1, 0])
+    iies = np.array([0, 3, 7, 11, 19, 25, 30])
+    prels = np.array([s.prel(i) for i in iies])
+    t1, i1 = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
+    it1, _ = ctx.last_stats(len(atoms))
+    for limit in (1, 12_000_000):
+        ctx.set_workspace_limit(limit)
+        t2, i2 = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
+        it2, _ = ctx.last_stats(len(atoms))
+        assert np.array_equal(t1, t2) and np.array_equal(i1, i2), limit
+        assert np.array_equal(it1, it2), limit
+    ctx.set_workspace_limit(0)
+    t0, i0 = ctx.tau00_batched(np.zeros(0, int), np.zeros(0, int), np.zeros(0, complex), 16,
+                               precision=mode)
+    ctx.close()
+    assert t0.shape == (0, 16, 16) and i0.shape == (0,)
+    assert np.all(i1 == 0) and np.all(it1 >= 2)
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_mixed_tile_counts_one_to_seven(mode):
     """Solves of 1, 2, 5 and 7 tiles (N = 16, 112, 304, 432) in one batch:
     the one-launch refinement sweep deals tile rows to eight waves, so this
