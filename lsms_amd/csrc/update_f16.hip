@@ -50,8 +50,10 @@ __device__ __forceinline__ f4 mfma_h(h8 a, h8 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
+// Sign flip of eight halves as four 32-bit XORs of the packed sign bits.
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ h8 neg8(h8 v) {
-  return -v;
+  return __builtin_bit_cast(h8, __builtin_bit_cast(u4v, v) ^ 0x80008000u);
 }
 
 // LDS k group of row r holding logical group g (8 halves): XOR swizzle of the
@@ -141,7 +143,10 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
   };
   auto compute = [&](int buf) __attribute__((always_inline)) {
     if (!valid) return;
-    h8 ur_h[2], ur_l[2], ui_h[2], ui_l[2];
+    // U operands of the wave's two 16-column blocks, with -Re(U) formed once
+    // here for the -Im(L) Re(U) term (cheaper than negating Im(L) per row
+    // block; Im(L) (-Re U) = -(Im(L) Re(U)) exactly)
+    h8 ur_h[2], ur_l[2], ui_h[2], ui_l[2], nur_h[2], nur_l[2];
 #pragma unroll
     for (int bj = 0; bj < 2; bj++) {
       const int col = wc + 16 * bj + l16, gg = swz_group(col, g);
@@ -149,13 +154,14 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
       ur_l[bj] = Us[buf][ct][1][col][gg];
       ui_h[bj] = Us[buf][ct][2][col][gg];
       ui_l[bj] = Us[buf][ct][3][col][gg];
+      nur_h[bj] = neg8(ur_h[bj]);
+      nur_l[bj] = neg8(ur_l[bj]);
     }
 #pragma unroll
     for (int bi = 0; bi < 4; bi++) {
       const int row = 16 * bi + l16, gg = swz_group(row, g);
       const h8 nar_h = Ls[buf][wt][0][row][gg], nar_l = Ls[buf][wt][1][row][gg];
       const h8 ai_h = Ls[buf][wt][2][row][gg], ai_l = Ls[buf][wt][3][row][gg];
-      const h8 nai_h = neg8(ai_h), nai_l = neg8(ai_l);
 #pragma unroll
       for (int bj = 0; bj < 2; bj++) {
         f4 r = cr[bi][bj], m = ci[bi][bj];
@@ -168,9 +174,9 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
         m = mfma_h(ui_h[bj], nar_h, m);
         m = mfma_h(ui_l[bj], nar_h, m);
         m = mfma_h(ui_h[bj], nar_l, m);
-        m = mfma_h(ur_h[bj], nai_h, m);
-        m = mfma_h(ur_l[bj], nai_h, m);
-        m = mfma_h(ur_h[bj], nai_l, m);
+        m = mfma_h(nur_h[bj], ai_h, m);
+        m = mfma_h(nur_l[bj], ai_h, m);
+        m = mfma_h(nur_h[bj], ai_l, m);
         cr[bi][bj] = r;
         ci[bi][bj] = m;
       }
