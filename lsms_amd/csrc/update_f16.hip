@@ -130,6 +130,7 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
   auto rescale = [&](int ch) __attribute__((always_inline)) {
     if (ch > 0 && !(ch & 1) && valid) {
       const int e2 = tile_exp(s_first + (ch >> 1));
+      if (e2 == e) return;  // a factor of exactly 1
       const float sc1 = ldexpf(1.f, e - e2);
 #pragma unroll
       for (int bi = 0; bi < 4; bi++)
