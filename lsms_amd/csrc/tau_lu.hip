@@ -317,9 +317,12 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
   __syncthreads();
   Acc3<T> q;
   zero_acc(q);
+  // operand k-steps in flight: three for FP64 (the inverse's L2 latency),
+  // one for FP32, where more registers would cost a wave per SIMD
+  constexpr int kPanelPF = sizeof(T) == 8 ? 3 : 1;
   if (lower) {
     // A(i,k) = Ts(wr+i, k); B(k,j) = U^-1(k, wc+j), upper incl. diagonal
-    wave_gemm3m<T, false, 1>([&](int i, int k) { return Ts[k][wr + i]; },
+    wave_gemm3m<T, false, kPanelPF>([&](int i, int k) { return Ts[k][wr + i]; },
                           [&](int k, int j) {
                             int jj = wc + j;
                             V v = czero<V>();
@@ -329,7 +332,7 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
                           q);
   } else {
     // A(i,k) = L^-1(wr+i, k), unit lower; B(k,j) = (P A_sj)(k, wc+j)
-    wave_gemm3m<T, false, 1>([&](int i, int k) {
+    wave_gemm3m<T, false, kPanelPF>([&](int i, int k) {
                             int ii = wr + i;
                             V v = czero<V>();
                             if (ii == k) v.x = T(1);
