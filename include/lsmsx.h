@@ -218,6 +218,36 @@ int lsmsx_autotune(lsmsx_ctx *ctx, int nsolve, const int *atom_ids, const int *i
                    const int *iies, const double *prel, int n_spin_cant, int n_spin_pola,
                    int precision, int *best, double *ms);
 
+/* Single-site scattering of npot spherical potentials at ne energies:
+ * calculateSingleScattererSolution (SingleSite/SingleSiteScattering.cpp:6-54)
+ * calls, per spin, single_scatterer_nonrel_ (SingleSite/single_scatterer_nonrel.f:4-122),
+ * which solves the radial equation through semrel (SingleSite/semrel_c.f:3-228,
+ * solver 2: Misc/fitpot.f, Misc/rwave.f).  One entry for the whole batch; a
+ * spin-polarized atom passes each spin's potential as its own p.
+ *   nrelv           0: scalar-relativistic, else non-relativistic (lsms.nrelv)
+ *   lmax            1..3 (atom.lmax)
+ *   vr, r_mesh      npot x iprpts: r v(r) and the radial mesh of potential p at
+ *                   [p*iprpts + j] (vr(0,is), atom.r_mesh); r_mesh[0] > 0, increasing
+ *   jmt, jws        1-based mesh indices of the muffin-tin and Wigner-Seitz radii;
+ *                   max(jmt+11, jws) <= iprpts (semrel_c.f:131-143)
+ *   r_sph           matching radius (atom.rInscribed, or atom.rws when mtasa > 0)
+ *   energy, prel    ne complex: E and sqrt(E (1 + E/c^2))
+ * Outputs (host; q = e*npot + p):
+ *   matom           [(q*(lmax+1) + l)] complex, 1/t_l (solution.matom(l, is))
+ *   tmat            may be NULL; [q*kkrsz*kkrsz + ...] column-major kkrsz x kkrsz,
+ *                   diagonal 1/matom_l with kkrsz = (lmax+1)^2 (solution.tmat_l)
+ *   zlr, jlr        may be NULL; [(q*(lmax+1) + l)*iprpts + j] complex for j < jws
+ *                   (solution.zlr, solution.jlr; other j are zero)
+ * Returns 0; 1 if an adaptive integration step underflowed somewhere (the
+ * reference prints a warning and goes on; the results are still written);
+ * < 0 on bad arguments.  The spin-canted frame rotation (trltog) stays with
+ * the caller. */
+int lsmsx_single_site_nonrel(lsmsx_ctx *ctx, int nrelv, int lmax, int npot, int iprpts,
+                             const double *vr, const double *r_mesh, const int *jmt,
+                             const int *jws, const double *r_sph, int ne, const double *energy,
+                             const double *prel, double *matom, double *tmat, double *zlr,
+                             double *jlr);
+
 #ifdef __cplusplus
 }
 #endif

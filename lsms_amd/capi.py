@@ -25,6 +25,7 @@ EXPORTED = [
     "lsmsx_get_profile", "lsmsx_set_refinement", "lsmsx_last_batch_stats",
     "lsmsx_upload_dtmats", "lsmsx_tau_full", "lsmsx_tau00_adjoint", "lsmsx_set_schedule",
     "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune", "lsmsx_set_pivot_guard",
+    "lsmsx_single_site_nonrel",
 ]
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
@@ -74,6 +75,8 @@ def load():
                                  _vp, _vp]
     L.lsmsx_tau00_adjoint.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                       _vp, _vp, _vp, _vp, _vp]
+    L.lsmsx_single_site_nonrel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
+                                           _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
     _lib = L
     return L
 
@@ -199,6 +202,41 @@ class Context:
         _ck(load().lsmsx_tau_full(self._h, atom_id, ispin, iie, _p(pr), self.nsc, self.nspola,
                                   _p(tau), nrmat_ns, C.byref(info)))
         return tau, info.value
+
+    def single_site_nonrel(self, nrelv, lmax, vr, r_mesh, jmt, jws, r_sph, energies, prels,
+                           radial=True):
+        """Single-site solve of npot potentials at ne energies on the GPU
+        (single_scatterer_nonrel.f via semrel_c.f, solver 2).  vr, r_mesh:
+        (npot, iprpts); jmt, jws: 1-based per potential; r_sph per potential.
+        Returns (matom[ne, npot, lmax+1], tmat[ne, npot, kkrsz, kkrsz],
+        zlr, jlr [ne, npot, lmax+1, iprpts] or None, status) where status 1
+        means an integration step underflowed somewhere."""
+        vr = np.ascontiguousarray(np.atleast_2d(vr), dtype=np.float64)
+        rm = np.ascontiguousarray(np.atleast_2d(r_mesh), dtype=np.float64)
+        npot, iprpts = vr.shape
+        if rm.shape != vr.shape:
+            raise ValueError("vr and r_mesh shapes differ")
+        jm = np.ascontiguousarray(np.broadcast_to(jmt, (npot,)), dtype=np.int32)
+        jw = np.ascontiguousarray(np.broadcast_to(jws, (npot,)), dtype=np.int32)
+        rs = np.ascontiguousarray(np.broadcast_to(r_sph, (npot,)), dtype=np.float64)
+        e = np.ascontiguousarray(np.atleast_1d(energies), dtype=np.complex128)
+        pr = np.ascontiguousarray(np.atleast_1d(prels), dtype=np.complex128)
+        ne, nl = len(e), lmax + 1
+        if len(pr) != ne:
+            raise ValueError("energies and prels lengths differ")
+        kk = nl * nl
+        matom = np.zeros((ne, npot, nl), dtype=np.complex128)
+        tmat = np.zeros((ne, npot, kk, kk), dtype=np.complex128)
+        zlr = np.zeros((ne, npot, nl, iprpts), dtype=np.complex128) if radial else None
+        jlr = np.zeros((ne, npot, nl, iprpts), dtype=np.complex128) if radial else None
+        rc = load().lsmsx_single_site_nonrel(
+            self._h, nrelv, lmax, npot, iprpts, _p(vr), _p(rm), _p(jm), _p(jw), _p(rs), ne,
+            _p(e), _p(pr), _p(matom), _p(tmat), _p(zlr) if radial else None,
+            _p(jlr) if radial else None)
+        if rc < 0:
+            _ck(rc)
+        # tmat blocks are column-major kkrsz x kkrsz
+        return matom, np.swapaxes(tmat, 2, 3), zlr, jlr, rc
 
     def tau00_adjoint(self, atom_ids, iies, energies, kkrsz_ns, ispins=None, lloyd=True,
                       tau00_bar=None, store_shape=None):

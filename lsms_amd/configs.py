@@ -340,3 +340,45 @@ def config_c4(npts=32, seed=20261202, natoms=64, rcuts=(11.5, 13.3)):
     return make_system("c4_bcc4096_alloy", "bcc", 5.42, (16, 16, 8), max(rcuts), 3, e, w,
                        nspecies=2, seed=seed, local_atoms=list(range(2 * natoms)),
                        vacancy=0.10, rcut_per_atom=r_per_atom, _max_local=natoms)
+
+
+def log_mesh(rmt, jmt, h=0.02, iprpts=1000):
+    """Logarithmic radial mesh r_j = exp(x0 + (j-1) h) with r_jmt = rmt
+    (1-based jmt), the form of AtomData::r_mesh (xstart, h, jmt)."""
+    x0 = np.log(rmt) - (jmt - 1) * h
+    return np.exp(x0 + h * np.arange(iprpts))
+
+
+def model_potential(z, rmt, jmt=450, jws=470, h=0.02, iprpts=1000, screen=0.6, vmtz=-1.2,
+                    shift=0.0):
+    """Synthetic spherical potential r v(r) = -2Z exp(-r/screen) + vmtz r + shift r^2
+    (Rydberg units) on log_mesh(rmt, jmt): nuclear -2Z at the origin, screened
+    outwards.  Returns (r_mesh, vr, jmt, jws, r_sph) with r_sph a little inside
+    the muffin-tin radius and between mesh points, as rInscribed usually is."""
+    r = log_mesh(rmt, jmt, h, iprpts)
+    vr = -2.0 * z * np.exp(-r / screen) + vmtz * r + shift * r * r
+    r_sph = 0.5 * (r[jmt - 2] + r[jmt - 1]) * 0.999
+    return r, vr, jmt, jws, r_sph
+
+
+def single_site_batch(nspecies=4, ne=48, seed=20261202, ebot=-0.4, etop=0.75):
+    """Single-site workload of the C3 shape: nspecies x 2 spins potentials at
+    ne energies on the C3 contour band (Z 22..29, rmt 2.2-2.5 bohr).
+    Returns (r_mesh[npot, iprpts], vr[npot, iprpts], jmt[npot], jws[npot],
+    r_sph[npot], energies[ne], prels[ne])."""
+    rng = np.random.default_rng(seed)
+    rs, vs, jm, jw, rsph = [], [], [], [], []
+    for s in range(nspecies):
+        z = 22 + int(rng.integers(0, 8))
+        rmt = 2.2 + 0.3 * float(rng.random())
+        for spin in (0, 1):
+            r, vr, jmt, jws, r_sph = model_potential(z, rmt, screen=0.55 + 0.1 * float(rng.random()),
+                                                      vmtz=-1.2 + 0.05 * (2 * spin - 1))
+            rs.append(r)
+            vs.append(vr)
+            jm.append(jmt)
+            jw.append(jws)
+            rsph.append(r_sph)
+    e, _ = contour_semicircle(ebot, etop, ne)
+    return (np.array(rs), np.array(vs), np.array(jm, dtype=np.int32),
+            np.array(jw, dtype=np.int32), np.array(rsph), e, prel_of(e))

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <chrono>
 #include <cmath>
 #include <tuple>
@@ -51,6 +52,10 @@ __global__ void k_ir_decide(const SolveDesc *, int, double);
 template <bool FIRST>
 __global__ void k_ir_sweep(const SolveDesc *, double);
 __global__ void k_copy_m(const SolveDesc *);
+__global__ void k_fitpot(const double *, const double *, const int *, int, int, double *);
+__global__ void k_single_site(const double *, const double *, const SingleSitePot *,
+                              const double2 *, const double2 *, int, int, int, int, int,
+                              double2 *, double2 *, double2 *, int *);
 __global__ void k_ir_extract(const SolveDesc *);
 __global__ void k_count_conv(const SolveDesc *, int, int *);
 __global__ void k_lloyd(const SolveDesc *, const double2 *, const double2 *, long, const double *,
@@ -184,6 +189,11 @@ struct lsmsx_ctx {
   // explicit inverse / adjoint
   DevBuf<double2> Minv, gx, lpart, lloyd, vbar, gblk, gstore, tfull;
   DevBuf<int> keyi, colmap;
+  // single-site scattering batch
+  DevBuf<double> ss_r, ss_v, ss_coef;
+  DevBuf<SingleSitePot> ss_pot;
+  DevBuf<int> ss_iend, ss_status;
+  DevBuf<double2> ss_e, ss_prel, ss_matom, ss_zlr, ss_jlr;
   DevBuf<long> keyl;
   std::vector<int> last_iters;
   std::vector<double> last_minpiv;
@@ -1158,6 +1168,17 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->cgnt.release();
   c->illp.release();
   c->tmat.release();
+  c->ss_r.release();
+  c->ss_v.release();
+  c->ss_coef.release();
+  c->ss_pot.release();
+  c->ss_iend.release();
+  c->ss_status.release();
+  c->ss_e.release();
+  c->ss_prel.release();
+  c->ss_matom.release();
+  c->ss_zlr.release();
+  c->ss_jlr.release();
   c->M.release();
   c->inv.release();
   c->tau.release();
@@ -1602,6 +1623,108 @@ int lsmsx_set_pivot_guard(lsmsx_ctx *c, double threshold) {
   if (!(threshold >= 0.0)) return fail(-2, "pivot guard must be >= 0");
   c->pivot_guard = threshold;
   return 0;
+}
+
+int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int iprpts,
+                             const double *vr, const double *r_mesh, const int *jmt,
+                             const int *jws, const double *r_sph, int ne, const double *energy,
+                             const double *prel, double *matom, double *tmat, double *zlr,
+                             double *jlr) {
+  if (!c) return fail(-1, "null context");
+  if (lmax < 1 || lmax > 3) return fail(-2, "single-site lmax must be 1..3");
+  if (npot < 0 || ne < 0 || iprpts < 1) return fail(-2, "negative batch size");
+  if (npot == 0 || ne == 0) return 0;
+  if (!vr || !r_mesh || !jmt || !jws || !r_sph || !energy || !prel || !matom)
+    return fail(-2, "null argument");
+  // per-potential checks and the matching window (semrel_c.f:130-143, cinterp.f:12-15)
+  std::vector<SingleSitePot> hp((size_t)npot);
+  std::vector<int> hiend((size_t)npot);
+  for (int p = 0; p < npot; p++) {
+    const double *r = r_mesh + (size_t)p * iprpts;
+    SingleSitePot &P = hp[p];
+    P.jmt = jmt[p];
+    P.jws = jws[p];
+    P.iend = std::max(P.jmt + 11, P.jws);
+    P.r_sph = r_sph[p];
+    if (P.jmt < 5 || P.jws < 1) return fail(-2, "jmt < 5 or jws < 1");
+    if (P.iend > iprpts) return fail(-2, "max(jmt+11, jws) exceeds iprpts");
+    if (!(r[0] > 0.0)) return fail(-2, "the radial mesh must start above r = 0");
+    for (int j = 0; j + 1 < P.iend; j++)
+      if (!(r[j + 1] > r[j])) return fail(-2, "the radial mesh must increase");
+    if (!(P.r_sph > r[0]) || !(P.r_sph <= r[P.iend - 1])) return fail(-2, "r_sph outside the mesh");
+    const int nr = P.jmt;
+    int ip1 = 1;
+    for (int i = 1; i <= nr - 2; i++)
+      if (P.r_sph > r[i]) ip1 = i + 1;
+    const int i = ip1 - 1;
+    P.ip1 = ip1;
+    P.i1 = (i > 0) ? i - 1 : std::min(4, nr) - 1;
+    P.i2 = (ip1 < nr - 1) ? ip1 + 1 : std::max(nr - 3, 1) - 1;
+    hiend[p] = P.iend;
+  }
+  const int nl = lmax + 1;
+  const size_t ntask = (size_t)ne * npot * nl;
+  const size_t npts = (size_t)npot * iprpts;
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->ss_r.ensure(npts)) return r;
+  if (int r = c->ss_v.ensure(npts)) return r;
+  if (int r = c->ss_coef.ensure(4 * npts)) return r;
+  if (int r = c->ss_pot.ensure(npot)) return r;
+  if (int r = c->ss_iend.ensure(npot)) return r;
+  if (int r = c->ss_status.ensure(1)) return r;
+  if (int r = c->ss_e.ensure(ne)) return r;
+  if (int r = c->ss_prel.ensure(ne)) return r;
+  if (int r = c->ss_matom.ensure(ntask)) return r;
+  if (zlr)
+    if (int r = c->ss_zlr.ensure(ntask * iprpts)) return r;
+  if (jlr)
+    if (int r = c->ss_jlr.ensure(ntask * iprpts)) return r;
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(c->ss_r.p, r_mesh, npts * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_v.p, vr, npts * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_pot.p, hp.data(), npot * sizeof(SingleSitePot),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_iend.p, hiend.data(), npot * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_e.p, energy, ne * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_prel.p, prel, ne * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(c->ss_status.p, 0, sizeof(int), st));
+  if (zlr) HIPCHK(hipMemsetAsync(c->ss_zlr.p, 0, ntask * iprpts * sizeof(double2), st));
+  if (jlr) HIPCHK(hipMemsetAsync(c->ss_jlr.p, 0, ntask * iprpts * sizeof(double2), st));
+  hipLaunchKernelGGL(k_fitpot, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, st, c->ss_r.p,
+                     c->ss_v.p, c->ss_iend.p, iprpts, npot, c->ss_coef.p);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_single_site, dim3((unsigned)((ntask + 63) / 64)), dim3(64), 0, st,
+                     c->ss_r.p, c->ss_coef.p, c->ss_pot.p, c->ss_e.p, c->ss_prel.p, npot, ne,
+                     lmax, iprpts, nrelv == 0 ? 1 : 0, c->ss_matom.p, zlr ? c->ss_zlr.p : nullptr,
+                     jlr ? c->ss_jlr.p : nullptr, c->ss_status.p);
+  HIPCHK(hipGetLastError());
+  int status = 0;
+  HIPCHK(hipMemcpyAsync(matom, c->ss_matom.p, ntask * sizeof(double2), hipMemcpyDeviceToHost, st));
+  if (zlr)
+    HIPCHK(hipMemcpyAsync(zlr, c->ss_zlr.p, ntask * iprpts * sizeof(double2),
+                          hipMemcpyDeviceToHost, st));
+  if (jlr)
+    HIPCHK(hipMemcpyAsync(jlr, c->ss_jlr.p, ntask * iprpts * sizeof(double2),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&status, c->ss_status.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (tmat) {  // diagonal t = 1/matom_l (single_scatterer_nonrel.f:100-111)
+    const int kk = nl * nl;
+    for (size_t q = 0; q < (size_t)ne * npot; q++) {
+      double *tq = tmat + 2 * q * kk * kk;
+      std::fill(tq, tq + 2 * (size_t)kk * kk, 0.0);
+      int lm = 0;
+      for (int l = 0; l <= lmax; l++) {
+        const std::complex<double> m(matom[2 * (q * nl + l)], matom[2 * (q * nl + l) + 1]);
+        const std::complex<double> t = 1.0 / m;
+        for (int mm = -l; mm <= l; mm++, lm++) {
+          tq[2 * ((size_t)lm + (size_t)lm * kk)] = t.real();
+          tq[2 * ((size_t)lm + (size_t)lm * kk) + 1] = t.imag();
+        }
+      }
+    }
+  }
+  return status ? 1 : 0;
 }
 
 }  // extern "C"

@@ -85,4 +85,13 @@ struct SolveDesc {
   double2 *lpart;         // numLIZ per-row-site parts of Tr(M^-1 dM/dE)
 };
 
+// one potential of a single-site batch: 1-based jmt, jws, iend = max(jmt+11, jws)
+// as in semrel_c.f, and the 0-based four-point window of the matching
+// interpolation at r_sph (cinterp.f: interval ip1-1..ip1, outside points i1, i2)
+struct SingleSitePot {
+  int jmt, jws, iend;
+  int ip1, i1, i2;
+  double r_sph;
+};
+
 }  // namespace lsmsx

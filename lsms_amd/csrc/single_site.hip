@@ -1,0 +1,598 @@
+// single_site.hip -- single-site scattering on the GPU: the t-matrix
+// t_l = 1/matom_l and the radial functions zlr, jlr of non-relativistic and
+// scalar-relativistic spherical potentials, for many (energy, potential)
+// pairs at once.
+//
+// Same algorithm as the reference's single_scatterer_nonrel
+// (SingleSite/single_scatterer_nonrel.f:4-122) through semrel with solver 2
+// (SingleSite/semrel_c.f:130-220): the potential r v(r) is fitted per mesh
+// interval to a rational form (Misc/fitpot.f), the radial equation is
+// integrated outwards from the first mesh point with an adaptive
+// Bulirsch-Stoer step per interval (Misc/rwave.f, initwave.f, dfv_new.f,
+// bulirschStoerIntegrator.cpp), matched at r_sph to j_l + alpha h_l
+// (Misc/cinterp.f, fit.f, zsphbesjh.f, zsphbesj.f), and the irregular
+// solution integrated inwards from r_sph.
+//
+// GPU mapping.  The work is a few hundred independent ODE solves of a few
+// hundred mesh intervals each: latency-bound scalar FP64, far from any HBM or
+// MFMA roofline.  k_fitpot gives one thread to each mesh interval;
+// k_single_site gives one thread to each (energy, potential, l) and keeps the
+// whole extrapolation tableau in registers (its loops are unrolled over the
+// fixed step sequence, so no per-thread array is indexed at run time).  Each
+// thread writes its own zlr/jlr column directly; the regular solution is
+// kept raw in the zlr column until the matching gives its normalisation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "lsmsx_device.hpp"
+
+namespace lsmsx {
+
+namespace {
+
+struct zd {
+  double x, y;
+};
+__device__ inline zd mk(double a, double b = 0.0) { return zd{a, b}; }
+__device__ inline zd operator+(zd a, zd b) { return zd{a.x + b.x, a.y + b.y}; }
+__device__ inline zd operator-(zd a, zd b) { return zd{a.x - b.x, a.y - b.y}; }
+__device__ inline zd operator-(zd a) { return zd{-a.x, -a.y}; }
+__device__ inline zd operator*(zd a, zd b) { return zd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline zd operator*(double s, zd a) { return zd{s * a.x, s * a.y}; }
+__device__ inline zd operator*(zd a, double s) { return zd{s * a.x, s * a.y}; }
+__device__ inline zd operator/(zd a, double s) { return zd{a.x / s, a.y / s}; }
+// complex quotient with Smith's scaling (no overflow of |b|^2)
+__device__ inline zd operator/(zd a, zd b) {
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double r = b.y / b.x, d = b.x + b.y * r;
+    return zd{(a.x + a.y * r) / d, (a.y - a.x * r) / d};
+  }
+  const double r = b.x / b.y, d = b.x * r + b.y;
+  return zd{(a.x * r + a.y) / d, (a.y * r - a.x) / d};
+}
+__device__ inline zd operator/(double s, zd b) { return mk(s) / b; }
+__device__ inline zd times_i(zd a) { return zd{-a.y, a.x}; }
+// principal square root
+__device__ inline zd zsqrt(zd z) {
+  const double m = hypot(z.x, z.y);
+  if (m == 0.0) return mk(0.0);
+  if (z.x >= 0.0) {
+    const double t = sqrt(0.5 * (m + z.x));
+    return zd{t, z.y / (2.0 * t)};
+  }
+  const double t = sqrt(0.5 * (m - z.x));
+  return zd{fabs(z.y) / (2.0 * t), copysign(t, z.y)};
+}
+__device__ inline zd zsin(zd z) { return zd{sin(z.x) * cosh(z.y), cos(z.x) * sinh(z.y)}; }
+__device__ inline zd zcos(zd z) { return zd{cos(z.x) * cosh(z.y), -sin(z.x) * sinh(z.y)}; }
+__device__ inline zd zexp_i(zd z) {  // e^{i z}
+  const double s = exp(-z.y);
+  return zd{s * cos(z.x), s * sin(z.x)};
+}
+
+// Bulirsch-Stoer step-count sequence n_j = 2 n_{j-2} (bulirschStoerIntegrator.cpp:66-67)
+__constant__ int kSeq[12] = {2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128};
+constexpr int kBsSteps = 11;
+constexpr int kBsTarget = 7;
+constexpr double kBsEps = 1.0e-12;
+
+struct RhsPar {
+  double er, ei, b, allp1;
+};
+
+// derivatives of (Re g, Im g, Re f, Im f) (dfv_new.f, nv = 4, eunit = 1)
+__device__ inline void rhs4(double r, const double *y, double *dy, double rn, const double *pc,
+                            const RhsPar &p) {
+  const double s1or = 1.0 / r, s1or2 = s1or * s1or;
+  const double r1 = r - rn;
+  const double v = (pc[0] * r1 + pc[1]) / (1.0 + pc[2] * r1 * r1) + pc[3];
+  const double s2vmer = v * s1or - p.er;
+  const double frelr = 1.0 - p.b * s2vmer;
+  const double s2vmei = -p.ei;
+  const double freli = -p.b * s2vmei;
+  const double den = p.allp1 / (frelr * frelr + freli * freli) * s1or2;
+  const double facr = den * frelr + s2vmer;
+  const double faci = -den * freli + s2vmei;
+  dy[0] = frelr * y[2] - freli * y[3] + s1or * y[0];
+  dy[1] = freli * y[2] + frelr * y[3] + s1or * y[1];
+  dy[2] = facr * y[0] - faci * y[1] - y[2] * s1or;
+  dy[3] = faci * y[0] + facr * y[1] - y[3] * s1or;
+}
+
+// modified midpoint rule with `steps` substeps over [x0, x1]
+__device__ inline void midpoint4(double x0, double x1, const double *y0, double *y1, int steps,
+                                 double rn, const double *pc, const RhsPar &p) {
+  double y2[4], dy[4];
+  const double h = (x1 - x0) / (double)steps, h2 = 2.0 * h;
+  double x = x0;
+  rhs4(x, y0, dy, rn, pc, p);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    y1[i] = y0[i];
+    y2[i] = y0[i] + h * dy[i];
+  }
+  for (int j = 1; j < steps; j++) {
+    x += h;
+    rhs4(x, y2, dy, rn, pc, p);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const double t = y1[i] + h2 * dy[i];
+      y1[i] = y2[i];
+      y2[i] = t;
+    }
+  }
+  x += h;
+  rhs4(x, y2, dy, rn, pc, p);
+#pragma unroll
+  for (int i = 0; i < 4; i++) y1[i] = 0.5 * (y1[i] + y2[i] + h * dy[i]);
+}
+
+// Adaptive Bulirsch-Stoer over [x0, x1] with polynomial extrapolation in h^2,
+// one tableau row kept in registers (T[i][k] needs only T[i][k-1] and
+// T[i-1][k-1]).  y is advanced in place; dy returns the derivative at x1.
+// Returns 1 if the step size underflowed (y then holds the last estimate).
+__device__ int bs_interval(double x0, double x1, double *y, double *dy, double rn,
+                           const double *pc, const RhsPar &p) {
+  const double xeps = 1.0e-12 * fabs(x0);
+  double x = x0;
+  const double h0 = x1 - x0;
+  double h = h0;
+  double row[kBsSteps][4] = {};
+  double hh[kBsSteps] = {};
+  int last = 0;  // diagonal index of the last tableau
+  int rc = 0;
+  do {
+    double err = HUGE_VAL;
+    int step = 0;
+#pragma unroll
+    for (int i = 0; i < kBsSteps; i++) {
+      if (err > kBsEps) {
+        double y1[4];
+        midpoint4(x, x + h, y, y1, kSeq[i], rn, pc, p);
+        hh[i] = h / (double)kSeq[i];
+        double e = 0.0, mag = 0.0;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+          double old = row[0][d];  // T[i-1][0]
+          row[0][d] = y1[d];
+#pragma unroll
+          for (int k = 1; k <= i; k++) {
+            const double rr = hh[i - k] / hh[i];
+            const double den = 1.0 / (rr * rr - 1.0);
+            const double keep = row[k][d];  // T[i-1][k] (unused when k == i)
+            row[k][d] = row[k - 1][d] + (row[k - 1][d] - old) * den;
+            if (k == i) e += fabs(row[i][d] - old);  // old = T[i-1][i-1]
+            old = keep;
+          }
+          mag += fabs(row[i][d]);
+        }
+        if (i == 0) e = 4.0;
+        if (mag == 0.0) mag = 1.0e-15;
+        err = e / mag;
+        step = i + 1;
+        last = i;
+      }
+    }
+    if (err < kBsEps) {
+      x = x + h;
+      if (step < kBsTarget)
+        h *= 1.25;
+      else
+        h *= (double)kSeq[kBsTarget] / (double)kSeq[step];
+      const double lim = fabs(x1 - x);
+      h = copysign(fabs(h) < lim ? fabs(h) : lim, h0);
+#pragma unroll
+      for (int k = 0; k < kBsSteps; k++)
+        if (k == last) {
+#pragma unroll
+          for (int d = 0; d < 4; d++) y[d] = row[k][d];
+        }
+    } else {
+      h = 0.25 * h;
+      if (h + x == x) {
+        rc = 1;
+        break;
+      }
+    }
+  } while (fabs(x - x1) > xeps);
+#pragma unroll
+  for (int k = 0; k < kBsSteps; k++)
+    if (k == last) {
+#pragma unroll
+      for (int d = 0; d < 4; d++) y[d] = row[k][d];
+    }
+  rhs4(x1, y, dy, rn, pc, p);
+  return rc;
+}
+
+// j_l(x) and h_l(x) = hm[l] e^{ix}/x for l = 0..LB (zsphbesj.f, zsphbesjh.f job 1)
+template <int LB>
+__device__ void bessel_jh(zd x, zd *bj, zd *hm, zd &eiz) {
+  zd bs[LB + 1];
+  const zd as = zsin(x), ac = zcos(x), x1 = 1.0 / x;
+  bj[0] = as * x1;
+  int k = ((int)(0.75 * (fabs(x.x) + fabs(x.y))) + 2) >> 2;
+  bs[0] = as;
+  bool done = false;
+  if (k >= 1) {
+    if (k > LB) k = LB;
+    bs[1] = bj[0] - ac;
+    bj[1] = bs[1] * x1;
+#pragma unroll
+    for (int l = 2; l <= LB; l++)
+      if (l <= k) {
+        bs[l] = (double)(2 * l - 1) * bj[l - 1] - bs[l - 2];
+        bj[l] = bs[l] * x1;
+      }
+    done = (k == LB);
+  }
+  if (!done) {
+    zd a = bs[0];
+#pragma unroll
+    for (int l = 1; l <= LB; l++)
+      if (l == k) a = bs[l];
+    const int nm = LB << 4;
+    zd aj = mk((double)(2 * nm + 3));
+    const zd x2 = x * x;
+    for (int l = nm; l >= LB + 2; l--) aj = mk((double)(2 * l + 1)) - x2 / aj;
+    bs[LB] = (double)(2 * LB + 3) * aj * x1 - x;
+    bj[LB] = bs[LB] * x1;
+    bs[LB - 1] = (double)(2 * LB + 1) * bj[LB] - aj;
+#pragma unroll
+    for (int l = LB - 1; l >= 1; l--)
+      if (l >= k + 1) {
+        bj[l] = bs[l] * x1;
+        bs[l - 1] = (double)(2 * l + 1) * bj[l] - bs[l + 1];
+      }
+    zd bk = bs[0];
+#pragma unroll
+    for (int l = 1; l <= LB; l++)
+      if (l == k) bk = bs[l];
+    aj = a / bk;
+#pragma unroll
+    for (int l = 1; l <= LB; l++)
+      if (l >= k + 1) bj[l] = aj * bj[l];
+  }
+  eiz = zexp_i(x);
+  hm[0] = mk(0.0, -1.0);
+  hm[1] = mk(-(1.0 - x1.y), -x1.x);
+#pragma unroll
+  for (int l = 2; l <= LB; l++) hm[l] = (double)(2 * l - 1) * hm[l - 1] * x1 - hm[l - 2];
+}
+
+// j_l, j_{l-1} (or j_1 at l = 0) and the same of hm, at argument x
+struct BesselPick {
+  zd bj_l, bj_o, hm_l, hm_o, hm_0, bj_0, eiz;
+};
+
+template <int LB>
+__device__ BesselPick bessel_pick_t(zd x, int l) {
+  zd bj[LB + 1], hm[LB + 1], eiz;
+  bessel_jh<LB>(x, bj, hm, eiz);
+  BesselPick b;
+  b.eiz = eiz;
+  b.bj_0 = bj[0];
+  b.hm_0 = hm[0];
+  b.bj_l = bj[0];
+  b.hm_l = hm[0];
+  b.bj_o = bj[1];
+  b.hm_o = hm[1];
+#pragma unroll
+  for (int q = 1; q <= LB; q++)
+    if (q == l) {
+      b.bj_l = bj[q];
+      b.hm_l = hm[q];
+      b.bj_o = bj[q - 1];
+      b.hm_o = hm[q - 1];
+    }
+  return b;
+}
+
+__device__ BesselPick bessel_pick(zd x, int l) {
+  if (l <= 1) return bessel_pick_t<1>(x, l);
+  if (l == 2) return bessel_pick_t<2>(x, l);
+  return bessel_pick_t<3>(x, l);
+}
+
+// regular-solution value and derivative factors outside the sphere
+// (rwave.f:153-162 and :228-236): j_l, d(r j_l)-like term, h_l r, its derivative
+__device__ inline void outside(const BesselPick &b, zd ka, zd kar, int l, zd &bj, zd &dbj,
+                               zd &bh1, zd &dbh1) {
+  bj = b.bj_l;
+  bh1 = b.hm_l * b.eiz / ka;
+  if (l == 0) {
+    dbj = -(ka * (b.bj_o - b.bj_0 / kar));
+    dbh1 = -((b.hm_o - b.hm_0 / kar) * b.eiz);
+  } else {
+    const zd cd = (double)l / kar;
+    dbj = ka * (b.bj_o - b.bj_l * cd);
+    dbh1 = (b.hm_o - b.hm_l * cd) * b.eiz;
+  }
+}
+
+// four-point rational fit of real f at 0-based point i (fit.f); the points it
+// reads are passed as values: f at i1, i, ip1, i2
+__device__ void fit4(const double *r, int nr, int i, int i1, int i2, double fi1, double fi,
+                     double fip1, double fi2, double *c) {
+  const int ip1 = i + 1;
+  const double dr = r[ip1] - r[i];
+  c[2] = (fip1 - fi) / dr;
+  c[0] = fi;
+  c[1] = 0.0;
+  c[3] = 0.0;
+  if (i1 != i2) {
+    c[1] = (fi2 - fip1) / (r[i2] - r[ip1]) - (fi1 - fi) / (r[i1] - r[i]);
+    if (c[1] * (fi2 - fi1) * (r[i2] - r[i1]) > 0.0) c[1] = -c[1];
+    double h1 = r[i1] - r[i];
+    double d0 = fi1 - fi;
+    const double a1 = d0 - c[2] * h1, a2 = d0 - c[1] * h1;
+    h1 = h1 * (r[ip1] - r[i1]);
+    const double e12 = -h1 * a2;
+    double h2 = r[i2] - r[i];
+    d0 = fi2 - fi;
+    const double a3 = d0 - c[2] * h2, a4 = d0 - c[1] * h2;
+    h2 = h2 * (r[ip1] - r[i2]);
+    const double e22 = -h2 * a4;
+    double gj = (a4 - a2) * h1 * h2;
+    if (gj == 0.0) {
+      c[3] = 0.0;
+    } else {
+      c[0] = (a1 * e22 - a3 * e12) / gj;
+      c[3] = (a1 * h2 - a3 * h1) / gj;
+    }
+    gj = c[3] * dr * dr;
+    if (gj > -4.0 && fabs(gj) > 1e-14 && gj < 1e14) {
+      c[0] = c[0] / c[3];
+    } else {
+      c[0] = fi;
+      c[3] = 0.0;
+    }
+  } else {
+    // three-point form (only for meshes of fewer than five points)
+    const double fj = 0.5 * (fi + fip1);
+    c[0] = fi - fj;
+    c[1] = 0.0;
+    const double rj = r[i] + 0.5 * dr;
+    int j1 = i1;
+    double fj1 = fi1;
+    if (fabs(r[i1] - rj) > fabs(r[i2] - rj)) {
+      j1 = i2;
+      fj1 = fi2;
+    }
+    double h1 = r[j1] - r[i];
+    const double a1 = fj1 - fi - c[2] * h1;
+    h1 = h1 * (r[ip1] - r[j1]);
+    if (fj1 != fj) {
+      c[3] = -a1 / ((fj1 - fj) * h1);
+      const double gj = c[3] * dr * dr;
+      if (gj <= -4.0 || fabs(gj) <= 1e-14 || gj >= 1e14) {
+        c[0] = fi;
+        c[3] = 0.0;
+      }
+    } else {
+      c[0] = fi;
+      c[3] = 0.0;
+    }
+  }
+}
+
+// cinterp.f value at rs from the four captured complex samples
+__device__ zd interp4(const double *r, int nr, int i, int i1, int i2, double rs, const zd *f4) {
+  double part[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    double c[4];
+    const double g1 = q ? f4[0].y : f4[0].x, g = q ? f4[1].y : f4[1].x;
+    const double gp = q ? f4[2].y : f4[2].x, g2 = q ? f4[3].y : f4[3].x;
+    fit4(r, nr, i, i1, i2, g1, g, gp, g2, c);
+    const double h1 = rs - r[i], h2 = r[i + 1] - rs;
+    const double den = 1.0 + c[3] * h1 * h2;
+    const double ps = ((c[2] - c[1]) * h1 + c[0]) / den;
+    part[q] = ps + g - c[0] + c[1] * h1;
+  }
+  return mk(part[0], part[1]);
+}
+
+}  // namespace
+
+// fitpot.f (nr = 4) for every potential: one thread per mesh interval.
+// coef[(p*iprpts + j)*4 + q]; intervals j >= iend-2 get zeros.
+__global__ void k_fitpot(const double *__restrict__ rmesh, const double *__restrict__ vr,
+                         const int *__restrict__ iend_of, int iprpts, int npot,
+                         double *__restrict__ coef) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)npot * iprpts) return;
+  const int p = (int)(t / iprpts), j = (int)(t % iprpts);
+  const double *r = rmesh + (long)p * iprpts, *rv = vr + (long)p * iprpts;
+  double *c = coef + 4 * t;
+  const int nfit = iend_of[p] - 1;
+  if (j >= nfit - 1) {
+    c[0] = c[1] = c[2] = c[3] = 0.0;
+    return;
+  }
+  const double r0 = r[j], dr = r[j + 1] - r0;
+  const double drvdr = (rv[j + 1] - rv[j]) / dr;
+  int i1 = (j == 0) ? 2 : j - 1;
+  if (j == nfit - 2) i1 -= 1;
+  double h1 = r[i1] - r0, h2 = r[i1] - r[j + 1];
+  const double e11 = h2, e12 = -h1 * (rv[i1] - rv[j + 1]);
+  const double rhs1 = (rv[i1] - rv[j]) / h1 - drvdr;
+  i1 += 1;
+  if (i1 == j) i1 += 2;
+  h1 = r[i1] - r0;
+  h2 = r[i1] - r[j + 1];
+  const double e21 = h2, e22 = -h1 * (rv[i1] - rv[j + 1]);
+  const double rhs2 = (rv[i1] - rv[j]) / h1 - drvdr;
+  const double det = e11 * e22 - e21 * e12;
+  if (fabs(det) > 1e-16) {
+    double b = -(e12 * rhs2 - e22 * rhs1) / det;
+    const double cc = (e11 * rhs2 - e21 * rhs1) / det;
+    const double tt = cc * dr * dr;
+    if (tt > -1.0 && fabs(tt) > 1e-14) {
+      const double a4 = rv[j + 1] + b / cc;
+      c[0] = drvdr - b * dr;
+      c[1] = rv[j] - a4;
+      c[2] = cc;
+      c[3] = a4;
+      return;
+    }
+  }
+  c[0] = drvdr;
+  c[1] = rv[j];
+  c[2] = 0.0;
+  c[3] = 0.0;
+}
+
+// One thread per (energy e, potential p, l): rwave.f with the outputs of
+// semrel_c.f (solver 2).  matom[(e*npot + p)*(lmax+1) + l];
+// zlr/jlr (may be null) columns of iprpts at ((e*npot + p)*(lmax+1) + l)*iprpts.
+__global__ void __launch_bounds__(64) k_single_site(
+    const double *__restrict__ rmesh, const double *__restrict__ coef,
+    const SingleSitePot *__restrict__ pots, const double2 *__restrict__ energy,
+    const double2 *__restrict__ prel, int npot, int ne, int lmax, int iprpts, int srel,
+    double2 *__restrict__ matom_out, double2 *__restrict__ zlr, double2 *__restrict__ jlr,
+    int *__restrict__ status) {
+  const int nl = lmax + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ne * npot * nl) return;
+  const int l = (int)(t % nl), p = (int)((t / nl) % npot), e = (int)(t / ((long)nl * npot));
+  const SingleSitePot P = pots[p];
+  const double *rad = rmesh + (long)p * iprpts;
+  const double *cf = coef + (long)p * iprpts * 4;
+  const zd E = mk(energy[e].x, energy[e].y);
+  const zd ka = zsqrt(E + mk(0.0, 1.0e-14));
+  const double al = (double)l;
+  const int jmt = P.jmt, jws = P.jws;
+  double2 *zcol = zlr ? zlr + t * iprpts : nullptr;
+  double2 *jcol = jlr ? jlr + t * iprpts : nullptr;
+  int rc = 0;
+
+  // initial values (initwave.f, core = false; r_1 > 0 so istart = 1)
+  RhsPar rp;
+  rp.er = E.x;
+  rp.ei = E.y;
+  double y[4], dy[4];
+  {
+    const double c = 137.0359895, c2 = c * c;
+    double z0 = cf[1] + cf[3];
+    z0 = (double)(int)(fabs(z0) + 0.1);
+    const double z = 0.5 * z0;
+    const bool rel = srel && !(z < 0.5);
+    rp.allp1 = fabs((double)(l * (l + 1)));
+    const double x = rad[0];
+    zd g1, g2;
+    if (rel) {
+      rp.b = 0.25 / c2;
+      g1 = mk(1.0);
+      g2 = mk(2.0 * c2 * (sqrt(rp.allp1 + 1.0 - z * z / c2) - 1.0) / z);
+    } else {
+      rp.b = 0.0;
+      zd v0 = 0.5 * (E - mk(cf[0]));
+      const double v1 = 2.0 * 0.5 * cf[1] * cf[2];
+      v0 = v0 - mk(0.5 * v1 * (rad[0] + rad[1]));
+      const double alp1 = al + 1.0;
+      const double a1 = -z / alp1;
+      const zd a2 = (mk(-z * a1) - v0) / (2.0 * al + 3.0);
+      const zd a3 = -((-a1) * (mk(z * z) - (3.0 * al + 4.0) * v0) + mk((2.0 * al + 3.0) * v1)) /
+                    (3.0 * (al + 2.0) * (2.0 * al + 3.0));
+      const double rl = pow(x, al);
+      g1 = (x * rl) * (mk(1.0) + x * (mk(a1) + x * (a2 + x * a3)));
+      g2 = rl * (mk(al) + x * (mk(alp1 * a1) + x * ((al + 2.0) * a2 + x * (al + 3.0) * a3)));
+    }
+    y[0] = g1.x;
+    y[1] = g1.y;
+    y[2] = g2.x;
+    y[3] = g2.y;
+  }
+  rhs4(rad[0], y, dy, rad[0], cf, rp);
+
+  // outward integration; the raw regular solution goes to the zlr column and
+  // the four-point matching window is captured in registers
+  zd wr[4], wd[4];  // rr, drr at i1, i, ip1, i2
+  const int wi = P.ip1 - 1;
+  auto capture = [&](int j) {
+    const zd g = mk(y[0], y[1]), d = mk(dy[0], dy[1]);
+    if (j == P.i1) { wr[0] = g; wd[0] = d; }
+    if (j == wi) { wr[1] = g; wd[1] = d; }
+    if (j == P.ip1) { wr[2] = g; wd[2] = d; }
+    if (j == P.i2) { wr[3] = g; wd[3] = d; }
+    if (zcol && j < jws) zcol[j] = make_double2(y[0], y[1]);
+  };
+  capture(0);
+  for (int j = 1; j < jmt; j++) {
+    rc |= bs_interval(rad[j - 1], rad[j], y, dy, rad[j - 1], cf + 4 * (j - 1), rp);
+    capture(j);
+  }
+
+  // match to j_l + alpha h_l at r_sph (rwave.f:123-171)
+  const double r_sph = P.r_sph;
+  const zd drrsph = interp4(rad, jmt, wi, P.i1, P.i2, r_sph, wd);
+  const zd rrsph = interp4(rad, jmt, wi, P.i1, P.i2, r_sph, wr);
+  const zd dfi = drrsph / rrsph, fi = rrsph / r_sph;
+  zd kar = ka * r_sph;
+  zd bj, dbj, bh1, dbh1;
+  BesselPick b = bessel_pick(kar, l);
+  outside(b, ka, kar, l, bj, dbj, bh1, dbh1);
+  const zd cdum = -((bj * dfi - dbj) / (bh1 * dfi - dbh1));
+  const zd cnorm = (bj + cdum * bh1) / fi;
+  const zd tatoml = r_sph * cdum;
+  const zd matom = -(times_i(ka) / tatoml);
+  matom_out[t] = make_double2(matom.x, matom.y);
+  const zd iprel = times_i(mk(prel[e].x, prel[e].y));
+
+  // zlr = (cnorm g) matom / r inside the sphere
+  if (zcol) {
+    const int jend = jmt - 1 < jws ? jmt - 1 : jws;
+    for (int j = 0; j < jend; j++) {
+      const double2 g = zcol[j];
+      const zd gn = cnorm * mk(g.x, g.y);
+      const zd z = (gn * matom) / rad[j];
+      zcol[j] = make_double2(z.x, z.y);
+    }
+  }
+
+  // irregular solution inwards from r_sph (rwave.f:181-223)
+  {
+    const zd hl = b.hm_l * b.eiz / ka;
+    zd dh;
+    if (l == 0) {
+      dh = -(b.hm_o * b.eiz);
+    } else {
+      const zd cd = (double)(l + 1) / kar;
+      dh = (b.hm_o - b.hm_l * cd) * b.eiz;
+    }
+    y[0] = hl.x;
+    y[1] = hl.y;
+    y[2] = dh.x;
+    y[3] = dh.y;
+    rhs4(r_sph, y, dy, r_sph, cf + 4 * (jmt - 1), rp);
+    for (int j = jmt - 2; j >= 0; j--) {
+      const double rfrom = (j + 1 == jmt - 1) ? r_sph : rad[j + 1];
+      rc |= bs_interval(rfrom, rad[j], y, dy, rad[j], cf + 4 * j, rp);
+      if (jcol && j < jws) {
+        const zd q = (iprel * mk(y[0], y[1])) / (rad[j] * matom);
+        jcol[j] = make_double2(q.x, q.y);
+      }
+    }
+  }
+
+  // outside the sphere: Bessel and Hankel forms (rwave.f:225-240)
+  for (int j = jmt - 1; j < jws; j++) {
+    kar = ka * rad[j];
+    b = bessel_pick(kar, l);
+    outside(b, ka, kar, l, bj, dbj, bh1, dbh1);
+    if (zcol) {
+      const zd g = bj * rad[j] + bh1 * tatoml;
+      const zd z = (g * matom) / rad[j];
+      zcol[j] = make_double2(z.x, z.y);
+    }
+    if (jcol) {
+      const zd q = (iprel * bh1) / (rad[j] * matom);
+      jcol[j] = make_double2(q.x, q.y);
+    }
+  }
+  if (rc) atomicOr(status, 1);
+}
+
+}  // namespace lsmsx

@@ -75,6 +75,17 @@ void or_trgtol(int nr, int nc, const zc *u, const zc *ud, const zc *mg,
 /* --- energy contour ---------------------------------------------------- */
 void or_congauss(double ebot, double etop, int ngauss, zc *egrd, zc *dele);
 
+/* --- single-site scattering (single_site_oracle.c) ---------------------- */
+/* rwave.f: regular/irregular radial solutions on rad[0..kmax), matom = 1/t_l */
+int or_rwave(int srel, int l, zc energy, const double *coef, const double *rad, int jmt,
+             int kmax, double r_sph, zc *rr, zc *drr, zc *ri, zc *dri, zc *matom);
+/* single_scatterer_nonrel.f through semrel_c.f (solver 2) */
+int or_single_scatterer_nonrel(int nrelv, int lmax, zc energy, zc prel, const double *vr,
+                               const double *r, int jmt, int jws, double r_sph, int iprpts,
+                               zc *matom, zc *tmat, zc *zlr, zc *jlr);
+void or_fitpot(const double *r, const double *rv, double *coef, int nfit, int iend);
+void or_sph_bessel_jh(int lmax, zc x, zc *bj, zc *hm, zc *eiz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,11 @@ def lib():
     L.or_spin_trafo.argtypes = [_dp, _zp, _zp]
     L.or_trgtol.argtypes = [C.c_int, C.c_int, _zp, _zp, _zp, _zp]
     L.or_congauss.argtypes = [C.c_double, C.c_double, C.c_int, _zp, _zp]
+    L.or_single_scatterer_nonrel.restype = C.c_int
+    L.or_single_scatterer_nonrel.argtypes = [C.c_int, C.c_int, _Z, _Z, _dp, _dp, C.c_int,
+                                             C.c_int, C.c_double, C.c_int, _zp, _zp, _zp, _zp]
+    L.or_fitpot.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int]
+    L.or_sph_bessel_jh.argtypes = [C.c_int, _Z, _zp, _zp, _zp]
     _lib = L
     return L
 
@@ -191,3 +196,40 @@ def congauss(ebot, etop, ngauss):
     w = np.zeros(ngauss, dtype=np.complex128, order="F")
     lib().or_congauss(ebot, etop, ngauss, e, w)
     return e, w
+
+
+def single_scatterer_nonrel(nrelv, lmax, energy, prel_, vr, r, jmt, jws, r_sph):
+    """t-matrix solve of one potential at one energy
+    (SingleSite/single_scatterer_nonrel.f:4-122 via semrel_c.f, solver 2).
+    vr = r v(r) and r on the radial mesh (length iprpts); jmt, jws 1-based
+    grid indices as in AtomData.  Returns (matom[lmax+1], tmat[kkrsz,kkrsz],
+    zlr[iprpts,lmax+1], jlr[iprpts,lmax+1], rc)."""
+    vr = _f(vr, np.float64)
+    r = _f(r, np.float64)
+    iprpts = len(r)
+    kkrsz = (lmax + 1) ** 2
+    matom = np.zeros(lmax + 1, dtype=np.complex128)
+    tmat = np.zeros((kkrsz, kkrsz), dtype=np.complex128, order="F")
+    zlr = np.zeros((iprpts, lmax + 1), dtype=np.complex128, order="F")
+    jlr = np.zeros((iprpts, lmax + 1), dtype=np.complex128, order="F")
+    rc = lib().or_single_scatterer_nonrel(nrelv, lmax, _z(energy), _z(prel_), vr, r, jmt, jws,
+                                          r_sph, iprpts, matom, tmat, zlr, jlr)
+    if rc < 0:
+        raise ValueError("or_single_scatterer_nonrel: bad sizes")
+    return matom, tmat, zlr, jlr, rc
+
+
+def fitpot(r, rv, nfit, iend):
+    """Misc/fitpot.f with nr = 4: coefficients (4, iend)."""
+    coef = np.zeros((4, iend), dtype=np.float64, order="F")
+    lib().or_fitpot(_f(r, np.float64), _f(rv, np.float64), coef, nfit, iend)
+    return coef
+
+
+def sph_bessel_jh(lmax, x):
+    """Misc/zsphbesjh.f (job = 1): j_l(x), h_l(x) x e^{-ix}, e^{ix}."""
+    bj = np.zeros(lmax + 1, dtype=np.complex128)
+    hm = np.zeros(lmax + 1, dtype=np.complex128)
+    eiz = np.zeros(1, dtype=np.complex128)
+    lib().or_sph_bessel_jh(lmax, _z(x), bj, hm, eiz)
+    return bj, hm, eiz[0]

@@ -159,3 +159,38 @@ def congauss(ebot, etop, eibot, npts):
     lib().congauss_(_d(ebot), _d(etop), _d(eibot), vp(egrd), vp(dele), _i(npts), C.byref(nume),
                     C.byref(pi), _i(ip), C.byref(iprint), _ISTOP, C.c_size_t(32))
     return egrd[: nume.value], dele[: nume.value]
+
+
+_SS_PATH = os.path.join(_HERE, "_ref", "libss_ref.so")
+_ss = None
+
+
+def ss_available():
+    return os.path.exists(_SS_PATH)
+
+
+def single_scatterer_nonrel(nrelv, lmax, energy, prel_, vr, r, jmt, jws, r_sph, clight=274.072):
+    """SingleSite/single_scatterer_nonrel.f as called by
+    calculateSingleScattererSolution (SingleSite/SingleSiteScattering.cpp:17-27).
+    Returns (matom[lmax+1], tmat[kkrsz,kkrsz], zlr, jlr)."""
+    global _ss
+    if _ss is None:
+        _ss = C.CDLL(_SS_PATH)
+    vr = np.asfortranarray(np.asarray(vr, dtype=np.float64))
+    r = np.asfortranarray(np.asarray(r, dtype=np.float64))
+    iprpts = len(r)
+    kkrsz = (lmax + 1) ** 2
+    tmat = np.zeros((kkrsz, kkrsz), dtype=np.complex128, order="F")
+    matom = np.zeros(lmax + 1, dtype=np.complex128, order="F")
+    zlr = np.zeros((iprpts, lmax + 1), dtype=np.complex128, order="F")
+    jlr = np.zeros((iprpts, lmax + 1), dtype=np.complex128, order="F")
+    e = _zarr(energy)
+    pr = _zarr(prel_)
+    pn = _zarr(prel_)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    h = np.log(r[1] / r[0])
+    _ss.single_scatterer_nonrel_(_i(nrelv), _d(clight), _i(lmax), _i(kkrsz), vp(e), vp(pr),
+                                 vp(pn), vp(vr), vp(r), _d(h), _i(jmt), _i(jws), vp(tmat),
+                                 vp(matom), vp(zlr), vp(jlr), _d(r_sph), _i(iprpts), _i(-1),
+                                 _ISTOP, C.c_size_t(32))
+    return matom, tmat, zlr, jlr

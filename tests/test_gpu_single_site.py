@@ -1,0 +1,97 @@
+"""Single-site scattering on the GPU (lsmsx_single_site_nonrel) against the
+oracle's restatement of single_scatterer_nonrel.f / semrel_c.f -- needs a GPU.
+
+The kernels follow the same fit, Bulirsch-Stoer steps and matching as the
+oracle in FP64; floating-point contraction on the GPU can move an adaptive
+step decision, so the bar is relative: 1e-10 on matom and on every zlr/jlr
+column (relative to the column's largest value), the accuracy bar of the
+tau00 path these t-matrices feed.
+"""
+import numpy as np
+import pytest
+
+from lsms_amd import capi
+from lsms_amd import configs as CF
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def colerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("nrelv", [0, 10])
+def test_batch_matches_oracle(ctx, nrelv):
+    rr, vr, jm, jw, rs, e, pr = CF.single_site_batch(nspecies=4, ne=48)
+    print(f"single-site batch nrelv={nrelv} npot={len(rr)} ne={len(e)}", flush=True)
+    matom, tmat, zlr, jlr, rc = ctx.single_site_nonrel(nrelv, 3, vr, rr, jm, jw, rs, e, pr)
+    assert rc == 0
+    worst = [0.0, 0.0, 0.0]
+    for q in range(len(rr)):
+        n = jw[q]
+        for k in range(len(e)):
+            m, t, z, j, orc = O.single_scatterer_nonrel(nrelv, 3, e[k], pr[k], vr[q], rr[q], jm[q],
+                                                        jw[q], rs[q])
+            assert orc == 0
+            worst[0] = max(worst[0], np.max(np.abs(matom[k, q] - m) / np.abs(m)))
+            assert np.abs(tmat[k, q] - t).max() <= TOL * np.abs(t).max()
+            for l in range(4):
+                worst[1] = max(worst[1], colerr(zlr[k, q, l, :n], z[:n, l]))
+                worst[2] = max(worst[2], colerr(jlr[k, q, l, :n], j[:n, l]))
+            assert not np.any(zlr[k, q, :, n:]) and not np.any(jlr[k, q, :, n:])
+    print("max rel err matom/zlr/jlr:", worst, flush=True)
+    assert max(worst) < TOL, worst
+
+
+def test_constant_potential_closed_form_on_gpu(ctx):
+    """Constant potential inside the sphere, matched on a mesh point: matom
+    against the spherical-Bessel closed form (the oracle's own pin)."""
+    from tests.test_single_site_cpu import square_well_matom
+    rmt, jmt = 2.3, 400
+    r = CF.log_mesh(rmt, jmt)
+    rs = r[jmt - 1]
+    es = np.array([0.6 + 0.05j, -0.3 + 0.4j, 1.2 + 0.001j])
+    vr = np.stack([-0.8 * r, 0.3 * r])
+    matom, _, _, _, rc = ctx.single_site_nonrel(10, 3, vr, np.stack([r, r]), jmt, jmt + 15, rs,
+                                                es, np.sqrt(es), radial=False)
+    assert rc == 0
+    for q, V0 in enumerate((-0.8, 0.3)):
+        for k, E in enumerate(es):
+            ref = square_well_matom(V0, E, rs, 3)
+            assert np.max(np.abs(matom[k, q] - ref) / np.abs(ref)) < 1e-12
+
+
+def test_repeat_and_subset_are_identical(ctx):
+    """A second call, and a call on a subset of the potentials and energies,
+    give bitwise the same numbers (one thread per (energy, potential, l))."""
+    rr, vr, jm, jw, rs, e, pr = CF.single_site_batch(nspecies=2, ne=8)
+    a = ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr)
+    b = ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr)
+    c = ctx.single_site_nonrel(0, 3, vr[1:3], rr[1:3], jm[1:3], jw[1:3], rs[1:3], e[2:5], pr[2:5])
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(x, y)
+    assert np.array_equal(c[0], a[0][2:5, 1:3])
+    assert np.array_equal(c[2], a[2][2:5, 1:3])
+
+
+def test_argument_errors(ctx):
+    r, vr, jmt, jws, rs = CF.model_potential(26, 2.3, iprpts=465)
+    e = np.array([0.5 + 0.01j])
+    with pytest.raises(capi.LsmsxError):  # jmt + 11 > iprpts
+        ctx.single_site_nonrel(0, 3, vr, r, jmt, jws, rs, e, np.sqrt(e))
+    r, vr, jmt, jws, rs = CF.model_potential(26, 2.3)
+    with pytest.raises(capi.LsmsxError):  # r_sph outside the mesh
+        ctx.single_site_nonrel(0, 3, vr, r, jmt, jws, -1.0, e, np.sqrt(e))
+    with pytest.raises(capi.LsmsxError):  # lmax above the path's 3
+        ctx.single_site_nonrel(0, 4, vr, r, jmt, jws, rs, e, np.sqrt(e))
+    m, _, _, _, rc = ctx.single_site_nonrel(0, 3, vr, r, jmt, jws, rs, e[:0], e[:0])
+    assert rc == 0 and m.shape == (0, 1, 4)
