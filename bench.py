@@ -141,10 +141,11 @@ def main():
     ap.add_argument("--schedule", type=int, default=-1,
                     help="schedule flags (LSMSX_SCHED_*) instead of the autotuner's choice; "
                          "0 = one stream, as used for the rocprofv3 summaries")
-    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "ss"), default="c2",
                     help="c2: the headline (fcc 135 sites, mixed vs FP64); c4: uneven 4096-site "
                          "bcc alloy batch (graph + autotuned schedule vs eager); c5: spin-canted "
-                         "bcc 137 sites (N=4384), plain tau00 and the adjoint route")
+                         "bcc 137 sites (N=4384), plain tau00 and the adjoint route; ss: "
+                         "single-site t-matrices of 64 species x 2 spins at 48 energies")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
@@ -180,6 +181,8 @@ def main():
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "ss":
+        return bench_ss(args)
 
     s = CF.config_c2(npts=args.nenergies, natoms=args.natoms)
     ctx = capi.Context(0)
@@ -515,6 +518,52 @@ def bench_c5(args):
                     "inverse_tflops": round(inv_fl * len(atoms) / t_adj / 1e12, 3),
                     "max_rel_diff_tau00_vs_plain": err},
     }
+    print(json.dumps(out))
+
+
+def bench_ss(args):
+    """Single-site scattering (single_scatterer_nonrel.f via semrel_c.f) of
+    64 species x 2 spins = 128 potentials (jmt = 450, iprpts = 1000) at 48
+    contour energies, lmax = 3, scalar-relativistic: one call per step.
+    Reports (potential, energy) solves/s for t-matrices only (what the tau00
+    path consumes) and with the radial functions zlr, jlr copied back."""
+    import numpy as np
+    from lsms_amd import capi
+    from lsms_amd import configs as CF
+    rr, vr, jm, jw, rs, e, pr = CF.single_site_batch(nspecies=64, ne=48)
+    ctx = capi.Context(0)
+    run_t = lambda: ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr, radial=False)
+    run_r = lambda: ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr, radial=True)
+    (m, _, _, _, rc), t_t = _timed_runs(run_t, args.steps, args.warmup)
+    (_, _, _, _, rc2), t_r = _timed_runs(run_r, max(1, args.steps // 2), 1)
+    ctx.close()
+    assert rc == 0 and rc2 == 0
+    npair = len(rr) * len(e)
+    out = {
+        "metric": "single-site solves/sec (potential x energy, lmax=3, scalar-relativistic)",
+        "value": round(npair / t_t, 1), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t_t, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(rr)} model potentials (64 species x 2 spins, Z 22-29, "
+                               f"jmt=450, iprpts=1000) x {len(e)} energies",
+                   "solves_per_step": npair},
+        "with_radial_functions": {"solves_per_s": round(npair / t_r, 1),
+                                  "ms_per_step": round(1e3 * t_r, 3)},
+    }
+    if not args.no_cpu:
+        import time as _t
+        from oracle import oracle as O
+        t0 = _t.perf_counter()
+        n = 0
+        while _t.perf_counter() - t0 < 3.0:
+            q, k = n % len(rr), (7 * n) % len(e)
+            O.single_scatterer_nonrel(0, 3, e[k], pr[k], vr[q], rr[q], jm[q], jw[q], rs[q])
+            n += 1
+        dt = _t.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / dt, 1), "unit": "solves/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{n} (potential, energy) solves of the same batch "
+                                         "through the oracle's restatement, one thread"}
     print(json.dumps(out))
 
 

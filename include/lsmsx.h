@@ -239,7 +239,9 @@ int lsmsx_autotune(lsmsx_ctx *ctx, int nsolve, const int *atom_ids, const int *i
  *   zlr, jlr        may be NULL; [(q*(lmax+1) + l)*iprpts + j] complex for j < jws
  *                   (solution.zlr, solution.jlr; other j are zero)
  * Returns 0; 1 if an adaptive integration step underflowed somewhere (the
- * reference prints a warning and goes on; the results are still written);
+ * reference prints a warning and goes on; the results are still written).
+ * The inward (irregular) integration feeds only jlr and runs only when jlr
+ * is requested;
  * < 0 on bad arguments.  The spin-canted frame rotation (trltog) stays with
  * the caller. */
 int lsmsx_single_site_nonrel(lsmsx_ctx *ctx, int nrelv, int lmax, int npot, int iprpts,

@@ -552,8 +552,9 @@ __global__ void __launch_bounds__(64) k_single_site(
     }
   }
 
-  // irregular solution inwards from r_sph (rwave.f:181-223)
-  {
+  // irregular solution inwards from r_sph (rwave.f:181-223); it feeds only
+  // jlr, so a call without radial functions skips it
+  if (jcol) {
     const zd hl = b.hm_l * b.eiz / ka;
     zd dh;
     if (l == 0) {
