@@ -250,6 +250,27 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *ctx, int nrelv, int lmax, int npot, int 
                              const double *prel, double *matom, double *tmat, double *zlr,
                              double *jlr);
 
+/* Muffin-tin density of states and spherical Green function of nsolve
+ * (atom, energy) pairs of one collinear spin channel, from tau00 and the
+ * single-site radial functions: the sphere part of green_function_
+ * (MultipleScattering/green_function.f:107-124 for isp = 1, gf_local.f:109-189,
+ * int_zz_zj.f:116-165, mdosms_c.f, mgreen_c.f), which is the whole-cell
+ * result for ASA potentials (mtasa = 1, int_zz_zj.f:226-237).  The
+ * interstitial part of muffin-tin potentials (inter_m, Voronoi cells) is
+ * not computed.
+ *   pot_of[b]       the potential (0..npot-1) whose mesh solve b uses
+ *   r_mesh, jmt, jws, r_sph   per potential, as for lsmsx_single_site_nonrel
+ *   tau00           nsolve x kkrsz x kkrsz complex, column-major, local frame
+ *                   (tau00_l), kkrsz = (lmax+1)^2
+ *   zlr, jlr        nsolve x (lmax+1) x iprpts complex (the single-site layout)
+ * Outputs: dos[b] (complex; -Im/pi convention of mdosms, times 2 when
+ * n_spin_pola = 1, as green_function.f:184-191 does), green (may be NULL):
+ * [b*iprpts + j] for j < jws, zero beyond. */
+int lsmsx_green_mt(lsmsx_ctx *ctx, int nsolve, int lmax, int iprpts, int n_spin_pola, int npot,
+                   const int *pot_of, const double *r_mesh, const int *jmt, const int *jws,
+                   const double *r_sph, const double *tau00, const double *zlr,
+                   const double *jlr, double *dos, double *green);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,7 +25,7 @@ EXPORTED = [
     "lsmsx_get_profile", "lsmsx_set_refinement", "lsmsx_last_batch_stats",
     "lsmsx_upload_dtmats", "lsmsx_tau_full", "lsmsx_tau00_adjoint", "lsmsx_set_schedule",
     "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune", "lsmsx_set_pivot_guard",
-    "lsmsx_single_site_nonrel",
+    "lsmsx_single_site_nonrel", "lsmsx_green_mt",
 ]
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
@@ -75,6 +75,8 @@ def load():
                                  _vp, _vp]
     L.lsmsx_tau00_adjoint.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                       _vp, _vp, _vp, _vp, _vp]
+    L.lsmsx_green_mt.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
+                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_single_site_nonrel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
                                            _vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
     _lib = L
@@ -237,6 +239,32 @@ class Context:
             _ck(rc)
         # tmat blocks are column-major kkrsz x kkrsz
         return matom, np.swapaxes(tmat, 2, 3), zlr, jlr, rc
+
+    def green_mt(self, lmax, tau00, zlr, jlr, pot_of, r_mesh, jmt, jws, r_sph, n_spin_pola=1,
+                 with_green=True):
+        """Muffin-tin DOS and spherical Green function of nsolve solves of one
+        collinear spin (green_function.f, sphere part).  tau00: (nsolve,
+        kkrsz, kkrsz); zlr, jlr: (nsolve, lmax+1, iprpts); pot_of: (nsolve,);
+        r_mesh: (npot, iprpts).  Returns (dos[nsolve], green[nsolve, iprpts])."""
+        rm = np.ascontiguousarray(np.atleast_2d(r_mesh), dtype=np.float64)
+        npot, iprpts = rm.shape
+        z = np.ascontiguousarray(zlr, dtype=np.complex128)
+        j = np.ascontiguousarray(jlr, dtype=np.complex128)
+        nsolve = z.shape[0]
+        if z.shape != (nsolve, lmax + 1, iprpts) or j.shape != z.shape:
+            raise ValueError("zlr/jlr must be (nsolve, lmax+1, iprpts)")
+        # column-major kkrsz x kkrsz blocks
+        t = np.ascontiguousarray(np.swapaxes(np.asarray(tau00, dtype=np.complex128), 1, 2))
+        po = np.ascontiguousarray(pot_of, dtype=np.int32)
+        jm = np.ascontiguousarray(np.broadcast_to(jmt, (npot,)), dtype=np.int32)
+        jw = np.ascontiguousarray(np.broadcast_to(jws, (npot,)), dtype=np.int32)
+        rs = np.ascontiguousarray(np.broadcast_to(r_sph, (npot,)), dtype=np.float64)
+        dos = np.zeros(nsolve, dtype=np.complex128)
+        green = np.zeros((nsolve, iprpts), dtype=np.complex128) if with_green else None
+        _ck(load().lsmsx_green_mt(self._h, nsolve, lmax, iprpts, n_spin_pola, npot, _p(po), _p(rm),
+                                  _p(jm), _p(jw), _p(rs), _p(t), _p(z), _p(j), _p(dos),
+                                  _p(green) if with_green else None))
+        return dos, green
 
     def tau00_adjoint(self, atom_ids, iies, energies, kkrsz_ns, ispins=None, lloyd=True,
                       tau00_bar=None, store_shape=None):

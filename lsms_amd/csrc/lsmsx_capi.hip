@@ -56,6 +56,8 @@ __global__ void k_fitpot(const double *, const double *, const int *, int, int, 
 __global__ void k_single_site(const double *, const double *, const SingleSitePot *,
                               const double2 *, const double2 *, int, int, int, int, int,
                               double2 *, double2 *, double2 *, int *);
+__global__ void k_green_mt(const double2 *, const double2 *, const double2 *, const double *,
+                           const GreenPot *, const int *, int, int, double, double2 *, double2 *);
 __global__ void k_ir_extract(const SolveDesc *);
 __global__ void k_count_conv(const SolveDesc *, int, int *);
 __global__ void k_lloyd(const SolveDesc *, const double2 *, const double2 *, long, const double *,
@@ -194,6 +196,9 @@ struct lsmsx_ctx {
   DevBuf<SingleSitePot> ss_pot;
   DevBuf<int> ss_iend, ss_status;
   DevBuf<double2> ss_e, ss_prel, ss_matom, ss_zlr, ss_jlr;
+  DevBuf<GreenPot> gf_pot;
+  DevBuf<int> gf_potof;
+  DevBuf<double2> gf_tau, gf_dos, gf_green;
   DevBuf<long> keyl;
   std::vector<int> last_iters;
   std::vector<double> last_minpiv;
@@ -1179,6 +1184,11 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->ss_matom.release();
   c->ss_zlr.release();
   c->ss_jlr.release();
+  c->gf_pot.release();
+  c->gf_potof.release();
+  c->gf_tau.release();
+  c->gf_dos.release();
+  c->gf_green.release();
   c->M.release();
   c->inv.release();
   c->tau.release();
@@ -1725,6 +1735,78 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int ip
     }
   }
   return status ? 1 : 0;
+}
+
+int lsmsx_green_mt(lsmsx_ctx *c, int nsolve, int lmax, int iprpts, int n_spin_pola, int npot,
+                   const int *pot_of, const double *r_mesh, const int *jmt, const int *jws,
+                   const double *r_sph, const double *tau00, const double *zlr,
+                   const double *jlr, double *dos, double *green) {
+  if (!c) return fail(-1, "null context");
+  if (lmax < 0 || lmax > 3) return fail(-2, "lmax must be 0..3");
+  if (nsolve < 0 || npot < 1 || iprpts < 1) return fail(-2, "bad batch size");
+  if (n_spin_pola != 1 && n_spin_pola != 2) return fail(-2, "n_spin_pola must be 1 or 2");
+  if (nsolve == 0) return 0;
+  if (!pot_of || !r_mesh || !jmt || !jws || !r_sph || !tau00 || !zlr || !jlr || !dos)
+    return fail(-2, "null argument");
+  std::vector<GreenPot> hp((size_t)npot);
+  for (int p = 0; p < npot; p++) {
+    const double *r = r_mesh + (size_t)p * iprpts;
+    GreenPot &P = hp[p];
+    P.jmt = jmt[p];
+    P.jws = jws[p];
+    if (P.jmt < 4 || P.jmt > iprpts || P.jws < 0 || P.jws > iprpts)
+      return fail(-2, "jmt or jws outside 4..iprpts");
+    if (!(r[0] > 0.0)) return fail(-2, "the radial mesh must start above r = 0");
+    for (int j = 0; j + 1 < P.jmt; j++)
+      if (!(r[j + 1] > r[j])) return fail(-2, "the radial mesh must increase");
+    if (!(r_sph[p] > 0.0)) return fail(-2, "r_sph must be > 0");
+    P.xs = std::sqrt(r_sph[p]);
+    // cinterp.f window on x_0 = 0, x_k = sqrt(r_k), k = 1..jmt
+    const int nr = P.jmt + 1;
+    int ip1 = 1;
+    for (int i = 1; i <= nr - 2; i++)
+      if (P.xs > std::sqrt(r[i - 1])) ip1 = i + 1;
+    const int i = ip1 - 1;
+    P.ip1 = ip1;
+    P.i1 = (i > 0) ? i - 1 : std::min(4, nr) - 1;
+    P.i2 = (ip1 < nr - 1) ? ip1 + 1 : std::max(nr - 3, 1) - 1;
+  }
+  for (int b = 0; b < nsolve; b++)
+    if (pot_of[b] < 0 || pot_of[b] >= npot) return fail(-2, "pot_of out of range");
+  const int nl = lmax + 1, kk = nl * nl;
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->ss_r.ensure((size_t)npot * iprpts)) return r;
+  if (int r = c->gf_pot.ensure(npot)) return r;
+  if (int r = c->gf_potof.ensure(nsolve)) return r;
+  if (int r = c->gf_tau.ensure((size_t)nsolve * kk * kk)) return r;
+  if (int r = c->ss_zlr.ensure((size_t)nsolve * nl * iprpts)) return r;
+  if (int r = c->ss_jlr.ensure((size_t)nsolve * nl * iprpts)) return r;
+  if (int r = c->gf_dos.ensure(nsolve)) return r;
+  if (green)
+    if (int r = c->gf_green.ensure((size_t)nsolve * iprpts)) return r;
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(c->ss_r.p, r_mesh, (size_t)npot * iprpts * sizeof(double),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gf_pot.p, hp.data(), npot * sizeof(GreenPot), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gf_potof.p, pot_of, nsolve * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gf_tau.p, tau00, (size_t)nsolve * kk * kk * sizeof(double2),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_zlr.p, zlr, (size_t)nsolve * nl * iprpts * sizeof(double2),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_jlr.p, jlr, (size_t)nsolve * nl * iprpts * sizeof(double2),
+                        hipMemcpyHostToDevice, st));
+  if (green)
+    HIPCHK(hipMemsetAsync(c->gf_green.p, 0, (size_t)nsolve * iprpts * sizeof(double2), st));
+  hipLaunchKernelGGL(k_green_mt, dim3((unsigned)nsolve), dim3(64), 0, st, c->gf_tau.p, c->ss_zlr.p,
+                     c->ss_jlr.p, c->ss_r.p, c->gf_pot.p, c->gf_potof.p, lmax, iprpts,
+                     n_spin_pola == 1 ? 2.0 : 1.0, c->gf_dos.p, green ? c->gf_green.p : nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dos, c->gf_dos.p, nsolve * sizeof(double2), hipMemcpyDeviceToHost, st));
+  if (green)
+    HIPCHK(hipMemcpyAsync(green, c->gf_green.p, (size_t)nsolve * iprpts * sizeof(double2),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
 }
 
 }  // extern "C"

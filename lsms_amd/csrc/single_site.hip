@@ -596,4 +596,177 @@ __global__ void __launch_bounds__(64) k_single_site(
   if (rc) atomicOr(status, 1);
 }
 
+// ---------------------------------------------------------------------------
+// Muffin-tin (ASA: whole-cell) DOS and Green function of one collinear spin
+// channel from tau00 and the single-site radial functions: gf_local.f with
+// the sphere integrals of int_zz_zj.f:116-165 (cnewint.f on x = sqrt(r),
+// cinterp.f at sqrt(r_sph)), mdosms_c.f and mgreen_c.f, times 2 when
+// n_spin_pola = 1 (green_function.f:184-191).
+//
+// One 64-thread workgroup per solve: threads 2l and 2l+1 run the running
+// integrals of 2 z_l z_l and 2 z_l j_l over the jmt+1 points of the x mesh
+// (a ring of four fitted points, as cnewint keeps them); thread 0 contracts
+// them with the diagonal of tau00; all threads then form green(r).
+
+namespace {
+
+// running integral step of cnewint.f for one interval: slots of the ring in
+// rs/fr/fi, interval from slot i0 to the next; returns the increment
+__device__ zd cnewint_step(const double *rs, const double *fr, const double *fi, int i0, int ip) {
+  const int i1 = (i0 + 3) & 3, ip1 = (i0 + 1) & 3, i2 = (i0 + 2) & 3;
+  double xl[4], cr[4], ci[4];
+  xl[0] = rs[i1];
+  xl[1] = rs[i0];
+  xl[2] = rs[ip1];
+  xl[3] = rs[i2];
+  fit4(xl, 4, 1, 0, 3, fr[i1], fr[i0], fr[ip1], fr[i2], cr);
+  fit4(xl, 4, 1, 0, 3, fi[i1], fi[i0], fi[ip1], fi[i2], ci);
+  const double x0 = rs[i0], rip1 = rs[ip1];
+  const double pip1 = 1.0 / (double)(ip + 1), pip2 = 1.0 / (double)(ip + 2);
+  double gj = 1.0, fj = rip1;
+  for (int j = 1; j <= ip; j++) {
+    gj = gj * x0 + fj;
+    fj = fj * rip1;
+  }
+  fj = (gj * x0 + fj) * pip2;
+  gj = gj * pip1;
+  double gr = (fr[i0] - cr[2] * x0) * gj + cr[2] * fj;
+  double gi = (fi[i0] - ci[2] * x0) * gj + ci[2] * fj;
+  const double dr = rip1 - x0;
+  const double h1 = dr / 6.0;
+  const double wt0 = 2.0 / 18.0, wt1 = 3.0 * wt0;
+  double h2 = 0.0, wt = wt0;
+#pragma unroll
+  for (int j = 1; j <= 5; j++) {
+    h2 = h2 + h1;
+    wt = wt1 - wt;
+    fj = wt;
+    const double rj = x0 + h2;
+    for (int k = 1; k <= ip; k++) fj = fj * rj;
+    gj = h2 * (dr - h2);
+    double c1 = (cr[2] - cr[1]) * h2 + cr[0];
+    double c2 = cr[3] * gj;
+    gr = gr - fj * c1 * c2 / (1.0 + c2);
+    c1 = (ci[2] - ci[1]) * h2 + ci[0];
+    c2 = ci[3] * gj;
+    gi = gi - fj * c1 * c2 / (1.0 + c2);
+  }
+  return mk(2.0 * dr * gr, 2.0 * dr * gi);
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(64) k_green_mt(
+    const double2 *__restrict__ tau00, const double2 *__restrict__ zlr,
+    const double2 *__restrict__ jlr, const double *__restrict__ rmesh,
+    const GreenPot *__restrict__ pots, const int *__restrict__ pot_of, int lmax, int iprpts,
+    double spin, double2 *__restrict__ dos, double2 *__restrict__ green) {
+  __shared__ zd s_int[2][4];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int nl = lmax + 1, kk = nl * nl;
+  const GreenPot P = pots[pot_of[b]];
+  const double *rm = rmesh + (long)pot_of[b] * iprpts;
+  const double2 *z0 = zlr + (long)b * nl * iprpts, *j0 = jlr + (long)b * nl * iprpts;
+  const double2 *tb = tau00 + (long)b * kk * kk;
+  const int jmt = P.jmt;
+  if (tid < 2 * nl) {
+    const int l = tid >> 1, which = tid & 1;
+    const double2 *zl = z0 + (long)l * iprpts, *ol = (which ? j0 : z0) + (long)l * iprpts;
+    // integrand on the x mesh (x_0 = 0, x_k = sqrt(r_k)), f_0 = f_1
+    auto fval = [&](int k) -> zd {
+      const int q = k == 0 ? 0 : k - 1;
+      const zd a = mk(zl[q].x, zl[q].y), c = mk(ol[q].x, ol[q].y);
+      return 2.0 * (a * c);
+    };
+    auto xval = [&](int k) -> double { return k == 0 ? 0.0 : sqrt(rm[k - 1]); };
+    const int nr = jmt + 1;
+    const int ip = 2 * 5 + 1;
+    double rs[4], fr[4], fi[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      rs[k] = sqrt(xval(k));
+      const zd f = fval(k);
+      fr[k] = f.x;
+      fi[k] = f.y;
+    }
+    zd g = mk(0.0), w[4];
+    int i4 = 3, i0 = 0, ip2 = 4;
+    auto capture = [&](int k) {
+      if (k == P.i1) w[0] = g;
+      if (k == P.ip1 - 1) w[1] = g;
+      if (k == P.ip1) w[2] = g;
+      if (k == P.i2) w[3] = g;
+    };
+    capture(0);
+    for (int i = 1; i <= nr - 1; i++) {
+      if (i > 2 && ip2 <= nr) {
+        const zd f = fval(ip2 - 1);
+        const double xv = sqrt(xval(ip2 - 1));
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (q == i4) {
+            rs[q] = xv;
+            fr[q] = f.x;
+            fi[q] = f.y;
+          }
+      }
+      g = g + cnewint_step(rs, fr, fi, i0, ip);
+      capture(i);
+      if (i > 1 && ip2 < nr) {
+        ip2 += 1;
+        i4 = (i4 + 1) & 3;  // the ring rotation i1..i4 advances i4 by one slot
+      }
+      i0 = (i0 + 1) & 3;
+    }
+    // cinterp.f at sqrt(r_sph) over the x mesh
+    double xw[4];
+    xw[0] = xval(P.i1);
+    xw[1] = xval(P.ip1 - 1);
+    xw[2] = xval(P.ip1);
+    xw[3] = xval(P.i2);
+    const int same = (P.i1 == P.i2);
+    s_int[which][l] = interp4(xw, 4, 1, 0, same ? 0 : 3, P.xs, w);
+  }
+  __syncthreads();
+  if (tid == 0) {  // mdosms: -(sum pzzck tau - pzjck)/pi
+    zd zz = mk(0.0), zj = mk(0.0);
+    for (int l = 0; l < nl; l++)
+      for (int m = -l; m <= l; m++) zj = zj + s_int[1][l];
+    int lm = 0;
+    for (int l = 0; l < nl; l++)
+      for (int m = -l; m <= l; m++, lm++) {
+        const double2 t = tb[lm + (long)lm * kk];
+        zz = zz + s_int[0][l] * mk(t.x, t.y);
+      }
+    const zd d = -(zz - zj) / 3.14159265358979323846;
+    dos[b] = make_double2(spin * d.x, spin * d.y);
+  }
+  if (green) {  // mgreen: sum over l from lmax down of z (z wt_l - j (2l+1))
+    zd wt[4];
+    int i = kk - 1;
+    for (int l = lmax; l >= 0; l--) {
+      zd s = mk(0.0);
+      for (int m = -l; m <= l; m++, i--) {
+        const double2 t = tb[i + (long)i * kk];
+        s = s + mk(t.x, t.y);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (q == l) wt[q] = s;
+    }
+    double2 *gb = green + (long)b * iprpts;
+    for (int ir = tid; ir < P.jws; ir += blockDim.x) {
+      zd acc = mk(0.0);
+#pragma unroll
+      for (int l = 3; l >= 0; l--) {
+        if (l > lmax) continue;
+        const double2 zv = z0[(long)l * iprpts + ir], jv = j0[(long)l * iprpts + ir];
+        const zd zz = mk(zv.x, zv.y), jj = mk(jv.x, jv.y);
+        acc = acc + zz * (zz * wt[l] - jj * (double)(2 * l + 1));
+      }
+      gb[ir] = make_double2(spin * acc.x, spin * acc.y);
+    }
+  }
+}
+
 }  // namespace lsmsx

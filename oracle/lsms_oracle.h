@@ -85,6 +85,12 @@ int or_single_scatterer_nonrel(int nrelv, int lmax, zc energy, zc prel, const do
                                zc *matom, zc *tmat, zc *zlr, zc *jlr);
 void or_fitpot(const double *r, const double *rv, double *coef, int nfit, int iend);
 void or_sph_bessel_jh(int lmax, zc x, zc *bj, zc *hm, zc *eiz);
+/* cnewint.f: running integral of f x^ip0 on the mesh x */
+void or_cnewint(int nr, const double *x, const zc *f, zc *g, int ip0);
+/* muffin-tin / ASA part of green_function.f (gf_local, int_zz_zj, mdosms, mgreen) */
+int or_green_mt(int lmax, const zc *tau00, const zc *zlr, const zc *jlr, const double *r_mesh,
+                int jmt, int jws, double r_sph, int iprpts, int n_spin_pola, zc *dos, zc *green,
+                zc *zz_out, zc *zj_out);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,10 @@ def lib():
                                              C.c_int, C.c_double, C.c_int, _zp, _zp, _zp, _zp]
     L.or_fitpot.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int]
     L.or_sph_bessel_jh.argtypes = [C.c_int, _Z, _zp, _zp, _zp]
+    L.or_cnewint.argtypes = [C.c_int, _dp, _zp, _zp, C.c_int]
+    L.or_green_mt.restype = C.c_int
+    L.or_green_mt.argtypes = [C.c_int, _zp, _zp, _zp, _dp, C.c_int, C.c_int, C.c_double, C.c_int,
+                              C.c_int, _zp, _zp, _zp, _zp]
     _lib = L
     return L
 
@@ -233,3 +237,31 @@ def sph_bessel_jh(lmax, x):
     eiz = np.zeros(1, dtype=np.complex128)
     lib().or_sph_bessel_jh(lmax, _z(x), bj, hm, eiz)
     return bj, hm, eiz[0]
+
+
+def cnewint(x, f, ip0):
+    """Misc/cnewint.f: running integral of f x^ip0 on the mesh x."""
+    x = _f(x, np.float64)
+    f = _f(f, np.complex128)
+    g = np.zeros(len(x), dtype=np.complex128)
+    lib().or_cnewint(len(x), x, f, g, ip0)
+    return g
+
+
+def green_mt(lmax, tau00, zlr, jlr, r_mesh, jmt, jws, r_sph, n_spin_pola=1):
+    """Muffin-tin / ASA part of green_function.f for one collinear spin:
+    (dos, green[jws], zlzl[lmax+1], zljl[lmax+1])."""
+    tau00 = _f(tau00, np.complex128)
+    zlr = _f(zlr, np.complex128)
+    jlr = _f(jlr, np.complex128)
+    r_mesh = _f(r_mesh, np.float64)
+    iprpts = zlr.shape[0]
+    dos = np.zeros(1, dtype=np.complex128)
+    green = np.zeros(max(jws, 1), dtype=np.complex128)
+    zz = np.zeros(lmax + 1, dtype=np.complex128)
+    zj = np.zeros(lmax + 1, dtype=np.complex128)
+    rc = lib().or_green_mt(lmax, tau00, zlr, jlr, r_mesh, jmt, jws, r_sph, iprpts, n_spin_pola,
+                           dos, green, zz, zj)
+    if rc != 0:
+        raise ValueError("or_green_mt: bad sizes")
+    return dos[0], green[:jws], zz, zj

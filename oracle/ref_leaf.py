@@ -194,3 +194,56 @@ def single_scatterer_nonrel(nrelv, lmax, energy, prel_, vr, r, jmt, jws, r_sph, 
                                  vp(matom), vp(zlr), vp(jlr), _d(r_sph), _i(iprpts), _i(-1),
                                  _ISTOP, C.c_size_t(32))
     return matom, tmat, zlr, jlr
+
+
+def _ssl():
+    global _ss
+    if _ss is None:
+        _ss = C.CDLL(_SS_PATH)
+    return _ss
+
+
+def cnewint(x, f, ip0):
+    """Misc/cnewint.f: running integral of f x^ip0 on the mesh x."""
+    x = np.asfortranarray(np.asarray(x, dtype=np.float64))
+    f = np.asfortranarray(np.asarray(f, dtype=np.complex128))
+    g = np.zeros(len(x), dtype=np.complex128, order="F")
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    _ssl().cnewint_(_i(len(x)), vp(x), vp(f), vp(g), _i(ip0))
+    return g
+
+
+def cinterp(x, f, xs):
+    """Misc/cinterp.f: value of f at xs."""
+    x = np.asfortranarray(np.asarray(x, dtype=np.float64))
+    f = np.asfortranarray(np.asarray(f, dtype=np.complex128))
+    work = np.zeros(len(x), dtype=np.float64, order="F")
+    ps = np.zeros(1, dtype=np.complex128)
+    dps = np.zeros(1, dtype=np.complex128)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    _ssl().cinterp_(vp(x), vp(f), _i(len(x)), _d(xs), vp(ps), vp(dps), vp(work))
+    return ps[0]
+
+
+def mdosms(zj_flag, zz, zj, tau):
+    """MultipleScattering/mdosms_c.f: -(sum zz tau - zj)/pi."""
+    zz = np.asfortranarray(zz, dtype=np.complex128)
+    tau = np.asfortranarray(tau, dtype=np.complex128)
+    zjv = _zarr(zj)
+    dos = np.zeros(1, dtype=np.complex128)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    _ssl().mdosms_(_i(zj_flag), _i(zz.size), vp(dos), vp(zz), vp(zjv), vp(tau),
+                   _d(np.pi), _i(-1), _ISTOP, C.c_size_t(32))
+    return dos[0]
+
+
+def mgreen(zj_flag, lmax, jws, zlr, jlr, tau):
+    """MultipleScattering/mgreen_c.f: green(r) for r < jws (left = right)."""
+    zlr = np.asfortranarray(zlr, dtype=np.complex128)
+    jlr = np.asfortranarray(jlr, dtype=np.complex128)
+    tau = np.asfortranarray(tau, dtype=np.complex128)
+    green = np.zeros(jws, dtype=np.complex128, order="F")
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    _ssl().mgreen_(_i(zj_flag), _i(tau.shape[0]), _i(lmax), _i(jws), _i(zlr.shape[0]), vp(zlr),
+                   vp(zlr), vp(jlr), vp(green), vp(tau), _i(-1), _ISTOP, C.c_size_t(32))
+    return green

@@ -81,7 +81,13 @@ static void fitpot4(const double *r, const double *rv, double *coef, int nfit, i
 static int fit4(const double *r, const double *f, int nr, int i, double c[4])
 {
     int ip1 = i + 1;
-    if (i < 0 || ip1 >= nr) return -1;
+    if (i < 0 || i >= nr) return -1;
+    if (ip1 >= nr) { /* wrap around on a ring of points (fit.f:53-62) */
+        if (r[0] > r[nr - 1])
+            ip1 = 0;
+        else
+            return -1;
+    }
     double dr = r[ip1] - r[i];
     if (dr == 0.0) return -1;
     c[2] = (f[ip1] - f[i]) / dr;
@@ -516,4 +522,139 @@ void or_fitpot(const double *r, const double *rv, double *coef, int nfit, int ie
 void or_sph_bessel_jh(int lmax, zc x, zc *bj, zc *hm, zc *eiz)
 {
     sph_bessel_jh(lmax, x, bj, hm, eiz);
+}
+
+/* ---- cnewint.f: running integral g(k) = int_{x_1}^{x_k} f(x) x^ip0 dx on an
+ * arbitrary increasing mesh x (here x = sqrt(r)), from the rational
+ * four-point fit of f in s = sqrt(x) on a ring of four points, its linear part
+ * integrated exactly and the rest by Simpson's rule on six sub-intervals. */
+void or_cnewint(int nr, const double *x, const zc *f, zc *g, int ip0)
+{
+    const int nj = 6;
+    const double dnj = 1.0 / nj, wt0 = 2.0 / (3.0 * nj), wt1 = 3.0 * wt0;
+    const int ip = 2 * ip0 + 1;
+    const double pip1 = 1.0 / (double)(ip + 1), pip2 = 1.0 / (double)(ip + 2);
+    double rs[4], fr[4], fi[4], cr[4], ci[4];
+    g[0] = 0.0;
+    int i1 = 0, i2 = 1, i3 = 2, i4 = 3, i0 = 0; /* ring slots */
+    for (int k = 0; k < 4; k++) {
+        rs[k] = sqrt(x[k]);
+        fr[k] = creal(f[k]);
+        fi[k] = cimag(f[k]);
+    }
+    int ip1 = 1, ip2 = 4; /* 1-based point counters */
+    (void)i1;
+    (void)i2;
+    (void)i3;
+    for (int i = 1; i <= nr - 1; i++) {
+        ip1 += 1;
+        if (i > 2 && ip2 <= nr) {
+            rs[i4] = sqrt(x[ip2 - 1]);
+            fr[i4] = creal(f[ip2 - 1]);
+            fi[i4] = cimag(f[ip2 - 1]);
+        }
+        fit4(rs, fr, 4, i0, cr);
+        fit4(rs, fi, 4, i0, ci);
+        const double rip1 = rs[(i0 + 1) % 4];
+        double gj = 1.0, fj = rip1;
+        for (int j = 1; j <= ip; j++) {
+            gj = gj * rs[i0] + fj;
+            fj = fj * rip1;
+        }
+        fj = (gj * rs[i0] + fj) * pip2;
+        gj = gj * pip1;
+        double gr = (fr[i0] - cr[2] * rs[i0]) * gj + cr[2] * fj;
+        double gi = (fi[i0] - ci[2] * rs[i0]) * gj + ci[2] * fj;
+        const double dr = rip1 - rs[i0];
+        const double h1 = dnj * dr;
+        double h2 = 0.0, wt = wt0;
+        for (int j = 1; j <= nj - 1; j++) {
+            h2 = h2 + h1;
+            wt = wt1 - wt;
+            fj = wt;
+            const double rj = rs[i0] + h2;
+            for (int k = 1; k <= ip; k++) fj = fj * rj;
+            gj = h2 * (dr - h2);
+            double c1 = (cr[2] - cr[1]) * h2 + cr[0];
+            double c2 = cr[3] * gj;
+            gr = gr - fj * c1 * c2 / (1.0 + c2);
+            c1 = (ci[2] - ci[1]) * h2 + ci[0];
+            c2 = ci[3] * gj;
+            gi = gi - fj * c1 * c2 / (1.0 + c2);
+        }
+        g[ip1 - 1] = g[i - 1] + 2.0 * dr * (gr + I * gi);
+        if (i > 1 && ip2 < nr) {
+            ip2 += 1;
+            int t = i1;
+            i1 = i2;
+            i2 = i3;
+            i3 = i4;
+            i4 = t;
+        }
+        i0 = (i0 + 1) % 4;
+    }
+}
+
+/* ---- the muffin-tin (ASA: whole-cell) part of green_function.f for one
+ * collinear spin channel: gf_local.f:109-189 with int_zz_zj.f:116-165
+ * (zlzl_l, zljl_l over the sphere, integrated in x = sqrt(r)), mdosms_c.f
+ * (dos = -(sum zz tau - zj)/pi) and mgreen_c.f (green(r) for r < jws), with
+ * the factor 2 of green_function.f:184-191 when n_spin_pola = 1.
+ * tau00 kkrsz x kkrsz column-major; zlr, jlr iprpts x (lmax+1). */
+int or_green_mt(int lmax, const zc *tau00, const zc *zlr, const zc *jlr, const double *r_mesh,
+                int jmt, int jws, double r_sph, int iprpts, int n_spin_pola, zc *dos, zc *green,
+                zc *zz_out, zc *zj_out)
+{
+    const int kkrsz = (lmax + 1) * (lmax + 1);
+    if (jmt < 4 || jmt > iprpts || jws > iprpts || lmax < 0) return -1;
+    const double pi = 3.14159265358979323846;
+    double *rtmp = (double *)malloc(sizeof(double) * (size_t)(jmt + 1));
+    double *work = (double *)malloc(sizeof(double) * (size_t)(jmt + 1));
+    zc *fr = (zc *)malloc(sizeof(zc) * (size_t)(jmt + 1));
+    zc *fint = (zc *)malloc(sizeof(zc) * (size_t)(jmt + 1));
+    rtmp[0] = 0.0;
+    for (int ir = 1; ir <= jmt; ir++) rtmp[ir] = sqrt(r_mesh[ir - 1]);
+    const double xs = sqrt(r_sph);
+    zc zzsum = 0.0, zjsum = 0.0;
+    zc *zzl = (zc *)malloc(sizeof(zc) * (size_t)(lmax + 1));
+    for (int l = 0; l <= lmax; l++) {
+        const zc *z = zlr + (size_t)l * iprpts, *j = jlr + (size_t)l * iprpts;
+        for (int ir = 1; ir <= jmt; ir++) fr[ir] = 2.0 * z[ir - 1] * z[ir - 1];
+        fr[0] = fr[1];
+        or_cnewint(jmt + 1, rtmp, fr, fint, 5);
+        const zc zlzl = cinterp_value(rtmp, fint, jmt + 1, xs, work);
+        for (int ir = 1; ir <= jmt; ir++) fr[ir] = 2.0 * z[ir - 1] * j[ir - 1];
+        fr[0] = fr[1];
+        or_cnewint(jmt + 1, rtmp, fr, fint, 5);
+        const zc zljl = cinterp_value(rtmp, fint, jmt + 1, xs, work);
+        zzl[l] = zlzl;
+        if (zz_out) zz_out[l] = zlzl;
+        if (zj_out) zj_out[l] = zljl;
+        for (int m = -l; m <= l; m++) zjsum += zljl;
+    }
+    /* mdosms: sum over the kkrsz^2 entries of the diagonal pzzck times tau00 */
+    int lm = 0;
+    for (int l = 0; l <= lmax; l++)
+        for (int m = -l; m <= l; m++, lm++) zzsum += zzl[l] * tau00[lm + (size_t)lm * kkrsz];
+    zc d = -(zzsum - zjsum) / pi;
+    const double spin = (n_spin_pola == 1) ? 2.0 : 1.0;
+    *dos = spin * d;
+    if (green) {
+        for (int ir = 0; ir < jws; ir++) green[ir] = 0.0;
+        int i = kkrsz - 1;
+        for (int l = lmax; l >= 0; l--) {
+            zc wt = 0.0;
+            for (int m = -l; m <= l; m++) wt += tau00[i + (size_t)i * kkrsz], i--;
+            const zc *z = zlr + (size_t)l * iprpts, *j = jlr + (size_t)l * iprpts;
+            for (int ir = 0; ir < jws; ir++)
+                green[ir] += z[ir] * (z[ir] * wt - j[ir] * (double)(2 * l + 1));
+        }
+        for (int ir = 0; ir < jws; ir++) green[ir] *= spin;
+    }
+    free(zzl);
+    free(rtmp);
+    free(work);
+    free(fr);
+    free(fint);
+    return 0;
 }

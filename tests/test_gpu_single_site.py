@@ -143,3 +143,81 @@ def test_gpu_tmatrices_feed_tau00(ctx):
         rhs[:kk] = R.oracle_t0(s, atoms[b], ies[b])
         ref = sla.solve(m, rhs)[:kk]
         assert np.linalg.norm(tau[b] - ref) / np.linalg.norm(ref) < 1e-10, (atoms[b], ies[b])
+
+
+def test_green_mt_matches_oracle(ctx):
+    """Muffin-tin DOS and green(r) on the GPU against the oracle, for radial
+    functions of four potentials at six energies and random tau00 near t,
+    both spin conventions; the GPU quadrature uses the same ring of fitted
+    points, so the bar is 1e-12 relative."""
+    rr, vr, jm, jw, rs, e, pr = CF.single_site_batch(nspecies=2, ne=6)
+    npot, ne = len(rr), len(e)
+    matom, tmat, zlr, jlr, rc = ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr)
+    assert rc == 0
+    rng = np.random.default_rng(11)
+    nsolve = npot * ne
+    pot_of = np.tile(np.arange(npot), ne)
+    z = zlr.reshape(nsolve, 4, -1)
+    j = jlr.reshape(nsolve, 4, -1)
+    tau = tmat.reshape(nsolve, 16, 16) + 0.05 * (rng.normal(size=(nsolve, 16, 16)) +
+                                                1j * rng.normal(size=(nsolve, 16, 16)))
+    for nspol in (1, 2):
+        dos, green = ctx.green_mt(3, tau, z, j, pot_of, rr, jm, jw, rs, n_spin_pola=nspol)
+        for b in range(nsolve):
+            p = pot_of[b]
+            d_or, g_or, _, _ = O.green_mt(3, tau[b], z[b].T, j[b].T, rr[p], jm[p], jw[p], rs[p],
+                                          nspol)
+            assert abs(dos[b] - d_or) <= 1e-12 * abs(d_or), (b, dos[b], d_or)
+            n = jw[p]
+            assert np.abs(green[b, :n] - g_or).max() <= 1e-12 * np.abs(g_or).max()
+            assert not np.any(green[b, n:])
+
+
+def test_dos_chain_on_gpu(ctx):
+    """The whole chain on the GPU for the bcc-15 system with model
+    potentials: single-site t and radial functions, tau00 of every atom at
+    five energies, then the muffin-tin DOS; against the oracle chain
+    (oracle t, LAPACK tau00 on the oracle's KKR matrices, oracle DOS) to
+    1e-10 relative."""
+    import scipy.linalg as sla
+    from tests import adjoint_ref as R
+    s = CF.config_c1()
+    pots = [CF.model_potential(26, 2.3), CF.model_potential(29, 2.4, screen=0.65)]
+    rr = np.stack([p[0] for p in pots])
+    vr = np.stack([p[1] for p in pots])
+    jm = np.array([p[2] for p in pots])
+    jw = np.array([p[3] for p in pots])
+    rs = np.array([p[4] for p in pots])
+    prels = np.array([s.prel(i) for i in range(len(s.energies))])
+    matom, tmat, zlr, jlr, rc = ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, s.energies, prels)
+    assert rc == 0
+    kk = 16
+    store = np.zeros_like(s.tmatStore)
+    for sp in range(2):
+        for ie in range(len(s.energies)):
+            base = ie * s.blkSizeTmatStore
+            store[base:base + kk * kk, sp] = tmat[ie, sp].flatten(order="F")
+    s.tmatStore = store
+    c2 = capi.Context(0)
+    c2.load_system(s)
+    na = len(s.atoms)
+    iies = [0, 15, 30]
+    atoms = np.repeat(np.arange(na), len(iies))
+    ies = np.tile(iies, na)
+    tau, info = c2.tau00_batched(atoms, ies, [s.prel(i) for i in ies], kk)
+    assert np.all(info == 0)
+    sp_of = np.array([int(s.atoms[a].LIZStoreIdx[0]) for a in atoms])
+    z = np.stack([zlr[ies[b], sp_of[b]] for b in range(len(atoms))])
+    j = np.stack([jlr[ies[b], sp_of[b]] for b in range(len(atoms))])
+    dos, _ = c2.green_mt(3, tau, z, j, sp_of, rr, jm, jw, rs, with_green=False)
+    c2.close()
+    for b in range(len(atoms)):
+        a, ie, sp = atoms[b], ies[b], sp_of[b]
+        m = R.oracle_m(s, a, ie)
+        rhs = np.zeros((m.shape[0], kk), dtype=np.complex128)
+        rhs[:kk] = R.oracle_t0(s, a, ie)
+        tref = sla.solve(m, rhs)[:kk]
+        _, _, zo, jo, _ = O.single_scatterer_nonrel(0, 3, s.energies[ie], prels[ie], vr[sp], rr[sp],
+                                                    jm[sp], jw[sp], rs[sp])
+        d_or = O.green_mt(3, tref, zo, jo, rr[sp], jm[sp], jw[sp], rs[sp], 1)[0]
+        assert abs(dos[b] - d_or) <= 1e-10 * abs(d_or), (a, ie, dos[b], d_or)

@@ -117,3 +117,77 @@ def test_bad_sizes_raise():
     r = CF.log_mesh(2.3, 400, iprpts=405)
     with pytest.raises(ValueError):
         O.single_scatterer_nonrel(10, 3, 0.5, np.sqrt(0.5), -r, r, 400, 400, r[399])
+
+
+# ---- muffin-tin Green function (green_function.f sphere part) ----------
+
+def _radial(z=26, E=0.45 + 0.05j, nrelv=0):
+    r, vr, jmt, jws, rs = CF.model_potential(z, 2.3)
+    pr = CF.prel_of(E)
+    m, t, zl, jl, rc = O.single_scatterer_nonrel(nrelv, 3, E, pr, vr, r, jmt, jws, rs)
+    return r, jmt, jws, rs, m, t, zl, jl
+
+
+@pytest.mark.parametrize("ip0", [0, 1, 2, 5])
+def test_cnewint_integrates_powers(ip0):
+    """int_0^x f(x') x'^ip0 dx' = x^(k+ip0+1)/(k+ip0+1) for f = x^k: exact
+    (to rounding) for a constant, whose fit is exactly its linear part; to
+    the quadrature's accuracy for k = 2."""
+    r = CF.log_mesh(2.3, 450)[:450]
+    x = np.concatenate([[0.0], np.sqrt(r)])
+    for k, tol in ((0, 1e-14), (2, 1e-6)):
+        g = O.cnewint(x, (x ** k).astype(np.complex128), ip0)
+        exact = x ** (k + ip0 + 1) / (k + ip0 + 1)
+        assert np.abs(g - exact).max() <= tol * exact.max()
+
+
+@pytest.mark.skipif(not RL.ss_available(), reason="oracle/_ref/libss_ref.so not built")
+def test_green_mt_matches_reference_routines():
+    """cnewint equal to the reference's, and dos / green(r) of the oracle
+    equal to the reference's int_zz_zj sphere integrals (cnewint + cinterp),
+    mdosms and mgreen composed as gf_local does for mtasa = 1."""
+    r, jmt, jws, rs, m, t, zl, jl = _radial()
+    x = np.concatenate([[0.0], np.sqrt(r[:jmt])])
+    f = np.concatenate([[0], 2 * zl[:jmt, 1] ** 2])
+    f[0] = f[1]
+    assert np.array_equal(O.cnewint(x, f, 5), RL.cnewint(x, f, 5))
+    rng = np.random.default_rng(3)
+    tau = (rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16))) * 0.1 + t
+    for nspol in (1, 2):
+        dos, green, zz, zj = O.green_mt(3, tau, zl, jl, r, jmt, jws, rs, nspol)
+        zzr, zjr = [], []
+        for l in range(4):
+            for src, out in ((zl, zzr), (jl, zjr)):
+                f = np.concatenate([[0], 2 * zl[:jmt, l] * src[:jmt, l]])
+                f[0] = f[1]
+                out.append(RL.cinterp(x, RL.cnewint(x, f, 5), np.sqrt(rs)))
+        pzz = np.zeros((16, 16), complex)
+        pzj = 0
+        lm = 0
+        for l in range(4):
+            for _ in range(2 * l + 1):
+                pzz[lm, lm] = zzr[l]
+                pzj += zjr[l]
+                lm += 1
+        s = 2.0 if nspol == 1 else 1.0
+        d2 = s * RL.mdosms(1, pzz, pzj, tau)
+        g2 = s * RL.mgreen(1, 3, jws, zl, jl, tau)
+        assert abs(dos - d2) <= 1e-14 * abs(d2)
+        assert np.abs(green - g2).max() <= 1e-14 * np.abs(g2).max()
+        assert np.abs(zz - np.array(zzr)).max() <= 1e-14 * np.abs(zzr).max()
+
+
+def test_green_mt_dos_is_the_integral_of_green():
+    """With the sphere ending on a mesh point, dos = -(1/pi) times the sphere
+    integral of green(r) r^2 (one spin, n_spin_pola = 2): the trace of the
+    Green function integrates to the DOS.  The quadrature's rational fit is
+    not linear in the integrand, so the two agree to its accuracy (5e-7 here;
+    a dropped term of either sum moves dos at order one)."""
+    r, jmt, jws, rs, m, t, zl, jl = _radial()
+    rs = r[jmt - 1]
+    dos, green, _, _ = O.green_mt(3, t, zl, jl, r, jmt, jws, rs, 2)
+    x = np.concatenate([[0.0], np.sqrt(r[:jmt])])
+    f = np.concatenate([[0], 2 * green[:jmt]])
+    f[0] = f[1]
+    g = O.cnewint(x, f, 5)
+    assert abs(-g[jmt] / np.pi - dos) <= 1e-5 * abs(dos)
