@@ -535,10 +535,16 @@ def bench_ss(args):
     run_t = lambda: ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr, radial=False)
     run_r = lambda: ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr, radial=True)
     (m, _, _, _, rc), t_t = _timed_runs(run_t, args.steps, args.warmup)
-    (_, _, _, _, rc2), t_r = _timed_runs(run_r, max(1, args.steps // 2), 1)
-    ctx.close()
+    (_, tm, zl, jl, rc2), t_r = _timed_runs(run_r, max(1, args.steps // 2), 1)
     assert rc == 0 and rc2 == 0
     npair = len(rr) * len(e)
+    # muffin-tin DOS and green(r) of every (potential, energy) with tau00 = t
+    # (the single-site limit), radial functions from the call above
+    pot_of = np.tile(np.arange(len(rr)), len(e))
+    gf = lambda: ctx.green_mt(3, tm.reshape(npair, 16, 16), zl.reshape(npair, 4, -1),
+                              jl.reshape(npair, 4, -1), pot_of, rr, jm, jw, rs)
+    _, t_g = _timed_runs(gf, max(1, args.steps // 2), 1)
+    ctx.close()
     out = {
         "metric": "single-site solves/sec (potential x energy, lmax=3, scalar-relativistic)",
         "value": round(npair / t_t, 1), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -549,6 +555,8 @@ def bench_ss(args):
                    "solves_per_step": npair},
         "with_radial_functions": {"solves_per_s": round(npair / t_r, 1),
                                   "ms_per_step": round(1e3 * t_r, 3)},
+        "green_mt": {"solves_per_s": round(npair / t_g, 1), "ms_per_call": round(1e3 * t_g, 3),
+                     "note": "host-array C-ABI call: 0.8 GB of zlr/jlr uploaded per call"},
     }
     if not args.no_cpu:
         import time as _t

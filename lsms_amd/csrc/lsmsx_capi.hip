@@ -57,7 +57,8 @@ __global__ void k_single_site(const double *, const double *, const SingleSitePo
                               const double2 *, const double2 *, int, int, int, int, int,
                               double2 *, double2 *, double2 *, int *);
 __global__ void k_green_mt(const double2 *, const double2 *, const double2 *, const double *,
-                           const GreenPot *, const int *, int, int, double, double2 *, double2 *);
+                           const GreenPot *, const int *, int, int, int, double, double2 *,
+                           double2 *);
 __global__ void k_ir_extract(const SolveDesc *);
 __global__ void k_count_conv(const SolveDesc *, int, int *);
 __global__ void k_lloyd(const SolveDesc *, const double2 *, const double2 *, long, const double *,
@@ -1797,8 +1798,9 @@ int lsmsx_green_mt(lsmsx_ctx *c, int nsolve, int lmax, int iprpts, int n_spin_po
                         hipMemcpyHostToDevice, st));
   if (green)
     HIPCHK(hipMemsetAsync(c->gf_green.p, 0, (size_t)nsolve * iprpts * sizeof(double2), st));
-  hipLaunchKernelGGL(k_green_mt, dim3((unsigned)nsolve), dim3(64), 0, st, c->gf_tau.p, c->ss_zlr.p,
-                     c->ss_jlr.p, c->ss_r.p, c->gf_pot.p, c->gf_potof.p, lmax, iprpts,
+  hipLaunchKernelGGL(k_green_mt, dim3((unsigned)((nsolve + kGreenSolves - 1) / kGreenSolves)),
+                     dim3(64), 0, st, c->gf_tau.p, c->ss_zlr.p, c->ss_jlr.p, c->ss_r.p,
+                     c->gf_pot.p, c->gf_potof.p, nsolve, lmax, iprpts,
                      n_spin_pola == 1 ? 2.0 : 1.0, c->gf_dos.p, green ? c->gf_green.p : nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(dos, c->gf_dos.p, nsolve * sizeof(double2), hipMemcpyDeviceToHost, st));

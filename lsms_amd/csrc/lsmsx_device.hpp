@@ -97,6 +97,7 @@ struct SingleSitePot {
 // one potential of a muffin-tin Green-function batch: 1-based jmt, jws, and
 // the 0-based window of the interpolation at xs = sqrt(r_sph) on the mesh
 // x_0 = 0, x_k = sqrt(r_k) (k = 1..jmt): interval ip1-1..ip1, outside points i1, i2
+constexpr int kGreenSolves = 8;  // solves per k_green_mt workgroup (8 lanes each)
 struct GreenPot {
   int jmt, jws;
   int ip1, i1, i2;

@@ -603,10 +603,13 @@ __global__ void __launch_bounds__(64) k_single_site(
 // cinterp.f at sqrt(r_sph)), mdosms_c.f and mgreen_c.f, times 2 when
 // n_spin_pola = 1 (green_function.f:184-191).
 //
-// One 64-thread workgroup per solve: threads 2l and 2l+1 run the running
-// integrals of 2 z_l z_l and 2 z_l j_l over the jmt+1 points of the x mesh
-// (a ring of four fitted points, as cnewint keeps them); thread 0 contracts
-// them with the diagonal of tau00; all threads then form green(r).
+// One 64-thread workgroup per eight solves (kGreenSolves): lane 8s+2l and
+// 8s+2l+1 run the running integrals of 2 z_l z_l and 2 z_l j_l of solve s
+// over the jmt+1 points of the x mesh (a ring of four fitted points, as
+// cnewint keeps them), so every lane of the wave carries one sequential
+// integral; lane s then contracts solve s's integrals with the diagonal of
+// tau00, and all lanes form green(r) solve by solve, reading zlr/jlr with
+// consecutive lanes on consecutive radial points.
 
 namespace {
 
@@ -659,112 +662,127 @@ __device__ zd cnewint_step(const double *rs, const double *fr, const double *fi,
 __global__ void __launch_bounds__(64) k_green_mt(
     const double2 *__restrict__ tau00, const double2 *__restrict__ zlr,
     const double2 *__restrict__ jlr, const double *__restrict__ rmesh,
-    const GreenPot *__restrict__ pots, const int *__restrict__ pot_of, int lmax, int iprpts,
-    double spin, double2 *__restrict__ dos, double2 *__restrict__ green) {
-  __shared__ zd s_int[2][4];
-  const int b = blockIdx.x, tid = threadIdx.x;
+    const GreenPot *__restrict__ pots, const int *__restrict__ pot_of, int nsolve, int lmax,
+    int iprpts, double spin, double2 *__restrict__ dos, double2 *__restrict__ green) {
+  __shared__ zd s_int[kGreenSolves][2][4];
+  const int tid = threadIdx.x;
   const int nl = lmax + 1, kk = nl * nl;
-  const GreenPot P = pots[pot_of[b]];
-  const double *rm = rmesh + (long)pot_of[b] * iprpts;
-  const double2 *z0 = zlr + (long)b * nl * iprpts, *j0 = jlr + (long)b * nl * iprpts;
-  const double2 *tb = tau00 + (long)b * kk * kk;
-  const int jmt = P.jmt;
-  if (tid < 2 * nl) {
-    const int l = tid >> 1, which = tid & 1;
-    const double2 *zl = z0 + (long)l * iprpts, *ol = (which ? j0 : z0) + (long)l * iprpts;
-    // integrand on the x mesh (x_0 = 0, x_k = sqrt(r_k)), f_0 = f_1
-    auto fval = [&](int k) -> zd {
-      const int q = k == 0 ? 0 : k - 1;
-      const zd a = mk(zl[q].x, zl[q].y), c = mk(ol[q].x, ol[q].y);
-      return 2.0 * (a * c);
-    };
-    auto xval = [&](int k) -> double { return k == 0 ? 0.0 : sqrt(rm[k - 1]); };
-    const int nr = jmt + 1;
-    const int ip = 2 * 5 + 1;
-    double rs[4], fr[4], fi[4];
+  const int b0 = blockIdx.x * kGreenSolves;
+  // phase 1: lane 8s + 2l + which runs integral `which` of channel l of solve b0 + s
+  {
+    const int sl = tid >> 3, l = (tid & 7) >> 1, which = tid & 1;
+    const int b = b0 + sl;
+    if (b < nsolve && l < nl) {
+      const GreenPot P = pots[pot_of[b]];
+      const double *rm = rmesh + (long)pot_of[b] * iprpts;
+      const double2 *zl = zlr + ((long)b * nl + l) * iprpts;
+      const double2 *ol = (which ? jlr : zlr) + ((long)b * nl + l) * iprpts;
+      // integrand on the x mesh (x_0 = 0, x_k = sqrt(r_k)), f_0 = f_1
+      auto fval = [&](int k) -> zd {
+        const int q = k == 0 ? 0 : k - 1;
+        const zd u = mk(zl[q].x, zl[q].y), c = mk(ol[q].x, ol[q].y);
+        return 2.0 * (u * c);
+      };
+      auto xval = [&](int k) -> double { return k == 0 ? 0.0 : sqrt(rm[k - 1]); };
+      const int nr = P.jmt + 1;
+      const int ip = 2 * 5 + 1;
+      double rs[4], fr[4], fi[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      rs[k] = sqrt(xval(k));
-      const zd f = fval(k);
-      fr[k] = f.x;
-      fi[k] = f.y;
-    }
-    zd g = mk(0.0), w[4];
-    int i4 = 3, i0 = 0, ip2 = 4;
-    auto capture = [&](int k) {
-      if (k == P.i1) w[0] = g;
-      if (k == P.ip1 - 1) w[1] = g;
-      if (k == P.ip1) w[2] = g;
-      if (k == P.i2) w[3] = g;
-    };
-    capture(0);
-    for (int i = 1; i <= nr - 1; i++) {
-      if (i > 2 && ip2 <= nr) {
-        const zd f = fval(ip2 - 1);
-        const double xv = sqrt(xval(ip2 - 1));
+      for (int k = 0; k < 4; k++) {
+        rs[k] = sqrt(xval(k));
+        const zd f = fval(k);
+        fr[k] = f.x;
+        fi[k] = f.y;
+      }
+      zd g = mk(0.0), w[4];
+      int i4 = 3, i0 = 0, ip2 = 4;
+      auto capture = [&](int k) {
+        if (k == P.i1) w[0] = g;
+        if (k == P.ip1 - 1) w[1] = g;
+        if (k == P.ip1) w[2] = g;
+        if (k == P.i2) w[3] = g;
+      };
+      capture(0);
+      for (int i = 1; i <= nr - 1; i++) {
+        if (i > 2 && ip2 <= nr) {
+          const zd f = fval(ip2 - 1);
+          const double xv = sqrt(xval(ip2 - 1));
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (q == i4) {
-            rs[q] = xv;
-            fr[q] = f.x;
-            fi[q] = f.y;
-          }
+          for (int q = 0; q < 4; q++)
+            if (q == i4) {
+              rs[q] = xv;
+              fr[q] = f.x;
+              fi[q] = f.y;
+            }
+        }
+        g = g + cnewint_step(rs, fr, fi, i0, ip);
+        capture(i);
+        if (i > 1 && ip2 < nr) {
+          ip2 += 1;
+          i4 = (i4 + 1) & 3;  // the ring rotation i1..i4 advances i4 by one slot
+        }
+        i0 = (i0 + 1) & 3;
       }
-      g = g + cnewint_step(rs, fr, fi, i0, ip);
-      capture(i);
-      if (i > 1 && ip2 < nr) {
-        ip2 += 1;
-        i4 = (i4 + 1) & 3;  // the ring rotation i1..i4 advances i4 by one slot
-      }
-      i0 = (i0 + 1) & 3;
+      // cinterp.f at sqrt(r_sph) over the x mesh
+      double xw[4];
+      xw[0] = xval(P.i1);
+      xw[1] = xval(P.ip1 - 1);
+      xw[2] = xval(P.ip1);
+      xw[3] = xval(P.i2);
+      const int same = (P.i1 == P.i2);
+      s_int[sl][which][l] = interp4(xw, 4, 1, 0, same ? 0 : 3, P.xs, w);
     }
-    // cinterp.f at sqrt(r_sph) over the x mesh
-    double xw[4];
-    xw[0] = xval(P.i1);
-    xw[1] = xval(P.ip1 - 1);
-    xw[2] = xval(P.ip1);
-    xw[3] = xval(P.i2);
-    const int same = (P.i1 == P.i2);
-    s_int[which][l] = interp4(xw, 4, 1, 0, same ? 0 : 3, P.xs, w);
   }
   __syncthreads();
-  if (tid == 0) {  // mdosms: -(sum pzzck tau - pzjck)/pi
+  // phase 2: mdosms, -(sum pzzck tau - pzjck)/pi, one lane per solve
+  if (tid < kGreenSolves && b0 + tid < nsolve) {
+    const int b = b0 + tid;
+    const double2 *tb = tau00 + (long)b * kk * kk;
     zd zz = mk(0.0), zj = mk(0.0);
     for (int l = 0; l < nl; l++)
-      for (int m = -l; m <= l; m++) zj = zj + s_int[1][l];
+      for (int m = -l; m <= l; m++) zj = zj + s_int[tid][1][l];
     int lm = 0;
     for (int l = 0; l < nl; l++)
       for (int m = -l; m <= l; m++, lm++) {
         const double2 t = tb[lm + (long)lm * kk];
-        zz = zz + s_int[0][l] * mk(t.x, t.y);
+        zz = zz + s_int[tid][0][l] * mk(t.x, t.y);
       }
     const zd d = -(zz - zj) / 3.14159265358979323846;
     dos[b] = make_double2(spin * d.x, spin * d.y);
   }
-  if (green) {  // mgreen: sum over l from lmax down of z (z wt_l - j (2l+1))
-    zd wt[4];
-    int i = kk - 1;
-    for (int l = lmax; l >= 0; l--) {
-      zd s = mk(0.0);
-      for (int m = -l; m <= l; m++, i--) {
-        const double2 t = tb[i + (long)i * kk];
-        s = s + mk(t.x, t.y);
-      }
+  // phase 3: mgreen, sum over l from lmax down of z (z wt_l - j (2l+1)),
+  // each solve's radial points spread over the 64 lanes
+  if (green) {
+    for (int sl = 0; sl < kGreenSolves; sl++) {
+      const int b = b0 + sl;
+      if (b >= nsolve) break;
+      const double2 *tb = tau00 + (long)b * kk * kk;
+      zd wt[4];
+      int i = kk - 1;
+      for (int l = lmax; l >= 0; l--) {
+        zd sum = mk(0.0);
+        for (int m = -l; m <= l; m++, i--) {
+          const double2 t = tb[i + (long)i * kk];
+          sum = sum + mk(t.x, t.y);
+        }
 #pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (q == l) wt[q] = s;
-    }
-    double2 *gb = green + (long)b * iprpts;
-    for (int ir = tid; ir < P.jws; ir += blockDim.x) {
-      zd acc = mk(0.0);
-#pragma unroll
-      for (int l = 3; l >= 0; l--) {
-        if (l > lmax) continue;
-        const double2 zv = z0[(long)l * iprpts + ir], jv = j0[(long)l * iprpts + ir];
-        const zd zz = mk(zv.x, zv.y), jj = mk(jv.x, jv.y);
-        acc = acc + zz * (zz * wt[l] - jj * (double)(2 * l + 1));
+        for (int q = 0; q < 4; q++)
+          if (q == l) wt[q] = sum;
       }
-      gb[ir] = make_double2(spin * acc.x, spin * acc.y);
+      const int jws = pots[pot_of[b]].jws;
+      const double2 *z0 = zlr + (long)b * nl * iprpts, *j0 = jlr + (long)b * nl * iprpts;
+      double2 *gb = green + (long)b * iprpts;
+      for (int ir = tid; ir < jws; ir += blockDim.x) {
+        zd acc = mk(0.0);
+#pragma unroll
+        for (int l = 3; l >= 0; l--) {
+          if (l > lmax) continue;
+          const double2 zv = z0[(long)l * iprpts + ir], jv = j0[(long)l * iprpts + ir];
+          const zd zz = mk(zv.x, zv.y), jj = mk(jv.x, jv.y);
+          acc = acc + zz * (zz * wt[l] - jj * (double)(2 * l + 1));
+        }
+        gb[ir] = make_double2(spin * acc.x, spin * acc.y);
+      }
     }
   }
 }
