@@ -544,6 +544,16 @@ def bench_ss(args):
     gf = lambda: ctx.green_mt(3, tm.reshape(npair, 16, 16), zl.reshape(npair, 4, -1),
                               jl.reshape(npair, 4, -1), pot_of, rr, jm, jw, rs)
     _, t_g = _timed_runs(gf, max(1, args.steps // 2), 1)
+    # the same chain with the radial functions kept in HBM: single-site solve
+    # (radial functions computed, not copied) then the Green function
+    ctx.set_single_site_resident(True)
+    tau_t = tm.reshape(npair, 16, 16)
+    pairs = np.arange(npair)
+
+    def chain():
+        ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr, radial=False)
+        return ctx.green_mt_resident(tau_t, pairs, rr.shape[1], with_green=False)
+    _, t_c = _timed_runs(chain, max(1, args.steps // 2), 1)
     ctx.close()
     out = {
         "metric": "single-site solves/sec (potential x energy, lmax=3, scalar-relativistic)",
@@ -557,6 +567,8 @@ def bench_ss(args):
                                   "ms_per_step": round(1e3 * t_r, 3)},
         "green_mt": {"solves_per_s": round(npair / t_g, 1), "ms_per_call": round(1e3 * t_g, 3),
                      "note": "host-array C-ABI call: 0.8 GB of zlr/jlr uploaded per call"},
+        "single_site_then_dos_resident": {"solves_per_s": round(npair / t_c, 1),
+                                          "ms_per_step": round(1e3 * t_c, 3)},
     }
     if not args.no_cpu:
         import time as _t

@@ -271,6 +271,23 @@ int lsmsx_green_mt(lsmsx_ctx *ctx, int nsolve, int lmax, int iprpts, int n_spin_
                    const double *r_sph, const double *tau00, const double *zlr,
                    const double *jlr, double *dos, double *green);
 
+/* Keep the radial functions of the next lsmsx_single_site_nonrel calls on
+ * the device (computed even when zlr/jlr are NULL) for
+ * lsmsx_green_mt_resident.  No reference counterpart: the reference keeps
+ * zlr/jlr in host memory between the single-site solve and green_function_
+ * (energyContourIntegration.cpp:426-436 and :527-546); this pair keeps the
+ * chain in HBM. */
+int lsmsx_set_single_site_resident(lsmsx_ctx *ctx, int on);
+
+/* As lsmsx_green_mt, with the radial functions, meshes and lmax of the last
+ * lsmsx_single_site_nonrel call made with residency on: solve b uses the
+ * (energy e, potential p) pair pair_of[b] = e*npot + p of that call.
+ * Returns -3 when no such call precedes it (an lsmsx_green_mt call in between
+ * reuses the buffers and also counts as none).  No reference counterpart
+ * (see lsmsx_set_single_site_resident). */
+int lsmsx_green_mt_resident(lsmsx_ctx *ctx, int nsolve, int n_spin_pola, const int *pair_of,
+                            const double *tau00, double *dos, double *green);
+
 #ifdef __cplusplus
 }
 #endif

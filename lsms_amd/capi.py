@@ -25,7 +25,8 @@ EXPORTED = [
     "lsmsx_get_profile", "lsmsx_set_refinement", "lsmsx_last_batch_stats",
     "lsmsx_upload_dtmats", "lsmsx_tau_full", "lsmsx_tau00_adjoint", "lsmsx_set_schedule",
     "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune", "lsmsx_set_pivot_guard",
-    "lsmsx_single_site_nonrel", "lsmsx_green_mt",
+    "lsmsx_single_site_nonrel", "lsmsx_green_mt", "lsmsx_set_single_site_resident",
+    "lsmsx_green_mt_resident",
 ]
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
@@ -75,6 +76,8 @@ def load():
                                  _vp, _vp]
     L.lsmsx_tau00_adjoint.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                       _vp, _vp, _vp, _vp, _vp]
+    L.lsmsx_set_single_site_resident.argtypes = [_vp, C.c_int]
+    L.lsmsx_green_mt_resident.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
     L.lsmsx_green_mt.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_single_site_nonrel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
@@ -264,6 +267,23 @@ class Context:
         _ck(load().lsmsx_green_mt(self._h, nsolve, lmax, iprpts, n_spin_pola, npot, _p(po), _p(rm),
                                   _p(jm), _p(jw), _p(rs), _p(t), _p(z), _p(j), _p(dos),
                                   _p(green) if with_green else None))
+        return dos, green
+
+    def set_single_site_resident(self, on):
+        """Keep the radial functions of later single_site_nonrel calls on the
+        device for green_mt_resident."""
+        _ck(load().lsmsx_set_single_site_resident(self._h, 1 if on else 0))
+
+    def green_mt_resident(self, tau00, pair_of, iprpts, n_spin_pola=1, with_green=True):
+        """green_mt on the radial functions left on the device by the last
+        single_site_nonrel call (residency on); pair_of[b] = e*npot + p."""
+        t = np.ascontiguousarray(np.swapaxes(np.asarray(tau00, dtype=np.complex128), 1, 2))
+        po = np.ascontiguousarray(pair_of, dtype=np.int32)
+        nsolve = len(po)
+        dos = np.zeros(nsolve, dtype=np.complex128)
+        green = np.zeros((nsolve, iprpts), dtype=np.complex128) if with_green else None
+        _ck(load().lsmsx_green_mt_resident(self._h, nsolve, n_spin_pola, _p(po), _p(t), _p(dos),
+                                           _p(green) if with_green else None))
         return dos, green
 
     def tau00_adjoint(self, atom_ids, iies, energies, kkrsz_ns, ispins=None, lloyd=True,

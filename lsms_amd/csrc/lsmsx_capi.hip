@@ -57,8 +57,8 @@ __global__ void k_single_site(const double *, const double *, const SingleSitePo
                               const double2 *, const double2 *, int, int, int, int, int,
                               double2 *, double2 *, double2 *, int *);
 __global__ void k_green_mt(const double2 *, const double2 *, const double2 *, const double *,
-                           const GreenPot *, const int *, int, int, int, double, double2 *,
-                           double2 *);
+                           const GreenPot *, const int *, const int *, int, int, int, double,
+                           double2 *, double2 *);
 __global__ void k_ir_extract(const SolveDesc *);
 __global__ void k_count_conv(const SolveDesc *, int, int *);
 __global__ void k_lloyd(const SolveDesc *, const double2 *, const double2 *, long, const double *,
@@ -167,6 +167,29 @@ struct DevBuf {
 
 int roundup(int x, int m) { return ((x + m - 1) / m) * m; }
 
+// checks of one potential's mesh for the muffin-tin Green function and the
+// cinterp.f window at sqrt(r_sph) on x_0 = 0, x_k = sqrt(r_k), k = 1..jmt
+const char *make_green_pot(const double *r, int jmt, int jws, double r_sph, int iprpts,
+                           GreenPot &P) {
+  P.jmt = jmt;
+  P.jws = jws;
+  if (jmt < 4 || jmt > iprpts || jws < 0 || jws > iprpts) return "jmt or jws outside 4..iprpts";
+  if (!(r[0] > 0.0)) return "the radial mesh must start above r = 0";
+  for (int j = 0; j + 1 < jmt; j++)
+    if (!(r[j + 1] > r[j])) return "the radial mesh must increase";
+  if (!(r_sph > 0.0)) return "r_sph must be > 0";
+  P.xs = std::sqrt(r_sph);
+  const int nr = jmt + 1;
+  int ip1 = 1;
+  for (int i = 1; i <= nr - 2; i++)
+    if (P.xs > std::sqrt(r[i - 1])) ip1 = i + 1;
+  const int i = ip1 - 1;
+  P.ip1 = ip1;
+  P.i1 = (i > 0) ? i - 1 : std::min(4, nr) - 1;
+  P.i2 = (ip1 < nr - 1) ? ip1 + 1 : std::max(nr - 3, 1) - 1;
+  return nullptr;
+}
+
 }  // namespace
 
 struct lsmsx_ctx {
@@ -198,7 +221,10 @@ struct lsmsx_ctx {
   DevBuf<int> ss_iend, ss_status;
   DevBuf<double2> ss_e, ss_prel, ss_matom, ss_zlr, ss_jlr;
   DevBuf<GreenPot> gf_pot;
-  DevBuf<int> gf_potof;
+  DevBuf<int> gf_potof, gf_src;
+  // radial functions of the last single-site call kept in ss_zlr/ss_jlr
+  int ss_resident = 0, ss_valid = 0, ss_ne = 0, ss_npot = 0, ss_lmax = 0, ss_iprpts = 0;
+  std::vector<GreenPot> ss_gpot;
   DevBuf<double2> gf_tau, gf_dos, gf_green;
   DevBuf<long> keyl;
   std::vector<int> last_iters;
@@ -1187,6 +1213,7 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->ss_jlr.release();
   c->gf_pot.release();
   c->gf_potof.release();
+  c->gf_src.release();
   c->gf_tau.release();
   c->gf_dos.release();
   c->gf_green.release();
@@ -1676,6 +1703,16 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int ip
   const int nl = lmax + 1;
   const size_t ntask = (size_t)ne * npot * nl;
   const size_t npts = (size_t)npot * iprpts;
+  const bool want_z = zlr || c->ss_resident, want_j = jlr || c->ss_resident;
+  std::vector<GreenPot> gp;
+  if (want_z && want_j) {
+    gp.resize((size_t)npot);
+    for (int p = 0; p < npot; p++)
+      if (const char *e = make_green_pot(r_mesh + (size_t)p * iprpts, jmt[p], jws[p], r_sph[p],
+                                         iprpts, gp[p]))
+        return fail(-2, e);
+  }
+  c->ss_valid = 0;
   HIPCHK(hipSetDevice(c->device));
   if (int r = c->ss_r.ensure(npts)) return r;
   if (int r = c->ss_v.ensure(npts)) return r;
@@ -1686,9 +1723,9 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int ip
   if (int r = c->ss_e.ensure(ne)) return r;
   if (int r = c->ss_prel.ensure(ne)) return r;
   if (int r = c->ss_matom.ensure(ntask)) return r;
-  if (zlr)
+  if (want_z)
     if (int r = c->ss_zlr.ensure(ntask * iprpts)) return r;
-  if (jlr)
+  if (want_j)
     if (int r = c->ss_jlr.ensure(ntask * iprpts)) return r;
   hipStream_t st = c->stream;
   HIPCHK(hipMemcpyAsync(c->ss_r.p, r_mesh, npts * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1699,15 +1736,16 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int ip
   HIPCHK(hipMemcpyAsync(c->ss_e.p, energy, ne * sizeof(double2), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->ss_prel.p, prel, ne * sizeof(double2), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(c->ss_status.p, 0, sizeof(int), st));
-  if (zlr) HIPCHK(hipMemsetAsync(c->ss_zlr.p, 0, ntask * iprpts * sizeof(double2), st));
-  if (jlr) HIPCHK(hipMemsetAsync(c->ss_jlr.p, 0, ntask * iprpts * sizeof(double2), st));
+  if (want_z) HIPCHK(hipMemsetAsync(c->ss_zlr.p, 0, ntask * iprpts * sizeof(double2), st));
+  if (want_j) HIPCHK(hipMemsetAsync(c->ss_jlr.p, 0, ntask * iprpts * sizeof(double2), st));
   hipLaunchKernelGGL(k_fitpot, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, st, c->ss_r.p,
                      c->ss_v.p, c->ss_iend.p, iprpts, npot, c->ss_coef.p);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_single_site, dim3((unsigned)((ntask + 63) / 64)), dim3(64), 0, st,
                      c->ss_r.p, c->ss_coef.p, c->ss_pot.p, c->ss_e.p, c->ss_prel.p, npot, ne,
-                     lmax, iprpts, nrelv == 0 ? 1 : 0, c->ss_matom.p, zlr ? c->ss_zlr.p : nullptr,
-                     jlr ? c->ss_jlr.p : nullptr, c->ss_status.p);
+                     lmax, iprpts, nrelv == 0 ? 1 : 0, c->ss_matom.p,
+                     want_z ? c->ss_zlr.p : nullptr, want_j ? c->ss_jlr.p : nullptr,
+                     c->ss_status.p);
   HIPCHK(hipGetLastError());
   int status = 0;
   HIPCHK(hipMemcpyAsync(matom, c->ss_matom.p, ntask * sizeof(double2), hipMemcpyDeviceToHost, st));
@@ -1719,6 +1757,14 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int ip
                           hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&status, c->ss_status.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (want_z && want_j) {
+    c->ss_valid = 1;
+    c->ss_ne = ne;
+    c->ss_npot = npot;
+    c->ss_lmax = lmax;
+    c->ss_iprpts = iprpts;
+    c->ss_gpot = gp;
+  }
   if (tmat) {  // diagonal t = 1/matom_l (single_scatterer_nonrel.f:100-111)
     const int kk = nl * nl;
     for (size_t q = 0; q < (size_t)ne * npot; q++) {
@@ -1750,31 +1796,14 @@ int lsmsx_green_mt(lsmsx_ctx *c, int nsolve, int lmax, int iprpts, int n_spin_po
   if (!pot_of || !r_mesh || !jmt || !jws || !r_sph || !tau00 || !zlr || !jlr || !dos)
     return fail(-2, "null argument");
   std::vector<GreenPot> hp((size_t)npot);
-  for (int p = 0; p < npot; p++) {
-    const double *r = r_mesh + (size_t)p * iprpts;
-    GreenPot &P = hp[p];
-    P.jmt = jmt[p];
-    P.jws = jws[p];
-    if (P.jmt < 4 || P.jmt > iprpts || P.jws < 0 || P.jws > iprpts)
-      return fail(-2, "jmt or jws outside 4..iprpts");
-    if (!(r[0] > 0.0)) return fail(-2, "the radial mesh must start above r = 0");
-    for (int j = 0; j + 1 < P.jmt; j++)
-      if (!(r[j + 1] > r[j])) return fail(-2, "the radial mesh must increase");
-    if (!(r_sph[p] > 0.0)) return fail(-2, "r_sph must be > 0");
-    P.xs = std::sqrt(r_sph[p]);
-    // cinterp.f window on x_0 = 0, x_k = sqrt(r_k), k = 1..jmt
-    const int nr = P.jmt + 1;
-    int ip1 = 1;
-    for (int i = 1; i <= nr - 2; i++)
-      if (P.xs > std::sqrt(r[i - 1])) ip1 = i + 1;
-    const int i = ip1 - 1;
-    P.ip1 = ip1;
-    P.i1 = (i > 0) ? i - 1 : std::min(4, nr) - 1;
-    P.i2 = (ip1 < nr - 1) ? ip1 + 1 : std::max(nr - 3, 1) - 1;
-  }
+  for (int p = 0; p < npot; p++)
+    if (const char *e = make_green_pot(r_mesh + (size_t)p * iprpts, jmt[p], jws[p], r_sph[p],
+                                       iprpts, hp[p]))
+      return fail(-2, e);
   for (int b = 0; b < nsolve; b++)
     if (pot_of[b] < 0 || pot_of[b] >= npot) return fail(-2, "pot_of out of range");
   const int nl = lmax + 1, kk = nl * nl;
+  c->ss_valid = 0;  // the radial-function buffers are reused below
   HIPCHK(hipSetDevice(c->device));
   if (int r = c->ss_r.ensure((size_t)npot * iprpts)) return r;
   if (int r = c->gf_pot.ensure(npot)) return r;
@@ -1800,7 +1829,61 @@ int lsmsx_green_mt(lsmsx_ctx *c, int nsolve, int lmax, int iprpts, int n_spin_po
     HIPCHK(hipMemsetAsync(c->gf_green.p, 0, (size_t)nsolve * iprpts * sizeof(double2), st));
   hipLaunchKernelGGL(k_green_mt, dim3((unsigned)((nsolve + kGreenSolves - 1) / kGreenSolves)),
                      dim3(64), 0, st, c->gf_tau.p, c->ss_zlr.p, c->ss_jlr.p, c->ss_r.p,
-                     c->gf_pot.p, c->gf_potof.p, nsolve, lmax, iprpts,
+                     c->gf_pot.p, c->gf_potof.p, nullptr, nsolve, lmax, iprpts,
+                     n_spin_pola == 1 ? 2.0 : 1.0, c->gf_dos.p, green ? c->gf_green.p : nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dos, c->gf_dos.p, nsolve * sizeof(double2), hipMemcpyDeviceToHost, st));
+  if (green)
+    HIPCHK(hipMemcpyAsync(green, c->gf_green.p, (size_t)nsolve * iprpts * sizeof(double2),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int lsmsx_set_single_site_resident(lsmsx_ctx *c, int on) {
+  if (!c) return fail(-1, "null context");
+  c->ss_resident = on ? 1 : 0;
+  return 0;
+}
+
+int lsmsx_green_mt_resident(lsmsx_ctx *c, int nsolve, int n_spin_pola, const int *pair_of,
+                            const double *tau00, double *dos, double *green) {
+  if (!c) return fail(-1, "null context");
+  if (nsolve < 0) return fail(-2, "bad batch size");
+  if (n_spin_pola != 1 && n_spin_pola != 2) return fail(-2, "n_spin_pola must be 1 or 2");
+  if (nsolve == 0) return 0;
+  if (!pair_of || !tau00 || !dos) return fail(-2, "null argument");
+  if (!c->ss_valid)
+    return fail(-3, "no radial functions on the device: call lsmsx_single_site_nonrel with "
+                    "lsmsx_set_single_site_resident on (and no lsmsx_green_mt in between)");
+  const int npair = c->ss_ne * c->ss_npot, lmax = c->ss_lmax, iprpts = c->ss_iprpts;
+  std::vector<int> pot_of((size_t)nsolve);
+  for (int b = 0; b < nsolve; b++) {
+    if (pair_of[b] < 0 || pair_of[b] >= npair) return fail(-2, "pair_of out of range");
+    pot_of[b] = pair_of[b] % c->ss_npot;
+  }
+  const int nl = lmax + 1, kk = nl * nl;
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->gf_pot.ensure(c->ss_npot)) return r;
+  if (int r = c->gf_potof.ensure(nsolve)) return r;
+  if (int r = c->gf_src.ensure(nsolve)) return r;
+  if (int r = c->gf_tau.ensure((size_t)nsolve * kk * kk)) return r;
+  if (int r = c->gf_dos.ensure(nsolve)) return r;
+  if (green)
+    if (int r = c->gf_green.ensure((size_t)nsolve * iprpts)) return r;
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(c->gf_pot.p, c->ss_gpot.data(), c->ss_npot * sizeof(GreenPot),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gf_potof.p, pot_of.data(), nsolve * sizeof(int), hipMemcpyHostToDevice,
+                        st));
+  HIPCHK(hipMemcpyAsync(c->gf_src.p, pair_of, nsolve * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gf_tau.p, tau00, (size_t)nsolve * kk * kk * sizeof(double2),
+                        hipMemcpyHostToDevice, st));
+  if (green)
+    HIPCHK(hipMemsetAsync(c->gf_green.p, 0, (size_t)nsolve * iprpts * sizeof(double2), st));
+  hipLaunchKernelGGL(k_green_mt, dim3((unsigned)((nsolve + kGreenSolves - 1) / kGreenSolves)),
+                     dim3(64), 0, st, c->gf_tau.p, c->ss_zlr.p, c->ss_jlr.p, c->ss_r.p,
+                     c->gf_pot.p, c->gf_potof.p, c->gf_src.p, nsolve, lmax, iprpts,
                      n_spin_pola == 1 ? 2.0 : 1.0, c->gf_dos.p, green ? c->gf_green.p : nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(dos, c->gf_dos.p, nsolve * sizeof(double2), hipMemcpyDeviceToHost, st));

@@ -662,8 +662,11 @@ __device__ zd cnewint_step(const double *rs, const double *fr, const double *fi,
 __global__ void __launch_bounds__(64) k_green_mt(
     const double2 *__restrict__ tau00, const double2 *__restrict__ zlr,
     const double2 *__restrict__ jlr, const double *__restrict__ rmesh,
-    const GreenPot *__restrict__ pots, const int *__restrict__ pot_of, int nsolve, int lmax,
-    int iprpts, double spin, double2 *__restrict__ dos, double2 *__restrict__ green) {
+    const GreenPot *__restrict__ pots, const int *__restrict__ pot_of,
+    const int *__restrict__ src_of, int nsolve, int lmax, int iprpts, double spin,
+    double2 *__restrict__ dos, double2 *__restrict__ green) {
+  // zlr/jlr row of solve b: src_of[b] (radial functions of a single-site
+  // batch left on the device) or b
   __shared__ zd s_int[kGreenSolves][2][4];
   const int tid = threadIdx.x;
   const int nl = lmax + 1, kk = nl * nl;
@@ -675,8 +678,9 @@ __global__ void __launch_bounds__(64) k_green_mt(
     if (b < nsolve && l < nl) {
       const GreenPot P = pots[pot_of[b]];
       const double *rm = rmesh + (long)pot_of[b] * iprpts;
-      const double2 *zl = zlr + ((long)b * nl + l) * iprpts;
-      const double2 *ol = (which ? jlr : zlr) + ((long)b * nl + l) * iprpts;
+      const long src = src_of ? src_of[b] : b;
+      const double2 *zl = zlr + (src * nl + l) * iprpts;
+      const double2 *ol = (which ? jlr : zlr) + (src * nl + l) * iprpts;
       // integrand on the x mesh (x_0 = 0, x_k = sqrt(r_k)), f_0 = f_1
       auto fval = [&](int k) -> zd {
         const int q = k == 0 ? 0 : k - 1;
@@ -770,7 +774,8 @@ __global__ void __launch_bounds__(64) k_green_mt(
           if (q == l) wt[q] = sum;
       }
       const int jws = pots[pot_of[b]].jws;
-      const double2 *z0 = zlr + (long)b * nl * iprpts, *j0 = jlr + (long)b * nl * iprpts;
+      const long src = src_of ? src_of[b] : b;
+      const double2 *z0 = zlr + src * nl * iprpts, *j0 = jlr + src * nl * iprpts;
       double2 *gb = green + (long)b * iprpts;
       for (int ir = tid; ir < jws; ir += blockDim.x) {
         zd acc = mk(0.0);

@@ -221,3 +221,30 @@ def test_dos_chain_on_gpu(ctx):
                                                     jm[sp], jw[sp], rs[sp])
         d_or = O.green_mt(3, tref, zo, jo, rr[sp], jm[sp], jw[sp], rs[sp], 1)[0]
         assert abs(dos[b] - d_or) <= 1e-10 * abs(d_or), (a, ie, dos[b], d_or)
+
+
+def test_green_mt_resident_equals_host_path():
+    """Radial functions left on the device by the single-site call give
+    bitwise the Green function of the host-array call, for solves picked
+    out of order; an intervening host-array call invalidates them."""
+    c = capi.Context(0)
+    rr, vr, jm, jw, rs, e, pr = CF.single_site_batch(nspecies=2, ne=5)
+    npot = len(rr)
+    with pytest.raises(capi.LsmsxError):  # nothing resident yet
+        c.green_mt_resident(np.zeros((1, 16, 16)), [0], rr.shape[1])
+    c.set_single_site_resident(True)
+    matom, tmat, zlr, jlr, rc = c.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr)
+    _, _, _, _, rc2 = c.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr, radial=False)
+    assert rc == 0 and rc2 == 0
+    pairs = np.array([7, 0, 19, 3, 3, 12, 15, 1, 8])
+    q_e, q_p = pairs // npot, pairs % npot
+    rng = np.random.default_rng(5)
+    tau = tmat[q_e, q_p] + 0.05 * (rng.normal(size=(len(pairs), 16, 16)) +
+                                   1j * rng.normal(size=(len(pairs), 16, 16)))
+    d_res, g_res = c.green_mt_resident(tau, pairs, rr.shape[1], n_spin_pola=2)
+    d_host, g_host = c.green_mt(3, tau, zlr[q_e, q_p], jlr[q_e, q_p], q_p, rr, jm, jw, rs,
+                                n_spin_pola=2)
+    assert np.array_equal(d_res, d_host) and np.array_equal(g_res, g_host)
+    with pytest.raises(capi.LsmsxError):  # the host-array call reused the buffers
+        c.green_mt_resident(tau, pairs, rr.shape[1])
+    c.close()
