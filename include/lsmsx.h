@@ -288,6 +288,19 @@ int lsmsx_set_single_site_resident(lsmsx_ctx *ctx, int on);
 int lsmsx_green_mt_resident(lsmsx_ctx *ctx, int nsolve, int n_spin_pola, const int *pair_of,
                             const double *tau00, double *dos, double *green);
 
+/* Fill the device t-matrix store from the last lsmsx_single_site_nonrel
+ * batch without a host round trip: the store (same dimensions as
+ * lsmsx_upload_tmats) is zeroed, as local.tmatStore = 0 before the
+ * single-site solve (energyContourIntegration.cpp:420), then energy e of
+ * potential p goes to slot e of column store_of_pot[p], spin block
+ * spin_of_pot[p] (offset spin*kkrsz^2, the collinear layout), as
+ * solveSingleScatterers fills &local.tmatStore(iie*blkSizeTmatStore, i)
+ * (Main/solveSingleScatterers.cpp:26-28, SingleSiteScattering.cpp:28-49).
+ * The spin-canted global-frame t (trltog) is not formed here. */
+int lsmsx_tmats_from_single_site(lsmsx_ctx *ctx, long ldTmatStore, int numStore,
+                                 long blkSizeTmatStore, int groupSize, const int *store_of_pot,
+                                 const int *spin_of_pot);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,7 @@ EXPORTED = [
     "lsmsx_upload_dtmats", "lsmsx_tau_full", "lsmsx_tau00_adjoint", "lsmsx_set_schedule",
     "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune", "lsmsx_set_pivot_guard",
     "lsmsx_single_site_nonrel", "lsmsx_green_mt", "lsmsx_set_single_site_resident",
-    "lsmsx_green_mt_resident",
+    "lsmsx_green_mt_resident", "lsmsx_tmats_from_single_site",
 ]
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
@@ -77,6 +77,7 @@ def load():
     L.lsmsx_tau00_adjoint.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                       _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_set_single_site_resident.argtypes = [_vp, C.c_int]
+    L.lsmsx_tmats_from_single_site.argtypes = [_vp, C.c_long, C.c_int, C.c_long, C.c_int, _vp, _vp]
     L.lsmsx_green_mt_resident.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
     L.lsmsx_green_mt.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
@@ -268,6 +269,14 @@ class Context:
                                   _p(jm), _p(jw), _p(rs), _p(t), _p(z), _p(j), _p(dos),
                                   _p(green) if with_green else None))
         return dos, green
+
+    def tmats_from_single_site(self, ld, num_store, blk, group, store_of_pot, spin_of_pot=None):
+        """Fill the device t-matrix store from the last single_site_nonrel
+        batch (zeroed first; energy e -> slot e)."""
+        so = np.ascontiguousarray(store_of_pot, dtype=np.int32)
+        sp = np.ascontiguousarray(np.zeros_like(so) if spin_of_pot is None else spin_of_pot,
+                                  dtype=np.int32)
+        _ck(load().lsmsx_tmats_from_single_site(self._h, ld, num_store, blk, group, _p(so), _p(sp)))
 
     def set_single_site_resident(self, on):
         """Keep the radial functions of later single_site_nonrel calls on the

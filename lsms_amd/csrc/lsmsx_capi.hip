@@ -56,6 +56,8 @@ __global__ void k_fitpot(const double *, const double *, const int *, int, int, 
 __global__ void k_single_site(const double *, const double *, const SingleSitePot *,
                               const double2 *, const double2 *, int, int, int, int, int,
                               double2 *, double2 *, double2 *, int *);
+__global__ void k_tmat_fill(const double2 *, const int *, const int *, int, int, int, long, long,
+                            double2 *);
 __global__ void k_green_mt(const double2 *, const double2 *, const double2 *, const double *,
                            const GreenPot *, const int *, const int *, int, int, int, double,
                            double2 *, double2 *);
@@ -224,6 +226,8 @@ struct lsmsx_ctx {
   DevBuf<int> gf_potof, gf_src;
   // radial functions of the last single-site call kept in ss_zlr/ss_jlr
   int ss_resident = 0, ss_valid = 0, ss_ne = 0, ss_npot = 0, ss_lmax = 0, ss_iprpts = 0;
+  int ss_have_matom = 0;  // ss_matom holds the last call's ss_ne x ss_npot x (ss_lmax+1)
+  DevBuf<int> ss_storeof;
   std::vector<GreenPot> ss_gpot;
   DevBuf<double2> gf_tau, gf_dos, gf_green;
   DevBuf<long> keyl;
@@ -1214,6 +1218,7 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->gf_pot.release();
   c->gf_potof.release();
   c->gf_src.release();
+  c->ss_storeof.release();
   c->gf_tau.release();
   c->gf_dos.release();
   c->gf_green.release();
@@ -1713,6 +1718,7 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int ip
         return fail(-2, e);
   }
   c->ss_valid = 0;
+  c->ss_have_matom = 0;
   HIPCHK(hipSetDevice(c->device));
   if (int r = c->ss_r.ensure(npts)) return r;
   if (int r = c->ss_v.ensure(npts)) return r;
@@ -1757,12 +1763,13 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int ip
                           hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&status, c->ss_status.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  c->ss_ne = ne;
+  c->ss_npot = npot;
+  c->ss_lmax = lmax;
+  c->ss_iprpts = iprpts;
+  c->ss_have_matom = 1;
   if (want_z && want_j) {
     c->ss_valid = 1;
-    c->ss_ne = ne;
-    c->ss_npot = npot;
-    c->ss_lmax = lmax;
-    c->ss_iprpts = iprpts;
     c->ss_gpot = gp;
   }
   if (tmat) {  // diagonal t = 1/matom_l (single_scatterer_nonrel.f:100-111)
@@ -1891,6 +1898,45 @@ int lsmsx_green_mt_resident(lsmsx_ctx *c, int nsolve, int n_spin_pola, const int
     HIPCHK(hipMemcpyAsync(green, c->gf_green.p, (size_t)nsolve * iprpts * sizeof(double2),
                           hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int lsmsx_tmats_from_single_site(lsmsx_ctx *c, long ldT, int numStore, long blk, int groupSize,
+                                 const int *store_of_pot, const int *spin_of_pot) {
+  if (!c) return fail(-1, "null context");
+  if (!c->ss_have_matom) return fail(-3, "no single-site batch on the device");
+  if (!store_of_pot || !spin_of_pot) return fail(-2, "null argument");
+  const int ne = c->ss_ne, npot = c->ss_npot, kkr = (c->ss_lmax + 1) * (c->ss_lmax + 1);
+  if (ldT < blk * groupSize || numStore <= 0 || blk <= 0 || groupSize <= 0)
+    return fail(-2, "inconsistent t-matrix store dimensions");
+  if (ne > groupSize) return fail(-2, "more energies than store slots");
+  for (int p = 0; p < npot; p++) {
+    if (store_of_pot[p] < 0 || store_of_pot[p] >= numStore) return fail(-2, "store column out of range");
+    if (spin_of_pot[p] < 0 || (long)(spin_of_pot[p] + 1) * kkr * kkr > blk)
+      return fail(-2, "spin block outside blkSizeTmatStore");
+  }
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->tmat.ensure((size_t)ldT * numStore)) return r;
+  if (int r = c->ss_storeof.ensure(2 * (size_t)npot)) return r;
+  std::vector<int> hs(2 * (size_t)npot);
+  std::copy(store_of_pot, store_of_pot + npot, hs.begin());
+  std::copy(spin_of_pot, spin_of_pot + npot, hs.begin() + npot);
+  hipStream_t st = c->stream;
+  // the store starts from zero, as local.tmatStore = 0 before the solve
+  // (energyContourIntegration.cpp:420)
+  HIPCHK(hipMemsetAsync(c->tmat.p, 0, sizeof(double2) * (size_t)ldT * numStore, st));
+  HIPCHK(hipMemcpyAsync(c->ss_storeof.p, hs.data(), 2 * npot * sizeof(int), hipMemcpyHostToDevice,
+                        st));
+  const long n = (long)ne * npot * kkr * kkr;
+  hipLaunchKernelGGL(k_tmat_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     c->ss_matom.p, c->ss_storeof.p, c->ss_storeof.p + npot, ne, npot, c->ss_lmax,
+                     ldT, blk, c->tmat.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  c->ldT = ldT;
+  c->blk = blk;
+  c->nstore = numStore;
+  c->group = groupSize;
   return 0;
 }
 

@@ -792,4 +792,29 @@ __global__ void __launch_bounds__(64) k_green_mt(
   }
 }
 
+// t-matrix store from a single-site batch: block (slot e, spin isp) of store
+// column s = kkr x kkr diagonal 1/matom_l (single_scatterer_nonrel.f:100-111)
+// at [s*ldT + e*blk + isp*kkr^2], the collinear layout the path reads.
+// One thread per block entry; the caller zeroes the store first.
+__global__ void k_tmat_fill(const double2 *__restrict__ matom, const int *__restrict__ store_of,
+                            const int *__restrict__ spin_of, int ne, int npot, int lmax,
+                            long ldT, long blk, double2 *__restrict__ store) {
+  const int nl = lmax + 1, kkr = nl * nl;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ne * npot * kkr * kkr) return;
+  const int idx = (int)(t % (kkr * kkr));
+  const long q = t / (kkr * kkr);
+  const int p = (int)(q % npot), e = (int)(q / npot);
+  const int i = idx % kkr, j = idx / kkr;
+  double2 v = make_double2(0.0, 0.0);
+  if (i == j) {
+    int l = 0;
+    while ((l + 1) * (l + 1) <= i) l++;
+    const double2 m = matom[q * nl + l];
+    const zd tv = 1.0 / mk(m.x, m.y);
+    v = make_double2(tv.x, tv.y);
+  }
+  store[(long)store_of[p] * ldT + (long)e * blk + (long)spin_of[p] * kkr * kkr + idx] = v;
+}
+
 }  // namespace lsmsx

@@ -248,3 +248,61 @@ def test_green_mt_resident_equals_host_path():
     with pytest.raises(capi.LsmsxError):  # the host-array call reused the buffers
         c.green_mt_resident(tau, pairs, rr.shape[1])
     c.close()
+
+
+@pytest.mark.parametrize("nspol", [1, 2])
+def test_store_filled_on_device(nspol):
+    """The device t-matrix store filled straight from the single-site batch
+    gives the tau00 of the same t uploaded from the host (to the rounding of
+    1/matom on either side), and meets LAPACK on the oracle's matrices;
+    collinear non-polarized and spin-polarized."""
+    import scipy.linalg as sla
+    from tests import adjoint_ref as R
+    e, w = CF.contour_semicircle(-0.5, 0.7, 8)
+    s = CF.make_system("small", "bcc", 5.42, (1, 1, 1), 6.0, 3, e, w, nspecies=2, seed=5,
+                       n_spin_pola=nspol)
+    pots, store_of, spin_of = [], [], []
+    for sp in range(2):
+        for isp in range(nspol):
+            pots.append(CF.model_potential(26 + 3 * sp, 2.3, vmtz=-1.2 + 0.1 * isp))
+            store_of.append(sp)
+            spin_of.append(isp)
+    rr = np.stack([p[0] for p in pots])
+    vr = np.stack([p[1] for p in pots])
+    jm = np.array([p[2] for p in pots])
+    jw = np.array([p[3] for p in pots])
+    rs = np.array([p[4] for p in pots])
+    prels = np.array([s.prel(i) for i in range(len(s.energies))])
+    c = capi.Context(0)
+    c.load_system(s)
+    _, tmat, _, _, rc = c.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, s.energies, prels,
+                                             radial=False)
+    assert rc == 0
+    c.tmats_from_single_site(s.tmatStore.shape[0], s.tmatStore.shape[1], s.blkSizeTmatStore,
+                             len(s.energies), store_of, spin_of)
+    store = np.zeros_like(s.tmatStore)
+    for p in range(len(pots)):
+        for ie in range(len(s.energies)):
+            base = ie * s.blkSizeTmatStore + spin_of[p] * 256
+            store[base:base + 256, store_of[p]] = tmat[ie, p].flatten(order="F")
+    s.tmatStore = store
+    na = len(s.atoms)
+    atoms = np.repeat(np.arange(na), 3 * nspol)
+    ies = np.tile(np.repeat([0, 3, 7], nspol), na)
+    isps = np.tile(np.arange(nspol), 3 * na)
+    pr = [s.prel(i) for i in ies]
+    tau_dev, info = c.tau00_batched(atoms, ies, pr, 16, ispins=isps)
+    c.close()
+    c2 = capi.Context(0)
+    c2.load_system(s)
+    tau_host, info2 = c2.tau00_batched(atoms, ies, pr, 16, ispins=isps)
+    c2.close()
+    assert np.all(info == 0) and np.all(info2 == 0)
+    for b in range(len(atoms)):
+        assert np.linalg.norm(tau_dev[b] - tau_host[b]) <= 1e-13 * np.linalg.norm(tau_host[b])
+    for b in (0, len(atoms) - 1):
+        m = R.oracle_m(s, atoms[b], ies[b], isps[b])
+        rhs = np.zeros((m.shape[0], 16), dtype=np.complex128)
+        rhs[:16] = R.oracle_t0(s, atoms[b], ies[b], isps[b])
+        ref = sla.solve(m, rhs)[:16]
+        assert np.linalg.norm(tau_dev[b] - ref) / np.linalg.norm(ref) < 1e-10
