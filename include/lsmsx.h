@@ -282,11 +282,15 @@ int lsmsx_set_single_site_resident(lsmsx_ctx *ctx, int on);
 /* As lsmsx_green_mt, with the radial functions, meshes and lmax of the last
  * lsmsx_single_site_nonrel call made with residency on: solve b uses the
  * (energy e, potential p) pair pair_of[b] = e*npot + p of that call.
+ * jmt_of_pot (may be NULL: the single-site jmt) sets the end of the sphere
+ * integrals per potential; the reference passes jws there for ASA
+ * (energyContourIntegration.cpp:519).
  * Returns -3 when no such call precedes it (an lsmsx_green_mt call in between
  * reuses the buffers and also counts as none).  No reference counterpart
  * (see lsmsx_set_single_site_resident). */
 int lsmsx_green_mt_resident(lsmsx_ctx *ctx, int nsolve, int n_spin_pola, const int *pair_of,
-                            const double *tau00, double *dos, double *green);
+                            const int *jmt_of_pot, const double *tau00, double *dos,
+                            double *green);
 
 /* Fill the device t-matrix store from the last lsmsx_single_site_nonrel
  * batch without a host round trip: the store (same dimensions as

@@ -78,7 +78,7 @@ def load():
                                       _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_set_single_site_resident.argtypes = [_vp, C.c_int]
     L.lsmsx_tmats_from_single_site.argtypes = [_vp, C.c_long, C.c_int, C.c_long, C.c_int, _vp, _vp]
-    L.lsmsx_green_mt_resident.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
+    L.lsmsx_green_mt_resident.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_green_mt.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_single_site_nonrel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
@@ -283,15 +283,19 @@ class Context:
         device for green_mt_resident."""
         _ck(load().lsmsx_set_single_site_resident(self._h, 1 if on else 0))
 
-    def green_mt_resident(self, tau00, pair_of, iprpts, n_spin_pola=1, with_green=True):
+    def green_mt_resident(self, tau00, pair_of, iprpts, n_spin_pola=1, with_green=True,
+                          jmt_of_pot=None):
         """green_mt on the radial functions left on the device by the last
-        single_site_nonrel call (residency on); pair_of[b] = e*npot + p."""
+        single_site_nonrel call (residency on); pair_of[b] = e*npot + p;
+        jmt_of_pot overrides the end of the sphere integrals (jws for ASA)."""
         t = np.ascontiguousarray(np.swapaxes(np.asarray(tau00, dtype=np.complex128), 1, 2))
         po = np.ascontiguousarray(pair_of, dtype=np.int32)
         nsolve = len(po)
         dos = np.zeros(nsolve, dtype=np.complex128)
         green = np.zeros((nsolve, iprpts), dtype=np.complex128) if with_green else None
-        _ck(load().lsmsx_green_mt_resident(self._h, nsolve, n_spin_pola, _p(po), _p(t), _p(dos),
+        jm = None if jmt_of_pot is None else np.ascontiguousarray(jmt_of_pot, dtype=np.int32)
+        _ck(load().lsmsx_green_mt_resident(self._h, nsolve, n_spin_pola, _p(po),
+                                           None if jm is None else _p(jm), _p(t), _p(dos),
                                            _p(green) if with_green else None))
         return dos, green
 

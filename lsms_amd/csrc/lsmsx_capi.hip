@@ -229,6 +229,7 @@ struct lsmsx_ctx {
   int ss_have_matom = 0;  // ss_matom holds the last call's ss_ne x ss_npot x (ss_lmax+1)
   DevBuf<int> ss_storeof;
   std::vector<GreenPot> ss_gpot;
+  std::vector<double> ss_hr, ss_hrsph;  // host copies of the last call's meshes and r_sph
   DevBuf<double2> gf_tau, gf_dos, gf_green;
   DevBuf<long> keyl;
   std::vector<int> last_iters;
@@ -1771,6 +1772,8 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int ip
   if (want_z && want_j) {
     c->ss_valid = 1;
     c->ss_gpot = gp;
+    c->ss_hr.assign(r_mesh, r_mesh + npts);
+    c->ss_hrsph.assign(r_sph, r_sph + npot);
   }
   if (tmat) {  // diagonal t = 1/matom_l (single_scatterer_nonrel.f:100-111)
     const int kk = nl * nl;
@@ -1854,7 +1857,8 @@ int lsmsx_set_single_site_resident(lsmsx_ctx *c, int on) {
 }
 
 int lsmsx_green_mt_resident(lsmsx_ctx *c, int nsolve, int n_spin_pola, const int *pair_of,
-                            const double *tau00, double *dos, double *green) {
+                            const int *jmt_of_pot, const double *tau00, double *dos,
+                            double *green) {
   if (!c) return fail(-1, "null context");
   if (nsolve < 0) return fail(-2, "bad batch size");
   if (n_spin_pola != 1 && n_spin_pola != 2) return fail(-2, "n_spin_pola must be 1 or 2");
@@ -1864,6 +1868,12 @@ int lsmsx_green_mt_resident(lsmsx_ctx *c, int nsolve, int n_spin_pola, const int
     return fail(-3, "no radial functions on the device: call lsmsx_single_site_nonrel with "
                     "lsmsx_set_single_site_resident on (and no lsmsx_green_mt in between)");
   const int npair = c->ss_ne * c->ss_npot, lmax = c->ss_lmax, iprpts = c->ss_iprpts;
+  std::vector<GreenPot> gp = c->ss_gpot;
+  if (jmt_of_pot)  // e.g. jws for ASA potentials (energyContourIntegration.cpp:519)
+    for (int p = 0; p < c->ss_npot; p++)
+      if (const char *e = make_green_pot(c->ss_hr.data() + (size_t)p * iprpts, jmt_of_pot[p],
+                                         gp[p].jws, c->ss_hrsph[p], iprpts, gp[p]))
+        return fail(-2, e);
   std::vector<int> pot_of((size_t)nsolve);
   for (int b = 0; b < nsolve; b++) {
     if (pair_of[b] < 0 || pair_of[b] >= npair) return fail(-2, "pair_of out of range");
@@ -1879,7 +1889,7 @@ int lsmsx_green_mt_resident(lsmsx_ctx *c, int nsolve, int n_spin_pola, const int
   if (green)
     if (int r = c->gf_green.ensure((size_t)nsolve * iprpts)) return r;
   hipStream_t st = c->stream;
-  HIPCHK(hipMemcpyAsync(c->gf_pot.p, c->ss_gpot.data(), c->ss_npot * sizeof(GreenPot),
+  HIPCHK(hipMemcpyAsync(c->gf_pot.p, gp.data(), c->ss_npot * sizeof(GreenPot),
                         hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->gf_potof.p, pot_of.data(), nsolve * sizeof(int), hipMemcpyHostToDevice,
                         st));

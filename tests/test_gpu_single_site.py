@@ -242,6 +242,12 @@ def test_green_mt_resident_equals_host_path():
     tau = tmat[q_e, q_p] + 0.05 * (rng.normal(size=(len(pairs), 16, 16)) +
                                    1j * rng.normal(size=(len(pairs), 16, 16)))
     d_res, g_res = c.green_mt_resident(tau, pairs, rr.shape[1], n_spin_pola=2)
+    # ASA: the sphere integrals run to jws (energyContourIntegration.cpp:519)
+    d_res2, _ = c.green_mt_resident(tau, pairs, rr.shape[1], n_spin_pola=2, jmt_of_pot=jw)
+    d_or = [O.green_mt(3, tau[b], zlr[q_e[b], q_p[b]].T, jlr[q_e[b], q_p[b]].T, rr[q_p[b]],
+                       jw[q_p[b]], jw[q_p[b]], rs[q_p[b]], 2)[0] for b in range(len(pairs))]
+    assert np.allclose(d_res2, d_or, rtol=1e-12, atol=0)
+    assert not np.allclose(d_res2, d_res, rtol=1e-6, atol=0)
     d_host, g_host = c.green_mt(3, tau, zlr[q_e, q_p], jlr[q_e, q_p], q_p, rr, jm, jw, rs,
                                 n_spin_pola=2)
     assert np.array_equal(d_res, d_host) and np.array_equal(g_res, g_host)
