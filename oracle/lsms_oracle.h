@@ -91,6 +91,10 @@ void or_cnewint(int nr, const double *x, const zc *f, zc *g, int ip0);
 int or_green_mt(int lmax, const zc *tau00, const zc *zlr, const zc *jlr, const double *r_mesh,
                 int jmt, int jws, double r_sph, int iprpts, int n_spin_pola, zc *dos, zc *green,
                 zc *zz_out, zc *zj_out);
+/* spin-canted muffin-tin part of green_function.f with the wx/wy/wz recombination */
+int or_green_mt_canted(int lmax, const zc *tau00_l, const zc *zlr, const zc *jlr,
+                       const double *r_mesh, int jmt, int jws, double r_sph, int iprpts,
+                       const zc *w, zc *dos, zc *green);
 
 #ifdef __cplusplus
 }

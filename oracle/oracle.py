@@ -67,6 +67,9 @@ def lib():
     L.or_fitpot.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int]
     L.or_sph_bessel_jh.argtypes = [C.c_int, _Z, _zp, _zp, _zp]
     L.or_cnewint.argtypes = [C.c_int, _dp, _zp, _zp, C.c_int]
+    L.or_green_mt_canted.restype = C.c_int
+    L.or_green_mt_canted.argtypes = [C.c_int, _zp, _zp, _zp, _dp, C.c_int, C.c_int, C.c_double,
+                                     C.c_int, _zp, _zp, _zp]
     L.or_green_mt.restype = C.c_int
     L.or_green_mt.argtypes = [C.c_int, _zp, _zp, _zp, _dp, C.c_int, C.c_int, C.c_double, C.c_int,
                               C.c_int, _zp, _zp, _zp, _zp]
@@ -265,3 +268,19 @@ def green_mt(lmax, tau00, zlr, jlr, r_mesh, jmt, jws, r_sph, n_spin_pola=1):
     if rc != 0:
         raise ValueError("or_green_mt: bad sizes")
     return dos[0], green[:jws], zz, zj
+
+
+def green_mt_canted(lmax, tau00_l, zlr, jlr, r_mesh, jmt, jws, r_sph, wx, wy, wz):
+    """Spin-canted muffin-tin part of green_function.f: tau00_l (kkrsz, kkrsz, 4)
+    column-major blocks, zlr/jlr (iprpts, lmax+1, 2).  Returns (dos[4], green[4, jws])."""
+    tau00_l = _f(tau00_l, np.complex128)
+    zlr = _f(zlr, np.complex128)
+    jlr = _f(jlr, np.complex128)
+    w = np.concatenate([np.asarray(wx), np.asarray(wy), np.asarray(wz)]).astype(np.complex128)
+    dos = np.zeros(4, dtype=np.complex128)
+    green = np.zeros(4 * max(jws, 1), dtype=np.complex128)
+    rc = lib().or_green_mt_canted(lmax, tau00_l, zlr, jlr, _f(r_mesh, np.float64), jmt, jws,
+                                  r_sph, zlr.shape[0], w, dos, green)
+    if rc != 0:
+        raise ValueError("or_green_mt_canted: bad sizes")
+    return dos, green[: 4 * jws].reshape(4, jws)

@@ -595,6 +595,72 @@ void or_cnewint(int nr, const double *x, const zc *f, zc *g, int ip0)
     }
 }
 
+/* ---- one spin channel (zsl, zsr) of gf_local.f for mtasa = 1 / the sphere
+ * part: int_zz_zj.f:116-165 (zlzl_l = int 2 z_l^L z_l^R, zljl_l = int
+ * 2 z_l^L j_l^R when zj_flag), mdosms_c.f and mgreen_c.f, without the spin
+ * factor.  tau: kkrsz x kkrsz column-major; zl_*, jl: iprpts x (lmax+1). */
+static void mt_channel(int lmax, const zc *tau, const zc *zl_left, const zc *zl_right,
+                       const zc *jl, int zj_flag, const double *r_mesh, int jmt, int jws,
+                       double r_sph, int iprpts, zc *dos, zc *green, zc *zz_out, zc *zj_out)
+{
+    const int kkrsz = (lmax + 1) * (lmax + 1);
+    const double pi = 3.14159265358979323846;
+    double *rtmp = (double *)malloc(sizeof(double) * (size_t)(jmt + 1));
+    double *work = (double *)malloc(sizeof(double) * (size_t)(jmt + 1));
+    zc *fr = (zc *)malloc(sizeof(zc) * (size_t)(jmt + 1));
+    zc *fint = (zc *)malloc(sizeof(zc) * (size_t)(jmt + 1));
+    zc zzl[16];
+    rtmp[0] = 0.0;
+    for (int ir = 1; ir <= jmt; ir++) rtmp[ir] = sqrt(r_mesh[ir - 1]);
+    const double xs = sqrt(r_sph);
+    zc zzsum = 0.0, zjsum = 0.0;
+    for (int l = 0; l <= lmax; l++) {
+        const zc *zL = zl_left + (size_t)l * iprpts, *zR = zl_right + (size_t)l * iprpts;
+        const zc *j = jl + (size_t)l * iprpts;
+        for (int ir = 1; ir <= jmt; ir++) fr[ir] = 2.0 * zL[ir - 1] * zR[ir - 1];
+        fr[0] = fr[1];
+        or_cnewint(jmt + 1, rtmp, fr, fint, 5);
+        const zc zlzl = cinterp_value(rtmp, fint, jmt + 1, xs, work);
+        zc zljl = 0.0;
+        if (zj_flag) {
+            for (int ir = 1; ir <= jmt; ir++) fr[ir] = 2.0 * zL[ir - 1] * j[ir - 1];
+            fr[0] = fr[1];
+            or_cnewint(jmt + 1, rtmp, fr, fint, 5);
+            zljl = cinterp_value(rtmp, fint, jmt + 1, xs, work);
+        }
+        zzl[l] = zlzl;
+        if (zz_out) zz_out[l] = zlzl;
+        if (zj_out) zj_out[l] = zljl;
+        for (int m = -l; m <= l; m++) zjsum += zljl;
+    }
+    /* mdosms: sum over the kkrsz^2 entries of the diagonal pzzck times tau */
+    int lm = 0;
+    for (int l = 0; l <= lmax; l++)
+        for (int m = -l; m <= l; m++, lm++) zzsum += zzl[l] * tau[lm + (size_t)lm * kkrsz];
+    zc d = zzsum;
+    if (zj_flag) d = d - zjsum;
+    *dos = -d / pi;
+    if (green) {
+        for (int ir = 0; ir < jws; ir++) green[ir] = 0.0;
+        int i = kkrsz - 1;
+        for (int l = lmax; l >= 0; l--) {
+            zc wt = 0.0;
+            for (int m = -l; m <= l; m++) wt += tau[i + (size_t)i * kkrsz], i--;
+            const zc *zL = zl_left + (size_t)l * iprpts, *zR = zl_right + (size_t)l * iprpts;
+            const zc *j = jl + (size_t)l * iprpts;
+            for (int ir = 0; ir < jws; ir++) {
+                zc t = zR[ir] * wt;
+                if (zj_flag) t = zR[ir] * wt - j[ir] * (double)(2 * l + 1);
+                green[ir] += zL[ir] * t;
+            }
+        }
+    }
+    free(rtmp);
+    free(work);
+    free(fr);
+    free(fint);
+}
+
 /* ---- the muffin-tin (ASA: whole-cell) part of green_function.f for one
  * collinear spin channel: gf_local.f:109-189 with int_zz_zj.f:116-165
  * (zlzl_l, zljl_l over the sphere, integrated in x = sqrt(r)), mdosms_c.f
@@ -605,56 +671,52 @@ int or_green_mt(int lmax, const zc *tau00, const zc *zlr, const zc *jlr, const d
                 int jmt, int jws, double r_sph, int iprpts, int n_spin_pola, zc *dos, zc *green,
                 zc *zz_out, zc *zj_out)
 {
-    const int kkrsz = (lmax + 1) * (lmax + 1);
-    if (jmt < 4 || jmt > iprpts || jws > iprpts || lmax < 0) return -1;
-    const double pi = 3.14159265358979323846;
-    double *rtmp = (double *)malloc(sizeof(double) * (size_t)(jmt + 1));
-    double *work = (double *)malloc(sizeof(double) * (size_t)(jmt + 1));
-    zc *fr = (zc *)malloc(sizeof(zc) * (size_t)(jmt + 1));
-    zc *fint = (zc *)malloc(sizeof(zc) * (size_t)(jmt + 1));
-    rtmp[0] = 0.0;
-    for (int ir = 1; ir <= jmt; ir++) rtmp[ir] = sqrt(r_mesh[ir - 1]);
-    const double xs = sqrt(r_sph);
-    zc zzsum = 0.0, zjsum = 0.0;
-    zc *zzl = (zc *)malloc(sizeof(zc) * (size_t)(lmax + 1));
-    for (int l = 0; l <= lmax; l++) {
-        const zc *z = zlr + (size_t)l * iprpts, *j = jlr + (size_t)l * iprpts;
-        for (int ir = 1; ir <= jmt; ir++) fr[ir] = 2.0 * z[ir - 1] * z[ir - 1];
-        fr[0] = fr[1];
-        or_cnewint(jmt + 1, rtmp, fr, fint, 5);
-        const zc zlzl = cinterp_value(rtmp, fint, jmt + 1, xs, work);
-        for (int ir = 1; ir <= jmt; ir++) fr[ir] = 2.0 * z[ir - 1] * j[ir - 1];
-        fr[0] = fr[1];
-        or_cnewint(jmt + 1, rtmp, fr, fint, 5);
-        const zc zljl = cinterp_value(rtmp, fint, jmt + 1, xs, work);
-        zzl[l] = zlzl;
-        if (zz_out) zz_out[l] = zlzl;
-        if (zj_out) zj_out[l] = zljl;
-        for (int m = -l; m <= l; m++) zjsum += zljl;
-    }
-    /* mdosms: sum over the kkrsz^2 entries of the diagonal pzzck times tau00 */
-    int lm = 0;
-    for (int l = 0; l <= lmax; l++)
-        for (int m = -l; m <= l; m++, lm++) zzsum += zzl[l] * tau00[lm + (size_t)lm * kkrsz];
-    zc d = -(zzsum - zjsum) / pi;
+    if (jmt < 4 || jmt > iprpts || jws > iprpts || lmax < 0 || lmax > 3) return -1;
+    zc d;
+    mt_channel(lmax, tau00, zlr, zlr, jlr, 1, r_mesh, jmt, jws, r_sph, iprpts, &d, green,
+               zz_out, zj_out);
     const double spin = (n_spin_pola == 1) ? 2.0 : 1.0;
     *dos = spin * d;
-    if (green) {
-        for (int ir = 0; ir < jws; ir++) green[ir] = 0.0;
-        int i = kkrsz - 1;
-        for (int l = lmax; l >= 0; l--) {
-            zc wt = 0.0;
-            for (int m = -l; m <= l; m++) wt += tau00[i + (size_t)i * kkrsz], i--;
-            const zc *z = zlr + (size_t)l * iprpts, *j = jlr + (size_t)l * iprpts;
-            for (int ir = 0; ir < jws; ir++)
-                green[ir] += z[ir] * (z[ir] * wt - j[ir] * (double)(2 * l + 1));
-        }
+    if (green)
         for (int ir = 0; ir < jws; ir++) green[ir] *= spin;
+    return 0;
+}
+
+/* ---- spin-canted muffin-tin part of green_function.f:84-148 (isp = 1..4:
+ * (zsl, zsr) = (1,1), (2,1), (1,2), (2,2), zj only on the diagonal) and the
+ * recombination with wx, wy, wz (u_sigma_u) of :150-180.
+ * tau00_l: 4 blocks kkrsz x kkrsz (trgtol's ml(nr,nc,4)); zlr, jlr:
+ * iprpts x (lmax+1) x 2 spins; w = wx[4], wy[4], wz[4];
+ * dos[4] = Tr, x, y, z; green[4*jws]: the same per r (may be NULL). */
+int or_green_mt_canted(int lmax, const zc *tau00_l, const zc *zlr, const zc *jlr,
+                       const double *r_mesh, int jmt, int jws, double r_sph, int iprpts,
+                       const zc *w, zc *dos, zc *green)
+{
+    if (jmt < 4 || jmt > iprpts || jws > iprpts || lmax < 0 || lmax > 3) return -1;
+    const int kkrsz = (lmax + 1) * (lmax + 1);
+    const size_t sp = (size_t)iprpts * (lmax + 1);
+    static const int zsl[4] = {0, 1, 0, 1}, zsr[4] = {0, 0, 1, 1}, flag[4] = {1, 0, 0, 1};
+    zc d[4];
+    zc *g = green ? (zc *)malloc(sizeof(zc) * 4 * (size_t)(jws > 0 ? jws : 1)) : NULL;
+    for (int isp = 0; isp < 4; isp++)
+        mt_channel(lmax, tau00_l + (size_t)isp * kkrsz * kkrsz, zlr + zsl[isp] * sp,
+                   zlr + zsr[isp] * sp, jlr + zsr[isp] * sp, flag[isp], r_mesh, jmt, jws, r_sph,
+                   iprpts, &d[isp], g ? g + (size_t)isp * jws : NULL, NULL, NULL);
+    const zc *wx = w, *wy = w + 4, *wz = w + 8;
+    dos[0] = d[0] + d[3];
+    dos[1] = d[0] * wx[0] + d[1] * wx[2] + d[2] * wx[1] + d[3] * wx[3];
+    dos[2] = d[0] * wy[0] + d[1] * wy[2] + d[2] * wy[1] + d[3] * wy[3];
+    dos[3] = d[0] * wz[0] + d[1] * wz[2] + d[2] * wz[1] + d[3] * wz[3];
+    if (g) {
+        for (int ir = 0; ir < jws; ir++) {
+            const zc g1 = g[ir], g2 = g[jws + ir], g3 = g[2 * (size_t)jws + ir],
+                     g4 = g[3 * (size_t)jws + ir];
+            green[ir] = g1 + g4;
+            green[jws + ir] = g1 * wx[0] + g2 * wx[2] + g3 * wx[1] + g4 * wx[3];
+            green[2 * (size_t)jws + ir] = g1 * wy[0] + g2 * wy[2] + g3 * wy[1] + g4 * wy[3];
+            green[3 * (size_t)jws + ir] = g1 * wz[0] + g2 * wz[2] + g3 * wz[1] + g4 * wz[3];
+        }
+        free(g);
     }
-    free(zzl);
-    free(rtmp);
-    free(work);
-    free(fr);
-    free(fint);
     return 0;
 }

@@ -191,3 +191,73 @@ def test_green_mt_dos_is_the_integral_of_green():
     f[0] = f[1]
     g = O.cnewint(x, f, 5)
     assert abs(-g[jmt] / np.pi - dos) <= 1e-5 * abs(dos)
+
+
+def _usu(u, ud):
+    """wx, wy, wz of Misc/u_sigma_u.f for the spin rotation (u, ud)."""
+    wx = [u[2] * ud[0] + u[0] * ud[1], u[3] * ud[0] + u[1] * ud[1],
+          u[2] * ud[2] + u[0] * ud[3], u[3] * ud[2] + u[1] * ud[3]]
+    wy = [1j * (u[2] * ud[0] - u[0] * ud[1]), 1j * (u[3] * ud[0] - u[1] * ud[1]),
+          1j * (u[2] * ud[2] - u[0] * ud[3]), 1j * (u[3] * ud[2] - u[1] * ud[3])]
+    wz = [u[0] * ud[0] - u[2] * ud[1], u[1] * ud[0] - u[3] * ud[1],
+          u[0] * ud[2] - u[2] * ud[3], u[1] * ud[2] - u[3] * ud[3]]
+    return np.array(wx), np.array(wy), np.array(wz)
+
+
+def _two_spins():
+    E = 0.45 + 0.05j
+    pr = CF.prel_of(E)
+    out = []
+    for vmtz in (-1.25, -1.15):
+        r, vr, jmt, jws, rs = CF.model_potential(26, 2.3, vmtz=vmtz)
+        m, t, zl, jl, rc = O.single_scatterer_nonrel(0, 3, E, pr, vr, r, jmt, jws, rs)
+        out.append((t, zl, jl))
+    return r, jmt, jws, rs, out
+
+
+@pytest.mark.parametrize("evec", [(0.0, 0.0, 1.0), (0.3, -0.5, 0.81)])
+def test_green_mt_canted_spin_diagonal_limit(evec):
+    """Spin-diagonal local tau00: the canted DOS and Green function reduce
+    to the two collinear channels -- the trace is their sum and the moment
+    (x, y, z parts) is their difference along the local moment direction."""
+    r, jmt, jws, rs, ((tu, zu, ju), (td, zd, jd)) = _two_spins()
+    ev = np.array(evec) / np.linalg.norm(evec)
+    wx, wy, wz = _usu(*CF.spin_rotation(ev))
+    tau = np.zeros((16, 16, 4), dtype=np.complex128)
+    tau[:, :, 0], tau[:, :, 3] = tu, td
+    zlr = np.stack([zu, zd], axis=2)
+    jlr = np.stack([ju, jd], axis=2)
+    dos, green = O.green_mt_canted(3, tau, zlr, jlr, r, jmt, jws, rs, wx, wy, wz)
+    du, gu, _, _ = O.green_mt(3, tu, zu, ju, r, jmt, jws, rs, 2)
+    dd, gd, _, _ = O.green_mt(3, td, zd, jd, r, jmt, jws, rs, 2)
+    assert abs(dos[0] - (du + dd)) <= 1e-14 * abs(du + dd)
+    for a in range(3):
+        assert abs(dos[1 + a] - ev[a] * (du - dd)) <= 1e-13 * abs(du - dd)
+        # green(r) cancels between the spins near the nucleus: bound by the terms
+        scale = (np.abs(gu) + np.abs(gd)).max()
+        assert np.abs(green[1 + a] - ev[a] * (gu - gd)).max() <= 1e-13 * scale
+    assert np.abs(green[0] - (gu + gd)).max() <= 1e-14 * np.abs(gu + gd).max()
+
+
+def test_green_mt_canted_off_diagonal_blocks():
+    """Identity spin frame with off-diagonal spin blocks and equal radial
+    functions for both spins: the x and y parts are Tr(D sigma_x),
+    Tr(D sigma_y) of the 2x2 spin matrix D of channel DOS values, each
+    channel -(sum zz_l tau_LL)/pi with the collinear sphere integrals."""
+    r, jmt, jws, rs, ((t, z, j), _) = _two_spins()
+    wx, wy, wz = _usu(*CF.spin_rotation(np.array([0.0, 0.0, 1.0])))
+    rng = np.random.default_rng(9)
+    tau = (rng.normal(size=(16, 16, 4)) + 1j * rng.normal(size=(16, 16, 4))) * 0.2
+    zlr = np.stack([z, z], axis=2)
+    jlr = np.stack([j, j], axis=2)
+    dos, _ = O.green_mt_canted(3, tau, zlr, jlr, r, jmt, jws, rs, wx, wy, wz)
+    _, _, zz, zj = O.green_mt(3, tau[:, :, 0], z, j, r, jmt, jws, rs, 2)
+    lof = np.repeat(np.arange(4), 2 * np.arange(4) + 1)
+    chan = [-(np.sum(zz[lof] * np.diag(tau[:, :, i]))) / np.pi for i in range(4)]
+    zjs = np.sum(zj[lof]) / np.pi
+    d1, d2, d3, d4 = chan[0] + zjs, chan[1], chan[2], chan[3] + zjs
+    # D = [[d1, d3], [d2, d4]] (rows: left spin)
+    assert abs(dos[0] - (d1 + d4)) <= 1e-13 * abs(d1 + d4)
+    assert abs(dos[1] - (d2 + d3)) <= 1e-13 * abs(d2 + d3)
+    assert abs(dos[2] - (-1j * d2 + 1j * d3)) <= 1e-13 * abs(d3 - d2)
+    assert abs(dos[3] - (d1 - d4)) <= 1e-13 * abs(d1 - d4)
