@@ -305,6 +305,23 @@ int lsmsx_tmats_from_single_site(lsmsx_ctx *ctx, long ldTmatStore, int numStore,
                                  long blkSizeTmatStore, int groupSize, const int *store_of_pot,
                                  const int *spin_of_pot);
 
+/* Spin-canted muffin-tin DOS and Green function: green_function_ with
+ * n_spin_cant = 2 (MultipleScattering/green_function.f:84-180), sphere part
+ * as lsmsx_green_mt: the four spin channels (zsl, zsr) = (1,1), (2,1),
+ * (1,2), (2,2) of gf_local, the irregular term on the diagonal ones only,
+ * then Tr and the x, y, z parts with wx, wy, wz (Misc/u_sigma_u.f).
+ *   tau00_l   nsolve x 4 x kkrsz x kkrsz complex: the four column-major
+ *             blocks trgtol.f writes (ml(nr,nc,4))
+ *   zlr, jlr  nsolve x 2 spins x (lmax+1) x iprpts complex
+ *   wxyz      nsolve x 12 complex: wx[4], wy[4], wz[4] of the solve's atom
+ * Outputs: dos[b*4 + a] (a = Tr, x, y, z), green (may be NULL)
+ * [(b*4 + a)*iprpts + j] for j < jws. */
+int lsmsx_green_mt_canted(lsmsx_ctx *ctx, int nsolve, int lmax, int iprpts, int npot,
+                          const int *pot_of, const double *r_mesh, const int *jmt,
+                          const int *jws, const double *r_sph, const double *tau00_l,
+                          const double *zlr, const double *jlr, const double *wxyz,
+                          double *dos, double *green);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,7 @@ EXPORTED = [
     "lsmsx_upload_dtmats", "lsmsx_tau_full", "lsmsx_tau00_adjoint", "lsmsx_set_schedule",
     "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune", "lsmsx_set_pivot_guard",
     "lsmsx_single_site_nonrel", "lsmsx_green_mt", "lsmsx_set_single_site_resident",
-    "lsmsx_green_mt_resident", "lsmsx_tmats_from_single_site",
+    "lsmsx_green_mt_resident", "lsmsx_tmats_from_single_site", "lsmsx_green_mt_canted",
 ]
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
@@ -77,6 +77,8 @@ def load():
     L.lsmsx_tau00_adjoint.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                       _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_set_single_site_resident.argtypes = [_vp, C.c_int]
+    L.lsmsx_green_mt_canted.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
+                                        _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_tmats_from_single_site.argtypes = [_vp, C.c_long, C.c_int, C.c_long, C.c_int, _vp, _vp]
     L.lsmsx_green_mt_resident.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_green_mt.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
@@ -277,6 +279,31 @@ class Context:
         sp = np.ascontiguousarray(np.zeros_like(so) if spin_of_pot is None else spin_of_pot,
                                   dtype=np.int32)
         _ck(load().lsmsx_tmats_from_single_site(self._h, ld, num_store, blk, group, _p(so), _p(sp)))
+
+    def green_mt_canted(self, lmax, tau00_l, zlr, jlr, wxyz, pot_of, r_mesh, jmt, jws, r_sph,
+                        with_green=True):
+        """Spin-canted muffin-tin DOS and Green function.  tau00_l: (nsolve, 4,
+        kkrsz, kkrsz); zlr, jlr: (nsolve, 2, lmax+1, iprpts); wxyz: (nsolve, 3, 4).
+        Returns (dos[nsolve, 4], green[nsolve, 4, iprpts]) with parts Tr, x, y, z."""
+        rm = np.ascontiguousarray(np.atleast_2d(r_mesh), dtype=np.float64)
+        npot, iprpts = rm.shape
+        z = np.ascontiguousarray(zlr, dtype=np.complex128)
+        j = np.ascontiguousarray(jlr, dtype=np.complex128)
+        nsolve = z.shape[0]
+        if z.shape != (nsolve, 2, lmax + 1, iprpts) or j.shape != z.shape:
+            raise ValueError("zlr/jlr must be (nsolve, 2, lmax+1, iprpts)")
+        t = np.ascontiguousarray(np.swapaxes(np.asarray(tau00_l, dtype=np.complex128), 2, 3))
+        w = np.ascontiguousarray(wxyz, dtype=np.complex128)
+        po = np.ascontiguousarray(pot_of, dtype=np.int32)
+        jm = np.ascontiguousarray(np.broadcast_to(jmt, (npot,)), dtype=np.int32)
+        jw = np.ascontiguousarray(np.broadcast_to(jws, (npot,)), dtype=np.int32)
+        rs = np.ascontiguousarray(np.broadcast_to(r_sph, (npot,)), dtype=np.float64)
+        dos = np.zeros((nsolve, 4), dtype=np.complex128)
+        green = np.zeros((nsolve, 4, iprpts), dtype=np.complex128) if with_green else None
+        _ck(load().lsmsx_green_mt_canted(self._h, nsolve, lmax, iprpts, npot, _p(po), _p(rm),
+                                         _p(jm), _p(jw), _p(rs), _p(t), _p(z), _p(j), _p(w),
+                                         _p(dos), _p(green) if with_green else None))
+        return dos, green
 
     def set_single_site_resident(self, on):
         """Keep the radial functions of later single_site_nonrel calls on the

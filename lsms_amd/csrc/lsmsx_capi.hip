@@ -56,6 +56,9 @@ __global__ void k_fitpot(const double *, const double *, const int *, int, int, 
 __global__ void k_single_site(const double *, const double *, const SingleSitePot *,
                               const double2 *, const double2 *, int, int, int, int, int,
                               double2 *, double2 *, double2 *, int *);
+__global__ void k_green_mt_canted(const double2 *, const double2 *, const double2 *,
+                                  const double2 *, const double *, const GreenPot *, const int *,
+                                  int, int, int, double2 *, double2 *);
 __global__ void k_tmat_fill(const double2 *, const int *, const int *, int, int, int, long, long,
                             double2 *);
 __global__ void k_green_mt(const double2 *, const double2 *, const double2 *, const double *,
@@ -230,7 +233,7 @@ struct lsmsx_ctx {
   DevBuf<int> ss_storeof;
   std::vector<GreenPot> ss_gpot;
   std::vector<double> ss_hr, ss_hrsph;  // host copies of the last call's meshes and r_sph
-  DevBuf<double2> gf_tau, gf_dos, gf_green;
+  DevBuf<double2> gf_tau, gf_dos, gf_green, gf_w;
   DevBuf<long> keyl;
   std::vector<int> last_iters;
   std::vector<double> last_minpiv;
@@ -1223,6 +1226,7 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->gf_tau.release();
   c->gf_dos.release();
   c->gf_green.release();
+  c->gf_w.release();
   c->M.release();
   c->inv.release();
   c->tau.release();
@@ -1947,6 +1951,65 @@ int lsmsx_tmats_from_single_site(lsmsx_ctx *c, long ldT, int numStore, long blk,
   c->blk = blk;
   c->nstore = numStore;
   c->group = groupSize;
+  return 0;
+}
+
+int lsmsx_green_mt_canted(lsmsx_ctx *c, int nsolve, int lmax, int iprpts, int npot,
+                          const int *pot_of, const double *r_mesh, const int *jmt,
+                          const int *jws, const double *r_sph, const double *tau00_l,
+                          const double *zlr, const double *jlr, const double *wxyz,
+                          double *dos, double *green) {
+  if (!c) return fail(-1, "null context");
+  if (lmax < 0 || lmax > 3) return fail(-2, "lmax must be 0..3");
+  if (nsolve < 0 || npot < 1 || iprpts < 1) return fail(-2, "bad batch size");
+  if (nsolve == 0) return 0;
+  if (!pot_of || !r_mesh || !jmt || !jws || !r_sph || !tau00_l || !zlr || !jlr || !wxyz || !dos)
+    return fail(-2, "null argument");
+  std::vector<GreenPot> hp((size_t)npot);
+  for (int p = 0; p < npot; p++)
+    if (const char *e = make_green_pot(r_mesh + (size_t)p * iprpts, jmt[p], jws[p], r_sph[p],
+                                       iprpts, hp[p]))
+      return fail(-2, e);
+  for (int b = 0; b < nsolve; b++)
+    if (pot_of[b] < 0 || pot_of[b] >= npot) return fail(-2, "pot_of out of range");
+  const int nl = lmax + 1, kk = nl * nl;
+  const size_t nrad = (size_t)nsolve * 2 * nl * iprpts;
+  c->ss_valid = 0;  // the radial-function buffers are reused below
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->ss_r.ensure((size_t)npot * iprpts)) return r;
+  if (int r = c->gf_pot.ensure(npot)) return r;
+  if (int r = c->gf_potof.ensure(nsolve)) return r;
+  if (int r = c->gf_tau.ensure((size_t)nsolve * 4 * kk * kk)) return r;
+  if (int r = c->ss_zlr.ensure(nrad)) return r;
+  if (int r = c->ss_jlr.ensure(nrad)) return r;
+  if (int r = c->gf_w.ensure((size_t)nsolve * 12)) return r;
+  if (int r = c->gf_dos.ensure((size_t)nsolve * 4)) return r;
+  if (green)
+    if (int r = c->gf_green.ensure((size_t)nsolve * 4 * iprpts)) return r;
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(c->ss_r.p, r_mesh, (size_t)npot * iprpts * sizeof(double),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gf_pot.p, hp.data(), npot * sizeof(GreenPot), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gf_potof.p, pot_of, nsolve * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gf_tau.p, tau00_l, (size_t)nsolve * 4 * kk * kk * sizeof(double2),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_zlr.p, zlr, nrad * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_jlr.p, jlr, nrad * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gf_w.p, wxyz, (size_t)nsolve * 12 * sizeof(double2),
+                        hipMemcpyHostToDevice, st));
+  if (green)
+    HIPCHK(hipMemsetAsync(c->gf_green.p, 0, (size_t)nsolve * 4 * iprpts * sizeof(double2), st));
+  hipLaunchKernelGGL(k_green_mt_canted, dim3((unsigned)((nsolve + 1) / 2)), dim3(64), 0, st,
+                     c->gf_tau.p, c->ss_zlr.p, c->ss_jlr.p, c->gf_w.p, c->ss_r.p, c->gf_pot.p,
+                     c->gf_potof.p, nsolve, lmax, iprpts, c->gf_dos.p,
+                     green ? c->gf_green.p : nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dos, c->gf_dos.p, (size_t)nsolve * 4 * sizeof(double2),
+                        hipMemcpyDeviceToHost, st));
+  if (green)
+    HIPCHK(hipMemcpyAsync(green, c->gf_green.p, (size_t)nsolve * 4 * iprpts * sizeof(double2),
+                          hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
 

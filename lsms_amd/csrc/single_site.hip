@@ -659,6 +659,69 @@ __device__ zd cnewint_step(const double *rs, const double *fr, const double *fi,
 
 }  // namespace
 
+namespace {
+
+// int_zz_zj.f:116-165 for one lane: the running integral (cnewint.f, ip0 = 5,
+// on x_0 = 0, x_k = sqrt(r_k)) of 2 a_l b_l with f_0 = f_1, interpolated at
+// sqrt(r_sph) (cinterp.f); a, b are radial columns, rm the mesh
+__device__ zd sphere_integral(const double2 *__restrict__ a, const double2 *__restrict__ bcol,
+                              const double *__restrict__ rm, const GreenPot &P) {
+  auto fval = [&](int k) -> zd {
+    const int q = k == 0 ? 0 : k - 1;
+    const zd u = mk(a[q].x, a[q].y), c = mk(bcol[q].x, bcol[q].y);
+    return 2.0 * (u * c);
+  };
+  auto xval = [&](int k) -> double { return k == 0 ? 0.0 : sqrt(rm[k - 1]); };
+  const int nr = P.jmt + 1;
+  const int ip = 2 * 5 + 1;
+  double rs[4], fr[4], fi[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    rs[k] = sqrt(xval(k));
+    const zd f = fval(k);
+    fr[k] = f.x;
+    fi[k] = f.y;
+  }
+  zd g = mk(0.0), w[4];
+  int i4 = 3, i0 = 0, ip2 = 4;
+  auto capture = [&](int k) {
+    if (k == P.i1) w[0] = g;
+    if (k == P.ip1 - 1) w[1] = g;
+    if (k == P.ip1) w[2] = g;
+    if (k == P.i2) w[3] = g;
+  };
+  capture(0);
+  for (int i = 1; i <= nr - 1; i++) {
+    if (i > 2 && ip2 <= nr) {
+      const zd f = fval(ip2 - 1);
+      const double xv = sqrt(xval(ip2 - 1));
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (q == i4) {
+          rs[q] = xv;
+          fr[q] = f.x;
+          fi[q] = f.y;
+        }
+    }
+    g = g + cnewint_step(rs, fr, fi, i0, ip);
+    capture(i);
+    if (i > 1 && ip2 < nr) {
+      ip2 += 1;
+      i4 = (i4 + 1) & 3;  // the ring rotation i1..i4 advances i4 by one slot
+    }
+    i0 = (i0 + 1) & 3;
+  }
+  double xw[4];
+  xw[0] = xval(P.i1);
+  xw[1] = xval(P.ip1 - 1);
+  xw[2] = xval(P.ip1);
+  xw[3] = xval(P.i2);
+  const int same = (P.i1 == P.i2);
+  return interp4(xw, 4, 1, 0, same ? 0 : 3, P.xs, w);
+}
+
+}  // namespace
+
 __global__ void __launch_bounds__(64) k_green_mt(
     const double2 *__restrict__ tau00, const double2 *__restrict__ zlr,
     const double2 *__restrict__ jlr, const double *__restrict__ rmesh,
@@ -681,60 +744,7 @@ __global__ void __launch_bounds__(64) k_green_mt(
       const long src = src_of ? src_of[b] : b;
       const double2 *zl = zlr + (src * nl + l) * iprpts;
       const double2 *ol = (which ? jlr : zlr) + (src * nl + l) * iprpts;
-      // integrand on the x mesh (x_0 = 0, x_k = sqrt(r_k)), f_0 = f_1
-      auto fval = [&](int k) -> zd {
-        const int q = k == 0 ? 0 : k - 1;
-        const zd u = mk(zl[q].x, zl[q].y), c = mk(ol[q].x, ol[q].y);
-        return 2.0 * (u * c);
-      };
-      auto xval = [&](int k) -> double { return k == 0 ? 0.0 : sqrt(rm[k - 1]); };
-      const int nr = P.jmt + 1;
-      const int ip = 2 * 5 + 1;
-      double rs[4], fr[4], fi[4];
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        rs[k] = sqrt(xval(k));
-        const zd f = fval(k);
-        fr[k] = f.x;
-        fi[k] = f.y;
-      }
-      zd g = mk(0.0), w[4];
-      int i4 = 3, i0 = 0, ip2 = 4;
-      auto capture = [&](int k) {
-        if (k == P.i1) w[0] = g;
-        if (k == P.ip1 - 1) w[1] = g;
-        if (k == P.ip1) w[2] = g;
-        if (k == P.i2) w[3] = g;
-      };
-      capture(0);
-      for (int i = 1; i <= nr - 1; i++) {
-        if (i > 2 && ip2 <= nr) {
-          const zd f = fval(ip2 - 1);
-          const double xv = sqrt(xval(ip2 - 1));
-#pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (q == i4) {
-              rs[q] = xv;
-              fr[q] = f.x;
-              fi[q] = f.y;
-            }
-        }
-        g = g + cnewint_step(rs, fr, fi, i0, ip);
-        capture(i);
-        if (i > 1 && ip2 < nr) {
-          ip2 += 1;
-          i4 = (i4 + 1) & 3;  // the ring rotation i1..i4 advances i4 by one slot
-        }
-        i0 = (i0 + 1) & 3;
-      }
-      // cinterp.f at sqrt(r_sph) over the x mesh
-      double xw[4];
-      xw[0] = xval(P.i1);
-      xw[1] = xval(P.ip1 - 1);
-      xw[2] = xval(P.ip1);
-      xw[3] = xval(P.i2);
-      const int same = (P.i1 == P.i2);
-      s_int[sl][which][l] = interp4(xw, 4, 1, 0, same ? 0 : 3, P.xs, w);
+      s_int[sl][which][l] = sphere_integral(zl, ol, rm, P);
     }
   }
   __syncthreads();
@@ -787,6 +797,133 @@ __global__ void __launch_bounds__(64) k_green_mt(
           acc = acc + zz * (zz * wt[l] - jj * (double)(2 * l + 1));
         }
         gb[ir] = make_double2(spin * acc.x, spin * acc.y);
+      }
+    }
+  }
+}
+
+// Spin-canted muffin-tin DOS and Green function (green_function.f:84-180):
+// the four spin channels isp = (zsl, zsr) = (1,1), (2,1), (1,2), (2,2), the
+// irregular term only on the diagonal ones, then Tr and the x, y, z parts
+// with wx, wy, wz (u_sigma_u.f).  Per solve: tau00_l as four kkrsz x kkrsz
+// blocks (trgtol.f), zlr/jlr as 2 spins x (lmax+1) x iprpts, w = wx, wy, wz.
+// One 64-thread workgroup per two solves: lane 32s + 8isp + 2l + which.
+__global__ void __launch_bounds__(64) k_green_mt_canted(
+    const double2 *__restrict__ tau00, const double2 *__restrict__ zlr,
+    const double2 *__restrict__ jlr, const double2 *__restrict__ wxyz,
+    const double *__restrict__ rmesh, const GreenPot *__restrict__ pots,
+    const int *__restrict__ pot_of, int nsolve, int lmax, int iprpts,
+    double2 *__restrict__ dos, double2 *__restrict__ green) {
+  __shared__ zd s_int[2][4][2][4];
+  __shared__ zd s_d[2][4];
+  const int tid = threadIdx.x;
+  const int nl = lmax + 1, kk = nl * nl;
+  const int b0 = blockIdx.x * 2;
+  {
+    const int sl = tid >> 5, isp = (tid >> 3) & 3, l = (tid & 7) >> 1, which = tid & 1;
+    const int b = b0 + sl;
+    const bool diag = (isp == 0 || isp == 3);
+    if (b < nsolve && l < nl && (which == 0 || diag)) {
+      const int zsl = (isp & 1), zsr = (isp >> 1);
+      const GreenPot P = pots[pot_of[b]];
+      const double *rm = rmesh + (long)pot_of[b] * iprpts;
+      const double2 *zl = zlr + (((long)b * 2 + zsl) * nl + l) * iprpts;
+      const double2 *ol = (which ? jlr : zlr) + (((long)b * 2 + zsr) * nl + l) * iprpts;
+      s_int[sl][isp][which][l] = sphere_integral(zl, ol, rm, P);
+    }
+  }
+  __syncthreads();
+  if (tid < 8) {  // mdosms per channel
+    const int sl = tid >> 2, isp = tid & 3, b = b0 + sl;
+    if (b < nsolve) {
+      const bool diag = (isp == 0 || isp == 3);
+      const double2 *tb = tau00 + ((long)b * 4 + isp) * kk * kk;
+      zd zz = mk(0.0), zj = mk(0.0);
+      for (int l = 0; l < nl; l++)
+        for (int m = -l; m <= l; m++) zj = zj + (diag ? s_int[sl][isp][1][l] : mk(0.0));
+      int lm = 0;
+      for (int l = 0; l < nl; l++)
+        for (int m = -l; m <= l; m++, lm++) {
+          const double2 t = tb[lm + (long)lm * kk];
+          zz = zz + s_int[sl][isp][0][l] * mk(t.x, t.y);
+        }
+      zd d = zz;
+      if (diag) d = d - zj;
+      s_d[sl][isp] = -d / 3.14159265358979323846;
+    }
+  }
+  __syncthreads();
+  if (tid < 2 && b0 + tid < nsolve) {  // recombination (green_function.f:150-162)
+    const int b = b0 + tid;
+    const zd d1 = s_d[tid][0], d2 = s_d[tid][1], d3 = s_d[tid][2], d4 = s_d[tid][3];
+    zd out[4];
+    out[0] = d1 + d4;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const double2 *w = wxyz + (long)b * 12 + 4 * a;
+      out[1 + a] = d1 * mk(w[0].x, w[0].y) + d2 * mk(w[2].x, w[2].y) +
+                   d3 * mk(w[1].x, w[1].y) + d4 * mk(w[3].x, w[3].y);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) dos[(long)b * 4 + a] = make_double2(out[a].x, out[a].y);
+  }
+  if (green) {  // mgreen per channel, then the same recombination per r
+    for (int sl = 0; sl < 2; sl++) {
+      const int b = b0 + sl;
+      if (b >= nsolve) break;
+      zd wt[4][4];
+#pragma unroll
+      for (int isp = 0; isp < 4; isp++) {
+        const double2 *tb = tau00 + ((long)b * 4 + isp) * kk * kk;
+        int i = kk - 1;
+        for (int l = lmax; l >= 0; l--) {
+          zd sum = mk(0.0);
+          for (int m = -l; m <= l; m++, i--) {
+            const double2 t = tb[i + (long)i * kk];
+            sum = sum + mk(t.x, t.y);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (q == l) wt[isp][q] = sum;
+        }
+      }
+      zd w[3][4];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const double2 v = wxyz[(long)b * 12 + 4 * a + q];
+          w[a][q] = mk(v.x, v.y);
+        }
+      const int jws = pots[pot_of[b]].jws;
+      const double2 *zb = zlr + (long)b * 2 * nl * iprpts, *jb = jlr + (long)b * 2 * nl * iprpts;
+      for (int ir = tid; ir < jws; ir += blockDim.x) {
+        zd g[4];
+#pragma unroll
+        for (int isp = 0; isp < 4; isp++) {
+          const int zsl = (isp & 1), zsr = (isp >> 1);
+          const bool diag = (isp == 0 || isp == 3);
+          zd acc = mk(0.0);
+#pragma unroll
+          for (int l = 3; l >= 0; l--) {
+            if (l > lmax) continue;
+            const double2 zL = zb[((long)zsl * nl + l) * iprpts + ir];
+            const double2 zR = zb[((long)zsr * nl + l) * iprpts + ir];
+            const double2 jR = jb[((long)zsr * nl + l) * iprpts + ir];
+            zd t = mk(zR.x, zR.y) * wt[isp][l];
+            if (diag) t = mk(zR.x, zR.y) * wt[isp][l] - mk(jR.x, jR.y) * (double)(2 * l + 1);
+            acc = acc + mk(zL.x, zL.y) * t;
+          }
+          g[isp] = acc;
+        }
+        double2 *gb = green + (long)b * 4 * iprpts;
+        const zd g0 = g[0] + g[3];
+        gb[ir] = make_double2(g0.x, g0.y);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          const zd v = g[0] * w[a][0] + g[1] * w[a][2] + g[2] * w[a][1] + g[3] * w[a][3];
+          gb[(long)(1 + a) * iprpts + ir] = make_double2(v.x, v.y);
+        }
       }
     }
   }

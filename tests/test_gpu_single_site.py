@@ -312,3 +312,37 @@ def test_store_filled_on_device(nspol):
         rhs[:16] = R.oracle_t0(s, atoms[b], ies[b], isps[b])
         ref = sla.solve(m, rhs)[:16]
         assert np.linalg.norm(tau_dev[b] - ref) / np.linalg.norm(ref) < 1e-10
+
+
+def test_green_mt_canted_matches_oracle(ctx):
+    """Spin-canted muffin-tin DOS and green(r) on the GPU against the oracle:
+    spin-up/down radial functions of two species, random local tau00 with
+    off-diagonal spin blocks, random moment directions; 1e-12 relative
+    (green(r) to 1e-12 of its largest value)."""
+    from tests.test_single_site_cpu import _usu
+    rr, vr, jm, jw, rs, e, pr = CF.single_site_batch(nspecies=2, ne=3)
+    matom, tmat, zlr, jlr, rc = ctx.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, e, pr)
+    assert rc == 0
+    rng = np.random.default_rng(21)
+    solves = [(ie, sp) for ie in range(3) for sp in range(2)] + [(1, 0)]
+    n = len(solves)
+    z = np.stack([np.stack([zlr[ie, 2 * sp], zlr[ie, 2 * sp + 1]]) for ie, sp in solves])
+    j = np.stack([np.stack([jlr[ie, 2 * sp], jlr[ie, 2 * sp + 1]]) for ie, sp in solves])
+    pot_of = np.array([2 * sp for _, sp in solves])
+    tau = 0.2 * (rng.normal(size=(n, 4, 16, 16)) + 1j * rng.normal(size=(n, 4, 16, 16)))
+    w = []
+    for b in range(n):
+        ev = rng.normal(size=3)
+        w.append(np.stack(_usu(*CF.spin_rotation(ev / np.linalg.norm(ev)))))
+    w = np.array(w)
+    dos, green = ctx.green_mt_canted(3, tau, z, j, w, pot_of, rr, jm, jw, rs)
+    for b in range(n):
+        p = pot_of[b]
+        # oracle layouts: tau (kkrsz, kkrsz, 4), radial (iprpts, lmax+1, 2)
+        d_or, g_or = O.green_mt_canted(3, np.moveaxis(tau[b], 0, 2), np.transpose(z[b], (2, 1, 0)),
+                                       np.transpose(j[b], (2, 1, 0)), rr[p], jm[p], jw[p], rs[p],
+                                       w[b, 0], w[b, 1], w[b, 2])
+        assert np.abs(dos[b] - d_or).max() <= 1e-12 * np.abs(d_or).max(), (b, dos[b], d_or)
+        nj = jw[p]
+        assert np.abs(green[b, :, :nj] - g_or).max() <= 1e-12 * np.abs(g_or).max()
+        assert not np.any(green[b, :, nj:])
