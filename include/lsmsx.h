@@ -242,8 +242,9 @@ int lsmsx_autotune(lsmsx_ctx *ctx, int nsolve, const int *atom_ids, const int *i
  * reference prints a warning and goes on; the results are still written).
  * The inward (irregular) integration feeds only jlr and runs only when jlr
  * is requested;
- * < 0 on bad arguments.  The spin-canted frame rotation (trltog) stays with
- * the caller. */
+ * < 0 on bad arguments.  The spin-canted frame rotation (trltog) is done
+ * by lsmsx_tmats_from_single_site_canted when the store is filled on the
+ * device. */
 int lsmsx_single_site_nonrel(lsmsx_ctx *ctx, int nrelv, int lmax, int npot, int iprpts,
                              const double *vr, const double *r_mesh, const int *jmt,
                              const int *jws, const double *r_sph, int ne, const double *energy,
@@ -321,6 +322,19 @@ int lsmsx_green_mt_canted(lsmsx_ctx *ctx, int nsolve, int lmax, int iprpts, int 
                           const int *jws, const double *r_sph, const double *tau00_l,
                           const double *zlr, const double *jlr, const double *wxyz,
                           double *dos, double *green);
+
+/* Spin-canted variant of lsmsx_tmats_from_single_site: column
+ * store_of_atom[a] gets, at every energy slot e, the global-frame
+ * 2kkrsz x 2kkrsz t = trltog(t_up, t_dn) (Misc/trltog.f:1-52, as
+ * calculateSingleScattererSolution does, SingleSite/SingleSiteScattering.cpp:44-46)
+ * of the atom's spin-up and spin-down potentials pot_up[a], pot_dn[a] of the
+ * last single-site batch, with the atom's ubr, ubrd (4 complex each,
+ * AtomData::ubr/ubrd) at [8a .. 8a+7] doubles; leading dimension 2kkrsz.
+ * The store is zeroed first. */
+int lsmsx_tmats_from_single_site_canted(lsmsx_ctx *ctx, long ldTmatStore, int numStore,
+                                        long blkSizeTmatStore, int groupSize, int natom,
+                                        const int *store_of_atom, const int *pot_up,
+                                        const int *pot_dn, const double *ubr, const double *ubrd);
 
 #ifdef __cplusplus
 }

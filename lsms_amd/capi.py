@@ -27,6 +27,7 @@ EXPORTED = [
     "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune", "lsmsx_set_pivot_guard",
     "lsmsx_single_site_nonrel", "lsmsx_green_mt", "lsmsx_set_single_site_resident",
     "lsmsx_green_mt_resident", "lsmsx_tmats_from_single_site", "lsmsx_green_mt_canted",
+    "lsmsx_tmats_from_single_site_canted",
 ]
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
@@ -77,6 +78,8 @@ def load():
     L.lsmsx_tau00_adjoint.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                       _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_set_single_site_resident.argtypes = [_vp, C.c_int]
+    L.lsmsx_tmats_from_single_site_canted.argtypes = [_vp, C.c_long, C.c_int, C.c_long, C.c_int,
+                                                      C.c_int, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_green_mt_canted.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                         _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_tmats_from_single_site.argtypes = [_vp, C.c_long, C.c_int, C.c_long, C.c_int, _vp, _vp]
@@ -304,6 +307,18 @@ class Context:
                                          _p(jm), _p(jw), _p(rs), _p(t), _p(z), _p(j), _p(w),
                                          _p(dos), _p(green) if with_green else None))
         return dos, green
+
+    def tmats_from_single_site_canted(self, ld, num_store, blk, group, store_of_atom, pot_up,
+                                      pot_dn, ubr, ubrd):
+        """Spin-canted store fill from the last single_site_nonrel batch:
+        trltog(t_up, t_dn) per atom with its (ubr, ubrd) (natom x 4 complex)."""
+        so = np.ascontiguousarray(store_of_atom, dtype=np.int32)
+        pu = np.ascontiguousarray(pot_up, dtype=np.int32)
+        pd = np.ascontiguousarray(pot_dn, dtype=np.int32)
+        u = np.ascontiguousarray(ubr, dtype=np.complex128).reshape(len(so), 4)
+        ud = np.ascontiguousarray(ubrd, dtype=np.complex128).reshape(len(so), 4)
+        _ck(load().lsmsx_tmats_from_single_site_canted(self._h, ld, num_store, blk, group, len(so),
+                                                       _p(so), _p(pu), _p(pd), _p(u), _p(ud)))
 
     def set_single_site_resident(self, on):
         """Keep the radial functions of later single_site_nonrel calls on the

@@ -59,6 +59,8 @@ __global__ void k_single_site(const double *, const double *, const SingleSitePo
 __global__ void k_green_mt_canted(const double2 *, const double2 *, const double2 *,
                                   const double2 *, const double *, const GreenPot *, const int *,
                                   int, int, int, double2 *, double2 *);
+__global__ void k_tmat_fill_canted(const double2 *, const int *, const double2 *, int, int, int,
+                                   int, long, long, double2 *);
 __global__ void k_tmat_fill(const double2 *, const int *, const int *, int, int, int, long, long,
                             double2 *);
 __global__ void k_green_mt(const double2 *, const double2 *, const double2 *, const double *,
@@ -231,6 +233,7 @@ struct lsmsx_ctx {
   int ss_resident = 0, ss_valid = 0, ss_ne = 0, ss_npot = 0, ss_lmax = 0, ss_iprpts = 0;
   int ss_have_matom = 0;  // ss_matom holds the last call's ss_ne x ss_npot x (ss_lmax+1)
   DevBuf<int> ss_storeof;
+  DevBuf<double2> ss_uu;
   std::vector<GreenPot> ss_gpot;
   std::vector<double> ss_hr, ss_hrsph;  // host copies of the last call's meshes and r_sph
   DevBuf<double2> gf_tau, gf_dos, gf_green, gf_w;
@@ -1223,6 +1226,7 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->gf_potof.release();
   c->gf_src.release();
   c->ss_storeof.release();
+  c->ss_uu.release();
   c->gf_tau.release();
   c->gf_dos.release();
   c->gf_green.release();
@@ -2010,6 +2014,57 @@ int lsmsx_green_mt_canted(lsmsx_ctx *c, int nsolve, int lmax, int iprpts, int np
     HIPCHK(hipMemcpyAsync(green, c->gf_green.p, (size_t)nsolve * 4 * iprpts * sizeof(double2),
                           hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int lsmsx_tmats_from_single_site_canted(lsmsx_ctx *c, long ldT, int numStore, long blk,
+                                        int groupSize, int natom, const int *store_of_atom,
+                                        const int *pot_up, const int *pot_dn, const double *ubr,
+                                        const double *ubrd) {
+  if (!c) return fail(-1, "null context");
+  if (!c->ss_have_matom) return fail(-3, "no single-site batch on the device");
+  if (natom <= 0 || !store_of_atom || !pot_up || !pot_dn || !ubr || !ubrd)
+    return fail(-2, "null argument or no atoms");
+  const int ne = c->ss_ne, npot = c->ss_npot, kkr = (c->ss_lmax + 1) * (c->ss_lmax + 1);
+  if (ldT < blk * groupSize || numStore <= 0 || blk <= 0 || groupSize <= 0)
+    return fail(-2, "inconsistent t-matrix store dimensions");
+  if (ne > groupSize) return fail(-2, "more energies than store slots");
+  if ((long)4 * kkr * kkr > blk) return fail(-2, "blkSizeTmatStore below (2 kkrsz)^2");
+  std::vector<int> map(3 * (size_t)natom);
+  std::vector<double2> uu(8 * (size_t)natom);
+  for (int a = 0; a < natom; a++) {
+    if (store_of_atom[a] < 0 || store_of_atom[a] >= numStore)
+      return fail(-2, "store column out of range");
+    if (pot_up[a] < 0 || pot_up[a] >= npot || pot_dn[a] < 0 || pot_dn[a] >= npot)
+      return fail(-2, "potential index out of range");
+    map[3 * a] = store_of_atom[a];
+    map[3 * a + 1] = pot_up[a];
+    map[3 * a + 2] = pot_dn[a];
+    for (int k = 0; k < 4; k++) {
+      uu[8 * a + k] = make_double2(ubr[2 * (4 * a + k)], ubr[2 * (4 * a + k) + 1]);
+      uu[8 * a + 4 + k] = make_double2(ubrd[2 * (4 * a + k)], ubrd[2 * (4 * a + k) + 1]);
+    }
+  }
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->tmat.ensure((size_t)ldT * numStore)) return r;
+  if (int r = c->ss_storeof.ensure(3 * (size_t)natom)) return r;
+  if (int r = c->ss_uu.ensure(8 * (size_t)natom)) return r;
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemsetAsync(c->tmat.p, 0, sizeof(double2) * (size_t)ldT * numStore, st));
+  HIPCHK(hipMemcpyAsync(c->ss_storeof.p, map.data(), map.size() * sizeof(int),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ss_uu.p, uu.data(), uu.size() * sizeof(double2), hipMemcpyHostToDevice,
+                        st));
+  const long n = (long)ne * natom * 4 * kkr * kkr;
+  hipLaunchKernelGGL(k_tmat_fill_canted, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     c->ss_matom.p, c->ss_storeof.p, c->ss_uu.p, ne, npot, natom, c->ss_lmax, ldT,
+                     blk, c->tmat.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  c->ldT = ldT;
+  c->blk = blk;
+  c->nstore = numStore;
+  c->group = groupSize;
   return 0;
 }
 

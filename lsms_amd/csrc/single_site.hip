@@ -954,4 +954,41 @@ __global__ void k_tmat_fill(const double2 *__restrict__ matom, const int *__rest
   store[(long)store_of[p] * ldT + (long)e * blk + (long)spin_of[p] * kkr * kkr + idx] = v;
 }
 
+// Spin-canted t-matrix store from a single-site batch: the global-frame
+// t_g = trltog(t_up, t_dn) of each atom (Misc/trltog.f, as
+// calculateSingleScattererSolution, SingleSiteScattering.cpp:44-46): block
+// (ri, ci) of the 2kkr x 2kkr store block is ud[ri] u[2ci] t_up +
+// ud[2+ri] u[2ci+1] t_dn, diagonal; stored at [s*ldT + e*blk + i + j*2kkr].
+// One thread per entry; the caller zeroes the store first.
+__global__ void k_tmat_fill_canted(const double2 *__restrict__ matom,
+                                   const int *__restrict__ atom_map, const double2 *__restrict__ uu,
+                                   int ne, int npot, int natom, int lmax, long ldT, long blk,
+                                   double2 *__restrict__ store) {
+  const int nl = lmax + 1, kkr = nl * nl, n2 = 2 * kkr;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ne * natom * n2 * n2) return;
+  const int idx = (int)(t % (n2 * n2));
+  const long q = t / (n2 * n2);
+  const int a = (int)(q % natom), e = (int)(q / natom);
+  const int i = idx % n2, j = idx / n2;
+  const int ri = i / kkr, ci = j / kkr, ii = i % kkr, jj = j % kkr;
+  double2 v = make_double2(0.0, 0.0);
+  if (ii == jj) {
+    int l = 0;
+    while ((l + 1) * (l + 1) <= ii) l++;
+    const int s_col = atom_map[3 * a], p_up = atom_map[3 * a + 1], p_dn = atom_map[3 * a + 2];
+    const double2 mu = matom[((long)e * npot + p_up) * nl + l];
+    const double2 md = matom[((long)e * npot + p_dn) * nl + l];
+    const zd tu = 1.0 / mk(mu.x, mu.y), td = 1.0 / mk(md.x, md.y);
+    const double2 *u = uu + (long)a * 8, *ud = u + 4;
+    const zd ca = mk(ud[ri].x, ud[ri].y) * mk(u[2 * ci].x, u[2 * ci].y);
+    const zd cb = mk(ud[2 + ri].x, ud[2 + ri].y) * mk(u[2 * ci + 1].x, u[2 * ci + 1].y);
+    const zd g = ca * tu + cb * td;
+    v = make_double2(g.x, g.y);
+    store[(long)s_col * ldT + (long)e * blk + i + (long)j * n2] = v;
+    return;
+  }
+  store[(long)atom_map[3 * a] * ldT + (long)e * blk + i + (long)j * n2] = v;
+}
+
 }  // namespace lsmsx

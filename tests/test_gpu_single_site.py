@@ -346,3 +346,57 @@ def test_green_mt_canted_matches_oracle(ctx):
         nj = jw[p]
         assert np.abs(green[b, :, :nj] - g_or).max() <= 1e-12 * np.abs(g_or).max()
         assert not np.any(green[b, :, nj:])
+
+
+def test_store_filled_on_device_spin_canted():
+    """Spin-canted store filled on the device (trltog of each species'
+    spin-up and spin-down t) gives the tau00 of the same global-frame t built
+    on the host (configs.local_to_global_t) and uploaded, and meets LAPACK."""
+    import scipy.linalg as sla
+    from tests import adjoint_ref as R
+    e, w = CF.contour_semicircle(-0.5, 0.7, 6)
+    s = CF.make_system("canted", "bcc", 5.42, (1, 1, 1), 6.0, 3, e, w, nspecies=2, seed=7,
+                       n_spin_cant=2, n_spin_pola=2)
+    pots = [CF.model_potential(26 + 3 * sp, 2.3, vmtz=-1.2 + 0.1 * isp)
+            for sp in range(2) for isp in range(2)]
+    rr = np.stack([p[0] for p in pots])
+    vr = np.stack([p[1] for p in pots])
+    jm = np.array([p[2] for p in pots])
+    jw = np.array([p[3] for p in pots])
+    rs = np.array([p[4] for p in pots])
+    prels = np.array([s.prel(i) for i in range(len(s.energies))])
+    uu = [CF.spin_rotation(s.evecs[sp]) for sp in range(2)]
+    c = capi.Context(0)
+    c.load_system(s)
+    _, tmat, _, _, rc = c.single_site_nonrel(0, 3, vr, rr, jm, jw, rs, s.energies, prels,
+                                             radial=False)
+    assert rc == 0
+    c.tmats_from_single_site_canted(s.tmatStore.shape[0], s.tmatStore.shape[1],
+                                    s.blkSizeTmatStore, len(s.energies), [0, 1], [0, 2], [1, 3],
+                                    np.array([u for u, _ in uu]), np.array([ud for _, ud in uu]))
+    store = np.zeros_like(s.tmatStore)
+    for sp in range(2):
+        for ie in range(len(s.energies)):
+            tg = CF.local_to_global_t(tmat[ie, 2 * sp], tmat[ie, 2 * sp + 1], s.evecs[sp])
+            base = ie * s.blkSizeTmatStore
+            store[base:base + 1024, sp] = tg.flatten(order="F")
+    s.tmatStore = store
+    na = len(s.atoms)
+    atoms = np.repeat(np.arange(na), 3)
+    ies = np.tile([0, 2, 5], na)
+    pr = [s.prel(i) for i in ies]
+    tau_dev, info = c.tau00_batched(atoms, ies, pr, 32)
+    c.close()
+    c2 = capi.Context(0)
+    c2.load_system(s)
+    tau_host, info2 = c2.tau00_batched(atoms, ies, pr, 32)
+    c2.close()
+    assert np.all(info == 0) and np.all(info2 == 0)
+    for b in range(len(atoms)):
+        assert np.linalg.norm(tau_dev[b] - tau_host[b]) <= 1e-13 * np.linalg.norm(tau_host[b])
+    for b in (0, len(atoms) - 1):
+        m = R.oracle_m(s, atoms[b], ies[b])
+        rhs = np.zeros((m.shape[0], 32), dtype=np.complex128)
+        rhs[:32] = R.oracle_t0(s, atoms[b], ies[b])
+        ref = sla.solve(m, rhs)[:32]
+        assert np.linalg.norm(tau_dev[b] - ref) / np.linalg.norm(ref) < 1e-10

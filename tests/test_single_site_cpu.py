@@ -261,3 +261,19 @@ def test_green_mt_canted_off_diagonal_blocks():
     assert abs(dos[1] - (d2 + d3)) <= 1e-13 * abs(d2 + d3)
     assert abs(dos[2] - (-1j * d2 + 1j * d3)) <= 1e-13 * abs(d3 - d2)
     assert abs(dos[3] - (d1 - d4)) <= 1e-13 * abs(d1 - d4)
+
+
+def test_local_to_global_t_inverts_trgtol():
+    """The global-frame t of trltog.f (configs.local_to_global_t, the layout
+    the spin-canted store fill writes) taken back to the local frame by the
+    oracle's trgtol (pinned to the reference's) returns diag(t_up, t_dn)."""
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        tu = np.diag(rng.normal(size=16) + 1j * rng.normal(size=16))
+        td = np.diag(rng.normal(size=16) + 1j * rng.normal(size=16))
+        ev = rng.normal(size=3)
+        ev /= np.linalg.norm(ev)
+        u, ud = O.spin_trafo(ev)
+        ml = O.trgtol(16, 16, u, ud, CF.local_to_global_t(tu, td, ev))
+        assert np.abs(ml[:, :, 0] - tu).max() < 1e-14 and np.abs(ml[:, :, 3] - td).max() < 1e-14
+        assert np.abs(ml[:, :, 1]).max() < 1e-14 and np.abs(ml[:, :, 2]).max() < 1e-14
