@@ -77,6 +77,20 @@ constexpr int kBsSteps = 11;
 constexpr int kBsTarget = 7;
 constexpr double kBsEps = 1.0e-12;
 
+// a/b from the hardware reciprocal refined by two Newton steps and one
+// residual correction (about 1 ulp for the well-scaled operands of the
+// radial equation); the IEEE division sequence is several dependent
+// instructions longer, and these divisions sit on each thread's latency chain
+__device__ inline double fdiv(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  const double q = a * r;
+  return fma(r, fma(-b, q, a), q);
+}
+
 struct RhsPar {
   double er, ei, b, allp1;
 };
@@ -84,14 +98,14 @@ struct RhsPar {
 // derivatives of (Re g, Im g, Re f, Im f) (dfv_new.f, nv = 4, eunit = 1)
 __device__ inline void rhs4(double r, const double *y, double *dy, double rn, const double *pc,
                             const RhsPar &p) {
-  const double s1or = 1.0 / r, s1or2 = s1or * s1or;
+  const double s1or = fdiv(1.0, r), s1or2 = s1or * s1or;
   const double r1 = r - rn;
-  const double v = (pc[0] * r1 + pc[1]) / (1.0 + pc[2] * r1 * r1) + pc[3];
+  const double v = fdiv(pc[0] * r1 + pc[1], 1.0 + pc[2] * r1 * r1) + pc[3];
   const double s2vmer = v * s1or - p.er;
   const double frelr = 1.0 - p.b * s2vmer;
   const double s2vmei = -p.ei;
   const double freli = -p.b * s2vmei;
-  const double den = p.allp1 / (frelr * frelr + freli * freli) * s1or2;
+  const double den = fdiv(p.allp1, frelr * frelr + freli * freli) * s1or2;
   const double facr = den * frelr + s2vmer;
   const double faci = -den * freli + s2vmei;
   dy[0] = frelr * y[2] - freli * y[3] + s1or * y[0];
@@ -317,12 +331,12 @@ __device__ void fit4(const double *r, int nr, int i, int i1, int i2, double fi1,
                      double fip1, double fi2, double *c) {
   const int ip1 = i + 1;
   const double dr = r[ip1] - r[i];
-  c[2] = (fip1 - fi) / dr;
+  c[2] = fdiv(fip1 - fi, dr);
   c[0] = fi;
   c[1] = 0.0;
   c[3] = 0.0;
   if (i1 != i2) {
-    c[1] = (fi2 - fip1) / (r[i2] - r[ip1]) - (fi1 - fi) / (r[i1] - r[i]);
+    c[1] = fdiv(fi2 - fip1, r[i2] - r[ip1]) - fdiv(fi1 - fi, r[i1] - r[i]);
     if (c[1] * (fi2 - fi1) * (r[i2] - r[i1]) > 0.0) c[1] = -c[1];
     double h1 = r[i1] - r[i];
     double d0 = fi1 - fi;
@@ -338,12 +352,12 @@ __device__ void fit4(const double *r, int nr, int i, int i1, int i2, double fi1,
     if (gj == 0.0) {
       c[3] = 0.0;
     } else {
-      c[0] = (a1 * e22 - a3 * e12) / gj;
-      c[3] = (a1 * h2 - a3 * h1) / gj;
+      c[0] = fdiv(a1 * e22 - a3 * e12, gj);
+      c[3] = fdiv(a1 * h2 - a3 * h1, gj);
     }
     gj = c[3] * dr * dr;
     if (gj > -4.0 && fabs(gj) > 1e-14 && gj < 1e14) {
-      c[0] = c[0] / c[3];
+      c[0] = fdiv(c[0], c[3]);
     } else {
       c[0] = fi;
       c[3] = 0.0;
@@ -364,7 +378,7 @@ __device__ void fit4(const double *r, int nr, int i, int i1, int i2, double fi1,
     const double a1 = fj1 - fi - c[2] * h1;
     h1 = h1 * (r[ip1] - r[j1]);
     if (fj1 != fj) {
-      c[3] = -a1 / ((fj1 - fj) * h1);
+      c[3] = fdiv(-a1, (fj1 - fj) * h1);
       const double gj = c[3] * dr * dr;
       if (gj <= -4.0 || fabs(gj) <= 1e-14 || gj >= 1e14) {
         c[0] = fi;
@@ -649,10 +663,10 @@ __device__ zd cnewint_step(const double *rs, const double *fr, const double *fi,
     gj = h2 * (dr - h2);
     double c1 = (cr[2] - cr[1]) * h2 + cr[0];
     double c2 = cr[3] * gj;
-    gr = gr - fj * c1 * c2 / (1.0 + c2);
+    gr = gr - fdiv(fj * c1 * c2, 1.0 + c2);
     c1 = (ci[2] - ci[1]) * h2 + ci[0];
     c2 = ci[3] * gj;
-    gi = gi - fj * c1 * c2 / (1.0 + c2);
+    gi = gi - fdiv(fj * c1 * c2, 1.0 + c2);
   }
   return mk(2.0 * dr * gr, 2.0 * dr * gi);
 }
