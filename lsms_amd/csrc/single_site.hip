@@ -118,7 +118,7 @@ __device__ inline void rhs4(double r, const double *y, double *dy, double rn, co
 __device__ inline void midpoint4(double x0, double x1, const double *y0, double *y1, int steps,
                                  double rn, const double *pc, const RhsPar &p) {
   double y2[4], dy[4];
-  const double h = (x1 - x0) / (double)steps, h2 = 2.0 * h;
+  const double h = fdiv(x1 - x0, (double)steps), h2 = 2.0 * h;
   double x = x0;
   rhs4(x, y0, dy, rn, pc, p);
 #pragma unroll
@@ -164,7 +164,7 @@ __device__ int bs_interval(double x0, double x1, double *y, double *dy, double r
       if (err > kBsEps) {
         double y1[4];
         midpoint4(x, x + h, y, y1, kSeq[i], rn, pc, p);
-        hh[i] = h / (double)kSeq[i];
+        hh[i] = fdiv(h, (double)kSeq[i]);
         double e = 0.0, mag = 0.0;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
@@ -172,8 +172,8 @@ __device__ int bs_interval(double x0, double x1, double *y, double *dy, double r
           row[0][d] = y1[d];
 #pragma unroll
           for (int k = 1; k <= i; k++) {
-            const double rr = hh[i - k] / hh[i];
-            const double den = 1.0 / (rr * rr - 1.0);
+            const double rr = fdiv(hh[i - k], hh[i]);
+            const double den = fdiv(1.0, rr * rr - 1.0);
             const double keep = row[k][d];  // T[i-1][k] (unused when k == i)
             row[k][d] = row[k - 1][d] + (row[k - 1][d] - old) * den;
             if (k == i) e += fabs(row[i][d] - old);  // old = T[i-1][i-1]
@@ -183,7 +183,7 @@ __device__ int bs_interval(double x0, double x1, double *y, double *dy, double r
         }
         if (i == 0) e = 4.0;
         if (mag == 0.0) mag = 1.0e-15;
-        err = e / mag;
+        err = fdiv(e, mag);
         step = i + 1;
         last = i;
       }
