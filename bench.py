@@ -162,7 +162,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by independent (atom, energy) solves; see DESIGN.md)",
+              "(the path shards by independent (atom, energy) solves, one batch per GPU)",
               file=sys.stderr)
         sys.exit(2)
 
