@@ -15,6 +15,7 @@ SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblsmsx.so")
 DRIVER = os.path.join(HERE, "_build", "host_mirror_driver")
+SS_DRIVER = os.path.join(HERE, "_build", "single_site_driver")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -64,6 +65,11 @@ def build(verbose=False):
     drv = os.path.join(os.path.dirname(HERE), "tests", "cpp", "host_mirror_driver.cpp")
     if os.path.exists(drv):
         _run(["g++", "-O2", "-std=c++17", "-I/opt/rocm/include", drv, "-o", DRIVER,
+              f"-L{HERE}", "-llsmsx", "-Wl,-rpath,$ORIGIN/..",
+              "-L/opt/rocm/lib", "-lamdhip64"])
+    drv2 = os.path.join(os.path.dirname(HERE), "tests", "cpp", "single_site_driver.cpp")
+    if os.path.exists(drv2):
+        _run(["g++", "-O2", "-std=c++17", "-I/opt/rocm/include", drv2, "-o", SS_DRIVER,
               f"-L{HERE}", "-llsmsx", "-Wl,-rpath,$ORIGIN/..",
               "-L/opt/rocm/lib", "-lamdhip64"])
     return LIB

@@ -15,6 +15,8 @@
 //   atom.numLIZ, atom.LIZPos(d, i), atom.LIZlmax[i], atom.LIZStoreIdx[i],
 //   atom.nrmat, atom.kkrsz, atom.ubr[4], atom.ubrd[4]   (AtomData.hpp:441-467)
 //   m(i, j), m.l_dim(), m.resize(n, n)                  (include/Matrix.hpp:126-148)
+//   lsms.nrelv, lsms.mtasa, atom.r_mesh, atom.jmt, atom.jws, atom.rInscribed,
+//   atom.rws, atom.lmax, vr(j, is)                      (single-site solve)
 //
 // Error behaviour: like the reference's solvers, a singular pivot (info != 0)
 // is printed and the computation continues (zblock_lu_cpp.cpp:58-68); a
@@ -244,6 +246,82 @@ void calculateTauFullMI355X(LSMS &lsms, Atom &atom, int localAtomIndex, int ispi
                        &info),
         "lsmsx_tau_full");
   if (info != 0) std::printf("lsmsx calculateTauFull: singular pivot info=%d\n", info);
+}
+
+// Single-site t-matrices of every local atom at the energies of one group,
+// written into local.tmatStore as solveSingleScatterers does for each energy
+// (Main/solveSingleScatterers.cpp:17-45, the omp loop of
+// energyContourIntegration.cpp:426-436): one batched GPU call replaces the
+// per-atom, per-spin single_scatterer_nonrel_ calls
+// (SingleSite/SingleSiteScattering.cpp:6-54).  Column i, slot iie gets atom
+// i's t: the spin blocks one after the other (collinear), or the
+// global-frame trltog(t_up, t_dn) for n_spin_cant = 2.  vr[i] is atom i's
+// potential Matrix (r_mesh.size() x n_spin_pola); all atoms share one lmax
+// and one mesh length.  Returns matom per (energy, atom, spin, l) for callers
+// that keep it (solution.matom).
+template <class LSMS, class Local, class VR>
+std::vector<cplx> solveSingleScatterersMI355X(LSMS &lsms, Local &local, std::vector<VR> &vr,
+                                              const std::vector<cplx> &energies) {
+  Runtime &rt = runtime();
+  const int nat = local.num_local, nsp = lsms.n_spin_pola, ne = (int)energies.size();
+  if (nat <= 0 || ne <= 0) return {};
+  const int lmax = local.atom[0].lmax, iprpts = (int)local.atom[0].r_mesh.size();
+  const int kkr = (lmax + 1) * (lmax + 1), nl = lmax + 1;
+  std::vector<double> rv, rm, rsph;
+  std::vector<int> jmt, jws;
+  for (int i = 0; i < nat; i++) {
+    auto &a = local.atom[i];
+    if (a.lmax != lmax || (int)a.r_mesh.size() != iprpts)
+      throw std::runtime_error("solveSingleScatterersMI355X: atoms differ in lmax or mesh length");
+    for (int is = 0; is < nsp; is++) {
+      for (int j = 0; j < iprpts; j++) rv.push_back(vr[i](j, is));
+      rm.insert(rm.end(), a.r_mesh.begin(), a.r_mesh.end());
+      jmt.push_back(a.jmt);
+      jws.push_back(a.jws);
+      rsph.push_back(lsms.mtasa > 0 ? a.rws : a.rInscribed);
+    }
+  }
+  // prel as solveSingleScatterers forms it (c = 2/alpha in Rydberg units)
+  const double c = 2.0 * 137.035999206, c2inv = 1.0 / (c * c);
+  std::vector<cplx> prel(ne);
+  for (int e = 0; e < ne; e++)
+    prel[e] = lsms.nrelv > 0 ? std::sqrt(energies[e])
+                             : std::sqrt(energies[e] * (1.0 + energies[e] * c2inv));
+  const int npot = nat * nsp;
+  std::vector<cplx> matom((size_t)ne * npot * nl), tmat((size_t)ne * npot * kkr * kkr);
+  check(lsmsx_single_site_nonrel(rt.ctx, lsms.nrelv, lmax, npot, iprpts, rv.data(), rm.data(),
+                                 jmt.data(), jws.data(), rsph.data(), ne,
+                                 reinterpret_cast<const double *>(energies.data()),
+                                 reinterpret_cast<const double *>(prel.data()),
+                                 reinterpret_cast<double *>(matom.data()),
+                                 reinterpret_cast<double *>(tmat.data()), nullptr, nullptr),
+        "lsmsx_single_site_nonrel");
+  const long blk = local.blkSizeTmatStore;
+  auto t_of = [&](int e, int i, int is) { return &tmat[(((size_t)e * npot) + i * nsp + is) * kkr * kkr]; };
+  for (int i = 0; i < nat; i++)
+    for (int e = 0; e < ne; e++) {
+      cplx *dst = &local.tmatStore((size_t)e * blk, i);
+      if (lsms.n_spin_cant == 2) {  // trltog.f: mg(2kkr x 2kkr) from the two local t
+        const cplx *u = local.atom[i].ubr, *ud = local.atom[i].ubrd;
+        const cplx *t1 = t_of(e, i, 0), *t2 = t_of(e, i, nsp - 1);
+        const int n2 = 2 * kkr;
+        for (int q = 0; q < n2 * n2; q++) dst[q] = 0.0;
+        for (int ri = 0; ri < 2; ri++)
+          for (int ci = 0; ci < 2; ci++) {
+            const cplx a = ud[ri] * u[2 * ci], b = ud[2 + ri] * u[2 * ci + 1];
+            for (int cc = 0; cc < kkr; cc++)
+              for (int rr = 0; rr < kkr; rr++)
+                dst[(ri * kkr + rr) + (size_t)(ci * kkr + cc) * n2] =
+                    a * t1[rr + (size_t)cc * kkr] + b * t2[rr + (size_t)cc * kkr];
+          }
+      } else {
+        for (int is = 0; is < nsp; is++) {
+          const cplx *t = t_of(e, i, is);
+          for (int q = 0; q < kkr * kkr; q++) dst[(size_t)is * kkr * kkr + q] = t[q];
+        }
+      }
+    }
+  return matom;
 }
 
 }  // namespace lsmsx

@@ -45,6 +45,11 @@ struct AtomData {
   std::vector<int> LIZlmax, LIZStoreIdx;
   int nrmat = 0, lmax = 0, kkrsz = 0;
   Complex ubr[4], ubrd[4];        // spin rotation u, u^dagger (spin_trafo.f)
+  // radial mesh of the single-site solve (AtomData.hpp: r_mesh, jmt, jws,
+  // rInscribed, rws)
+  std::vector<Real> r_mesh;
+  int jmt = 0, jws = 0;
+  Real rInscribed = 0, rws = 0;
 };
 
 struct LocalTypeInfo {
@@ -58,6 +63,7 @@ struct LocalTypeInfo {
 struct LSMSSystemParameters {
   int n_spin_cant = 1, n_spin_pola = 1;
   int maxlmax = 3;
+  int nrelv = 0, mtasa = 0;       // SystemParameters.hpp: relativity switch, ASA flag
 };
 
 }  // namespace types

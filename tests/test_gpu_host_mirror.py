@@ -105,3 +105,56 @@ def test_host_mirror_matches_oracle(name, s):
     tf = out[pos:pos + n * n].reshape((n, n), order="F")
     ref_f = R.tau_full(s, 0, 0)
     assert np.linalg.norm(tf - ref_f) <= 1e-10 * np.linalg.norm(ref_f)
+
+
+@pytest.mark.parametrize("nsc,nsp", [(1, 1), (1, 2), (2, 2)])
+def test_single_site_host_mirror_fills_store(nsc, nsp):
+    """solveSingleScatterersMI355X driven from C++ as energyContourIntegration
+    would: every atom's slot of the t-matrix store equals the oracle's
+    single-site t (single_scatterer_nonrel.f) laid out as
+    solveSingleScatterers does -- spin blocks one after the other, or
+    trltog(t_up, t_dn) for spin canting -- to 1e-10."""
+    if not os.path.exists(B.SS_DRIVER):
+        pytest.fail("single_site_driver not built (python -m lsms_amd.build)")
+    e, _ = CF.contour_semicircle(-0.4, 0.7, 4)
+    nat, lmax, kkr = 2, 3, 16
+    blk = (2 * kkr) ** 2
+    ldim = blk * len(e)
+    rng = np.random.default_rng(4)
+    atoms = []
+    for i in range(nat):
+        pots = [CF.model_potential(26 + 3 * i, 2.3, vmtz=-1.2 + 0.1 * s) for s in range(nsp)]
+        ev = rng.normal(size=3)
+        ev /= np.linalg.norm(ev)
+        atoms.append((pots, ev))
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            iprpts = len(atoms[0][0][0][0])
+            np.array([0, 0, nsc, nsp, nat, lmax, iprpts, len(e), blk, ldim],
+                     dtype=np.int32).tofile(f)
+            e.astype(np.complex128).tofile(f)
+            for pots, ev in atoms:
+                r, _, jmt, jws, rs = pots[0]
+                r.astype(np.float64).tofile(f)
+                np.array([jmt, jws], dtype=np.int32).tofile(f)
+                np.array([rs, r[jws - 1]], dtype=np.float64).tofile(f)
+                u, ud = O.spin_trafo(ev)
+                u.astype(np.complex128).tofile(f)
+                ud.astype(np.complex128).tofile(f)
+                np.concatenate([p[1] for p in pots]).astype(np.float64).tofile(f)
+        r = subprocess.run([B.SS_DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        store = np.fromfile(fout, dtype=np.complex128).reshape((ldim, nat), order="F")
+    for i, (pots, ev) in enumerate(atoms):
+        for ie, E in enumerate(e):
+            pr = CF.prel_of(E)
+            ts = [O.single_scatterer_nonrel(0, 3, E, pr, p[1], p[0], p[2], p[3], p[4])[1]
+                  for p in pots]
+            if nsc == 2:
+                ref = CF.local_to_global_t(ts[0], ts[1], ev).flatten(order="F")
+            else:
+                ref = np.concatenate([t.flatten(order="F") for t in ts])
+            got = store[ie * blk: ie * blk + len(ref), i]
+            assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max(), (i, ie)
+            assert not np.any(store[ie * blk + len(ref):(ie + 1) * blk, i])
