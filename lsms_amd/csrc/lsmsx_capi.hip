@@ -61,6 +61,8 @@ __global__ void k_green_mt_canted(const double2 *, const double2 *, const double
                                   int, int, int, double2 *, double2 *);
 __global__ void k_tmat_fill_canted(const double2 *, const int *, const double2 *, int, int, int,
                                    int, long, long, double2 *);
+__global__ void k_jlr_scale(const double *, const SingleSitePot *, const double2 *,
+                            const double2 *, int, int, int, int, double2 *);
 __global__ void k_tmat_fill(const double2 *, const int *, const int *, int, int, int, long, long,
                             double2 *);
 __global__ void k_green_mt(const double2 *, const double2 *, const double2 *, const double *,
@@ -1756,12 +1758,20 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *c, int nrelv, int lmax, int npot, int ip
   hipLaunchKernelGGL(k_fitpot, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, st, c->ss_r.p,
                      c->ss_v.p, c->ss_iend.p, iprpts, npot, c->ss_coef.p);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_single_site, dim3((unsigned)((ntask + 63) / 64)), dim3(64), 0, st,
+  const size_t nthr = (want_j ? 2 : 1) * ntask;  // inward integrations in threads of their own
+  hipLaunchKernelGGL(k_single_site, dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st,
                      c->ss_r.p, c->ss_coef.p, c->ss_pot.p, c->ss_e.p, c->ss_prel.p, npot, ne,
                      lmax, iprpts, nrelv == 0 ? 1 : 0, c->ss_matom.p,
                      want_z ? c->ss_zlr.p : nullptr, want_j ? c->ss_jlr.p : nullptr,
                      c->ss_status.p);
   HIPCHK(hipGetLastError());
+  if (want_j) {
+    const size_t npt = ntask * iprpts;
+    hipLaunchKernelGGL(k_jlr_scale, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, st,
+                       c->ss_r.p, c->ss_pot.p, c->ss_prel.p, c->ss_matom.p, npot, ne, lmax, iprpts,
+                       c->ss_jlr.p);
+    HIPCHK(hipGetLastError());
+  }
   int status = 0;
   HIPCHK(hipMemcpyAsync(matom, c->ss_matom.p, ntask * sizeof(double2), hipMemcpyDeviceToHost, st));
   if (zlr)

@@ -19,8 +19,10 @@
 // k_single_site gives one thread to each (energy, potential, l) and keeps the
 // whole extrapolation tableau in registers (its loops are unrolled over the
 // fixed step sequence, so no per-thread array is indexed at run time).  Each
-// thread writes its own zlr/jlr column directly; the regular solution is
-// kept raw in the zlr column until the matching gives its normalisation.
+// thread writes its own zlr column directly; the regular solution is kept
+// raw in the zlr column until the matching gives its normalisation.  When
+// jlr is asked for, a second set of threads runs the inward integrations at
+// the same time, and k_jlr_scale applies their 1/matom factor afterwards.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -467,9 +469,17 @@ __global__ void __launch_bounds__(64) k_single_site(
     const double2 *__restrict__ prel, int npot, int ne, int lmax, int iprpts, int srel,
     double2 *__restrict__ matom_out, double2 *__restrict__ zlr, double2 *__restrict__ jlr,
     int *__restrict__ status) {
+  // threads [0, ntask): outward integration, matching, zlr and the jlr
+  // beyond the sphere; with jlr, threads [ntask, 2 ntask) run the inward
+  // (irregular) integration of the same task at the same time, since it
+  // starts from the Hankel function at r_sph and does not need the regular
+  // solution.  k_jlr_scale divides their raw values by matom afterwards.
   const int nl = lmax + 1;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)ne * npot * nl) return;
+  const long ntask = (long)ne * npot * nl;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (jlr ? 2 : 1) * ntask) return;
+  const bool inward = t >= ntask;
+  if (inward) t -= ntask;
   const int l = (int)(t % nl), p = (int)((t / nl) % npot), e = (int)(t / ((long)nl * npot));
   const SingleSitePot P = pots[p];
   const double *rad = rmesh + (long)p * iprpts;
@@ -519,6 +529,30 @@ __global__ void __launch_bounds__(64) k_single_site(
     y[2] = g2.x;
     y[3] = g2.y;
   }
+  if (inward) {  // irregular solution inwards from r_sph (rwave.f:181-223), raw
+    const zd kar = ka * P.r_sph;
+    const BesselPick b = bessel_pick(kar, l);
+    const zd hl = b.hm_l * b.eiz / ka;
+    zd dh;
+    if (l == 0) {
+      dh = -(b.hm_o * b.eiz);
+    } else {
+      const zd cd = (double)(l + 1) / kar;
+      dh = (b.hm_o - b.hm_l * cd) * b.eiz;
+    }
+    y[0] = hl.x;
+    y[1] = hl.y;
+    y[2] = dh.x;
+    y[3] = dh.y;
+    rhs4(P.r_sph, y, dy, P.r_sph, cf + 4 * (jmt - 1), rp);
+    for (int j = jmt - 2; j >= 0; j--) {
+      const double rfrom = (j + 1 == jmt - 1) ? P.r_sph : rad[j + 1];
+      rc |= bs_interval(rfrom, rad[j], y, dy, rad[j], cf + 4 * j, rp);
+      if (j < jws) jcol[j] = make_double2(y[0], y[1]);
+    }
+    if (rc) atomicOr(status, 1);
+    return;
+  }
   rhs4(rad[0], y, dy, rad[0], cf, rp);
 
   // outward integration; the raw regular solution goes to the zlr column and
@@ -566,32 +600,6 @@ __global__ void __launch_bounds__(64) k_single_site(
     }
   }
 
-  // irregular solution inwards from r_sph (rwave.f:181-223); it feeds only
-  // jlr, so a call without radial functions skips it
-  if (jcol) {
-    const zd hl = b.hm_l * b.eiz / ka;
-    zd dh;
-    if (l == 0) {
-      dh = -(b.hm_o * b.eiz);
-    } else {
-      const zd cd = (double)(l + 1) / kar;
-      dh = (b.hm_o - b.hm_l * cd) * b.eiz;
-    }
-    y[0] = hl.x;
-    y[1] = hl.y;
-    y[2] = dh.x;
-    y[3] = dh.y;
-    rhs4(r_sph, y, dy, r_sph, cf + 4 * (jmt - 1), rp);
-    for (int j = jmt - 2; j >= 0; j--) {
-      const double rfrom = (j + 1 == jmt - 1) ? r_sph : rad[j + 1];
-      rc |= bs_interval(rfrom, rad[j], y, dy, rad[j], cf + 4 * j, rp);
-      if (jcol && j < jws) {
-        const zd q = (iprel * mk(y[0], y[1])) / (rad[j] * matom);
-        jcol[j] = make_double2(q.x, q.y);
-      }
-    }
-  }
-
   // outside the sphere: Bessel and Hankel forms (rwave.f:225-240)
   for (int j = jmt - 1; j < jws; j++) {
     kar = ka * rad[j];
@@ -608,6 +616,30 @@ __global__ void __launch_bounds__(64) k_single_site(
     }
   }
   if (rc) atomicOr(status, 1);
+}
+
+// jlr = i prel g_irr / (r matom) (semrel_c.f:214-217) for the points the
+// inward threads of k_single_site left raw (j < min(jmt-1, jws)); one
+// thread per (task, mesh point)
+__global__ void k_jlr_scale(const double *__restrict__ rmesh,
+                            const SingleSitePot *__restrict__ pots,
+                            const double2 *__restrict__ prel, const double2 *__restrict__ matom,
+                            int npot, int ne, int lmax, int iprpts, double2 *__restrict__ jlr) {
+  const int nl = lmax + 1;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ntask = (long)ne * npot * nl;
+  if (g >= ntask * iprpts) return;
+  const long t = g / iprpts;
+  const int j = (int)(g % iprpts);
+  const int p = (int)((t / nl) % npot), e = (int)(t / ((long)nl * npot));
+  const SingleSitePot P = pots[p];
+  const int jend = P.jmt - 1 < P.jws ? P.jmt - 1 : P.jws;
+  if (j >= jend) return;
+  const zd iprel = times_i(mk(prel[e].x, prel[e].y));
+  const zd m = mk(matom[t].x, matom[t].y);
+  const double2 v = jlr[t * iprpts + j];
+  const zd q = (iprel * mk(v.x, v.y)) / (rmesh[(long)p * iprpts + j] * m);
+  jlr[t * iprpts + j] = make_double2(q.x, q.y);
 }
 
 // ---------------------------------------------------------------------------
