@@ -41,3 +41,25 @@ def test_null_context_is_an_error_not_a_crash():
     lib = capi.load()
     assert lib.lsmsx_set_profiling(None, 1) < 0
     assert b"null" in lib.lsmsx_last_error()
+
+
+def test_single_site_and_green_entry_points_reject_a_null_context():
+    """The single-site, store-fill and Green-function entry points return an
+    error for a null context before touching the device (no GPU needed)."""
+    lib = capi.load()
+    calls = [
+        lambda: lib.lsmsx_single_site_nonrel(None, 0, 3, 1, 10, None, None, None, None, None, 1,
+                                             None, None, None, None, None, None),
+        lambda: lib.lsmsx_green_mt(None, 1, 3, 10, 1, 1, None, None, None, None, None, None, None,
+                                   None, None, None),
+        lambda: lib.lsmsx_green_mt_canted(None, 1, 3, 10, 1, None, None, None, None, None, None,
+                                          None, None, None, None, None),
+        lambda: lib.lsmsx_green_mt_resident(None, 1, 1, None, None, None, None, None),
+        lambda: lib.lsmsx_set_single_site_resident(None, 1),
+        lambda: lib.lsmsx_tmats_from_single_site(None, 1, 1, 1, 1, None, None),
+        lambda: lib.lsmsx_tmats_from_single_site_canted(None, 1, 1, 1, 1, 1, None, None, None,
+                                                        None, None),
+    ]
+    for call in calls:
+        assert call() < 0
+        assert b"null" in lib.lsmsx_last_error()
