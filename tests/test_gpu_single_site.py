@@ -400,3 +400,34 @@ def test_store_filled_on_device_spin_canted():
         rhs[:32] = R.oracle_t0(s, atoms[b], ies[b])
         ref = sla.solve(m, rhs)[:32]
         assert np.linalg.norm(tau_dev[b] - ref) / np.linalg.norm(ref) < 1e-10
+
+
+@pytest.mark.parametrize("lmax", [1, 2])
+def test_lower_lmax_matches_oracle(ctx, lmax):
+    """lmax = 1 and 2: matom, t, zlr, jlr and the muffin-tin DOS / green(r)
+    against the oracle (the kernels' channel loops stop at lmax)."""
+    rr, vr, jm, jw, rs, e, pr = CF.single_site_batch(nspecies=2, ne=4)
+    nl, kk = lmax + 1, (lmax + 1) ** 2
+    matom, tmat, zlr, jlr, rc = ctx.single_site_nonrel(0, lmax, vr, rr, jm, jw, rs, e, pr)
+    assert rc == 0 and matom.shape == (4, len(rr), nl) and tmat.shape == (4, len(rr), kk, kk)
+    for q in range(len(rr)):
+        n = jw[q]
+        for k in range(len(e)):
+            m, t, z, j, _ = O.single_scatterer_nonrel(0, lmax, e[k], pr[k], vr[q], rr[q], jm[q],
+                                                      jw[q], rs[q])
+            assert np.max(np.abs(matom[k, q] - m) / np.abs(m)) < TOL
+            assert np.abs(tmat[k, q] - t).max() <= TOL * np.abs(t).max()
+            for l in range(nl):
+                assert colerr(zlr[k, q, l, :n], z[:n, l]) < TOL
+                assert colerr(jlr[k, q, l, :n], j[:n, l]) < TOL
+    nsolve = len(rr) * len(e)
+    pot_of = np.tile(np.arange(len(rr)), len(e))
+    z = zlr.reshape(nsolve, nl, -1)
+    j = jlr.reshape(nsolve, nl, -1)
+    tau = tmat.reshape(nsolve, kk, kk)
+    dos, green = ctx.green_mt(lmax, tau, z, j, pot_of, rr, jm, jw, rs)
+    for b in range(0, nsolve, 3):
+        p = pot_of[b]
+        d_or, g_or, _, _ = O.green_mt(lmax, tau[b], z[b].T, j[b].T, rr[p], jm[p], jw[p], rs[p], 1)
+        assert abs(dos[b] - d_or) <= 1e-12 * abs(d_or)
+        assert np.abs(green[b, :jw[p]] - g_or).max() <= 1e-12 * np.abs(g_or).max()
