@@ -287,11 +287,13 @@ int lsmsx_set_single_site_resident(lsmsx_ctx *ctx, int on);
  * integrals per potential; the reference passes jws there for ASA
  * (energyContourIntegration.cpp:519).
  * Returns -3 when no such call precedes it (an lsmsx_green_mt call in between
- * reuses the buffers and also counts as none).  No reference counterpart
+ * reuses the buffers and also counts as none).  lmax and iprpts size the
+ * caller's tau00 (nsolve x kkrsz^2) and green (nsolve x iprpts) buffers and
+ * must equal those of that call (-2 otherwise).  No reference counterpart
  * (see lsmsx_set_single_site_resident). */
-int lsmsx_green_mt_resident(lsmsx_ctx *ctx, int nsolve, int n_spin_pola, const int *pair_of,
-                            const int *jmt_of_pot, const double *tau00, double *dos,
-                            double *green);
+int lsmsx_green_mt_resident(lsmsx_ctx *ctx, int nsolve, int lmax, int iprpts, int n_spin_pola,
+                            const int *pair_of, const int *jmt_of_pot, const double *tau00,
+                            double *dos, double *green);
 
 /* Fill the device t-matrix store from the last lsmsx_single_site_nonrel
  * batch without a host round trip: the store (same dimensions as

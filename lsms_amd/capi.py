@@ -83,7 +83,8 @@ def load():
     L.lsmsx_green_mt_canted.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                         _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_tmats_from_single_site.argtypes = [_vp, C.c_long, C.c_int, C.c_long, C.c_int, _vp, _vp]
-    L.lsmsx_green_mt_resident.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]
+    L.lsmsx_green_mt_resident.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
+                                          _vp, _vp, _vp]
     L.lsmsx_green_mt.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_single_site_nonrel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
@@ -104,6 +105,18 @@ def _ck(rc):
 
 def _p(a):
     return a.ctypes.data_as(_vp)
+
+
+def _lmax_of_tau(tau00, nsolve):
+    """lmax of a (nsolve, kkrsz, kkrsz) stack of tau00 blocks, kkrsz = (lmax+1)^2;
+    ValueError for any other shape."""
+    sh = np.shape(tau00)
+    if len(sh) != 3 or sh[0] != nsolve or sh[1] != sh[2]:
+        raise ValueError("tau00 must be (nsolve, kkrsz, kkrsz)")
+    nl = int(round(np.sqrt(sh[1])))
+    if nl * nl != sh[1] or not 1 <= nl <= 4:
+        raise ValueError("tau00 blocks must be (lmax+1)^2 square with lmax 0..3")
+    return nl - 1
 
 
 def compute_tables(lmax):
@@ -262,9 +275,13 @@ class Context:
         nsolve = z.shape[0]
         if z.shape != (nsolve, lmax + 1, iprpts) or j.shape != z.shape:
             raise ValueError("zlr/jlr must be (nsolve, lmax+1, iprpts)")
+        if _lmax_of_tau(tau00, nsolve) != lmax:
+            raise ValueError("tau00 must be (nsolve, (lmax+1)^2, (lmax+1)^2)")
         # column-major kkrsz x kkrsz blocks
         t = np.ascontiguousarray(np.swapaxes(np.asarray(tau00, dtype=np.complex128), 1, 2))
         po = np.ascontiguousarray(pot_of, dtype=np.int32)
+        if po.shape != (nsolve,):
+            raise ValueError("pot_of must hold one potential per solve")
         jm = np.ascontiguousarray(np.broadcast_to(jmt, (npot,)), dtype=np.int32)
         jw = np.ascontiguousarray(np.broadcast_to(jws, (npot,)), dtype=np.int32)
         rs = np.ascontiguousarray(np.broadcast_to(r_sph, (npot,)), dtype=np.float64)
@@ -295,9 +312,16 @@ class Context:
         nsolve = z.shape[0]
         if z.shape != (nsolve, 2, lmax + 1, iprpts) or j.shape != z.shape:
             raise ValueError("zlr/jlr must be (nsolve, 2, lmax+1, iprpts)")
+        kk = (lmax + 1) ** 2
+        if np.shape(tau00_l) != (nsolve, 4, kk, kk):
+            raise ValueError("tau00_l must be (nsolve, 4, (lmax+1)^2, (lmax+1)^2)")
+        if np.shape(wxyz) != (nsolve, 3, 4):
+            raise ValueError("wxyz must be (nsolve, 3, 4)")
         t = np.ascontiguousarray(np.swapaxes(np.asarray(tau00_l, dtype=np.complex128), 2, 3))
         w = np.ascontiguousarray(wxyz, dtype=np.complex128)
         po = np.ascontiguousarray(pot_of, dtype=np.int32)
+        if po.shape != (nsolve,):
+            raise ValueError("pot_of must hold one potential per solve")
         jm = np.ascontiguousarray(np.broadcast_to(jmt, (npot,)), dtype=np.int32)
         jw = np.ascontiguousarray(np.broadcast_to(jws, (npot,)), dtype=np.int32)
         rs = np.ascontiguousarray(np.broadcast_to(r_sph, (npot,)), dtype=np.float64)
@@ -330,14 +354,15 @@ class Context:
         """green_mt on the radial functions left on the device by the last
         single_site_nonrel call (residency on); pair_of[b] = e*npot + p;
         jmt_of_pot overrides the end of the sphere integrals (jws for ASA)."""
-        t = np.ascontiguousarray(np.swapaxes(np.asarray(tau00, dtype=np.complex128), 1, 2))
         po = np.ascontiguousarray(pair_of, dtype=np.int32)
         nsolve = len(po)
+        lmax = _lmax_of_tau(tau00, nsolve)
+        t = np.ascontiguousarray(np.swapaxes(np.asarray(tau00, dtype=np.complex128), 1, 2))
         dos = np.zeros(nsolve, dtype=np.complex128)
         green = np.zeros((nsolve, iprpts), dtype=np.complex128) if with_green else None
         jm = None if jmt_of_pot is None else np.ascontiguousarray(jmt_of_pot, dtype=np.int32)
-        _ck(load().lsmsx_green_mt_resident(self._h, nsolve, n_spin_pola, _p(po),
-                                           None if jm is None else _p(jm), _p(t), _p(dos),
+        _ck(load().lsmsx_green_mt_resident(self._h, nsolve, lmax, int(iprpts), n_spin_pola,
+                                           _p(po), None if jm is None else _p(jm), _p(t), _p(dos),
                                            _p(green) if with_green else None))
         return dos, green
 

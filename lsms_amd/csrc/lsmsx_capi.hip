@@ -1874,9 +1874,9 @@ int lsmsx_set_single_site_resident(lsmsx_ctx *c, int on) {
   return 0;
 }
 
-int lsmsx_green_mt_resident(lsmsx_ctx *c, int nsolve, int n_spin_pola, const int *pair_of,
-                            const int *jmt_of_pot, const double *tau00, double *dos,
-                            double *green) {
+int lsmsx_green_mt_resident(lsmsx_ctx *c, int nsolve, int lmax, int iprpts, int n_spin_pola,
+                            const int *pair_of, const int *jmt_of_pot, const double *tau00,
+                            double *dos, double *green) {
   if (!c) return fail(-1, "null context");
   if (nsolve < 0) return fail(-2, "bad batch size");
   if (n_spin_pola != 1 && n_spin_pola != 2) return fail(-2, "n_spin_pola must be 1 or 2");
@@ -1885,7 +1885,13 @@ int lsmsx_green_mt_resident(lsmsx_ctx *c, int nsolve, int n_spin_pola, const int
   if (!c->ss_valid)
     return fail(-3, "no radial functions on the device: call lsmsx_single_site_nonrel with "
                     "lsmsx_set_single_site_resident on (and no lsmsx_green_mt in between)");
-  const int npair = c->ss_ne * c->ss_npot, lmax = c->ss_lmax, iprpts = c->ss_iprpts;
+  // the caller's buffers are sized by its lmax and iprpts: they must be the
+  // resident batch's, or the copies below would over- or under-run them
+  if (lmax != c->ss_lmax || iprpts != c->ss_iprpts)
+    return fail(-2, "lmax/iprpts differ from those of the resident single-site batch (lmax " +
+                        std::to_string(c->ss_lmax) + ", iprpts " + std::to_string(c->ss_iprpts) +
+                        ")");
+  const int npair = c->ss_ne * c->ss_npot;
   std::vector<GreenPot> gp = c->ss_gpot;
   if (jmt_of_pot)  // e.g. jws for ASA potentials (energyContourIntegration.cpp:519)
     for (int p = 0; p < c->ss_npot; p++)

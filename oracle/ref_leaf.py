@@ -1,6 +1,7 @@
 """ctypes bindings to oracle/_ref/liblsms_ref.so: the reference's own leaf
 routines of the tau00 path (makegij_c.f, setgij.f, cgaunt_c.f90, gaunt.f,
-ifacts_c.f, congauss_c.f, associatedLegendreFunction.cpp, quadrature.cpp),
+ifacts_c.f, congauss_c.f, spin_trafo.f, associatedLegendreFunction.cpp,
+quadrature.cpp),
 compiled from their sources by `make -C oracle ref`.
 
 TEST INFRASTRUCTURE ONLY: used to pin the oracle's restatement against the
@@ -62,6 +63,17 @@ def gauss_legendre_points(n, x1=-1.0, x2=1.0):
     lib().gauss_legendre_points_(_d(x1), _d(x2), x.ctypes.data_as(C.c_void_p),
                                  w.ctypes.data_as(C.c_void_p), _i(n))
     return x, w
+
+
+def spin_trafo(evec):
+    """Misc/spin_trafo.f: the spin rotation u and u^dagger (4 complex each,
+    column-major 2 x 2) of a moment direction."""
+    ev = np.asfortranarray(np.asarray(evec, dtype=np.float64))
+    u = np.zeros(4, dtype=np.complex128, order="F")
+    ud = np.zeros(4, dtype=np.complex128, order="F")
+    lib().spin_trafo_(ev.ctypes.data_as(C.c_void_p), u.ctypes.data_as(C.c_void_p),
+                      ud.ctypes.data_as(C.c_void_p))
+    return u, ud
 
 
 def plm_normalized(lmax, x):

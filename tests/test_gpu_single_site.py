@@ -241,6 +241,15 @@ def test_green_mt_resident_equals_host_path():
     rng = np.random.default_rng(5)
     tau = tmat[q_e, q_p] + 0.05 * (rng.normal(size=(len(pairs), 16, 16)) +
                                    1j * rng.normal(size=(len(pairs), 16, 16)))
+    # buffers sized for another iprpts or kkrsz are refused, not over-run
+    with pytest.raises(capi.LsmsxError):
+        c.green_mt_resident(tau, pairs, rr.shape[1] - 1, n_spin_pola=2)
+    with pytest.raises(capi.LsmsxError):  # lmax 2 blocks against the resident lmax 3
+        c.green_mt_resident(tau[:, :9, :9], pairs, rr.shape[1], n_spin_pola=2)
+    with pytest.raises(ValueError):  # not a (lmax+1)^2 block, or one block short
+        c.green_mt_resident(tau[:, :10, :10], pairs, rr.shape[1], n_spin_pola=2)
+    with pytest.raises(ValueError):
+        c.green_mt_resident(tau[1:], pairs, rr.shape[1], n_spin_pola=2)
     d_res, g_res = c.green_mt_resident(tau, pairs, rr.shape[1], n_spin_pola=2)
     # ASA: the sphere integrals run to jws (energyContourIntegration.cpp:519)
     d_res2, _ = c.green_mt_resident(tau, pairs, rr.shape[1], n_spin_pola=2, jmt_of_pot=jw)

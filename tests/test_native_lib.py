@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from lsms_amd import capi
 from oracle import oracle as O
@@ -54,7 +55,7 @@ def test_single_site_and_green_entry_points_reject_a_null_context():
                                    None, None, None),
         lambda: lib.lsmsx_green_mt_canted(None, 1, 3, 10, 1, None, None, None, None, None, None,
                                           None, None, None, None, None),
-        lambda: lib.lsmsx_green_mt_resident(None, 1, 1, None, None, None, None, None),
+        lambda: lib.lsmsx_green_mt_resident(None, 1, 3, 10, 1, None, None, None, None, None),
         lambda: lib.lsmsx_set_single_site_resident(None, 1),
         lambda: lib.lsmsx_tmats_from_single_site(None, 1, 1, 1, 1, None, None),
         lambda: lib.lsmsx_tmats_from_single_site_canted(None, 1, 1, 1, 1, 1, None, None, None,
@@ -63,3 +64,22 @@ def test_single_site_and_green_entry_points_reject_a_null_context():
     for call in calls:
         assert call() < 0
         assert b"null" in lib.lsmsx_last_error()
+
+
+REF_TREE = "/root/reference"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF_TREE, "src", "Main")),
+                    reason="reference tree not present")
+def test_host_interface_compiles_against_reference_types():
+    """lsms_amd/host/lsmsx_lsms.hpp instantiates with the reference's own
+    LSMSSystemParameters, LocalTypeInfo, AtomData and Matrix<Complex>
+    (tests/cpp/reference_types_check.cpp, compile only: the headers need the
+    image's MPI headers, nothing is linked or run)."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "reference_types_check.cpp")
+    inc = [f"-I{REF_TREE}/include", f"-I{REF_TREE}/src"]
+    inc += [f"-I{d}" for d in ("/opt/conda/include",) if os.path.exists(os.path.join(d, "mpi.h"))]
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", *inc, src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]

@@ -389,6 +389,23 @@ def test_spin_trafo_unitary():
         assert np.allclose(UD, U.conj().T) and np.allclose(U @ UD, np.eye(2))
 
 
+@needs_ref
+def test_spin_trafo_equals_reference():
+    """or_spin_trafo against Misc/spin_trafo.f compiled from source, bit for
+    bit: random directions, the axes, and directions within the tolerance of
+    the south pole (the branch u(1) < 1e-8)."""
+    rng = np.random.default_rng(11)
+    dirs = [rng.normal(size=3) for _ in range(200)]
+    dirs += [np.array(v, float) for v in ((0, 0, 1), (0, 0, -1), (1, 0, 0), (0, 1, 0),
+                                          (0, -1, 0), (-1, 0, 0))]
+    dirs += [np.array([1e-9, 2e-9, -1.0]), np.array([3e-7, -1e-7, -1.0])]
+    for v in dirs:
+        v = v / np.linalg.norm(v)
+        u, ud = O.spin_trafo(v)
+        ur, udr = R.spin_trafo(v)
+        assert np.array_equal(u, ur) and np.array_equal(ud, udr), v
+
+
 def test_trgtol_brute_force():
     """ml = (U (x) 1) mg (U^dagger (x) 1), stored as the 4 spin blocks."""
     rng = np.random.default_rng(4)
