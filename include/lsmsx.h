@@ -108,6 +108,22 @@ int lsmsx_tau00_batched(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
 int lsmsx_solve_tau00_matrix(lsmsx_ctx *ctx, int n, int k, const double *m,
                              int ldm, const double *t0, double *tau00, int *info);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lsmsx_solve_tau00_matrix with every buffer on the
+ * device -- d_m (n x n, leading dimension ldm, central block first, not
+ * modified), d_t0 (k x k), d_tau00 (k x k, written) and d_info (one int,
+ * written: 0 or the first singular pivot) -- and all scratch in a fixed
+ * per-GPU static device array: no hipMalloc, hipFree or allocating library
+ * call.  Limit: roundup(n, 64) <= 1024 (returns -9 beyond it).  The array is
+ * shared by every call on one GPU and keeps the last call's contents: calls
+ * that overlap on one GPU (several host threads, or several contexts of one
+ * device) race and must be serialized by the caller.  Runs on the context's
+ * stream and returns once the solve is done. */
+int lsmsx_solve_tau00_matrix_noalloc(lsmsx_ctx *ctx, int n, int k, const double *d_m,
+                                     int ldm, const double *d_t0, double *d_tau00,
+                                     int *d_info);
+
 /* Refinement settings of LSMSX_PREC_MIXED_FP32: at most max_sweeps
  * residual/correction sweeps (default 12), converged when the central rows
  * satisfy ||dX||_F <= tol ||X||_F (default 1e-12).  Solves that do not

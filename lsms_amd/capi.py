@@ -27,7 +27,7 @@ EXPORTED = [
     "lsmsx_get_schedule", "lsmsx_set_graph", "lsmsx_autotune", "lsmsx_set_pivot_guard",
     "lsmsx_single_site_nonrel", "lsmsx_green_mt", "lsmsx_set_single_site_resident",
     "lsmsx_green_mt_resident", "lsmsx_tmats_from_single_site", "lsmsx_green_mt_canted",
-    "lsmsx_tmats_from_single_site_canted",
+    "lsmsx_tmats_from_single_site_canted", "lsmsx_solve_tau00_matrix_noalloc",
 ]
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
@@ -60,6 +60,8 @@ def load():
     L.lsmsx_tau00_batched.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                       C.c_int, _vp, _vp]
     L.lsmsx_solve_tau00_matrix.argtypes = [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]
+    L.lsmsx_solve_tau00_matrix_noalloc.argtypes = [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp,
+                                                   _vp, _vp]
     L.lsmsx_set_workspace_limit.argtypes = [_vp, C.c_long]
     L.lsmsx_set_profiling.argtypes = [_vp, C.c_int]
     L.lsmsx_reset_profile.argtypes = [_vp]
@@ -213,6 +215,15 @@ class Context:
         _ck(load().lsmsx_solve_tau00_matrix(self._h, n, k, _p(mm), n, _p(tt), _p(tau),
                                             C.byref(info)))
         return tau, info.value
+
+    def solve_tau00_matrix_noalloc(self, n, k, d_m, ldm, d_t0, d_tau00, d_info):
+        """lsmsx_solve_tau00_matrix_noalloc on device addresses (integers):
+        d_m n x n (leading dimension ldm, central block first), d_t0 k x k,
+        d_tau00 k x k (written) and d_info one int32 (written), all
+        column-major complex128.  Nothing is allocated on the device."""
+        _ck(load().lsmsx_solve_tau00_matrix_noalloc(self._h, n, k, C.c_void_p(d_m), ldm,
+                                                    C.c_void_p(d_t0), C.c_void_p(d_tau00),
+                                                    C.c_void_p(d_info)))
 
     def upload_dtmats(self, dtmat_store, blk, group):
         ts = np.asfortranarray(dtmat_store, dtype=np.complex128)

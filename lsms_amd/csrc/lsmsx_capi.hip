@@ -17,6 +17,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -120,6 +121,26 @@ __global__ void k_init_status(const SolveDesc *__restrict__ descs, int n) {
     if (descs[i].conv) *descs[i].conv = 0;
   }
 }
+
+// Scratch of lsmsx_solve_tau00_matrix_noalloc, one copy per GPU for the life
+// of the process: the solver-layout copy of the caller's matrix (at most
+// kNoAllocMaxNs^2 complex), the one reused diagonal-tile inverse, the row
+// permutations, the smallest pivot and the solve descriptor.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (noalloc_scratch_for_device), never through the name.
+constexpr int kNoAllocMaxNs = 1024;
+constexpr size_t kNoAllocInvOff = (size_t)kNoAllocMaxNs * kNoAllocMaxNs * sizeof(double2);
+constexpr size_t kNoAllocPermOff = kNoAllocInvOff + (size_t)kTile * kTile * sizeof(double2);
+constexpr size_t kNoAllocMinOff = kNoAllocPermOff + (size_t)kNoAllocMaxNs * sizeof(int);
+constexpr size_t kNoAllocDescOff = kNoAllocMinOff + 256;
+constexpr size_t kNoAllocBytes = kNoAllocDescOff + ((sizeof(SolveDesc) + 255) / 256) * 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// The descriptor travels as a kernel argument and is stored by the device,
+// so the no-allocation entry point needs no staging buffer for it.
+__global__ void k_put_desc(SolveDesc d, SolveDesc *__restrict__ dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = d;
+}
 }  // namespace lsmsx
 
 using namespace lsmsx;
@@ -203,6 +224,7 @@ const char *make_green_pot(const double *r, int jmt, int jws, double r_sph, int 
 
 struct lsmsx_ctx {
   int device = 0;
+  unsigned char *noalloc = nullptr;  // this device's copy of `scratch`
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // second half of a chunk (LSMSX_SCHED_TWO_STREAMS)
   hipEvent_t ev_asm = nullptr, ev_join = nullptr;  // second-half start and join
@@ -1168,6 +1190,27 @@ int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
 
 }  // namespace
 
+namespace {
+// Device address of `scratch` on device dev, looked up once per device with
+// that device current (each GPU holds its own copy at its own address).
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char *g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch_for_device(int dev, unsigned char **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(-1, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(-1, "hipGetSymbolAddress returned a null scratch address");
+    g_noalloc_ptr[dev] = (unsigned char *)p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 const char *lsmsx_last_error(void) { return g_err.c_str(); }
@@ -1191,6 +1234,14 @@ int lsmsx_ctx_create(int device, lsmsx_ctx **out) {
     if (c->ev_asm) (void)hipEventDestroy(c->ev_asm);
     delete c;
     return fail(-100 - (int)e, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  if (int r = noalloc_scratch_for_device(device, &c->noalloc)) {
+    (void)hipEventDestroy(c->ev_join);
+    (void)hipEventDestroy(c->ev_asm);
+    (void)hipStreamDestroy(c->stream2);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return r;
   }
   *out = c;
   return 0;
@@ -1459,6 +1510,51 @@ int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ld
   HIPCHK(hipStreamSynchronize(c->stream));
   drain_profile(c);
   if (info) *info = hinfo;
+  return 0;
+}
+
+int lsmsx_solve_tau00_matrix_noalloc(lsmsx_ctx *c, int n, int k, const double *d_m, int ldm,
+                                     const double *d_t0, double *d_tau00, int *d_info) {
+  if (!c) return fail(-1, "null context");
+  if (k <= 0 || k > kTile || n < k || ldm < n) return fail(-2, "bad matrix dimensions");
+  if (!d_m || !d_t0 || !d_tau00 || !d_info) return fail(-2, "null device pointer");
+  const int Ns = roundup(n, kTile);
+  if (Ns > kNoAllocMaxNs)
+    return fail(-9, "n = " + std::to_string(n) + " exceeds the no-allocation limit (padded size " +
+                        std::to_string(kNoAllocMaxNs) + ")");
+  if (!c->noalloc) return fail(-1, "context has no scratch address");
+  HIPCHK(hipSetDevice(c->device));
+  unsigned char *base = c->noalloc;
+  SolveDesc d{};
+  d.kc = k;
+  d.ldt = k;
+  d.M = (double2 *)base;
+  d.A = d.M;
+  d.ld = Ns;
+  d.Ns = Ns;
+  d.pad = Ns - n;
+  d.nt = Ns / kTile;
+  d.group = lu_group(false);
+  d.inv = base + kNoAllocInvOff;
+  d.inv_stride = 0;
+  d.perm = (int *)(base + kNoAllocPermOff);
+  d.minpiv = (double *)(base + kNoAllocMinOff);
+  d.info = d_info;
+  d.tau00 = (double2 *)d_tau00;
+  d.t0 = (const double2 *)d_t0;
+  SolveDesc *dd = (SolveDesc *)(base + kNoAllocDescOff);
+  hipLaunchKernelGGL(k_put_desc, dim3(1), dim3(64), 0, c->stream, d, dd);
+  hipLaunchKernelGGL(k_init_status, dim3(1), dim3(64), 0, c->stream, dd, 1);
+  hipLaunchKernelGGL(k_permute_in, dim3(Ns), dim3(256), 0, c->stream, dd, (const double2 *)d_m,
+                     (long)ldm, n, k);
+  if (int r = check_launch("k_permute_in")) return r;
+  // no profiling events here: recording them may create events
+  const bool prof = c->profiling;
+  c->profiling = false;
+  int r = launch_lu<double>(c, dd, 1, d.nt);
+  c->profiling = prof;
+  if (r) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
