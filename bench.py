@@ -211,6 +211,11 @@ def main():
                          "bcc alloy batch (graph + autotuned schedule vs eager); c5: spin-canted "
                          "bcc 137 sites (N=4384), plain tau00 and the adjoint route; ss: "
                          "single-site t-matrices of 64 species x 2 spins at 48 energies")
+    ap.add_argument("--profile-step", choices=("fp64", "fp32", "f16"), default=None,
+                    help="only run one untimed and one timed eager step of the C2 batch in this "
+                         "precision (one stream, no graph, no autotuner, no CPU leg; --schedule "
+                         "flags apply, except two streams): the command under rocprofv3 for the "
+                         "kernel summaries")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
@@ -259,6 +264,22 @@ def main():
     nsolve = len(atoms)
     N = s.nrmat_ns(0)
     k = s.kkrsz_ns
+
+    if args.profile_step:
+        prec = {"fp64": capi.PREC_FP64, "fp32": capi.PREC_MIXED_FP32,
+                "f16": capi.PREC_MIXED_F16}[args.profile_step]
+        ctx.set_graph(False)
+        ctx.set_schedule(max(args.schedule, 0) & ~capi.SCHED_TWO_STREAMS)
+        ctx.tau00_batched(atoms, iies, prels, k, precision=prec)
+        t0 = time.perf_counter()
+        tau, info = ctx.tau00_batched(atoms, iies, prels, k, precision=prec)
+        el = time.perf_counter() - t0
+        iters, _ = ctx.last_stats(nsolve)
+        ctx.close()
+        print(json.dumps({"profile_step": args.profile_step, "solves": nsolve,
+                          "ms_per_step": round(1e3 * el, 3), "info_nonzero": int(np.sum(info != 0)),
+                          "mean_sweeps": round(float(np.mean(iters)), 3)}), flush=True)
+        return
 
     schedules = {}
 

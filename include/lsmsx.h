@@ -206,9 +206,11 @@ int lsmsx_tau00_adjoint(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
  *   LSMSX_SCHED_RIGHT_ALIGN  solves of different size start their elimination
  *                            late enough that all finish on the same step, so
  *                            the small final steps of every solve run together;
- *   LSMSX_SCHED_XCD_SWIZZLE  trailing-update workgroups are remapped so that
- *                            each of the 8 XCDs works on whole solves and
- *                            reads their panels into its own L2;
+ *   LSMSX_SCHED_XCD_SWIZZLE  trailing-update workgroups (and, with split-FP16
+ *                            updates, the cross and strip updates) are
+ *                            remapped so that each of the 8 XCDs works on
+ *                            whole solves and reads their panels into its
+ *                            own L2;
  *   LSMSX_SCHED_TWO_STREAMS  the two halves of a chunk run on two streams (of
  *                            default priority), so one half's diagonal-tile
  *                            and panel steps overlap the other's updates.

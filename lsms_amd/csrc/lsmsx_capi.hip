@@ -87,8 +87,8 @@ __global__ void k_grad_reduce(const double2 *, int, const int *, const long *, c
                               const int *, double2 *, long, int);
 __global__ void k_zero_c(double2 *, long);
 __global__ void k_lu_trail_h(const SolveDesc *, int, int);
-__global__ void k_lu_cross_h(const SolveDesc *, int, int);
-__global__ void k_lu_strip_h(const SolveDesc *, int, int);
+__global__ void k_lu_cross_h(const SolveDesc *, int, int, int);
+__global__ void k_lu_strip_h(const SolveDesc *, int, int, int);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout.
 __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
@@ -410,7 +410,7 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
           const int n1 = nt_max - s - u;
           if (split_f16)
             hipLaunchKernelGGL(k_lu_cross_h, dim3((n1 + 1) / 2 + n1 / 2, nb), dim3(256), 0, st,
-                               dd, s, u);
+                               dd, s, u, swz);
           else
             hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * n1 - 1, nb), dim3(256), 0, st, dd, s, u);
         }
@@ -426,7 +426,8 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
       ProfScope p(c, 9);
       const int hs = s + h, b = std::min(hs + h - 1, nt_max - 2);
       const int w2 = (b - hs + 2) / 2, nr2 = (nt_max - hs + 1) / 2, nc2 = (nt_max - b) / 2;
-      hipLaunchKernelGGL(k_lu_strip_h, dim3(nr2 * w2 + w2 * nc2, nb), dim3(512), 0, st, dd, s, h);
+      hipLaunchKernelGGL(k_lu_strip_h, dim3(nr2 * w2 + w2 * nc2, nb), dim3(512), 0, st, dd, s, h,
+                         swz);
       if (int r = check_launch("k_lu_strip_h")) return r;
     }
     return block(s + h, h);

@@ -306,12 +306,15 @@ __global__ __launch_bounds__(512) void k_lu_trail_h(const SolveDesc *__restrict_
 // (K = 64 ns), so that the second half starts from columns and rows updated
 // by the whole first half, like a block of its own.  The group's trailing
 // update then applies all of its steps to the tiles beyond the group.
-// Nothing to do when the block has no second half.
+// Nothing to do when the block has no second half.  xcd_swizzle as in
+// k_lu_cross_h.
 __global__ __launch_bounds__(512) void k_lu_strip_h(const SolveDesc *__restrict__ descs, int step,
-                                                    int ns) {
+                                                    int ns, int xcd_swizzle) {
   __shared__ h8 Ls[2][2][4][kTile][4];
   __shared__ h8 Us[2][2][4][kTile][4];
-  const SolveDesc &d = descs[blockIdx.y];
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (xcd_swizzle) xcd_remap(bx, by);
+  const SolveDesc &d = descs[by];
   const int g0 = step - d.step0, nt = d.nt;
   if (g0 < 0) return;
   if (d.conv && *d.conv) return;
@@ -321,7 +324,7 @@ __global__ __launch_bounds__(512) void k_lu_strip_h(const SolveDesc *__restrict_
   const int w2 = (b - h + 2) / 2;            // column (row) pairs of the strip
   const int nr2 = (nt - h + 1) / 2;          // row pairs of its column part
   const int nc2 = (nt - 1 - b + 1) / 2;      // column pairs of its row part
-  const int x = blockIdx.x;
+  const int x = bx;
   if (x < nr2 * w2) {
     const int rp = x / w2, cp = x % w2;
     const int r0 = h + 2 * rp, c0 = h + 2 * cp;
@@ -338,15 +341,19 @@ __global__ __launch_bounds__(512) void k_lu_strip_h(const SolveDesc *__restrict_
 // g .. t-1, 256 threads per workgroup: tile column t (rows >= t) in 128 x 64
 // regions (two row tiles), then tile row t (columns > t) in 64 x 128 regions
 // (two column tiles, sharing the row's L planes), so every wave has a tile.
+// With xcd_swizzle the workgroups of one solve run on one XCD (xcd_remap), so
+// the planes its regions share are fetched into one L2.
 __global__ __launch_bounds__(256) void k_lu_cross_h(const SolveDesc *__restrict__ descs, int step,
-                                                    int u) {
+                                                    int u, int xcd_swizzle) {
   // three tiles' planes: (L, L, U) for the column part, (L, U, U) for the row part
   __shared__ h8 stage[3][4][kTile][4];
-  const SolveDesc &d = descs[blockIdx.y];
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (xcd_swizzle) xcd_remap(bx, by);
+  const SolveDesc &d = descs[by];
   const int g0 = step - d.step0, t = g0 + u;
   if (g0 < 0 || u >= min(kCrossH, d.nt - 1 - g0)) return;
   if (d.conv && *d.conv) return;
-  const int n1 = d.nt - t, nc = (n1 + 1) / 2, nr = n1 / 2, x = blockIdx.x;
+  const int n1 = d.nt - t, nc = (n1 + 1) / 2, nr = n1 / 2, x = bx;
   if (x < nc) {
     auto &Ls = *reinterpret_cast<h8(*)[1][2][4][kTile][4]>(&stage[0]);
     auto &Us = *reinterpret_cast<h8(*)[1][1][4][kTile][4]>(&stage[2]);

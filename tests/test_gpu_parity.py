@@ -149,7 +149,7 @@ def test_solve_matrix_needs_pivoting():
     ctx.close()
     to, _ = O.solve_tau00_zgetrf(A, t0)
     assert info == 0
-    assert rel(tau, to) < 1e-9
+    assert rel(tau, to) < TAU_RTOL
 
 
 def test_singular_reports_info():

@@ -1,0 +1,99 @@
+"""Pivoting and conditioning cases of the tau00 solve -- needs a GPU.
+
+The LU kernels pivot inside 64 x 64 diagonal tiles.  These cases check that
+choice against LAPACK's full partial pivoting (zgetrf/zgetrs) at the FP64 bar
+of 1e-10 relative (Frobenius):
+
+- the reference's inversion test, type 1 (Test/DevTests/inversionTest/
+  inversionTest.cpp:112-133,386-394): Hilbert t-matrices and a Hilbert G0 of
+  blockSize 18 x 113 blocks (N = 2034).  As written there, zeroDiagonalBlocks
+  (:95-107) zeroes every block of G0, so M = 1 and tau00 = t0; the case its
+  name describes (only the diagonal blocks zeroed) is run as well, and a
+  blockSize 16 variant (127 blocks) of it;
+- fcc-135 KKR matrices (N = 2160) at Im E = 1e-4 Ry, ten times nearer the real
+  axis than the C3 energies, on the plain and both mixed paths.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from lsms_amd import capi
+from lsms_amd import configs as CF
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TAU_RTOL = 1e-10
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def hilbert(n):
+    i = np.arange(n)
+    return 1.0 / (i[:, None] + i[None, :] + 1.0)
+
+
+def inversion_type1(block, nblocks, zero_all_blocks):
+    """M = 1 - t_i G0_ij with Hilbert t (block x block) and Hilbert G0, whose
+    diagonal blocks (or, as zeroDiagonalBlocks is written, all blocks) are
+    zero (inversionTest.cpp:112-133)."""
+    n = block * nblocks
+    g0 = np.zeros((n, n)) if zero_all_blocks else hilbert(n)
+    if not zero_all_blocks:
+        for b in range(nblocks):
+            g0[b * block:(b + 1) * block, b * block:(b + 1) * block] = 0.0
+    t = hilbert(block)
+    m = np.eye(n, dtype=complex)
+    for b in range(nblocks):
+        m[b * block:(b + 1) * block, :] -= t @ g0[b * block:(b + 1) * block, :]
+    return m, t.astype(complex)
+
+
+@pytest.mark.parametrize("block,nblocks,zero_all", [(18, 113, True), (18, 113, False),
+                                                     (16, 127, False)])
+def test_inversion_test_type1(block, nblocks, zero_all):
+    m, t = inversion_type1(block, nblocks, zero_all)
+    ctx = capi.Context(0)
+    tau, info = ctx.solve_tau00_matrix(m, t)
+    ctx.close()
+    ref, _ = O.solve_tau00_zgetrf(m, t)
+    assert info == 0
+    err = rel(tau, ref)
+    print(f"inversionTest type 1 block {block} x {nblocks} zero_all={zero_all}: {err:.2e}")
+    assert err < TAU_RTOL
+    if zero_all:  # M = 1: tau00 = t0
+        assert rel(tau, t) < 1e-14
+
+
+def lapack_tau00(s, a, ie):
+    at = s.atoms[a]
+    m = O.build_kkr(s.maxlmax, at.LIZPos, at.LIZlmax, at.LIZStoreIdx, s.tmatStore,
+                    s.blkSizeTmatStore, ie, s.prel(ie))
+    t0 = O.t0(s.maxlmax, int(at.LIZStoreIdx[0]), s.tmatStore, s.blkSizeTmatStore, ie)
+    lu, piv, _ = sl.lapack.zgetrf(m)
+    rhs = np.zeros((m.shape[0], t0.shape[0]), complex)
+    rhs[:t0.shape[0]] = t0
+    x, _ = sl.lapack.zgetrs(lu, piv, rhs)
+    return x[:t0.shape[0]]
+
+
+@pytest.mark.parametrize("mode", [capi.PREC_FP64, capi.PREC_MIXED_FP32, capi.PREC_MIXED_F16])
+def test_fcc135_at_im_e_1e_4(mode):
+    """N = 2160 at Re E = -0.2, 0.3, 0.65 Ry and Im E = 1e-4 Ry."""
+    base = CF.config_c2(npts=4, natoms=2)
+    s = base.at_energies(np.array([-0.2 + 1e-4j, 0.3 + 1e-4j, 0.65 + 1e-4j]))
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    atoms = np.array([0, 1, 0])
+    iies = np.array([0, 1, 2])
+    prels = np.array([s.prel(i) for i in iies])
+    tau, info = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
+    iters, minpiv = ctx.last_stats(3)
+    ctx.close()
+    assert np.all(info == 0)
+    for n in range(3):
+        err = rel(tau[n], lapack_tau00(s, atoms[n], iies[n]))
+        print(f"Im E = 1e-4, mode {mode}, E = {s.energies[iies[n]]}: {err:.2e}, "
+              f"sweeps {iters[n]}, min tile pivot {minpiv[n]:.3g}")
+        assert err < TAU_RTOL
