@@ -35,6 +35,10 @@ typedef struct lsmsx_ctx lsmsx_ctx;
                                    the plain path's guarded solves and the mixed path's
                                    unconverged solves */
 
+/* negative per-solve info values of the batched calls (see lsmsx_tau00_batched) */
+#define LSMSX_INFO_SWEEP_STALL -2
+#define LSMSX_INFO_REFINE_UNCONVERGED -3
+
 /* linear-solver selector ids for the reference's 16-bit
  * lsms.global.linearSolver field (linearSolvers.hpp:18-27,196-199;
  * buildKKRMatrix.hpp:22-30): solver bits 0x0fff, build bits 0xf000. */
@@ -95,7 +99,10 @@ int lsmsx_tau00(lsmsx_ctx *ctx, int atom_id, int ispin, int iie,
  * nsolve blocks of kkrsz_ns^2 complex; info nsolve ints (0 ok, > 0 the
  * first singular pivot, -2 an internal wait bound of the mixed path's
  * refinement sweep was hit: never expected, the solve's tau00 is then not
- * valid).  Replaces the OpenMP loop of calculateAllTauMatrices
+ * valid; -3 the FP64 refinement of the solve -- LSMSX_PREC_FP64_REFINED, or
+ * the redo of a guarded plain solve or of an unconverged mixed solve -- did
+ * not converge within the sweep limit, and tau00 is the unrefined plain
+ * FP64 result).  Replaces the OpenMP loop of calculateAllTauMatrices
  * (calculateTauMatrix.cpp:841-945). */
 int lsmsx_tau00_batched(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
                         const int *ispins, const int *iies, const double *prel,
@@ -129,6 +136,21 @@ int lsmsx_solve_tau00_matrix_noalloc(lsmsx_ctx *ctx, int n, int k, const double 
  * satisfy ||dX||_F <= tol ||X||_F (default 1e-12).  Solves that do not
  * converge are redone in FP64. */
 int lsmsx_set_refinement(lsmsx_ctx *ctx, int max_sweeps, double tol);
+
+/* Residual form of the mixed modes (no reference counterpart):
+ *   LSMSX_RESIDUAL_AUTO    (default) when every solve of a batch is collinear
+ *                          with 16 right-hand sides and every LIZ site has
+ *                          lmax 3, the assembly stores the structure
+ *                          constants G_ij once per site pair (G_ji =
+ *                          (-1)^(l1+l2) G_ij entry by entry) instead of the
+ *                          FP64 M, and each residual is formed as
+ *                          B - X + t (G X) from it: half the bytes of M;
+ *                          other batches use M;
+ *   LSMSX_RESIDUAL_MATRIX  always the FP64 M.
+ * The two forms round differently; both refine to the same tolerance. */
+#define LSMSX_RESIDUAL_AUTO 0
+#define LSMSX_RESIDUAL_MATRIX 1
+int lsmsx_set_residual_form(lsmsx_ctx *ctx, int form);
 
 /* Pivot guard of the plain FP64 path: a solve whose smallest tile-local pivot
  * (|u_kk|, pivoting inside 64x64 diagonal tiles) is below threshold is redone

@@ -28,7 +28,12 @@ EXPORTED = [
     "lsmsx_single_site_nonrel", "lsmsx_green_mt", "lsmsx_set_single_site_resident",
     "lsmsx_green_mt_resident", "lsmsx_tmats_from_single_site", "lsmsx_green_mt_canted",
     "lsmsx_tmats_from_single_site_canted", "lsmsx_solve_tau00_matrix_noalloc",
+    "lsmsx_set_residual_form",
 ]
+INFO_SWEEP_STALL = -2          # per-solve info: refinement wait bound hit (tau00 invalid)
+INFO_REFINE_UNCONVERGED = -3   # per-solve info: FP64 refinement did not converge (plain result)
+RESIDUAL_AUTO = 0
+RESIDUAL_MATRIX = 1
 SCHED_RIGHT_ALIGN = 1
 SCHED_XCD_SWIZZLE = 2
 SCHED_TWO_STREAMS = 4
@@ -67,6 +72,7 @@ def load():
     L.lsmsx_reset_profile.argtypes = [_vp]
     L.lsmsx_get_profile.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
     L.lsmsx_set_refinement.argtypes = [_vp, C.c_int, C.c_double]
+    L.lsmsx_set_residual_form.argtypes = [_vp, C.c_int]
     L.lsmsx_last_batch_stats.argtypes = [_vp, C.c_int, _vp, _vp]
     L.lsmsx_upload_dtmats.argtypes = [_vp, _vp, C.c_long, C.c_int, C.c_long, C.c_int]
     L.lsmsx_tau_full.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
@@ -429,6 +435,11 @@ class Context:
         _ck(load().lsmsx_autotune(self._h, n, _p(a), _p(s), _p(e), _p(pr), self.nsc, self.nspola,
                                   precision, C.byref(best), _p(ms)))
         return best.value, ms
+
+    def set_residual_form(self, form):
+        """RESIDUAL_AUTO (structure constants once per site pair where the
+        batch allows it) or RESIDUAL_MATRIX (always the FP64 M)."""
+        _ck(load().lsmsx_set_residual_form(self._h, form))
 
     def set_refinement(self, max_sweeps=12, tol=1e-12):
         _ck(load().lsmsx_set_refinement(self._h, int(max_sweeps), float(tol)))

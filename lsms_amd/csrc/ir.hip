@@ -147,6 +147,217 @@ __global__ __launch_bounds__(256) void k_ir_residual(const SolveDesc *__restrict
   }
 }
 
+// (-1)^l of the lm index a = l(l+1) + m, a < 16.
+__device__ __forceinline__ double lsign(int a) { return (a == 0 || (a >= 4 && a < 9)) ? 1.0 : -1.0; }
+
+// Residual of the mixed path without an FP64 M, first half: Y = G X from the
+// structure constants stored once per site pair (d.gh, G_pq for solver
+// positions p < q; G_qp = D G_pq D, D = diag (-1)^l), each block read once
+// and used twice:
+//   y_p += G_pq X_q            (row part, kept in the wave's registers)
+//   z_q += D G_pq D X_p        (the mirrored block, handed on through LDS)
+// Sites go in bands of kGBand; workgroup blockIdx.x takes band rows wg and
+// nbands-1-wg (equal work), each band row b the band blocks (b, c >= b).
+// Wave w owns sites 2w, 2w+1 of band b; in stage st it takes the sites
+// j = (w + st) mod 8 of band c, adding to their z the sums its predecessor
+// left in LDS, so every z is summed in one fixed order.  y of band b goes to
+// d.ybuf, the z of band block (b, c) to slot (c, b) of d.gpart (complex
+// values in the MFMA accumulator layout: element r*64 + lane is row
+// kq + 4r, column li).  k_ir_residual_g_finish adds them up.
+__global__ __launch_bounds__(512) void k_ir_residual_g(const SolveDesc *__restrict__ descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (*d.conv) return;
+  const int n = d.numLIZ, nbands = (n + kGBand - 1) / kGBand;
+  const int wg = blockIdx.x;
+  if (wg >= (nbands + 1) / 2) return;
+  __shared__ double2 zs[kGWaves][kGSites][256];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, kq = lane >> 4;
+  const int Ns = d.Ns, pad = d.pad;
+  const gd2 *__restrict__ X = (const gd2 *)d.X;
+  const gd2 *__restrict__ GH = (const gd2 *)d.gh;
+  gd2 *__restrict__ YB = (gd2 *)d.ybuf;
+  gd2 *__restrict__ GP = (gd2 *)d.gpart;
+  double sk[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) sk[u] = lsign(4 * u + kq);
+  for (int pass = 0; pass < 2; pass++) {
+    const int b = pass ? nbands - 1 - wg : wg;
+    if (pass && b <= wg) break;
+    d4 y[kGSites][3];
+    double2 xp[kGSites][4];  // D X_p, b-operand (k row 4u + kq, column li)
+    int pp[kGSites];
+#pragma unroll
+    for (int sp = 0; sp < kGSites; sp++) {
+      const int p = kGBand * b + kGSites * w + sp;
+      pp[sp] = p;
+#pragma unroll
+      for (int m = 0; m < 3; m++) y[sp][m] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        double2 v = make_double2(0.0, 0.0);
+        if (p < n) v = ldg<double>(X + (pad + 16 * p + 4 * u + kq) + (long)Ns * li);
+        xp[sp][u] = make_double2(sk[u] * v.x, sk[u] * v.y);
+      }
+    }
+    // the band blocks (b, c >= b), stage by stage, one site q at a time: the
+    // next site's blocks (p, q) and X_q are loaded before this site's MFMAs
+    const int nit = (nbands - b) * kGWaves * kGSites;
+    auto site_q = [&](int it) {
+      const int sq = it % kGSites, st = (it / kGSites) % kGWaves;
+      const int c = b + it / (kGSites * kGWaves);
+      return kGBand * c + kGSites * ((w + st) % kGWaves) + sq;
+    };
+    auto load_q = [&](int q, double2 (&gg)[kGSites][4], double2 (&xx)[4]) {
+#pragma unroll
+      for (int sp = 0; sp < kGSites; sp++)
+        if (pp[sp] < q && q < n) {
+          const gd2 *__restrict__ G = GH + gh_index(pp[sp], q, n) * 256;
+#pragma unroll
+          for (int u = 0; u < 4; u++) gg[sp][u] = ldg<double>(G + li + 16 * (4 * u + kq));
+        }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        xx[u] = q < n ? ldg<double>(X + (pad + 16 * q + 4 * u + kq) + (long)Ns * li)
+                      : make_double2(0.0, 0.0);
+    };
+    double2 g[kGSites][4], gn[kGSites][4], xq[4], xqn[4];
+    load_q(site_q(0), g, xq);
+    for (int it = 0; it < nit; it++) {
+      const int sq = it % kGSites, st = (it / kGSites) % kGWaves;
+      const int c = b + it / (kGSites * kGWaves);
+      const int j = (w + st) % kGWaves, q = kGBand * c + kGSites * j + sq;
+      if (it + 1 < nit) load_q(site_q(it + 1), gn, xqn);
+      d4 z[3];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        double2 v = make_double2(0.0, 0.0);
+        if (st > 0) v = zs[j][sq][r * 64 + lane];
+        z[0][r] = v.x;
+        z[1][r] = 0.0;
+        z[2][r] = v.x + v.y;
+      }
+#pragma unroll
+      for (int sp = 0; sp < kGSites; sp++) {
+        if (!(pp[sp] < q && q < n)) continue;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const double gs = g[sp][u].x + g[sp][u].y;
+          const double2 a = xq[u], e = xp[sp][u];
+          y[sp][0] = mfma4(g[sp][u].x, a.x, y[sp][0]);
+          y[sp][1] = mfma4(g[sp][u].y, a.y, y[sp][1]);
+          y[sp][2] = mfma4(gs, a.x + a.y, y[sp][2]);
+          z[0] = mfma4(g[sp][u].x, e.x, z[0]);
+          z[1] = mfma4(g[sp][u].y, e.y, z[1]);
+          z[2] = mfma4(gs, e.x + e.y, z[2]);
+        }
+      }
+      // hand the z sums on, or write the band block's slot (c, b)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = make_double2(z[0][r] - z[1][r], z[2][r] - z[0][r] - z[1][r]);
+        if (st < kGWaves - 1) {
+          zs[j][sq][r * 64 + lane] = v;
+        } else if (q < n) {
+          const long slot = (long)c * (c + 1) / 2 + b;
+          GP[(slot * kGBand + kGSites * j + sq) * 256 + r * 64 + lane] = v;
+        }
+      }
+      if (sq == kGSites - 1) __syncthreads();  // the stage's z sums are in LDS
+#pragma unroll
+      for (int sp = 0; sp < kGSites; sp++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) g[sp][u] = gn[sp][u];
+#pragma unroll
+      for (int u = 0; u < 4; u++) xq[u] = xqn[u];
+    }
+#pragma unroll
+    for (int sp = 0; sp < kGSites; sp++)
+      if (pp[sp] < n)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          YB[(long)pp[sp] * 256 + r * 64 + lane] =
+              make_double2(y[sp][0][r] - y[sp][1][r], y[sp][2][r] - y[sp][0][r] - y[sp][1][r]);
+  }
+}
+
+// Second half: for site q of band c = blockIdx.x (wave w: sites 2w, 2w+1),
+//   Y_q = ybuf_q + D sum_{b <= c} gpart(c, b)_q          (fixed order)
+//   R_q = B_q - X_q + t_q Y_q                            (M = 1 - t G)
+// and the padding rows R = -X (identity block, B = 0); R stored in FP32.
+__global__ __launch_bounds__(512) void k_ir_residual_g_finish(const SolveDesc *__restrict__ descs) {
+  const SolveDesc &d = descs[blockIdx.y];
+  if (*d.conv) return;
+  const int n = d.numLIZ, nbands = (n + kGBand - 1) / kGBand, c = blockIdx.x;
+  if (c >= nbands) return;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, kq = lane >> 4;
+  const int Ns = d.Ns, pad = d.pad, kc = d.kc;
+  const gd2 *__restrict__ X = (const gd2 *)d.X;
+  const gd2 *__restrict__ YB = (const gd2 *)d.ybuf;
+  const gd2 *__restrict__ GP = (const gd2 *)d.gpart;
+  GCx<float> *__restrict__ R = (GCx<float> *)d.R;
+  if (c == 0)
+    for (int e = tid; e < pad * kc; e += 512) {
+      const int row = e % pad, col = e / pad;
+      const double2 x = ldg<double>(X + row + (long)Ns * col);
+      float2 o;
+      o.x = (float)-x.x;
+      o.y = (float)-x.y;
+      R[row + (long)Ns * col] = o;
+    }
+#pragma unroll
+  for (int sq = 0; sq < kGSites; sq++) {
+    const int q = kGBand * c + kGSites * w + sq;
+    if (q >= n) continue;
+    double2 yv[4], zv[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      yv[r] = ldg<double>(YB + (long)q * 256 + r * 64 + lane);
+      zv[r] = make_double2(0.0, 0.0);
+    }
+    for (int b = 0; b <= c; b++) {
+      const long slot = (long)c * (c + 1) / 2 + b;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const double2 v = ldg<double>(GP + (slot * kGBand + (q - kGBand * c)) * 256 + r * 64 + lane);
+        zv[r].x += v.x;
+        zv[r].y += v.y;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const double sgn = lsign(kq + 4 * r);
+      yv[r].x += sgn * zv[r].x;
+      yv[r].y += sgn * zv[r].y;
+    }
+    // t_q Y_q: a-operand t(li, 4u + kq), b-operand Y(4u + kq, li) = yv[u]
+    const int site = site_at_pos(q, n);
+    const gd2 *__restrict__ T = (const gd2 *)(d.tmat + d.ldT * d.store[site] + d.t_off);
+    d4 q1 = {0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double2 tv = ldg<double>(T + li + (long)d.ldt * (4 * u + kq));
+      q1 = mfma4(tv.x, yv[u].x, q1);
+      q2 = mfma4(tv.y, yv[u].y, q2);
+      q3 = mfma4(tv.x + tv.y, yv[u].x + yv[u].y, q3);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = pad + 16 * q + kq + 4 * r;
+      const double2 x = ldg<double>(X + row + (long)Ns * li);
+      double br = 0.0, bi = 0.0;
+      if (row >= Ns - kc) {
+        const double2 t = d.t0[(row - (Ns - kc)) + (long)d.ldt * li];
+        br = t.x;
+        bi = t.y;
+      }
+      float2 o;
+      o.x = (float)(br - x.x + (q1[r] - q2[r]));
+      o.y = (float)(bi - x.y + (q3[r] - q1[r] - q2[r]));
+      R[row + (long)Ns * li] = o;
+    }
+  }
+}
+
 // Wave-level 3M product for the refinement sweeps: acc(16 x 16nb) of rows
 // i0.. += sum_k Bop(k, j) * Aop(i0 + i, k), transposed MFMA form as in the LU
 // update (a-operand Bop(k0+kq, j0+li), b-operand Aop(i0+li, k0+kq)).

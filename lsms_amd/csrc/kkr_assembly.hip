@@ -299,12 +299,31 @@ __device__ __forceinline__ void store_block(const SolveDesc &d, const double2 (&
           if (jc < kkrc) {
             const double re = q1[q] - q2[q], im = q3[q] - q1[q] - q2[q];
             const int row = offr + bi * kkrr + li, col = offc + bj * kkrc + jc;
-            M[row + ld * col] = make_double2(-re, -im);
+            if (M) M[row + ld * col] = make_double2(-re, -im);
             if (W) W[tiled_index(row, col, d.nt)] = make_float2((float)-re, (float)-im);
           }
         }
       }
     }
+}
+
+// The pair's structure constants once (mixed path without an FP64 M), in the
+// orientation of the smaller solver position: block gh_index(p, q), p < q,
+// holds G_pq (rows: the site at p), column-major.  Lane (li, kq) holds
+// g[u] = G_ij(4u + kq, li) (rows: site i); G_ji = (-1)^(l1+l2) G_ij entry by
+// entry (sg), so both orientations come from the same entries.  Collinear,
+// every site with the central site's lmax = 3 (16 x 16 blocks).
+__device__ __forceinline__ void store_gh(const SolveDesc &d, const double2 (&g)[4],
+                                         const double (&sg)[4], int i, int j, int n) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+  const int pi = solver_pos(i, n), pj = solver_pos(j, n);
+  const bool fwd = pi < pj;
+  double2 *__restrict__ blk = d.gh + gh_index(fwd ? pi : pj, fwd ? pj : pi, n) * 256;
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const double f = fwd ? 1.0 : sg[u];
+    blk[(4 * u + kq) + 16 * li] = make_double2(f * g[u].x, f * g[u].y);
+  }
 }
 
 // Assemble for row site i = blockIdx.x of solve blockIdx.y the identity
@@ -352,7 +371,8 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
   if (w == 0) {
     for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
       int rr = e % kkri_ns, cc = e / kkri_ns;
-      d.M[(offi + rr) + (long)d.ld * (offi + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
+      if (d.M)
+        d.M[(offi + rr) + (long)d.ld * (offi + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
       if (d.w32)
         d.w32[tiled_index(offi + rr, offi + cc, d.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
     }
@@ -387,6 +407,7 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
     wave_sync();
     double2 g[4];
     gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[w], g);
+    if (d.gh) store_gh(d, g, sg, i, j, n);
     store_block<NSC>(d, g, tvi, offi, kkri, offj, kkrj);
 #pragma unroll
     for (int u = 0; u < 4; u++) g[u] = make_double2(sg[u] * g[u].x, sg[u] * g[u].y);
@@ -582,12 +603,12 @@ __global__ __launch_bounds__(256) void k_pad_identity(const SolveDesc *__restric
   // columns 0..pad-1: full columns; columns pad..Ns-1: rows 0..pad-1
   for (long e = threadIdx.x; e < (long)pad * d.Ns; e += blockDim.x) {
     int rr = (int)(e % d.Ns), cc = (int)(e / d.Ns);
-    M[rr + ld * cc] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
+    if (M) M[rr + ld * cc] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
     if (W) W[tiled_index(rr, cc, d.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
   }
   for (long e = threadIdx.x; e < (long)pad * (d.Ns - pad); e += blockDim.x) {
     int rr = (int)(e % pad), cc = pad + (int)(e / pad);
-    M[rr + ld * cc] = make_double2(0.0, 0.0);
+    if (M) M[rr + ld * cc] = make_double2(0.0, 0.0);
     if (W) W[tiled_index(rr, cc, d.nt)] = make_float2(0.f, 0.f);
   }
 }

@@ -24,6 +24,9 @@
 #include "../../include/lsmsx.h"
 #include "lsmsx_device.hpp"
 
+static_assert(lsmsx::kInfoSweepStall == LSMSX_INFO_SWEEP_STALL, "info code");
+static_assert(lsmsx::kInfoRefineUnconverged == LSMSX_INFO_REFINE_UNCONVERGED, "info code");
+
 namespace lsmsx {
 template <int NSC>
 __global__ void k_assemble(const SolveDesc *, const double2 *, long, const double *,
@@ -52,6 +55,8 @@ __global__ void k_ir_accum(const SolveDesc *, int);
 __global__ void k_ir_decide(const SolveDesc *, int, double);
 template <bool FIRST>
 __global__ void k_ir_sweep(const SolveDesc *, double);
+__global__ void k_ir_residual_g(const SolveDesc *);
+__global__ void k_ir_residual_g_finish(const SolveDesc *);
 __global__ void k_copy_m(const SolveDesc *);
 __global__ void k_fitpot(const double *, const double *, const int *, int, int, double *);
 __global__ void k_single_site(const double *, const double *, const SingleSitePot *,
@@ -266,6 +271,9 @@ struct lsmsx_ctx {
   std::vector<double> last_minpiv;
   long ws_limit = 0;
   int ir_max_sweeps = 12;
+  // residual form of the mixed path (LSMSX_RESIDUAL_*) and the G-half buffers
+  int residual_form = LSMSX_RESIDUAL_AUTO;
+  DevBuf<double2> GH, YB, GP;
   double ir_tol = 1e-12;
   double pivot_guard = 1e-2;  // plain FP64 solves with a smaller tile pivot are refined
   // schedule (LSMSX_SCHED_* flags) and hipGraph replay of the batched path
@@ -516,12 +524,17 @@ int refine(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int kc, int ma
 // so a solve's arithmetic and sweep count are those of refine<float>.  The
 // part's refinement can then run beside another part's factorization.
 int refine_async(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int max_sweeps,
-                 double tol, hipStream_t st) {
+                 double tol, hipStream_t st, int nbands_max) {
   hipLaunchKernelGGL(k_ir_init<float>, dim3(nb), dim3(256), 0, st, dd);
   for (int it = 0; it < max_sweeps; it++) {
     if (it > 0) {
       ProfScope p(c, 5);
-      hipLaunchKernelGGL((k_ir_residual<float, 1>), dim3(nt_max, nb), dim3(256), 0, st, dd);
+      if (nbands_max > 0) {  // structure constants once per site pair
+        hipLaunchKernelGGL(k_ir_residual_g, dim3((nbands_max + 1) / 2, nb), dim3(512), 0, st, dd);
+        hipLaunchKernelGGL(k_ir_residual_g_finish, dim3(nbands_max, nb), dim3(512), 0, st, dd);
+      } else {
+        hipLaunchKernelGGL((k_ir_residual<float, 1>), dim3(nt_max, nb), dim3(256), 0, st, dd);
+      }
     }
     ProfScope p(c, 6);
     if (it == 0)
@@ -570,6 +583,23 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
   // per-solve sizes (bytes of all per-solve device buffers)
   std::vector<long> bytes(nsolve);
   int kc = -1;
+  // mixed path with 16 right-hand sides, collinear, every site at lmax 3 and
+  // at most kSweepMaxTiles tiles: no FP64 M, the structure constants once per
+  // site pair (the residual forms B - X + t G X)
+  bool gmode = mixed && nsc == 1 && c->residual_form != LSMSX_RESIDUAL_MATRIX;
+  for (int i = 0; i < nsolve && gmode; i++) {
+    auto it = c->atoms.find(atom_ids[i]);
+    if (it == c->atoms.end()) break;  // reported below
+    const AtomDev &a = it->second;
+    if (a.kkrsz != 16 || roundup(a.nrmat, kTile) / kTile > kSweepMaxTiles) gmode = false;
+    for (int s2 = 0; s2 < a.numLIZ && gmode; s2++)
+      if (a.lmax[s2] != 3) gmode = false;
+  }
+  auto g_blocks = [](int n) { return (long)n * (n - 1) / 2; };
+  auto g_slots = [](int n) {
+    const long nb = (n + kGBand - 1) / kGBand;
+    return nb * (nb + 1) / 2 * kGBand;
+  };
   for (int i = 0; i < nsolve; i++) {
     auto it = c->atoms.find(atom_ids[i]);
     if (it == c->atoms.end()) return fail(-5, "unknown atom id " + std::to_string(atom_ids[i]));
@@ -587,7 +617,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     for (int s = 0; s < a.numLIZ; s++)
       if (a.store[s] < 0 || a.store[s] >= c->nstore) return fail(-7, "LIZStoreIdx out of range");
     long Ns = roundup(nsc * a.nrmat, kTile), nt = Ns / kTile;
-    bytes[i] = Ns * Ns * 16;
+    bytes[i] = gmode ? (g_blocks(a.numLIZ) + a.numLIZ + g_slots(a.numLIZ)) * 256 * 16
+                     : Ns * Ns * 16;
     if (plain) bytes[i] += kTile * kTile * 16;
     if (mixed) bytes[i] += Ns * Ns * 8 + nt * kTile * kTile * 8 + Ns * kc * 24;
     if (refined64) bytes[i] += Ns * Ns * 16 + nt * kTile * kTile * 16 + Ns * kc * 32;
@@ -597,7 +628,9 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
   if (cap <= 0) {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
-    long held = (long)((c->M.n + c->inv.n + c->A64.n) * sizeof(double2) + c->W.n * sizeof(float2));
+    long held = (long)((c->M.n + c->inv.n + c->A64.n + c->GH.n + c->YB.n + c->GP.n) *
+                           sizeof(double2) +
+                       c->W.n * sizeof(float2));
     cap = (long)((double)(fr + held) * 0.7);
   }
   int done = 0;
@@ -607,8 +640,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     int end = done;
     while (end < nsolve && (end == done || acc + bytes[end] <= cap)) acc += bytes[end++];
     int nb = end - done;
-    long mtot = 0, xtot = 0, ivtot = 0, htot = 0;
-    int maxliz = 1, nt_max = 1;
+    long mtot = 0, xtot = 0, ivtot = 0, htot = 0, ghtot = 0, ybtot = 0, gptot = 0;
+    int maxliz = 1, nt_max = 1, nbands_max = 0;
     for (int b = 0; b < nb; b++) {
       const AtomDev &a = c->atoms[atom_ids[done + b]];
       long Ns = roundup(nsc * a.nrmat, kTile);
@@ -618,8 +651,20 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       htot += 2 * lu_group(split) * (Ns / kTile);
       maxliz = std::max(maxliz, a.numLIZ);
       nt_max = std::max(nt_max, (int)(Ns / kTile));
+      if (gmode) {
+        ghtot += g_blocks(a.numLIZ) * 256;
+        ybtot += (long)a.numLIZ * 256;
+        gptot += g_slots(a.numLIZ) * 256;
+        nbands_max = std::max(nbands_max, (a.numLIZ + kGBand - 1) / kGBand);
+      }
     }
-    if (int r = c->M.ensure(mtot)) return r;
+    if (gmode) {
+      if (int r = c->GH.ensure(std::max(ghtot, 1L))) return r;
+      if (int r = c->YB.ensure(ybtot)) return r;
+      if (int r = c->GP.ensure(gptot)) return r;
+    } else if (int r = c->M.ensure(mtot)) {
+      return r;
+    }
     // inv holds double2 elements; FP32 inverses take half as many
     if (int r = c->inv.ensure(mixed ? (ivtot + 1) / 2 : ivtot)) return r;
     if (int r = c->perm.ensure((size_t)nb * nt_max * kTile)) return r;
@@ -647,7 +692,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (int r = c->he.ensure((size_t)htot)) return r;
     }
     std::vector<SolveDesc> hd(nb);
-    long moff = 0, xoff = 0, ioff = 0, hoff = 0;
+    long moff = 0, xoff = 0, ioff = 0, hoff = 0, ghoff = 0, yboff = 0, gpoff = 0;
     for (int b = 0; b < nb; b++) {
       int i = done + b;
       AtomDev &a = c->atoms[atom_ids[i]];
@@ -672,7 +717,17 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       long jsm = (nspola != nsc) ? (long)kc * kc * ispins[i] : 0;
       d.t_off = (long)iies[i] * c->blk + jsm;
       d.ldt = kc;
-      d.M = c->M.p + moff;
+      d.M = gmode ? nullptr : c->M.p + moff;
+      if (gmode) {
+        d.gh = c->GH.p + ghoff;
+        d.ybuf = c->YB.p + yboff;
+        d.gpart = c->GP.p + gpoff;
+        ghoff += g_blocks(a.numLIZ) * 256;
+        yboff += (long)a.numLIZ * 256;
+        gpoff += g_slots(a.numLIZ) * 256;
+      }
+      d.tmat = c->tmat.p;
+      d.ldT = c->ldT;
       d.ld = Ns;
       d.Ns = Ns;
       d.pad = Ns - N;
@@ -749,7 +804,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       }
       if (!mixed) return launch_lu<double>(c, dd, n, ntm, false, st);
       if (int r = launch_lu<float>(c, dd, n, ntm, split, st)) return r;
-      if (ir_inline) return refine_async(c, dd, n, ntm, c->ir_max_sweeps, c->ir_tol, st);
+      if (ir_inline)
+        return refine_async(c, dd, n, ntm, c->ir_max_sweeps, c->ir_tol, st, gmode ? nbands_max : 0);
       return 0;
     };
     // two halves on two streams, the second started once the first half's
@@ -772,7 +828,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       std::memcpy(&tol_bits, &c->ir_tol, sizeof(tol_bits));
       std::vector<long> key = {nb, nt_max, maxliz, precision, c->sched, (long)c->desc.p,
                                (long)c->tmat.p, c->ldT, (long)c->cgnt.p, (long)c->illp.p,
-                               c->lmax_tab, c->ir_max_sweeps, tol_bits};
+                               c->lmax_tab, c->ir_max_sweeps, tol_bits, gmode ? nbands_max : -1};
       auto it = c->graphs.find(key);
       if (it == c->graphs.end()) {
         const bool prof = c->profiling;
@@ -798,6 +854,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (int r = enqueue()) return r;
     }
     std::vector<int> hiters(nb, 0);
+    std::vector<char> unconverged64(nb, 0);
     if (refining) {
       int left = 0;
       int r = ir_inline ? 0
@@ -813,7 +870,10 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
                             c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
       for (int b = 0; b < nb; b++)
-        if (!hconv[b]) hiters[b] = -1;
+        if (!hconv[b]) {
+          hiters[b] = -1;
+          if (refined64) unconverged64[b] = 1;
+        }
       if (refined64 && left > 0) {
         // last resort for a refinement that did not converge: the plain
         // FP64 result from the untouched M
@@ -850,6 +910,9 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     HIPCHK(hipStreamSynchronize(c->stream));
     drain_profile(c);
     for (int b = 0; b < nb; b++) {
+      // a refinement that did not converge returns the plain FP64 result,
+      // flagged so that callers can tell it from a refined one
+      if (unconverged64[b] && hinfo[b] == 0) hinfo[b] = kInfoRefineUnconverged;
       info_out[done + b] = hinfo[b];
       c->last_iters[done + b] = hiters[b];
       c->last_minpiv[done + b] = hmin[b];
@@ -1286,6 +1349,9 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->gf_green.release();
   c->gf_w.release();
   c->M.release();
+  c->GH.release();
+  c->YB.release();
+  c->GP.release();
   c->inv.release();
   c->tau.release();
   c->tmpM.release();
@@ -1564,6 +1630,14 @@ int lsmsx_set_refinement(lsmsx_ctx *c, int max_sweeps, double tol) {
   if (max_sweeps < 1 || max_sweeps > 100 || !(tol > 0.0)) return fail(-2, "bad refinement settings");
   c->ir_max_sweeps = max_sweeps;
   c->ir_tol = tol;
+  return 0;
+}
+
+int lsmsx_set_residual_form(lsmsx_ctx *c, int form) {
+  if (!c) return fail(-1, "null context");
+  if (form != LSMSX_RESIDUAL_AUTO && form != LSMSX_RESIDUAL_MATRIX)
+    return fail(-2, "unknown residual form");
+  c->residual_form = form;
   return 0;
 }
 

@@ -22,6 +22,9 @@ constexpr int kSweepWaves = 8;
 constexpr int kSweepMaxTiles = 128;
 constexpr unsigned kSweepSpinLimit = 1u << 24;  // bound of one k_ir_sweep flag wait
 constexpr int kInfoSweepStall = -2;             // per-solve info when that bound is hit
+// per-solve info of a solve whose FP64 refinement did not converge: its
+// tau00 is the unrefined FP64 result
+constexpr int kInfoRefineUnconverged = -3;  // LSMSX_INFO_REFINE_UNCONVERGED
 
 // Element (r, c) of a tile-major matrix with nt tiles per tile column: every
 // 64 x 64 tile is one contiguous run (column-major inside), tiles ordered by
@@ -83,7 +86,29 @@ struct SolveDesc {
   double dprel_re, dprel_im;  // d prel / dE
   double2 *gx;            // Ns x kc: the rows of site i hold (G X)_i, X = M^-1 [0; t0]
   double2 *lpart;         // numLIZ per-row-site parts of Tr(M^-1 dM/dE)
+  // mixed path with the structure constants stored once per site pair (no
+  // FP64 M): the residual is formed as B - X + T (G X)
+  double2 *gh;            // n(n-1)/2 blocks G_pq, p < q solver positions (gh_index), 16 x 16
+                          // column-major (rows: site p), so G_qp = D G_pq D, D = (-1)^l
+  double2 *ybuf;          // n site blocks: the G X rows of each band's own band row
+  double2 *gpart;         // band slots (c, b <= c) x 16 sites: rows of band c from band b
+  const double2 *tmat;    // t-matrix store and its leading dimension (site t of the residual)
+  long ldT;
 };
+
+// G-half residual: sites per band, waves per workgroup and sites per wave
+constexpr int kGBand = 16;
+constexpr int kGWaves = 8;
+constexpr int kGSites = kGBand / kGWaves;
+
+// Block of the solver-position pair p < q among n sites (row-major upper triangle).
+__host__ __device__ inline long gh_index(int p, int q, int n) {
+  return (long)p * (2 * n - p - 1) / 2 + (q - p - 1);
+}
+// Solver position of LIZ site s (site 0, the central one, last).
+__host__ __device__ inline int solver_pos(int s, int n) { return s ? s - 1 : n - 1; }
+// LIZ site at solver position p.
+__host__ __device__ inline int site_at_pos(int p, int n) { return p == n - 1 ? 0 : p + 1; }
 
 // one potential of a single-site batch: 1-based jmt, jws, iend = max(jmt+11, jws)
 // as in semrel_c.f, and the 0-based four-point window of the matching

@@ -18,9 +18,11 @@
 //   lsms.nrelv, lsms.mtasa, atom.r_mesh, atom.jmt, atom.jws, atom.rInscribed,
 //   atom.rws, atom.lmax, vr(j, is)                      (single-site solve)
 //
-// Error behaviour: like the reference's solvers, a singular pivot (info != 0)
-// is printed and the computation continues (zblock_lu_cpp.cpp:58-68); a
-// device or argument error throws std::runtime_error instead of exit(1).
+// Error behaviour: like the reference's solvers, a singular pivot (info > 0)
+// is printed and the computation continues (zblock_lu_cpp.cpp:58-68), and so
+// is an FP64 refinement that did not converge (info -3: the unrefined FP64
+// tau00 is used); a refinement wait bound (info -2: tau00 not valid) and a
+// device or argument error throw std::runtime_error instead of exit(1).
 #pragma once
 
 #include <complex>
@@ -130,6 +132,20 @@ void buildKKRMatrixMI355X(LSMS &lsms, Atom &atom, int localAtomIndex, int ispin,
         "lsmsx_build_kkr");
 }
 
+// Per-solve status of a solve (see the error behaviour above).
+inline void reportInfo(const char *where, int atom, int info) {
+  if (info == 0) return;
+  if (info > 0) {
+    std::printf("lsmsx %s: atom %d singular pivot info=%d\n", where, atom, info);
+  } else if (info == LSMSX_INFO_REFINE_UNCONVERGED) {
+    std::printf("lsmsx %s: atom %d FP64 refinement did not converge; unrefined FP64 tau00 used\n",
+                where, atom);
+  } else {
+    throw std::runtime_error(std::string("lsmsx ") + where + ": atom " + std::to_string(atom) +
+                             " solve status " + std::to_string(info) + " (tau00 not valid)");
+  }
+}
+
 // solveTau00zgetrf replacement (linearSolvers_CPU.cpp:98-123): tau00 =
 // [M^-1 [t0;0]]_00 from the caller's KKR matrix m (m is left unchanged).
 template <class LSMS, class Local, class Atom, class Mat>
@@ -142,7 +158,7 @@ void solveTau00MI355X(LSMS &lsms, Local &local, Atom &atom, int iie, Mat &m, Mat
                                  (int)m.l_dim(), reinterpret_cast<const double *>(t0.data()),
                                  reinterpret_cast<double *>(out.data()), &info),
         "lsmsx_solve_tau00_matrix");
-  if (info != 0) std::printf("lsmsx solveTau00: singular pivot info=%d\n", info);
+  reportInfo("solveTau00", -1, info);
   for (int j = 0; j < k; j++)
     for (int i = 0; i < k; i++) tau00(i, j) = out[i + k * j];
 }
@@ -186,7 +202,7 @@ void calculateTauMatrixMI355X(LSMS &lsms, Local &local, Atom &atom, int localAto
   check(lsmsx_tau00(runtime().ctx, localAtomIndex, ispin, iie, pr, lsms.n_spin_cant, lsms.n_spin_pola,
                     LSMSX_PREC_FP64, reinterpret_cast<double *>(tau.data()), &info),
         "lsmsx_tau00");
-  if (info != 0) std::printf("lsmsx calculateTauMatrix: singular pivot info=%d\n", info);
+  reportInfo("calculateTauMatrix", localAtomIndex, info);
   postprocessTau00(lsms, atom, tau, centralT(lsms, local, atom, iie, ispin), tau00_l);
 }
 
@@ -220,7 +236,7 @@ void calculateAllTauMatricesMI355X(LSMS &lsms, Local &local, cplx energy, int ii
                             reinterpret_cast<double *>(tau.data()), info.data()),
         "lsmsx_tau00_batched");
   for (int b = 0; b < n; b++) {
-    if (info[b] != 0) std::printf("lsmsx calculateAllTauMatrices: atom %d singular pivot info=%d\n", ids[b], info[b]);
+    reportInfo("calculateAllTauMatrices", ids[b], info[b]);
     auto &atom = local.atom[ids[b]];
     std::vector<cplx> one(tau.begin() + (size_t)b * k * k, tau.begin() + (size_t)(b + 1) * k * k);
     postprocessTau00(lsms, atom, one, centralT(lsms, local, atom, iie, spins[b]),
@@ -245,7 +261,7 @@ void calculateTauFullMI355X(LSMS &lsms, Atom &atom, int localAtomIndex, int ispi
                        lsms.n_spin_pola, reinterpret_cast<double *>(&tau(0, 0)), (int)tau.l_dim(),
                        &info),
         "lsmsx_tau_full");
-  if (info != 0) std::printf("lsmsx calculateTauFull: singular pivot info=%d\n", info);
+  reportInfo("calculateTauFull", -1, info);
 }
 
 // Single-site t-matrices of every local atom at the energies of one group,

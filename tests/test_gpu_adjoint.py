@@ -188,3 +188,36 @@ def test_c5_full_size_adjoint():
         assert rel(tb[b], ref) < 1e-10
         tr = R.lloyd_trace(s, a, s.energies[ie])
         assert abs(ll[b] - tr) < 1e-8 * abs(tr), (a, ie, ll[b], tr)
+
+
+def test_c5_full_size_gradient_and_lloyd_by_differences():
+    """N = 4384 (spin-canted bcc 137 sites), one solve: the tau00 gradient
+    against central differences of the GPU's FP64 tau00 along a random
+    complex store direction and its i-rotation (h = 1e-5: truncation
+    O(h^2) ~ 1e-10, rounding ~1e-12 / h ~ 1e-7 relative), and the Lloyd
+    term against the central difference of LAPACK log-determinants of the
+    oracle's M(E +- h), h = 1e-6 Ry."""
+    s = CF.config_c5(natoms=1)
+    ctx = engine(s)
+    a, ie = 0, 11
+    k = 32
+    rng = np.random.default_rng(29)
+    V = rng.normal(size=(1, k, k)) + 1j * rng.normal(size=(1, k, k))
+    _, ll, g, info = ctx.tau00_adjoint([a], [ie], s.energies[[ie]], k, tau00_bar=V,
+                                       store_shape=s.tmatStore.shape)
+    assert np.all(info == 0)
+    h = 1e-5
+    D = rng.normal(size=s.tmatStore.shape) + 1j * rng.normal(size=s.tmatStore.shape)
+    D *= np.abs(s.tmatStore) > 0
+    for direction in (D, 1j * D):
+        tp = tau_batch(ctx, s, s.tmatStore + h * direction, [a], [ie], None)
+        tm = tau_batch(ctx, s, s.tmatStore - h * direction, [a], [ie], None)
+        fd = np.real(np.vdot(V, (tp - tm) / (2 * h)))
+        an = np.real(np.vdot(g, direction))
+        print(f"C5 gradient: adjoint {an:.10e} differences {fd:.10e}")
+        assert abs(an - fd) < 1e-6 * abs(fd), (an, fd)
+    ctx.upload_tmats(s.tmatStore, s.blkSizeTmatStore, len(s.energies))
+    ctx.close()
+    fd_ll = R.lloyd_logdet_fd(s, a, s.energies[ie], h=1e-6)
+    print(f"C5 Lloyd: adjoint {ll[0]} log-det differences {fd_ll}")
+    assert abs(ll[0] - fd_ll) < 1e-6 * abs(fd_ll), (ll[0], fd_ll)

@@ -110,6 +110,8 @@ def test_mixed_fallback_to_fp64(mode):
     iters, _ = ctx.last_stats(3)
     ctx.close()
     assert np.all(iters == -1)
+    # the FP64 redo is limited to one sweep too: flagged, plain FP64 result
+    assert np.all(info == capi.INFO_REFINE_UNCONVERGED), info
     for n in range(3):
         assert rel(tau[n], lapack_tau00(s, atoms[n], iies[n])) < TAU_RTOL
 
@@ -240,3 +242,33 @@ def test_lloyd_idos_c1_against_oracle():
         n_ref = contour.idos_lloyd(ref, s.weights)
         assert abs(n_gpu - n_ref) < 1e-9 * max(1.0, abs(n_ref)), (a, n_gpu, n_ref)
     ctx.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_residual_forms_fcc135(mode):
+    """The residual from the structure constants stored once per site pair
+    (B - X + t G X, the default for these batches) and from the FP64 M both
+    refine fcc-135 solves (N = 2160, contour and Im E = 0.001 Ry) to the
+    bar against LAPACK, in about the same number of sweeps."""
+    s = CF.config_c3(natoms=2)
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    sel = [(0, 3), (1, 20), (0, 32), (1, 47)]
+    atoms = np.array([a for a, _ in sel])
+    iies = np.array([e for _, e in sel])
+    prels = np.array([s.prel(i) for i in iies])
+    out = {}
+    for form in (capi.RESIDUAL_AUTO, capi.RESIDUAL_MATRIX):
+        ctx.set_residual_form(form)
+        tau, info = ctx.tau00_batched(atoms, iies, prels, 16, precision=mode)
+        iters, _ = ctx.last_stats(len(sel))
+        assert np.all(info == 0) and np.all(iters > 0), (form, info, iters)
+        out[form] = (tau, iters)
+    ctx.close()
+    for n, (a, ie) in enumerate(sel):
+        ref = lapack_tau00(s, a, ie)
+        for form, (tau, iters) in out.items():
+            err = rel(tau[n], ref)
+            print(f"form {form} mode {mode} solve {a},{ie}: {err:.2e} sweeps {iters[n]}")
+            assert err < TAU_RTOL, (form, a, ie, err)
+    assert abs(int(out[0][1].sum()) - int(out[1][1].sum())) <= len(sel)

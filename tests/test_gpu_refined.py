@@ -102,3 +102,33 @@ def test_pivot_guard_on_small_tile_pivot():
         ref = lapack_refined_tau00(s, a, ie)
         assert rel(tau[b], ref) < 1e-10, (a, ie)
         assert rel(tau_adj[b], ref) < 1e-10, (a, ie)
+
+
+def test_unconverged_refinement_is_flagged():
+    """A refinement held to one sweep cannot converge: the FP64-refined mode
+    then returns the unrefined FP64 tau00 with info INFO_REFINE_UNCONVERGED,
+    and so does a plain solve that the pivot guard sends there (a guard of
+    1e3 catches every solve); the values are still the plain FP64 ones."""
+    s = CF.config_c1()
+    ctx = capi.Context(0)
+    ctx.load_system(s)
+    atoms = np.array([0, 1, 1])
+    iies = np.array([4, 12, 27])
+    prels = [s.prel(i) for i in iies]
+    plain, info0 = ctx.tau00_batched(atoms, iies, prels, 16)
+    ctx.set_refinement(max_sweeps=1, tol=1e-12)
+    tau_r, info_r = ctx.tau00_batched(atoms, iies, prels, 16, precision=REFINED)
+    it_r, _ = ctx.last_stats(3)
+    ctx.set_pivot_guard(1e3)
+    tau_g, info_g = ctx.tau00_batched(atoms, iies, prels, 16)
+    ctx.set_pivot_guard(1e-2)
+    ctx.set_refinement()
+    tau_ok, info_ok = ctx.tau00_batched(atoms, iies, prels, 16, precision=REFINED)
+    ctx.close()
+    assert np.all(info0 == 0) and np.all(info_ok == 0)
+    assert np.all(info_r == capi.INFO_REFINE_UNCONVERGED) and np.all(it_r == -1)
+    assert np.all(info_g == capi.INFO_REFINE_UNCONVERGED)
+    for n in range(3):
+        assert rel(tau_r[n], plain[n]) < 1e-13
+        assert rel(tau_g[n], plain[n]) < 1e-13
+        assert rel(tau_ok[n], plain[n]) < 1e-10
