@@ -69,6 +69,24 @@ int lsmsx_compute_tables(int maxlmax, double *cgnt, double *illp, double *ilp1);
 int lsmsx_upload_tmats(lsmsx_ctx *ctx, const double *tmat, long ldTmatStore,
                        int numStore, long blkSizeTmatStore, int groupSize);
 
+/* Relativistic store (relativity = full): tmat holds the reference's
+ * kappa-mu t-matrices, (2 kkrsz)^2 per energy slot, and energies[2 e],
+ * energies[2 e + 1] the complex energy of slot e.  The KKR blocks are then
+ * M_ij = -t_i (prel^2/E) relmtrx(G_ij) (setgij.f:39-55, relmtrx.f): the
+ * batched and single tau00 calls (n_spin_cant = n_spin_pola = 2, every LIZ
+ * site at the central lmax) return the kappa-mu tau00 = [M^-1 [t0;0]]_00,
+ * which the caller rotates with dmatp tau00 dmat (tau_inv_postproc.f:79-130;
+ * no tau00 - t), and lsmsx_build_kkr dumps the kappa-mu M.  Computed as the
+ * orthogonal similarity M = U~ M' U~^T of the spin-canted matrix of
+ * t' = (prel^2/E) U^T t U (U: the Clebsch-Gordan matrix of clebsch.f), so
+ * every precision mode applies.  Full tau and the adjoint route reject it.
+ * A later lsmsx_upload_tmats returns to the non-relativistic blocks.
+ * Replaces the relativistic branch of the F77 build (calculateTauMatrix.cpp:
+ * 216-242) and of the old solvers' post-processing (:818-821). */
+int lsmsx_upload_tmats_rel(lsmsx_ctx *ctx, const double *tmat, long ldTmatStore,
+                           int numStore, long blkSizeTmatStore, int groupSize,
+                           const double *energies);
+
 /* Register the LIZ of local atom `atom_id`: LIZPos (3 x numLIZ, relative to
  * the central site 0), LIZlmax, LIZStoreIdx (AtomData fields,
  * SingleSite/AtomData.hpp:441-467).  Replaces DeviceAtom::copyFromAtom

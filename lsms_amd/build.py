@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["kkr_assembly.hip", "tau_lu.hip", "ir.hip", "inverse.hip", "lsmsx_capi.hip",
-               "update_f16.hip", "single_site.hip"]
+               "update_f16.hip", "single_site.hip", "relativistic.hip"]
 # sources whose MFMA accumulators stay in AGPRs (no -amdgpu-mfma-vgpr-form)
 AGPR_SOURCES = {"update_f16.hip"}
 CXX_SOURCES = ["tables.cpp"]

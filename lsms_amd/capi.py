@@ -28,7 +28,7 @@ EXPORTED = [
     "lsmsx_single_site_nonrel", "lsmsx_green_mt", "lsmsx_set_single_site_resident",
     "lsmsx_green_mt_resident", "lsmsx_tmats_from_single_site", "lsmsx_green_mt_canted",
     "lsmsx_tmats_from_single_site_canted", "lsmsx_solve_tau00_matrix_noalloc",
-    "lsmsx_set_residual_form",
+    "lsmsx_set_residual_form", "lsmsx_upload_tmats_rel",
 ]
 INFO_SWEEP_STALL = -2          # per-solve info: refinement wait bound hit (tau00 invalid)
 INFO_REFINE_UNCONVERGED = -3   # per-solve info: FP64 refinement did not converge (plain result)
@@ -73,6 +73,7 @@ def load():
     L.lsmsx_get_profile.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
     L.lsmsx_set_refinement.argtypes = [_vp, C.c_int, C.c_double]
     L.lsmsx_set_residual_form.argtypes = [_vp, C.c_int]
+    L.lsmsx_upload_tmats_rel.argtypes = [_vp, _vp, C.c_long, C.c_int, C.c_long, C.c_int, _vp]
     L.lsmsx_last_batch_stats.argtypes = [_vp, C.c_int, _vp, _vp]
     L.lsmsx_upload_dtmats.argtypes = [_vp, _vp, C.c_long, C.c_int, C.c_long, C.c_int]
     L.lsmsx_tau_full.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
@@ -164,6 +165,15 @@ class Context:
         il = np.asfortranarray(illp, dtype=np.complex128)
         i1 = np.ascontiguousarray(ilp1, dtype=np.complex128)
         _ck(load().lsmsx_set_tables(self._h, lmax, _p(cg), _p(il), _p(i1)))
+
+    def upload_tmats_rel(self, tmat_store, blk, group, energies):
+        """Relativistic (kappa-mu) t-matrix store of `group` energy slots
+        with their complex energies."""
+        ts = np.asfortranarray(tmat_store, dtype=np.complex128)
+        en = np.ascontiguousarray(energies, dtype=np.complex128)
+        assert len(en) == group
+        _ck(load().lsmsx_upload_tmats_rel(self._h, _p(ts), ts.shape[0], ts.shape[1], blk, group,
+                                          _p(en)))
 
     def upload_tmats(self, tmat_store, blk, group):
         ts = np.asfortranarray(tmat_store, dtype=np.complex128)

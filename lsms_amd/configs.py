@@ -382,3 +382,45 @@ def single_site_batch(nspecies=4, ne=48, seed=20261202, ebot=-0.4, etop=0.75):
     e, _ = contour_semicircle(ebot, etop, ne)
     return (np.array(rs), np.array(vs), np.array(jm, dtype=np.int32),
             np.array(jw, dtype=np.int32), np.array(rsph), e, prel_of(e))
+
+
+def make_rel_tmat_store(nstore, energies, lmax, seed=20261203, delta_range=1.2, coupling=0.05):
+    """Synthetic relativistic (kappa-mu) t-matrix store: per store column,
+    phase shifts per (l, j) with t_kappa-mu = -sin(d) e^{i d} / kappa on the
+    diagonal (within each l the 2l rows of j = l - 1/2, then the 2l + 2 rows
+    of j = l + 1/2, the clebsch.f row order) plus a fixed complex symmetric
+    coupling of size `coupling` (as spin-orbit and exchange mix kappa-mu
+    states), all over kappa = prel(E).  Blocks of (2 kkr)^2 per energy."""
+    rng = np.random.default_rng(seed)
+    kkr = (lmax + 1) ** 2
+    kk2 = 2 * kkr
+    blk = kk2 * kk2
+    deltas = rng.uniform(-delta_range, delta_range, size=(nstore, lmax + 1, 2))
+    cpl = coupling * (rng.normal(size=(nstore, kk2, kk2)) + 1j * rng.normal(size=(nstore, kk2, kk2)))
+    cpl = 0.5 * (cpl + np.transpose(cpl, (0, 2, 1)))
+    ne = len(energies)
+    store = np.zeros((blk * ne, nstore), dtype=np.complex128, order="F")
+    for s in range(nstore):
+        dvec = []
+        for l in range(lmax + 1):
+            dvec += [deltas[s, l, 0]] * (2 * l) + [deltas[s, l, 1]] * (2 * l + 2)
+        dvec = np.array(dvec)
+        base_t = np.diag(-np.sin(dvec) * np.exp(1j * dvec)) + cpl[s]
+        for ie, e in enumerate(energies):
+            t = base_t / complex(prel_of(e))
+            store[ie * blk:(ie + 1) * blk, s] = t.flatten(order="F")
+    return store, blk
+
+
+def config_rel(npts=4, seed=20261203, rcut=6.0, ncell=(1, 1, 1)):
+    """Relativistic test system (relativity = full): bcc a=5.42 bohr,
+    n_spin_cant = n_spin_pola = 2, lmax 3 on every LIZ site (r=6.0: 15
+    sites, N = 480), synthetic kappa-mu t-matrices per species on a
+    semicircle contour."""
+    e, w = contour_semicircle(-0.5, 0.7, npts)
+    base = make_system("rel_bcc", "bcc", 5.42, ncell, rcut, 3, e, w, nspecies=2, seed=seed,
+                       n_spin_cant=2, n_spin_pola=2)
+    store, blk = make_rel_tmat_store(base.tmatStore.shape[1], e, 3, seed=seed)
+    return System(name="rel_bcc", maxlmax=3, n_spin_cant=2, n_spin_pola=2, atoms=base.atoms,
+                  energies=base.energies, weights=base.weights, tmatStore=store,
+                  blkSizeTmatStore=blk, deltas=base.deltas, evecs=base.evecs)

@@ -93,6 +93,9 @@ __global__ void k_grad_reduce(const double2 *, int, const int *, const long *, c
 __global__ void k_zero_c(double2 *, long);
 __global__ void k_lu_trail_h(const SolveDesc *, int, int);
 __global__ void k_lu_cross_h(const SolveDesc *, int, int, int);
+__global__ void k_rel_tstore(double2 *, long, long, int, const double *, const double2 *);
+__global__ void k_rel_tau(double2 *, int, const double *, const double2 *);
+__global__ void k_rel_blocks(double2 *, long, int, const double *);
 __global__ void k_lu_strip_h(const SolveDesc *, int, int, int);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout.
@@ -274,6 +277,12 @@ struct lsmsx_ctx {
   // residual form of the mixed path (LSMSX_RESIDUAL_*) and the G-half buffers
   int residual_form = LSMSX_RESIDUAL_AUTO;
   DevBuf<double2> GH, YB, GP;
+  // relativistic store (lsmsx_upload_tmats_rel): the solver store holds
+  // t' = f U^T t U, f = prel^2/E per energy slot; rel_finv[slot] = 1/f
+  int rel = 0, rel_kk2 = 0;
+  std::vector<double2> rel_finv;
+  DevBuf<double> relU;
+  DevBuf<double2> relf;
   double ir_tol = 1e-12;
   double pivot_guard = 1e-2;  // plain FP64 solves with a smaller tile pivot are refined
   // schedule (LSMSX_SCHED_* flags) and hipGraph replay of the batched path
@@ -546,6 +555,33 @@ int refine_async(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, int max_
   return check_launch("refine_async");
 }
 
+// The (l m s) -> (kappa mu) Clebsch-Gordan matrix of clebsch.f as a dense
+// kk2 x kk2 column-major U, kk2 = 2 (lmax+1)^2: kappa-mu row ir (0-based
+// 2 kappa^2 + kappa + m - 1, and the next row for spin up), column s kkr + lm
+// with s = 0 for the spin-down coefficient u1 and s = 1 for the spin-up u2,
+// so that relmtrx(G) = U diag_spin(G) U^T.
+std::vector<double> clebsch_u(int lmax) {
+  const int kkr = (lmax + 1) * (lmax + 1), kk2 = 2 * kkr;
+  std::vector<double> U((size_t)kk2 * kk2, 0.0);
+  auto at = [&](int r, int col) -> double & { return U[r + (size_t)kk2 * col]; };
+  for (int l = 0; l <= lmax; l++) {
+    const double d = 2.0 * l + 1.0;
+    for (int m = -l; m <= l; m++) {
+      const int lm = l * l + l + m;
+      if (l > 0) {  // j = l - 1/2, kappa = l
+        const int r = 2 * l * l + l + m - 1;
+        at(r, lm) = std::sqrt((l + m) / d);
+        at(r + 1, kkr + lm) = -std::sqrt((l - m) / d);
+      }
+      const int kap = -l - 1;  // j = l + 1/2
+      const int r = 2 * kap * kap + kap + m - 1;
+      at(r, lm) = std::sqrt((l - m + 1) / d);
+      at(r + 1, kkr + lm) = std::sqrt((l + m + 1) / d);
+    }
+  }
+  return U;
+}
+
 int validate_common(lsmsx_ctx *c, int nsc, int nspola) {
   if (!c) return fail(-1, "null context");
   if (c->lmax_tab < 0) return fail(-2, "tables not set");
@@ -577,6 +613,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
   const bool mixed = precision == LSMSX_PREC_MIXED_FP32 || split;
   const bool refining = mixed || refined64;
   if (nsolve <= 0) return 0;
+  if (c->rel && (nsc != 2 || nspola != 2))
+    return fail(-4, "the relativistic store needs n_spin_cant = n_spin_pola = 2");
   HIPCHK(hipSetDevice(c->device));
   c->last_iters.assign(nsolve, 0);
   c->last_minpiv.assign(nsolve, 0.0);
@@ -612,6 +650,12 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       return fail(-6, "refined precision modes need kkrsz_ns = 16 or 32");
     if (a.kkrsz > (c->lmax_tab + 1) * (c->lmax_tab + 1))
       return fail(-6, "atom lmax exceeds the lmax of the uploaded tables");
+    if (c->rel) {
+      if (2 * a.kkrsz != c->rel_kk2) return fail(-6, "atom lmax differs from the relativistic store's");
+      for (int s = 0; s < a.numLIZ; s++)
+        if (a.lmax[s] != a.lmax[0])
+          return fail(-6, "relativistic blocks need every LIZ site at the central lmax");
+    }
     if (iies[i] < 0 || iies[i] >= c->group) return fail(-7, "energy slot out of range");
     if (ispins[i] < 0 || ispins[i] >= nspola) return fail(-7, "spin index out of range");
     for (int s = 0; s < a.numLIZ; s++)
@@ -899,6 +943,17 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
         if (int r2 = launch_lu<double>(c, c->desc2.p, nf, fnt)) return r2;
       }
     }
+    if (c->rel) {  // kappa-mu tau00 = (E/prel^2) U tau00' U^T
+      std::vector<double2> fi(nb);
+      for (int b = 0; b < nb; b++) fi[b] = c->rel_finv[iies[done + b]];
+      if (int r = c->relf.ensure((size_t)nb)) return r;
+      HIPCHK(hipMemcpyAsync(c->relf.p, fi.data(), nb * sizeof(double2), hipMemcpyHostToDevice,
+                            c->stream));
+      hipLaunchKernelGGL(k_rel_tau, dim3(nb), dim3(256), 0, c->stream, c->tau.p, kc, c->relU.p,
+                         c->relf.p);
+      if (int r = check_launch("k_rel_tau")) return r;
+      HIPCHK(hipStreamSynchronize(c->stream));  // fi is read by the copy above
+    }
     HIPCHK(hipMemcpyAsync(tau_out + (long)done * kc * kc * 2, c->tau.p,
                           (size_t)nb * kc * kc * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
     std::vector<int> hinfo(nb);
@@ -1017,6 +1072,7 @@ int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
                 const double *energies, const double *prel, int nsc, int nspola, double *tau_out,
                 double *lloyd_out, const double *tau_bar, double *tmat_grad, int *info_out) {
   if (int r = validate_common(c, nsc, nspola)) return r;
+  if (c->rel) return fail(-6, "the adjoint route does not take a relativistic store");
   if (nsolve <= 0) return 0;
   if (lloyd_out && !c->dtmat.p) return fail(-3, "Lloyd term requested but no dt/dE store uploaded");
   if (lloyd_out && !energies) return fail(-2, "Lloyd term needs the energies");
@@ -1328,6 +1384,8 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->cgnt.release();
   c->illp.release();
   c->tmat.release();
+  c->relU.release();
+  c->relf.release();
   c->ss_r.release();
   c->ss_v.release();
   c->ss_coef.release();
@@ -1423,6 +1481,44 @@ int lsmsx_upload_tmats(lsmsx_ctx *c, const double *tmat, long ldT, int numStore,
   c->blk = blk;
   c->nstore = numStore;
   c->group = groupSize;
+  c->rel = 0;
+  return 0;
+}
+
+int lsmsx_upload_tmats_rel(lsmsx_ctx *c, const double *tmat, long ldT, int numStore, long blk,
+                           int groupSize, const double *energies) {
+  if (!c) return fail(-1, "null context");
+  if (c->lmax_tab < 0) return fail(-2, "tables not set");
+  if (!energies) return fail(-2, "null energies");
+  const int kk2 = 2 * (c->lmax_tab + 1) * (c->lmax_tab + 1);
+  if (blk < (long)kk2 * kk2) return fail(-2, "blkSizeTmatStore smaller than (2 kkrsz)^2");
+  if (int r = lsmsx_upload_tmats(c, tmat, ldT, numStore, blk, groupSize)) return r;
+  const double c2inv = 1.0 / ((2.0 * 137.035999206) * (2.0 * 137.035999206));
+  std::vector<double2> fac(groupSize);
+  c->rel_finv.assign(groupSize, make_double2(0.0, 0.0));
+  for (int e = 0; e < groupSize; e++) {
+    const std::complex<double> E(energies[2 * e], energies[2 * e + 1]);
+    if (E == 0.0) return fail(-2, "relativistic store at E = 0");
+    // setgij.f:52: fac = psq / ce, psq = prel^2 (calculateTauMatrix.cpp:219)
+    const std::complex<double> prel = std::sqrt(E * (1.0 + E * c2inv));
+    const std::complex<double> f = prel * prel / E;
+    fac[e] = make_double2(f.real(), f.imag());
+    const std::complex<double> g = 1.0 / f;
+    c->rel_finv[e] = make_double2(g.real(), g.imag());
+  }
+  const std::vector<double> U = clebsch_u(c->lmax_tab);
+  if (int r = c->relU.ensure(U.size())) return r;
+  if (int r = c->relf.ensure((size_t)groupSize)) return r;
+  HIPCHK(hipMemcpy(c->relU.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->relf.p, fac.data(), groupSize * sizeof(double2), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_rel_tstore, dim3(numStore, groupSize), dim3(256), 0, c->stream, c->tmat.p,
+                     ldT, blk, kk2, c->relU.p, c->relf.p);
+  if (int r = check_launch("k_rel_tstore")) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
+  c->rel = 1;
+  c->rel_kk2 = kk2;
   return 0;
 }
 
@@ -1508,6 +1604,16 @@ int lsmsx_build_kkr(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double 
     launch_assemble(c, nsc, dim3(a.numLIZ, 1), c->stream, c->desc.p);
   }
   if (int r = check_launch("k_assemble")) return r;
+  if (c->rel) {  // the reference's kappa-mu blocks U M'_ij U^T
+    if (nsc != 2 || 2 * a.kkrsz != c->rel_kk2)
+      return fail(-6, "the relativistic store needs n_spin_cant = 2 and the store's lmax");
+    for (int s = 0; s < a.numLIZ; s++)
+      if (a.lmax[s] != a.lmax[0])
+        return fail(-6, "relativistic blocks need every LIZ site at the central lmax");
+    hipLaunchKernelGGL(k_rel_blocks, dim3(a.numLIZ, a.numLIZ), dim3(256), 0, c->stream, c->M.p,
+                       (long)N, c->rel_kk2, c->relU.p);
+    if (int r = check_launch("k_rel_blocks")) return r;
+  }
   HIPCHK(hipMemcpy2DAsync(m, sizeof(double2) * ldm, c->M.p, sizeof(double2) * N,
                           sizeof(double2) * N, N, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1704,6 +1810,7 @@ int lsmsx_tau00_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int
 int lsmsx_tau_full(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *prel, int nsc,
                    int nspola, double *tau, int ldtau, int *info) {
   if (int r = validate_common(c, nsc, nspola)) return r;
+  if (c->rel) return fail(-6, "full tau does not take a relativistic store");
   auto it = c->atoms.find(atom_id);
   if (it == c->atoms.end()) return fail(-5, "unknown atom id");
   AtomDev &a = it->second;

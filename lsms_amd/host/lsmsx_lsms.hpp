@@ -87,6 +87,23 @@ void copyTmatStoreToDevice(Local &local, int groupSize) {
         "lsmsx_upload_tmats");
 }
 
+// The same for relativity = full: the kappa-mu store with the group's
+// energies (egrd of the energy group), which set the prel^2/E factor of the
+// relativistic blocks (setgij.f:39-55).
+template <class LSMS, class Local>
+void copyTmatStoreToDeviceRelativistic(LSMS &lsms, Local &local, int groupSize,
+                                       const cplx *energies) {
+  ensureTables(lsms);  // the kappa-mu transform is sized by the tables' lmax
+  Runtime &rt = runtime();
+  const cplx *p = &local.tmatStore(0, 0);
+  int nstore = (int)local.tmatStore.n_col();
+  check(lsmsx_upload_tmats_rel(rt.ctx, reinterpret_cast<const double *>(p),
+                               (long)local.tmatStore.l_dim(), nstore,
+                               (long)local.blkSizeTmatStore, groupSize,
+                               reinterpret_cast<const double *>(energies)),
+        "lsmsx_upload_tmats_rel");
+}
+
 // Counterpart of DeviceAtom::copyFromAtom (DeviceStorage_HIP.cpp:380-390) for
 // every local atom; the local index is the device atom id.
 template <class Local>
@@ -164,11 +181,29 @@ void solveTau00MI355X(LSMS &lsms, Local &local, Atom &atom, int iie, Mat &m, Mat
 }
 
 // tau00 - t0, then the local-frame rotation: calculateTau00MinusT and
-// rotateTau00ToLocalFrameNonRelativistic (tau00Postprocess.cpp:52-101; trgtol.f).
+// rotateTau00ToLocalFrameNonRelativistic (tau00Postprocess.cpp:52-101; trgtol.f);
+// for relativity = full, tau00 itself rotated as dmatp tau00 dmat
+// (rotateTau00ToLocalFrameRelativistic, tau00Postprocess.cpp:185-192; tripmt).
 template <class LSMS, class Atom>
 void postprocessTau00(LSMS &lsms, Atom &atom, const std::vector<cplx> &tau00, const std::vector<cplx> &t0,
                       cplx *tau00_l) {
   int k = lsms.n_spin_cant * atom.kkrsz;
+  if ((int)lsms.relativity == 2) {  // Relativity::full
+    std::vector<cplx> c((size_t)k * k);
+    for (int j = 0; j < k; j++)
+      for (int i = 0; i < k; i++) {
+        cplx x = 0.0;
+        for (int q = 0; q < k; q++) x += atom.dmatp(i, q) * tau00[q + k * j];
+        c[i + k * j] = x;
+      }
+    for (int j = 0; j < k; j++)
+      for (int i = 0; i < k; i++) {
+        cplx x = 0.0;
+        for (int q = 0; q < k; q++) x += c[i + k * q] * atom.dmat(q, j);
+        tau00_l[i + k * j] = x;
+      }
+    return;
+  }
   std::vector<cplx> d(k * k);
   for (int i = 0; i < k * k; i++) d[i] = tau00[i] - t0[i];
   if (lsms.n_spin_cant == 2) {

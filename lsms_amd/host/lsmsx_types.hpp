@@ -45,6 +45,7 @@ struct AtomData {
   std::vector<int> LIZlmax, LIZStoreIdx;
   int nrmat = 0, lmax = 0, kkrsz = 0;
   Complex ubr[4], ubrd[4];        // spin rotation u, u^dagger (spin_trafo.f)
+  Matrix<Complex> dmat, dmatp;    // spin rotation of the relativistic (kappa-mu) case
   // radial mesh of the single-site solve (AtomData.hpp: r_mesh, jmt, jws,
   // rInscribed, rws)
   std::vector<Real> r_mesh;
@@ -64,6 +65,7 @@ struct LSMSSystemParameters {
   int n_spin_cant = 1, n_spin_pola = 1;
   int maxlmax = 3;
   int nrelv = 0, mtasa = 0;       // SystemParameters.hpp: relativity switch, ASA flag
+  int relativity = 0;             // Relativity: none = 0, scalar = 1, full = 2
 };
 
 }  // namespace types

@@ -599,3 +599,139 @@ void or_congauss(double ebot, double etop, int ngauss, zc *egrd, zc *dele) {
   free(x);
   free(w);
 }
+
+/* ---- relativistic KKR blocks (relativity = full) ------------------------ */
+
+/*
+ * Clebsch-Gordan tables of the (l m s) -> (kappa mu) change of basis
+ * (Misc/clebsch.f:1-60): for every l <= 12 and m, the kappa-mu row
+ * ir = 2 kappa^2 + kappa + m (+1 for the spin-up component), 1-based, gets
+ *   j = l - 1/2 (kappa = l, l > 0):  u1 = sqrt((l+m)/(2l+1)),  u2 = -sqrt((l-m)/(2l+1))
+ *   j = l + 1/2 (kappa = -l-1):      u1 = sqrt((l-m+1)/(2l+1)), u2 = sqrt((l+m+1)/(2l+1))
+ * with ind1/ind2 = the 1-based lm index l^2 + l + m + 1; an index whose
+ * coefficient is below 1e-15 is set to 1.  Arrays of 500 as the common block
+ * /cgc/ (Misc/cgc.h); element ir is at [ir - 1].
+ */
+void or_clebsch(double *u1, double *u2, int *ind1, int *ind2) {
+  for (int i = 0; i < 500; i++) {
+    u1[i] = 0.0;
+    u2[i] = 0.0;
+    ind1[i] = 1;
+    ind2[i] = 1;
+  }
+  int inr = 0;
+  for (int l = 0; l <= 12; l++) {
+    const double twolp1 = 2.0 * l + 1.0;
+    for (int m = -l; m <= l; m++) {
+      inr++;
+      if (l > 0) {
+        const int kap = l;
+        int ir = 2 * kap * kap + kap + m;
+        u1[ir - 1] = sqrt((double)(l + m) / twolp1);
+        ind1[ir - 1] = inr;
+        ir++;
+        u2[ir - 1] = -sqrt((double)(l - m) / twolp1);
+        ind2[ir - 1] = inr;
+      }
+      const int kap = -l - 1;
+      int ir = 2 * kap * kap + kap + m;
+      u1[ir - 1] = sqrt((double)(l - m + 1) / twolp1);
+      ind1[ir - 1] = inr;
+      ir++;
+      u2[ir - 1] = sqrt((double)(l + m + 1) / twolp1);
+      ind2[ir - 1] = inr;
+    }
+  }
+  for (int ir = 0; ir < inr; ir++) {
+    if (fabs(u1[ir]) < 1e-15) ind1[ir] = 1;
+    if (fabs(u2[ir]) < 1e-15) ind2[ir] = 1;
+  }
+}
+
+/*
+ * kappa-mu form of a non-relativistic kkr x kkr block a
+ * (MultipleScattering/relmtrx.f:1-45):
+ *   b(i, j) = u1(i) a(ind1(i), ind1(j)) u1(j) + u2(i) a(ind2(i), ind2(j)) u2(j),
+ * i, j = 1 .. 2 kkr (b is 2kkr x 2kkr, column-major).
+ */
+void or_relmtrx(const zc *a, int kkr, zc *b) {
+  double u1[500], u2[500];
+  int ind1[500], ind2[500];
+  or_clebsch(u1, u2, ind1, ind2);
+  const int n = 2 * kkr;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      const zc t1 = u1[i] * a[(ind1[i] - 1) + (long)kkr * (ind1[j] - 1)] * u1[j];
+      const zc t2 = u2[i] * a[(ind2[i] - 1) + (long)kkr * (ind2[j] - 1)] * u2[j];
+      b[i + (long)n * j] = t1 + t2;
+    }
+}
+
+/*
+ * Relativistic KKR matrix, F77 route (calculateTauMatrix.cpp:216-242 with
+ * nrel_rel = 1; setgij.f:39-55): every LIZ site at lmax = maxlmax,
+ * bgij = (prel^2 / E) relmtrx(gij), M(site i, site j) = -t_i bgij with t_i
+ * the full 2 kkrsz x 2 kkrsz kappa-mu t-matrix of the store.  Returns
+ * nrmat_ns (= 2 nrmat), or -1 if a site has another lmax.
+ */
+int or_build_kkr_rel(int maxlmax, int numLIZ, const double *LIZPos, const int *LIZlmax,
+                     const int *LIZStoreIdx, const zc *tmatStore, long ldTmatStore,
+                     long blkSizeTmatStore, int iie, zc prel, zc energy, zc *m, int ldm) {
+  for (int s = 0; s < numLIZ; s++)
+    if (LIZlmax[s] != maxlmax) return -1;
+  const int kkr = (maxlmax + 1) * (maxlmax + 1), kk2 = 2 * kkr;
+  const int nrmat_ns = kk2 * numLIZ;
+  double *cgnt = malloc(sizeof(double) * (maxlmax + 1) * kkr * kkr);
+  zc *illp = malloc(sizeof(zc) * kkr * kkr);
+  zc ilp1[64];
+  zc *gij = malloc(sizeof(zc) * kkr * kkr);
+  zc *bgij = malloc(sizeof(zc) * kk2 * kk2);
+  zc *tmat_n = malloc(sizeof(zc) * kk2 * kk2);
+  or_cgaunt(maxlmax, cgnt);
+  or_ifacts(maxlmax, illp, ilp1);
+  const zc fac = prel * prel / energy;
+  for (int j = 0; j < nrmat_ns; j++)
+    for (int i = 0; i < nrmat_ns; i++) m[i + (long)ldm * j] = (i == j) ? 1.0 : 0.0;
+  for (int ir1 = 0; ir1 < numLIZ; ir1++) {
+    pack_tmat(kkr, kkr, 2, 2, 0, tmatStore + (long)ldTmatStore * LIZStoreIdx[ir1],
+              (long)iie * blkSizeTmatStore, tmat_n);
+    for (int ir2 = 0; ir2 < numLIZ; ir2++) {
+      if (ir1 == ir2) continue;
+      double rij[3];
+      for (int d = 0; d < 3; d++) rij[d] = LIZPos[d + 3 * ir1] - LIZPos[d + 3 * ir2];
+      or_makegij(maxlmax, maxlmax, maxlmax, prel, rij, cgnt, illp, ilp1, gij);
+      or_relmtrx(gij, kkr, bgij);
+      for (int e = 0; e < kk2 * kk2; e++) bgij[e] = fac * bgij[e];
+      for (int c = 0; c < kk2; c++)
+        for (int r = 0; r < kk2; r++) {
+          zc s = 0.0;
+          for (int k = 0; k < kk2; k++) s += tmat_n[r + kk2 * k] * bgij[k + kk2 * c];
+          m[(kk2 * ir1 + r) + (long)ldm * (kk2 * ir2 + c)] = -s;
+        }
+    }
+  }
+  free(cgnt); free(illp); free(gij); free(bgij); free(tmat_n);
+  return nrmat_ns;
+}
+
+/*
+ * b <- u b ust for n x n matrices (SingleSite/matops.f tripmt): the local
+ * frame of the relativistic tau00, tau00_l = dmatp tau00 dmat
+ * (tau00Postprocess.cpp:185-192; tau_inv_postproc.f:79-130).
+ */
+void or_tripmt(const zc *u, zc *b, const zc *ust, int n) {
+  zc *c = malloc(sizeof(zc) * n * n);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      zc x = 0.0;
+      for (int k = 0; k < n; k++) x += b[k + (long)n * j] * u[i + (long)n * k];
+      c[i + (long)n * j] = x;
+    }
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      zc x = 0.0;
+      for (int k = 0; k < n; k++) x += ust[k + (long)n * j] * c[i + (long)n * k];
+      b[i + (long)n * j] = x;
+    }
+  free(c);
+}

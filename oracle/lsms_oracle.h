@@ -96,7 +96,16 @@ int or_green_mt_canted(int lmax, const zc *tau00_l, const zc *zlr, const zc *jlr
                        const double *r_mesh, int jmt, int jws, double r_sph, int iprpts,
                        const zc *w, zc *dos, zc *green);
 
+/* relativistic KKR blocks (relativity = full) */
+void or_clebsch(double *u1, double *u2, int *ind1, int *ind2);
+void or_relmtrx(const zc *a, int kkr, zc *b);
+int or_build_kkr_rel(int maxlmax, int numLIZ, const double *LIZPos, const int *LIZlmax,
+                     const int *LIZStoreIdx, const zc *tmatStore, long ldTmatStore,
+                     long blkSizeTmatStore, int iie, zc prel, zc energy, zc *m, int ldm);
+void or_tripmt(const zc *u, zc *b, const zc *ust, int n);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

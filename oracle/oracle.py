@@ -73,6 +73,12 @@ def lib():
     L.or_green_mt.restype = C.c_int
     L.or_green_mt.argtypes = [C.c_int, _zp, _zp, _zp, _dp, C.c_int, C.c_int, C.c_double, C.c_int,
                               C.c_int, _zp, _zp, _zp, _zp]
+    L.or_clebsch.argtypes = [_dp, _dp, _ip, _ip]
+    L.or_relmtrx.argtypes = [_zp, C.c_int, _zp]
+    L.or_build_kkr_rel.restype = C.c_int
+    L.or_build_kkr_rel.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _zp, C.c_long, C.c_long,
+                                   C.c_int, _Z, _Z, _zp, C.c_int]
+    L.or_tripmt.argtypes = [_zp, _zp, _zp, C.c_int]
     _lib = L
     return L
 
@@ -284,3 +290,56 @@ def green_mt_canted(lmax, tau00_l, zlr, jlr, r_mesh, jmt, jws, r_sph, wx, wy, wz
     if rc != 0:
         raise ValueError("or_green_mt_canted: bad sizes")
     return dos, green[: 4 * jws].reshape(4, jws)
+
+
+# ---- relativistic KKR blocks -------------------------------------------------
+
+def clebsch():
+    """(u1, u2, ind1, ind2) of Misc/clebsch.f, 500 entries each (ind 1-based)."""
+    u1 = np.zeros(500, order="F")
+    u2 = np.zeros(500, order="F")
+    i1 = np.zeros(500, dtype=np.int32, order="F")
+    i2 = np.zeros(500, dtype=np.int32, order="F")
+    lib().or_clebsch(u1, u2, i1, i2)
+    return u1, u2, i1, i2
+
+
+def clebsch_matrix(lmax):
+    """U (2kkr x 2kkr): row = kappa-mu index, column = s*kkr + lm (s = 0 for
+    the u1, s = 1 for the u2 component), so relmtrx(a) = U diag_spin(a) U^T."""
+    kkr = (lmax + 1) ** 2
+    u1, u2, i1, i2 = clebsch()
+    U = np.zeros((2 * kkr, 2 * kkr))
+    for i in range(2 * kkr):
+        U[i, i1[i] - 1] += u1[i]
+        U[i, kkr + i2[i] - 1] += u2[i]
+    return U
+
+
+def relmtrx(a):
+    kkr = a.shape[0]
+    b = np.zeros((2 * kkr, 2 * kkr), dtype=np.complex128, order="F")
+    lib().or_relmtrx(_f(a, np.complex128), kkr, b)
+    return b
+
+
+def build_kkr_rel(maxlmax, liz_pos, liz_lmax, liz_store, tmat_store, blk_size, iie, prel_,
+                  energy):
+    """Relativistic M = 1 - t (prel^2/E) relmtrx(G) (setgij.f:39-55)."""
+    liz_pos = _f(liz_pos, np.float64)
+    liz_lmax = _f(liz_lmax, np.int32)
+    liz_store = _f(liz_store, np.int32)
+    ts = _f(tmat_store, np.complex128)
+    n = 2 * (maxlmax + 1) ** 2 * liz_pos.shape[1]
+    m = np.zeros((n, n), dtype=np.complex128, order="F")
+    rc = lib().or_build_kkr_rel(maxlmax, liz_pos.shape[1], liz_pos, liz_lmax, liz_store, ts,
+                                ts.shape[0], blk_size, iie, _z(prel_), _z(energy), m, n)
+    if rc < 0:
+        raise ValueError("relativistic build needs every LIZ site at maxlmax")
+    return m
+
+
+def tripmt(u, b, ust):
+    out = np.array(b, dtype=np.complex128, order="F", copy=True)
+    lib().or_tripmt(_f(u, np.complex128), out, _f(ust, np.complex128), out.shape[0])
+    return out

@@ -1,7 +1,7 @@
 """ctypes bindings to oracle/_ref/liblsms_ref.so: the reference's own leaf
-routines of the tau00 path (makegij_c.f, setgij.f, cgaunt_c.f90, gaunt.f,
-ifacts_c.f, congauss_c.f, spin_trafo.f, associatedLegendreFunction.cpp,
-quadrature.cpp),
+routines of the tau00 path (makegij_c.f, setgij.f, relmtrx.f, clebsch.f,
+cgaunt_c.f90, gaunt.f, ifacts_c.f, congauss_c.f, spin_trafo.f,
+associatedLegendreFunction.cpp, quadrature.cpp),
 compiled from their sources by `make -C oracle ref`.
 
 TEST INFRASTRUCTURE ONLY: used to pin the oracle's restatement against the
@@ -156,6 +156,45 @@ def setgij(gij, n_spin_cant, energy):
     vp = lambda a: a.ctypes.data_as(C.c_void_p)
     lib().setgij_(vp(g), vp(b), _i(kkr1), _i(kkr1 * n_spin_cant), _i(kkr2),
                   _i(kkr2 * n_spin_cant), _i(n_spin_cant), _i(0), vp(psq), vp(ce))
+    return b
+
+
+def clebsch():
+    """Misc/clebsch.f: fills the common block /cgc/ (Misc/cgc.h); returns
+    copies of u1, u2, ind1, ind2 (500 each)."""
+    lib().clebsch_()
+
+    class _Cgc(C.Structure):
+        _fields_ = [("u1", C.c_double * 500), ("u2", C.c_double * 500),
+                    ("ind1", C.c_int * 500), ("ind2", C.c_int * 500)]
+    c = _Cgc.in_dll(lib(), "cgc_")
+    return (np.array(c.u1[:]), np.array(c.u2[:]), np.array(c.ind1[:], dtype=np.int32),
+            np.array(c.ind2[:], dtype=np.int32))
+
+
+def relmtrx(a):
+    """MultipleScattering/relmtrx.f (after clebsch_ has filled /cgc/)."""
+    lib().clebsch_()
+    kkr1, kkr2 = a.shape
+    g = np.asfortranarray(a, dtype=np.complex128)
+    b = np.zeros((2 * kkr1, 2 * kkr2), dtype=np.complex128, order="F")
+    lib().relmtrx_(g.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p), _i(kkr1),
+                   _i(kkr2))
+    return b
+
+
+def setgij_rel(gij, energy, prel):
+    """MultipleScattering/setgij.f, relativistic branch (nrel_rel = 1):
+    (prel^2 / E) relmtrx(gij)."""
+    lib().clebsch_()
+    kkr = gij.shape[0]
+    b = np.zeros((2 * kkr, 2 * kkr), dtype=np.complex128, order="F")
+    g = np.asfortranarray(gij, dtype=np.complex128)
+    psq = _zarr(prel * prel)
+    ce = _zarr(energy)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib().setgij_(vp(g), vp(b), _i(kkr), _i(2 * kkr), _i(kkr), _i(2 * kkr), _i(2), _i(1),
+                  vp(psq), vp(ce))
     return b
 
 

@@ -5,13 +5,16 @@
 // the oracle.
 //
 // input  (binary, little endian): int32 maxlmax, n_spin_cant, n_spin_pola,
-//   num_local, num_store, groupSize, blkSize, lDim; complex tmatStore[lDim*num_store];
-//   complex energies[groupSize]; per atom: int32 numLIZ, double pos[3*numLIZ],
-//   int32 lmax[numLIZ], int32 store[numLIZ], complex ubr[4], complex ubrd[4]
+//   num_local, num_store, groupSize, blkSize, lDim, relativity; complex
+//   tmatStore[lDim*num_store]; complex energies[groupSize]; per atom: int32
+//   numLIZ, double pos[3*numLIZ], int32 lmax[numLIZ], int32 store[numLIZ],
+//   complex ubr[4], complex ubrd[4], and for relativity = 2 (full) complex
+//   dmat[k*k], dmatp[k*k]
 // output: complex tau00_l[groupSize][k*k][num_local*nspin] (calculateAllTauMatricesMI355X),
 //   then for atom 0, energy 0: complex M[n*n] (buildKKRMatrixMI355X),
 //   complex tau00[k*k] (solveTau00MI355X on that M), complex tau00_l[k*k]
-//   (calculateTauMatrixMI355X), complex tau[n*n] (calculateTauFullMI355X)
+//   (calculateTauMatrixMI355X), and, without relativity, complex tau[n*n]
+//   (calculateTauFullMI355X)
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -36,8 +39,8 @@ int main(int argc, char **argv) {
   }
   FILE *fi = std::fopen(argv[1], "rb");
   if (!fi) return 2;
-  int hdr[8];
-  rd(fi, hdr, 8);
+  int hdr[9];
+  rd(fi, hdr, 9);
   LSMSSystemParameters lsms;
   lsms.maxlmax = hdr[0];
   lsms.n_spin_cant = hdr[1];
@@ -47,6 +50,7 @@ int main(int argc, char **argv) {
   int nstore = hdr[4], group = hdr[5];
   local.blkSizeTmatStore = hdr[6];
   local.lDimTmatStore = hdr[7];
+  lsms.relativity = hdr[8];
   local.tmatStore.resize(local.lDimTmatStore, nstore);
   rd(fi, &local.tmatStore(0, 0), (size_t)local.lDimTmatStore * nstore);
   std::vector<Complex> energies(group);
@@ -64,13 +68,23 @@ int main(int argc, char **argv) {
     rd(fi, a.ubrd, 4);
     a.lmax = a.LIZlmax[0];
     a.kkrsz = (a.lmax + 1) * (a.lmax + 1);
+    if (lsms.relativity == 2) {
+      const int kk = lsms.n_spin_cant * a.kkrsz;
+      a.dmat.resize(kk, kk);
+      a.dmatp.resize(kk, kk);
+      rd(fi, &a.dmat(0, 0), (size_t)kk * kk);
+      rd(fi, &a.dmatp(0, 0), (size_t)kk * kk);
+    }
     a.nrmat = 0;
     for (int s = 0; s < a.numLIZ; s++) a.nrmat += (a.LIZlmax[s] + 1) * (a.LIZlmax[s] + 1);
   }
   std::fclose(fi);
 
   // the per-energy-group uploads of energyContourIntegration.cpp:462-475
-  lsmsx::copyTmatStoreToDevice(local, group);
+  if (lsms.relativity == 2)
+    lsmsx::copyTmatStoreToDeviceRelativistic(lsms, local, group, energies.data());
+  else
+    lsmsx::copyTmatStoreToDevice(local, group);
   lsmsx::copyLIZToDevice(local);
 
   int k = lsms.n_spin_cant * local.atom[0].kkrsz;
@@ -92,9 +106,11 @@ int main(int argc, char **argv) {
   std::vector<Complex> tl(k * k);
   lsmsx::calculateTauMatrixMI355X(lsms, local, local.atom[0], 0, 0, prel, tl.data(), 0);
   std::fwrite(tl.data(), sizeof(Complex), (size_t)k * k, fo);
-  Matrix<Complex> tfull;
-  lsmsx::calculateTauFullMI355X(lsms, local.atom[0], 0, 0, prel, tfull, 0);
-  std::fwrite(&tfull(0, 0), sizeof(Complex), tfull.n_row() * tfull.n_col(), fo);
+  if (lsms.relativity != 2) {
+    Matrix<Complex> tfull;
+    lsmsx::calculateTauFullMI355X(lsms, local.atom[0], 0, 0, prel, tfull, 0);
+    std::fwrite(&tfull(0, 0), sizeof(Complex), tfull.n_row() * tfull.n_col(), fo);
+  }
   std::fclose(fo);
   std::printf("host mirror driver ok: %d atoms x %d energies, k=%d\n", local.num_local, group, k);
   return 0;

@@ -13,6 +13,8 @@ using lsmsx::cplx;
 // the argument types of the reference call sites each template replaces
 template void lsmsx::ensureTables<LSMSSystemParameters>(LSMSSystemParameters &);
 template void lsmsx::copyTmatStoreToDevice<LocalTypeInfo>(LocalTypeInfo &, int);
+template void lsmsx::copyTmatStoreToDeviceRelativistic<LSMSSystemParameters, LocalTypeInfo>(
+    LSMSSystemParameters &, LocalTypeInfo &, int, const cplx *);
 template void lsmsx::copyLIZToDevice<LocalTypeInfo>(LocalTypeInfo &);
 template void lsmsx::buildKKRMatrixMI355X<LSMSSystemParameters, AtomData, Matrix<Complex>>(
     LSMSSystemParameters &, AtomData &, int, int, int, cplx, Matrix<Complex> &);

@@ -21,15 +21,23 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def write_input(path, s):
+def dmat_of(a, k):
+    """A seeded unitary spin rotation of atom a for the relativistic local
+    frame (AtomData dmat; dmatp = dmat^dagger)."""
+    rng = np.random.default_rng(100 + a)
+    q, _ = np.linalg.qr(rng.normal(size=(k, k)) + 1j * rng.normal(size=(k, k)))
+    return q, q.conj().T
+
+
+def write_input(path, s, relativity=0):
     nsc, nsp = s.n_spin_cant, s.n_spin_pola
     ts = np.asfortranarray(s.tmatStore)
     with open(path, "wb") as f:
         np.array([s.maxlmax, nsc, nsp, len(s.atoms), ts.shape[1], len(s.energies),
-                  s.blkSizeTmatStore, ts.shape[0]], dtype=np.int32).tofile(f)
+                  s.blkSizeTmatStore, ts.shape[0], relativity], dtype=np.int32).tofile(f)
         ts.flatten(order="F").astype(np.complex128).tofile(f)
         s.energies.astype(np.complex128).tofile(f)
-        for at in s.atoms:
+        for a, at in enumerate(s.atoms):
             np.array([at.numLIZ], dtype=np.int32).tofile(f)
             np.asfortranarray(at.LIZPos).flatten(order="F").tofile(f)
             at.LIZlmax.astype(np.int32).tofile(f)
@@ -37,6 +45,10 @@ def write_input(path, s):
             u, ud = O.spin_trafo(at.evec)
             u.astype(np.complex128).tofile(f)
             ud.astype(np.complex128).tofile(f)
+            if relativity == 2:
+                dm, dmp = dmat_of(a, s.kkrsz_ns)
+                dm.flatten(order="F").astype(np.complex128).tofile(f)
+                dmp.flatten(order="F").astype(np.complex128).tofile(f)
 
 
 def oracle_tau00_l(s, a, ie, ispin):
@@ -54,12 +66,12 @@ def oracle_tau00_l(s, a, ie, ispin):
     return d.flatten(order="F"), tau, M
 
 
-def run_driver(s):
+def run_driver(s, relativity=0):
     if not os.path.exists(B.DRIVER):
         pytest.fail("host_mirror_driver not built (python -m lsms_amd.build)")
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        write_input(fin, s)
+        write_input(fin, s, relativity)
         r = subprocess.run([B.DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         return np.fromfile(fout, dtype=np.complex128)
@@ -105,6 +117,48 @@ def test_host_mirror_matches_oracle(name, s):
     tf = out[pos:pos + n * n].reshape((n, n), order="F")
     ref_f = R.tau_full(s, 0, 0)
     assert np.linalg.norm(tf - ref_f) <= 1e-10 * np.linalg.norm(ref_f)
+
+
+def test_host_mirror_relativistic():
+    """relativity = full: the kappa-mu store uploaded with its energies,
+    calculateAllTauMatricesMI355X and calculateTauMatrixMI355X returning
+    dmatp tau00 dmat (no tau00 - t; tau00Postprocess.cpp:185-192), and the
+    kappa-mu M of buildKKRMatrixMI355X, against the oracle's relativistic
+    build (relmtrx.f, setgij.f:39-55) with LAPACK."""
+    s = CF.config_rel(npts=3)
+    out = run_driver(s, relativity=2)
+    na, ne, k = len(s.atoms), len(s.energies), s.kkrsz_ns
+    blk = s.blkSizeTmatStore
+
+    def ref(a, ie):
+        at = s.atoms[a]
+        m = O.build_kkr_rel(3, at.LIZPos, at.LIZlmax, at.LIZStoreIdx, s.tmatStore, blk, ie,
+                            s.prel(ie), s.energies[ie])
+        t0 = s.tmatStore[ie * blk:(ie + 1) * blk, at.LIZStoreIdx[0]].reshape(k, k, order="F")
+        tau, _ = O.solve_tau00_zgetrf(m, t0)
+        dm, dmp = dmat_of(a, k)
+        return O.tripmt(dmp, tau, dm), tau, m
+
+    pos = 0
+    for ie in range(ne):
+        blkout = out[pos:pos + k * k * na].reshape((k * k, na), order="F")
+        pos += k * k * na
+        for a in range(na):
+            tl, _, _ = ref(a, ie)
+            tl = tl.flatten(order="F")
+            assert np.linalg.norm(blkout[:, a] - tl) <= 1e-10 * np.linalg.norm(tl), (ie, a)
+    n = s.nrmat_ns(0)
+    M = out[pos:pos + n * n].reshape((n, n), order="F")
+    pos += n * n
+    ref_l, ref_tau, ref_M = ref(0, 0)
+    assert np.abs(M - ref_M).max() <= 1e-12 * np.abs(ref_M).max()
+    tau = out[pos:pos + k * k].reshape((k, k), order="F")  # solveTau00MI355X on that M
+    pos += k * k
+    assert np.linalg.norm(tau - ref_tau) <= 1e-10 * np.linalg.norm(ref_tau)
+    tl = out[pos:pos + k * k]
+    pos += k * k
+    assert np.linalg.norm(tl - ref_l.flatten(order="F")) <= 1e-10 * np.linalg.norm(ref_l)
+    assert pos == len(out)
 
 
 @pytest.mark.parametrize("nsc,nsp", [(1, 1), (1, 2), (2, 2)])

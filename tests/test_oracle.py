@@ -451,3 +451,113 @@ def test_congauss_vs_reference():
     e2, w2 = R.congauss(-0.5, 0.7, 0.0, 31)
     assert len(e2) == 32 and w2[31] == 0
     assert np.array_equal(e, e2[:31]) and np.array_equal(w, w2[:31])
+
+
+# ------------------------------------------------- relativistic KKR blocks
+@needs_ref
+def test_clebsch_equals_compiled_reference():
+    """Misc/clebsch.f: the oracle's tables equal the common block /cgc/ the
+    compiled routine fills, bit for bit, all 500 entries."""
+    a = O.clebsch()
+    b = R.clebsch()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+@needs_ref
+@pytest.mark.parametrize("lmax", [0, 1, 2, 3])
+def test_relmtrx_equals_compiled_reference(lmax):
+    rng = np.random.default_rng(lmax + 40)
+    kkr = (lmax + 1) ** 2
+    a = rng.normal(size=(kkr, kkr)) + 1j * rng.normal(size=(kkr, kkr))
+    assert np.array_equal(O.relmtrx(a), R.relmtrx(a))
+
+
+@pytest.mark.parametrize("lmax", [0, 1, 2, 3])
+def test_clebsch_matrix_orthogonal_and_relmtrx_similarity(lmax):
+    """U is real orthogonal (a change of basis (l m s) -> (kappa mu)), each
+    kappa-mu row has the norm of a Clebsch-Gordan column, and relmtrx(a) is
+    U diag_spin(a) U^T."""
+    U = O.clebsch_matrix(lmax)
+    kkr = (lmax + 1) ** 2
+    assert np.abs(U.T @ U - np.eye(2 * kkr)).max() < 1e-15
+    rng = np.random.default_rng(lmax)
+    a = rng.normal(size=(kkr, kkr)) + 1j * rng.normal(size=(kkr, kkr))
+    ds = np.kron(np.eye(2), a)
+    assert np.abs(O.relmtrx(a) - U @ ds @ U.T).max() < 1e-14 * np.abs(a).max()
+
+
+def test_clebsch_closed_form_j_blocks():
+    """Within each l the first 2l kappa-mu rows (j = l - 1/2) and the next
+    2l + 2 (j = l + 1/2) span the l block of (m, s) states: the diagonal
+    blocks of U diag(l) U^T are l on every row, and a scalar (l-independent)
+    operator stays scalar."""
+    lmax = 3
+    U = O.clebsch_matrix(lmax)
+    kkr = (lmax + 1) ** 2
+    lvec = np.array([math.isqrt(k) for k in range(kkr)])
+    L = U @ np.diag(np.concatenate([lvec, lvec])) @ U.T
+    rows = np.concatenate([[l] * (4 * l + 2) for l in range(lmax + 1)])
+    assert np.abs(L - np.diag(rows)).max() < 1e-14
+
+
+@needs_ref
+def test_build_kkr_rel_vs_reference_blocks():
+    """The relativistic M, block by block from the compiled makegij_ and the
+    relativistic branch of setgij_ (relmtrx, prel^2/E) with the full kappa-mu
+    t of each site (calculateTauMatrix.cpp:216-242)."""
+    s = CF.config_rel()
+    at = s.atoms[0]
+    ie = 2
+    e, p = s.energies[ie], s.prel(ie)
+    M = O.build_kkr_rel(3, at.LIZPos, at.LIZlmax, at.LIZStoreIdx, s.tmatStore,
+                        s.blkSizeTmatStore, ie, p, e)
+    n = at.numLIZ
+    Mr = np.eye(32 * n, dtype=complex)
+    cg = R.cgaunt(3)
+    il, i1 = R.ifacts(3)
+    blk = s.blkSizeTmatStore
+    for i in range(n):
+        t = s.tmatStore[ie * blk:(ie + 1) * blk, at.LIZStoreIdx[i]].reshape(32, 32, order="F")
+        for j in range(n):
+            if i == j:
+                continue
+            g = R.makegij(3, 3, 3, p, at.LIZPos[:, i] - at.LIZPos[:, j], cg, il, i1)
+            Mr[32 * i:32 * i + 32, 32 * j:32 * j + 32] = -t @ R.setgij_rel(g, e, p)
+    assert np.abs(M - Mr).max() <= 1e-14 * np.abs(Mr).max()
+
+
+def test_relativistic_matrix_is_a_similarity_of_the_spin_canted_one():
+    """M_rel = U~ M' U~^T with M' the spin-canted matrix of t' = (prel^2/E)
+    U^T t U, and tau00_rel = (E/prel^2) U tau00'(t0') U^T against LAPACK on
+    M_rel: the identity the device path computes with."""
+    s = CF.config_rel()
+    at = s.atoms[1]
+    ie = 1
+    e, p = s.energies[ie], s.prel(ie)
+    f = p * p / e
+    U = O.clebsch_matrix(3)
+    Mrel = O.build_kkr_rel(3, at.LIZPos, at.LIZlmax, at.LIZStoreIdx, s.tmatStore,
+                           s.blkSizeTmatStore, ie, p, e)
+    blk = s.blkSizeTmatStore
+    tp = np.array(s.tmatStore, order="F", copy=True)
+    for col in range(tp.shape[1]):
+        for k in range(len(s.energies)):
+            t = tp[k * blk:(k + 1) * blk, col].reshape(32, 32, order="F")
+            fk = s.prel(k) ** 2 / s.energies[k]
+            tp[k * blk:(k + 1) * blk, col] = (fk * U.T @ t @ U).flatten(order="F")
+    Mp = O.build_kkr(3, at.LIZPos, at.LIZlmax, at.LIZStoreIdx, tp, blk, ie, p, 2, 2)
+    Ut = np.kron(np.eye(at.numLIZ), U)
+    assert np.abs(Ut @ Mp @ Ut.T - Mrel).max() < 1e-13 * np.abs(Mrel).max()
+    t0 = s.tmatStore[ie * blk:(ie + 1) * blk, at.LIZStoreIdx[0]].reshape(32, 32, order="F")
+    ref, _ = O.solve_tau00_zgetrf(Mrel, t0)
+    t0p = tp[ie * blk:(ie + 1) * blk, at.LIZStoreIdx[0]].reshape(32, 32, order="F")
+    taup, _ = O.solve_tau00_zgetrf(Mp, t0p)
+    got = U @ taup @ U.T / f
+    assert np.linalg.norm(got - ref) < 1e-12 * np.linalg.norm(ref)
+
+
+def test_tripmt_is_the_triple_product():
+    rng = np.random.default_rng(8)
+    u, b, v = (rng.normal(size=(32, 32)) + 1j * rng.normal(size=(32, 32)) for _ in range(3))
+    assert np.abs(O.tripmt(u, b, v) - u @ b @ v).max() < 1e-12 * np.abs(u @ b @ v).max()
