@@ -314,9 +314,9 @@ int lsmsx_single_site_nonrel(lsmsx_ctx *ctx, int nrelv, int lmax, int npot, int 
  * single-site radial functions: the sphere part of green_function_
  * (MultipleScattering/green_function.f:107-124 for isp = 1, gf_local.f:109-189,
  * int_zz_zj.f:116-165, mdosms_c.f, mgreen_c.f), which is the whole-cell
- * result for ASA potentials (mtasa = 1, int_zz_zj.f:226-237).  The
- * interstitial part of muffin-tin potentials (inter_m, Voronoi cells) is
- * not computed.
+ * result for ASA potentials (mtasa = 1, int_zz_zj.f:226-237).  For
+ * muffin-tin potentials lsmsx_green_interstitial adds the interstitial part
+ * (inter_m) of the cell.
  *   pot_of[b]       the potential (0..npot-1) whose mesh solve b uses
  *   r_mesh, jmt, jws, r_sph   per potential, as for lsmsx_single_site_nonrel
  *   tau00           nsolve x kkrsz x kkrsz complex, column-major, local frame
@@ -329,6 +329,29 @@ int lsmsx_green_mt(lsmsx_ctx *ctx, int nsolve, int lmax, int iprpts, int n_spin_
                    const int *pot_of, const double *r_mesh, const int *jmt, const int *jws,
                    const double *r_sph, const double *tau00, const double *zlr,
                    const double *jlr, double *dos, double *green);
+
+/* Interstitial part of the cell integrals of muffin-tin potentials and its
+ * density of states, for nsolve (atom, energy) pairs: inter_m /
+ * interstitial (VORPOL/inter.f, VORPOL/interstitial.f), which int_zz_zj adds
+ * to the sphere part when mtasa != 1 (int_zz_zj.f:230-252).  The whole-cell
+ * DOS is the sum of lsmsx_green_mt's dos and this one (mdosms is linear in
+ * pzz and pzj).  Per solve b: prel[b] (complex), matom_left/right[b][l]
+ * (complex, l = 0..lmax; equal for collinear spins), tau00[b] (kkrsz x
+ * kkrsz complex, column-major, local frame), cell_of[b].  Per cell k (as
+ * VORPOL passes them): r_sph[k], ncrit[k] (<= maxreg + 1), grwylm and
+ * gwwylm [k][ir][ng] (radial Gauss points and weights, ngaussr per region,
+ * maxreg regions per cell, points with r <= 0 skipped), wylm [k][ir][ng][i]
+ * (complex step-function coefficients, i = l(l+1)/2 + m, m >= 0, l <=
+ * 2 lmax).  The Gaunt numbers are the uploaded tables (lmax <= their lmax).
+ * Outputs: dos[b] (complex, -Im/pi convention of mdosms, times 2 when
+ * n_spin_pola = 1 as for lsmsx_green_mt), and if not NULL pzz[b] (kkrsz x
+ * kkrsz, pzz(L2, L1) at [L2 + kkrsz L1]) and pzj[b] (complex). */
+int lsmsx_green_interstitial(lsmsx_ctx *ctx, int nsolve, int lmax, int zj_flag,
+                             int n_spin_pola, const double *prel, const double *matom_left,
+                             const double *matom_right, int ncell, const int *cell_of,
+                             const double *r_sph, const int *ncrit, int ngaussr, int maxreg,
+                             const double *grwylm, const double *gwwylm, const double *wylm,
+                             const double *tau00, double *pzz, double *pzj, double *dos);
 
 /* Keep the radial functions of the next lsmsx_single_site_nonrel calls on
  * the device (computed even when zlr/jlr are NULL) for

@@ -28,7 +28,7 @@ EXPORTED = [
     "lsmsx_single_site_nonrel", "lsmsx_green_mt", "lsmsx_set_single_site_resident",
     "lsmsx_green_mt_resident", "lsmsx_tmats_from_single_site", "lsmsx_green_mt_canted",
     "lsmsx_tmats_from_single_site_canted", "lsmsx_solve_tau00_matrix_noalloc",
-    "lsmsx_set_residual_form", "lsmsx_upload_tmats_rel",
+    "lsmsx_set_residual_form", "lsmsx_upload_tmats_rel", "lsmsx_green_interstitial",
 ]
 INFO_SWEEP_STALL = -2          # per-solve info: refinement wait bound hit (tau00 invalid)
 INFO_REFINE_UNCONVERGED = -3   # per-solve info: FP64 refinement did not converge (plain result)
@@ -74,6 +74,9 @@ def load():
     L.lsmsx_set_refinement.argtypes = [_vp, C.c_int, C.c_double]
     L.lsmsx_set_residual_form.argtypes = [_vp, C.c_int]
     L.lsmsx_upload_tmats_rel.argtypes = [_vp, _vp, C.c_long, C.c_int, C.c_long, C.c_int, _vp]
+    L.lsmsx_green_interstitial.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
+                                           C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp,
+                                           _vp, _vp, _vp, _vp, _vp]
     L.lsmsx_last_batch_stats.argtypes = [_vp, C.c_int, _vp, _vp]
     L.lsmsx_upload_dtmats.argtypes = [_vp, _vp, C.c_long, C.c_int, C.c_long, C.c_int]
     L.lsmsx_tau_full.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
@@ -165,6 +168,43 @@ class Context:
         il = np.asfortranarray(illp, dtype=np.complex128)
         i1 = np.ascontiguousarray(ilp1, dtype=np.complex128)
         _ck(load().lsmsx_set_tables(self._h, lmax, _p(cg), _p(il), _p(i1)))
+
+    def green_interstitial(self, lmax, prels, matom_left, matom_right, tau00, cells, cell_of,
+                           r_sph, zj_flag=1, n_spin_pola=1):
+        """Interstitial cell integrals and DOS: (dos (n,), pzz (n, kk, kk) row-major
+        blocks, pzj (n,)).  cells: list of (ncrit, ngaussr, grwylm (ngaussr, nreg),
+        gwwylm, wylm (nw, ngaussr, nreg)) sharing ngaussr; cell_of (n,); r_sph per cell."""
+        n = len(prels)
+        kk = (lmax + 1) ** 2
+        nw = (2 * lmax + 1) * (lmax + 1)
+        ngaussr = cells[0][1]
+        maxreg = max(cl[0] - 1 for cl in cells)
+        nc = len(cells)
+        gr = np.zeros((nc, maxreg, ngaussr))
+        gw = np.zeros((nc, maxreg, ngaussr))
+        wy = np.zeros((nc, maxreg, ngaussr, nw), dtype=np.complex128)
+        for k, (ncrit, ng, g, w, y) in enumerate(cells):
+            assert ng == ngaussr
+            gr[k, :ncrit - 1] = np.asarray(g).T
+            gw[k, :ncrit - 1] = np.asarray(w).T
+            wy[k, :ncrit - 1] = np.transpose(np.asarray(y), (2, 1, 0))
+        ncrit = np.array([cl[0] for cl in cells], dtype=np.int32)
+        pr = np.ascontiguousarray(prels, dtype=np.complex128)
+        ml = np.ascontiguousarray(matom_left, dtype=np.complex128).reshape(n, lmax + 1)
+        mr = np.ascontiguousarray(matom_right, dtype=np.complex128).reshape(n, lmax + 1)
+        tau = np.ascontiguousarray(np.transpose(np.asarray(tau00, dtype=np.complex128), (0, 2, 1)))
+        co = np.ascontiguousarray(cell_of, dtype=np.int32)
+        rs = np.ascontiguousarray(r_sph, dtype=np.float64)
+        dos = np.zeros(n, dtype=np.complex128)
+        pzz = np.zeros((n, kk, kk), dtype=np.complex128)
+        pzj = np.zeros(n, dtype=np.complex128)
+        _ck(load().lsmsx_green_interstitial(self._h, n, lmax, zj_flag, n_spin_pola, _p(pr), _p(ml),
+                                            _p(mr), nc, _p(co), _p(rs), _p(ncrit), ngaussr,
+                                            maxreg, _p(np.ascontiguousarray(gr)),
+                                            _p(np.ascontiguousarray(gw)),
+                                            _p(np.ascontiguousarray(wy)), _p(tau), _p(pzz),
+                                            _p(pzj), _p(dos)))
+        return dos, np.ascontiguousarray(np.transpose(pzz, (0, 2, 1))), pzj
 
     def upload_tmats_rel(self, tmat_store, blk, group, energies):
         """Relativistic (kappa-mu) t-matrix store of `group` energy slots

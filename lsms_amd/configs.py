@@ -424,3 +424,62 @@ def config_rel(npts=4, seed=20261203, rcut=6.0, ncell=(1, 1, 1)):
     return System(name="rel_bcc", maxlmax=3, n_spin_cant=2, n_spin_pola=2, atoms=base.atoms,
                   energies=base.energies, weights=base.weights, tmatStore=store,
                   blkSizeTmatStore=blk, deltas=base.deltas, evecs=base.evecs)
+
+
+def _gl(n, a, b):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (b - a) * x + 0.5 * (b + a), 0.5 * (b - a) * w
+
+
+def sphere_cell(lmax, r_lo, r_hi, ngauss=12):
+    """Cell data (ncrit, ngaussr, grwylm, gwwylm, wylm) of a spherical cell of
+    radius r_hi, integrated from r_lo (which may lie inside the muffin-tin
+    sphere): one region, Gauss-Legendre points, weights times r^2 (the cell
+    integral's Jacobian), step function sigma_00 = sqrt(4 pi), other
+    sigma_lm = 0."""
+    nw = (2 * lmax + 1) * (lmax + 1)
+    r, w = _gl(ngauss, r_lo, r_hi)
+    gr = r.reshape(ngauss, 1)
+    gw = (w * r * r).reshape(ngauss, 1)
+    wy = np.zeros((nw, ngauss, 1), dtype=np.complex128)
+    wy[0] = np.sqrt(4.0 * np.pi)
+    return 2, ngauss, np.asfortranarray(gr), np.asfortranarray(gw), np.asfortranarray(wy)
+
+
+def cube_cell(lmax, a, r_sph, ngauss=8, nth=96, nph=192):
+    """Cell data of a cube of side a (the simple-cubic Voronoi cell) outside
+    the muffin-tin radius r_sph <= a/2: regions between r_sph and the
+    critical radii h, h sqrt 2, h sqrt 3 (h = a/2), Gauss-Legendre points with
+    weights times r^2, and the step function's expansion
+    sigma_lm(r) = int dOmega Theta(r rhat) conj(Y_lm(rhat)) for m >= 0 (index
+    l(l+1)/2 + m, l <= 2 lmax) by Gauss-Legendre in cos(theta) and a uniform
+    phi grid (synthetic cell data of the shape VORPOL passes on)."""
+    from scipy.special import sph_harm_y
+    h = 0.5 * a
+    crit = [r_sph, h, h * np.sqrt(2.0), h * np.sqrt(3.0)]
+    ncrit = len(crit)
+    lw = 2 * lmax
+    nw = (lw + 1) * (lmax + 1)
+    ct, wt = np.polynomial.legendre.leggauss(nth)
+    ph = (np.arange(nph) + 0.5) * 2 * np.pi / nph
+    th = np.arccos(ct)
+    TH, PH = np.meshgrid(th, ph, indexing="ij")
+    W = np.outer(wt, np.full(nph, 2 * np.pi / nph))
+    dirs = np.stack([np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)], axis=-1)
+    ylm = {}
+    for l in range(lw + 1):
+        for m in range(0, l + 1):
+            ylm[(l, m)] = np.conj(sph_harm_y(l, m, TH, PH))
+    gr = np.zeros((ngauss, ncrit - 1))
+    gw = np.zeros((ngauss, ncrit - 1))
+    wy = np.zeros((nw, ngauss, ncrit - 1), dtype=np.complex128)
+    for ir in range(ncrit - 1):
+        r, w = _gl(ngauss, crit[ir], crit[ir + 1])
+        gr[:, ir] = r
+        gw[:, ir] = w * r * r
+        for ng in range(ngauss):
+            inside = np.all(np.abs(r[ng] * dirs) <= h, axis=-1).astype(float)
+            for l in range(lw + 1):
+                for m in range(0, l + 1):
+                    wy[l * (l + 1) // 2 + m, ng, ir] = np.sum(W * inside * ylm[(l, m)])
+    return ncrit, ngauss, np.asfortranarray(gr), np.asfortranarray(gw), np.asfortranarray(wy)

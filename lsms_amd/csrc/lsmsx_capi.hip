@@ -93,6 +93,10 @@ __global__ void k_grad_reduce(const double2 *, int, const int *, const long *, c
 __global__ void k_zero_c(double2 *, long);
 __global__ void k_lu_trail_h(const SolveDesc *, int, int);
 __global__ void k_lu_cross_h(const SolveDesc *, int, int, int);
+__global__ void k_interstitial(int, int, const double2 *, const double2 *, const double2 *,
+                               const double *, const int *, const int *, int, int, const double *,
+                               const double *, const double2 *, const double *, int,
+                               const double2 *, double, double2 *, double2 *, double2 *);
 __global__ void k_rel_tstore(double2 *, long, long, int, const double *, const double2 *);
 __global__ void k_rel_tau(double2 *, int, const double *, const double2 *);
 __global__ void k_rel_blocks(double2 *, long, int, const double *);
@@ -283,6 +287,10 @@ struct lsmsx_ctx {
   std::vector<double2> rel_finv;
   DevBuf<double> relU;
   DevBuf<double2> relf;
+  // interstitial cell integrals (lsmsx_green_interstitial)
+  DevBuf<double2> it_c2;  // prel, matom left/right, wylm, tau00, pzz, pzj, dos (one buffer)
+  DevBuf<double> it_d;    // r_sph, grwylm, gwwylm
+  DevBuf<int> it_i;       // cell_of, ncrit
   double ir_tol = 1e-12;
   double pivot_guard = 1e-2;  // plain FP64 solves with a smaller tile pivot are refined
   // schedule (LSMSX_SCHED_* flags) and hipGraph replay of the batched path
@@ -1386,6 +1394,9 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->tmat.release();
   c->relU.release();
   c->relf.release();
+  c->it_c2.release();
+  c->it_d.release();
+  c->it_i.release();
   c->ss_r.release();
   c->ss_v.release();
   c->ss_coef.release();
@@ -2142,6 +2153,69 @@ int lsmsx_green_mt(lsmsx_ctx *c, int nsolve, int lmax, int iprpts, int n_spin_po
   if (green)
     HIPCHK(hipMemcpyAsync(green, c->gf_green.p, (size_t)nsolve * iprpts * sizeof(double2),
                           hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int lsmsx_green_interstitial(lsmsx_ctx *c, int nsolve, int lmax, int zj_flag, int n_spin_pola,
+                             const double *prel, const double *matom_left,
+                             const double *matom_right, int ncell, const int *cell_of,
+                             const double *r_sph, const int *ncrit, int ngaussr, int maxreg,
+                             const double *grwylm, const double *gwwylm, const double *wylm,
+                             const double *tau00, double *pzz, double *pzj, double *dos) {
+  if (!c) return fail(-1, "null context");
+  if (c->lmax_tab < 0) return fail(-2, "tables not set");
+  if (lmax < 0 || lmax > 3 || lmax > c->lmax_tab) return fail(-2, "lmax must be 0..3 and <= the tables' lmax");
+  if (zj_flag != 0 && zj_flag != 1) return fail(-2, "zj_flag must be 0 or 1");
+  if (n_spin_pola != 1 && n_spin_pola != 2) return fail(-2, "n_spin_pola must be 1 or 2");
+  if (nsolve < 0 || ncell < 1 || ngaussr < 1 || maxreg < 1) return fail(-2, "bad batch size");
+  if (nsolve == 0) return 0;
+  if (!prel || !matom_left || !matom_right || !cell_of || !r_sph || !ncrit || !grwylm ||
+      !gwwylm || !wylm || !tau00 || !dos)
+    return fail(-2, "null argument");
+  for (int k = 0; k < ncell; k++)
+    if (ncrit[k] < 1 || ncrit[k] - 1 > maxreg) return fail(-2, "ncrit outside 1..maxreg+1");
+  for (int b = 0; b < nsolve; b++)
+    if (cell_of[b] < 0 || cell_of[b] >= ncell) return fail(-2, "cell_of out of range");
+  const int nl = lmax + 1, kk = nl * nl, nw = (2 * lmax + 1) * (lmax + 1);
+  const size_t npt = (size_t)ncell * maxreg * ngaussr;
+  // one complex buffer: prel | ml | mr | wylm | tau | pzz | pzj | dos
+  const size_t o_ml = nsolve, o_mr = o_ml + (size_t)nsolve * nl, o_wy = o_mr + (size_t)nsolve * nl,
+               o_tau = o_wy + npt * nw, o_pzz = o_tau + (size_t)nsolve * kk * kk,
+               o_pzj = o_pzz + (size_t)nsolve * kk * kk, o_dos = o_pzj + nsolve,
+               n_c2 = o_dos + nsolve;
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = c->it_c2.ensure(n_c2)) return r;
+  if (int r = c->it_d.ensure(ncell + 2 * npt)) return r;
+  if (int r = c->it_i.ensure(nsolve + ncell)) return r;
+  double2 *C2 = c->it_c2.p;
+  double *D = c->it_d.p;
+  int *Iv = c->it_i.p;
+  hipStream_t st = c->stream;
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+  };
+  HIPCHK(up(C2, prel, nsolve * sizeof(double2)));
+  HIPCHK(up(C2 + o_ml, matom_left, (size_t)nsolve * nl * sizeof(double2)));
+  HIPCHK(up(C2 + o_mr, matom_right, (size_t)nsolve * nl * sizeof(double2)));
+  HIPCHK(up(C2 + o_wy, wylm, npt * nw * sizeof(double2)));
+  HIPCHK(up(C2 + o_tau, tau00, (size_t)nsolve * kk * kk * sizeof(double2)));
+  HIPCHK(up(D, r_sph, ncell * sizeof(double)));
+  HIPCHK(up(D + ncell, grwylm, npt * sizeof(double)));
+  HIPCHK(up(D + ncell + npt, gwwylm, npt * sizeof(double)));
+  HIPCHK(up(Iv, cell_of, nsolve * sizeof(int)));
+  HIPCHK(up(Iv + nsolve, ncrit, ncell * sizeof(int)));
+  hipLaunchKernelGGL(k_interstitial, dim3(nsolve), dim3(256), 0, st, lmax, zj_flag, C2, C2 + o_ml,
+                     C2 + o_mr, D, Iv, Iv + nsolve, ngaussr, maxreg, D + ncell, D + ncell + npt,
+                     C2 + o_wy, c->cgnt.p, c->lmax_tab, C2 + o_tau, n_spin_pola == 1 ? 2.0 : 1.0,
+                     C2 + o_pzz, C2 + o_pzj, C2 + o_dos);
+  if (int r = check_launch("k_interstitial")) return r;
+  HIPCHK(hipMemcpyAsync(dos, C2 + o_dos, nsolve * sizeof(double2), hipMemcpyDeviceToHost, st));
+  if (pzz)
+    HIPCHK(hipMemcpyAsync(pzz, C2 + o_pzz, (size_t)nsolve * kk * kk * sizeof(double2),
+                          hipMemcpyDeviceToHost, st));
+  if (pzj)
+    HIPCHK(hipMemcpyAsync(pzj, C2 + o_pzj, nsolve * sizeof(double2), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return 0;
 }

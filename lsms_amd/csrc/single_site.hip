@@ -1037,4 +1037,130 @@ __global__ void k_tmat_fill_canted(const double2 *__restrict__ matom,
   store[(long)atom_map[3 * a] * ldT + (long)e * blk + i + (long)j * n2] = v;
 }
 
+// Interstitial part of the cell integrals of muffin-tin potentials
+// (VORPOL/inter.f -> interstitial.f, added by int_zz_zj.f:244 for mtasa != 1)
+// and its density of states, one workgroup per (atom, energy) solve b:
+// thread e = lm1 kkr + lm2 accumulates pzz(lm2, lm1) over the cell's radial
+// Gauss points (region-major, as interstitial.f), the diagonal threads also
+// their pzj part; the spherical Bessel functions of each point are formed
+// once (zsphbesjh, as the single-site matching) and shared through LDS.  Then
+//   dos_b = -scale (sum_e pzz_e tau00_e - pzj) / pi      (mdosms_c.f)
+// with fixed-order sums.  Cell c = cell_of[b]: ncrit[c] critical radii,
+// grwylm/gwwylm [c][ir][ng] (ngaussr x maxreg per cell), wylm [c][ir][ng][nw].
+template <int LB>
+__device__ void interstitial_solve(int b, int zj_flag, const double2 *__restrict__ prel,
+                                   const double2 *__restrict__ mleft,
+                                   const double2 *__restrict__ mright,
+                                   const double *__restrict__ r_sph, const int *__restrict__ cell_of,
+                                   const int *__restrict__ ncrit, int ngaussr, int maxreg,
+                                   const double *__restrict__ grwylm,
+                                   const double *__restrict__ gwwylm,
+                                   const double2 *__restrict__ wylm,
+                                   const double *__restrict__ cgnt, int lmax_cg,
+                                   const double2 *__restrict__ tau00, double scale,
+                                   double2 *__restrict__ pzz_out, double2 *__restrict__ pzj_out,
+                                   double2 *__restrict__ dos_out) {
+  constexpr int kk = (LB + 1) * (LB + 1), nw = (2 * LB + 1) * (LB + 1);
+  __shared__ zd sbj[LB + 1], sbh[LB + 1], seiz;
+  __shared__ double red[2][256];
+  __shared__ zd szj[kk];
+  const int e = threadIdx.x, lm1 = e / kk, lm2 = e % kk;
+  const bool act = e < kk * kk;
+  const int l1 = act ? (int)sqrt((double)lm1 + 0.5) : 0, l2 = act ? (int)sqrt((double)lm2 + 0.5) : 0;
+  const int m1 = lm1 - l1 * (l1 + 1), m2 = lm2 - l2 * (l2 + 1);
+  const int c = cell_of[b], nreg = ncrit[c] - 1;
+  const double rs = r_sph[c], rt4pi = sqrt(4.0 * 3.141592653589793);
+  const int kkr_cg = (lmax_cg + 1) * (lmax_cg + 1);
+  const zd p = mk(prel[b].x, prel[b].y);
+  const zd ml = act ? mk(mleft[(long)b * (LB + 1) + l1].x, mleft[(long)b * (LB + 1) + l1].y) : mk(1.0);
+  const zd mr = act ? mk(mright[(long)b * (LB + 1) + l2].x, mright[(long)b * (LB + 1) + l2].y) : mk(1.0);
+  zd acc = mk(0.0), zj = mk(0.0);
+  for (int ir = 0; ir < nreg; ir++)
+    for (int ng = 0; ng < ngaussr; ng++) {
+      const long pt = ((long)c * maxreg + ir) * ngaussr + ng;
+      const double rg = grwylm[pt];
+      if (!(rg > 0.0)) continue;  // uniform across the workgroup
+      __syncthreads();            // the previous point's functions are read
+      if (e == 0) {
+        zd bj[LB + 1], bh[LB + 1], eiz;
+        bessel_jh<LB>(rg * p, bj, bh, eiz);
+#pragma unroll
+        for (int l = 0; l <= LB; l++) {
+          sbj[l] = bj[l];
+          sbh[l] = bh[l];
+        }
+        seiz = times_i(eiz) / rg;
+      }
+      __syncthreads();
+      if (!act) continue;
+      const double w = gwwylm[pt];
+      const double2 *__restrict__ wy = wylm + pt * nw;
+      const zd ez = seiz;
+      const zd zlm1 = ml * sbj[l1] - ez * sbh[l1];
+      const zd zlm2 = mr * sbj[l2] - ez * sbh[l2];
+      const int m3 = m1 - m2, am3 = m3 < 0 ? -m3 : m3;
+      const int dl = l1 > l2 ? l1 - l2 : l2 - l1, llow = am3 > dl ? am3 : dl;
+      zd zlzl = mk(0.0);
+      for (int l3 = l1 + l2; l3 >= llow; l3 -= 2) {
+        const double2 wv = wy[(l3 * (l3 + 1)) / 2 + am3];
+        zd step = (m3 >= 0) ? mk(wv.x, wv.y) : (double)(1 - 2 * (am3 & 1)) * mk(wv.x, -wv.y);
+        if (l3 == 0) {
+          if (fabs(rg - rs) < 1e-10) step = step - mk(0.5 * rt4pi);
+          else if (rg < rs) step = step - mk(rt4pi);
+        }
+        zlzl = zlzl + cgnt[l3 / 2 + (lmax_cg + 1) * (lm2 + kkr_cg * lm1)] * step;
+      }
+      acc = acc + (zlzl * zlm1 * zlm2) * w;
+      if (lm1 == lm2 && zj_flag == 1) zj = zj + (zlzl * zlm1 * (ez * sbh[l1] / ml)) * w;
+    }
+  // outputs: pzz, then the fixed-order sums of pzj and of pzz . tau00
+  zd pt = mk(0.0);
+  if (act) {
+    if (pzz_out) pzz_out[(long)b * kk * kk + lm2 + kk * lm1] = make_double2(acc.x, acc.y);
+    const double2 t = tau00[(long)b * kk * kk + lm2 + kk * lm1];
+    pt = acc * mk(t.x, t.y);
+    if (lm1 == lm2) szj[lm1] = zj;
+  }
+  red[0][e] = pt.x;
+  red[1][e] = pt.y;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (e < s) {
+      red[0][e] += red[0][e + s];
+      red[1][e] += red[1][e + s];
+    }
+    __syncthreads();
+  }
+  if (e == 0) {
+    zd pj = mk(0.0);
+    if (zj_flag == 1)
+      for (int l = 0; l < kk; l++) pj = pj + szj[l];
+    if (pzj_out) pzj_out[b] = make_double2(pj.x, pj.y);
+    const zd d = mk(red[0][0], red[1][0]) - pj;
+    dos_out[b] = make_double2(-scale * d.x / 3.141592653589793, -scale * d.y / 3.141592653589793);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_interstitial(
+    int lmax, int zj_flag, const double2 *__restrict__ prel, const double2 *__restrict__ mleft,
+    const double2 *__restrict__ mright, const double *__restrict__ r_sph,
+    const int *__restrict__ cell_of, const int *__restrict__ ncrit, int ngaussr, int maxreg,
+    const double *__restrict__ grwylm, const double *__restrict__ gwwylm,
+    const double2 *__restrict__ wylm, const double *__restrict__ cgnt, int lmax_cg,
+    const double2 *__restrict__ tau00, double scale, double2 *__restrict__ pzz_out,
+    double2 *__restrict__ pzj_out, double2 *__restrict__ dos_out) {
+  const int b = blockIdx.x;
+#define LSMSX_INTER(LB)                                                                        \
+  interstitial_solve<LB>(b, zj_flag, prel, mleft, mright, r_sph, cell_of, ncrit, ngaussr,      \
+                         maxreg, grwylm, gwwylm, wylm, cgnt, lmax_cg, tau00, scale, pzz_out,   \
+                         pzj_out, dos_out)
+  switch (lmax) {
+    case 0: LSMSX_INTER(0); break;
+    case 1: LSMSX_INTER(1); break;
+    case 2: LSMSX_INTER(2); break;
+    default: LSMSX_INTER(3); break;
+  }
+#undef LSMSX_INTER
+}
+
 }  // namespace lsmsx

@@ -96,6 +96,11 @@ int or_green_mt_canted(int lmax, const zc *tau00_l, const zc *zlr, const zc *jlr
                        const double *r_mesh, int jmt, int jws, double r_sph, int iprpts,
                        const zc *w, zc *dos, zc *green);
 
+void or_interstitial(int zj_flag, int lmax, zc pnrel, const zc *matom_left,
+                     const zc *matom_right, double r_sph, int ncrit, int ngaussr,
+                     const double *grwylm, const double *gwwylm, const zc *wylm,
+                     const double *cgnt, int lmax_cg, zc *pzz, zc *pzj);
+
 /* relativistic KKR blocks (relativity = full) */
 void or_clebsch(double *u1, double *u2, int *ind1, int *ind2);
 void or_relmtrx(const zc *a, int kkr, zc *b);

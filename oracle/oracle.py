@@ -73,6 +73,8 @@ def lib():
     L.or_green_mt.restype = C.c_int
     L.or_green_mt.argtypes = [C.c_int, _zp, _zp, _zp, _dp, C.c_int, C.c_int, C.c_double, C.c_int,
                               C.c_int, _zp, _zp, _zp, _zp]
+    L.or_interstitial.argtypes = [C.c_int, C.c_int, _Z, _zp, _zp, C.c_double, C.c_int, C.c_int,
+                                  _dp, _dp, _zp, _dp, C.c_int, _zp, _zp]
     L.or_clebsch.argtypes = [_dp, _dp, _ip, _ip]
     L.or_relmtrx.argtypes = [_zp, C.c_int, _zp]
     L.or_build_kkr_rel.restype = C.c_int
@@ -343,3 +345,19 @@ def tripmt(u, b, ust):
     out = np.array(b, dtype=np.complex128, order="F", copy=True)
     lib().or_tripmt(_f(u, np.complex128), out, _f(ust, np.complex128), out.shape[0])
     return out
+
+
+def interstitial(zj_flag, lmax, pnrel, matom_left, matom_right, r_sph, cell, cgnt=None):
+    """Interstitial cell integrals (VORPOL/interstitial.f): (pzz, pzj).
+    cell = (ncrit, ngaussr, grwylm (ngaussr, ncrit-1), gwwylm, wylm (nw, ngaussr, ncrit-1))."""
+    ncrit, ngaussr, gr, gw, wy = cell
+    if cgnt is None:
+        cgnt = cgaunt(lmax)
+    kkr = (lmax + 1) ** 2
+    pzz = np.zeros((kkr, kkr), dtype=np.complex128, order="F")
+    pzj = np.zeros(1, dtype=np.complex128)
+    lib().or_interstitial(int(zj_flag), lmax, _z(pnrel), _f(matom_left, np.complex128),
+                          _f(matom_right, np.complex128), float(r_sph), ncrit, ngaussr,
+                          _f(gr, np.float64), _f(gw, np.float64), _f(wy, np.complex128),
+                          _f(cgnt, np.float64), cgnt.shape[0] - 1, pzz, pzj)
+    return pzz, complex(pzj[0])

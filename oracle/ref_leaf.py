@@ -298,3 +298,25 @@ def mgreen(zj_flag, lmax, jws, zlr, jlr, tau):
     _ssl().mgreen_(_i(zj_flag), _i(tau.shape[0]), _i(lmax), _i(jws), _i(zlr.shape[0]), vp(zlr),
                    vp(zlr), vp(jlr), vp(green), vp(tau), _i(-1), _ISTOP, C.c_size_t(32))
     return green
+
+
+def inter_m(zj_flag, lmax, pnrel, matom_left, matom_right, r_sph, cell, cgnt):
+    """VORPOL/inter.f (inter_m -> interstitial): (pzz, pzj, pzj_full)."""
+    ncrit, ngaussr, gr, gw, wy = cell
+    kkr = (lmax + 1) ** 2
+    lcg = cgnt.shape[0] - 1
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    ml = np.asfortranarray(matom_left, dtype=np.complex128)
+    mr = np.asfortranarray(matom_right, dtype=np.complex128)
+    g = np.asfortranarray(gr, dtype=np.float64)
+    w = np.asfortranarray(gw, dtype=np.float64)
+    y = np.asfortranarray(wy, dtype=np.complex128)
+    cg = np.asfortranarray(cgnt, dtype=np.float64)
+    pzz = np.zeros((kkr, kkr), dtype=np.complex128, order="F")
+    pzj = np.zeros(1, dtype=np.complex128)
+    pzf = np.zeros(kkr, dtype=np.complex128)
+    pn = _zarr(pnrel)
+    _ssl().inter_m_(_i(zj_flag), _i(lmax), vp(pn), vp(ml), vp(mr), _d(r_sph), _i(ngaussr),
+                    vp(cg), _i(lcg), vp(pzz), vp(pzj), vp(pzf), _i(ncrit), vp(g), vp(w), vp(y),
+                    _i(0), _ISTOP, C.c_size_t(32))
+    return pzz, complex(pzj[0]), pzf

@@ -720,3 +720,71 @@ int or_green_mt_canted(int lmax, const zc *tau00_l, const zc *zlr, const zc *jlr
     }
     return 0;
 }
+
+/* ---- the interstitial part of the cell integrals (VORPOL/inter.f,
+ * VORPOL/interstitial.f:1-162): for muffin-tin potentials (mtasa != 1)
+ * int_zz_zj.f:244 adds, over the region between the muffin-tin sphere and
+ * the circumscribing sphere of the Voronoi cell,
+ *   pzz(L2,L1) = sum_points w Z_L1 Z_L2 sum_l3 cgnt(l3/2, L2, L1) sigma_l3,m1-m2(r)
+ *   pzj        = sum_points w sum_L1 Z_L1 J_L1 sum_l3 cgnt(l3/2, L1, L1) sigma_l3,0(r)
+ * with Z_l = matom_l j_l(pr) - (i e^{ipr}/r) h_l(pr), J_l = (i e^{ipr}/r)
+ * h_l(pr) / matom_l (zsphbesjh: h_l scaled by e^{ix}/x), the cell's step
+ * function sigma_lm(r) from wylm (m >= 0 stored, sigma_{l,-m} = (-1)^m
+ * conj(sigma_lm)), the muffin-tin sphere taken out of sigma_00 (by sqrt(4 pi)
+ * inside r_sph, half of it on r_sph to 1e-10), and Gauss points grwylm with
+ * weights gwwylm per region (points with r <= 0 skipped).
+ *   cgnt:  Gaunt table cgnt[l3/2 + (lmax_cg+1)(j1 + kkr_cg j2)], kkr_cg = (lmax_cg+1)^2
+ *   grwylm, gwwylm: ngaussr x (ncrit-1); wylm: (2lmax+1)(lmax+1) x ngaussr x (ncrit-1)
+ *   pzz: (lmax+1)^2 x (lmax+1)^2 column-major, pzz(L2, L1) at [L2 + kkr L1]. */
+void or_interstitial(int zj_flag, int lmax, zc pnrel, const zc *matom_left,
+                     const zc *matom_right, double r_sph, int ncrit, int ngaussr,
+                     const double *grwylm, const double *gwwylm, const zc *wylm,
+                     const double *cgnt, int lmax_cg, zc *pzz, zc *pzj)
+{
+    const int kkr = (lmax + 1) * (lmax + 1), kkr_cg = (lmax_cg + 1) * (lmax_cg + 1);
+    const int nw = (2 * lmax + 1) * (lmax + 1);
+    const double pi = 3.141592653589793, rt4pi = sqrt(4.0 * pi);
+    zc bj[16], bh[16], eiz;
+    for (int e = 0; e < kkr * kkr; e++) pzz[e] = 0.0;
+    *pzj = 0.0;
+    for (int ir = 0; ir < ncrit - 1; ir++)
+        for (int ng = 0; ng < ngaussr; ng++) {
+            const double rg = grwylm[ng + (long)ngaussr * ir];
+            if (!(rg > 0.0)) continue;
+            const double w = gwwylm[ng + (long)ngaussr * ir];
+            const zc *wy = wylm + (long)nw * (ng + (long)ngaussr * ir);
+            sph_bessel_jh(lmax, pnrel * rg, bj, bh, &eiz);
+            eiz = I * eiz / rg;
+            int lm1 = 0;
+            for (int l1 = 0; l1 <= lmax; l1++) {
+                const zc zlm1 = matom_left[l1] * bj[l1] - eiz * bh[l1];
+                const zc jlm1 = eiz * bh[l1] / matom_left[l1];
+                for (int m1 = -l1; m1 <= l1; m1++, lm1++) {
+                    zc zljl = 0.0;
+                    int lm2 = 0;
+                    for (int l2 = 0; l2 <= lmax; l2++) {
+                        const zc zlm2 = matom_right[l2] * bj[l2] - eiz * bh[l2];
+                        for (int m2 = -l2; m2 <= l2; m2++, lm2++) {
+                            zc zlzl = 0.0;
+                            const int m3 = m1 - m2, am3 = m3 < 0 ? -m3 : m3;
+                            const int dl = l1 > l2 ? l1 - l2 : l2 - l1;
+                            const int llow = am3 > dl ? am3 : dl;
+                            for (int l3 = l1 + l2; l3 >= llow; l3 -= 2) {
+                                const zc wv = wy[(l3 * (l3 + 1)) / 2 + am3];
+                                zc step = (m3 >= 0) ? wv : (double)(1 - 2 * (am3 % 2)) * conj(wv);
+                                if (l3 == 0) {
+                                    if (fabs(rg - r_sph) < 1e-10) step -= 0.5 * rt4pi;
+                                    else if (rg < r_sph) step -= rt4pi;
+                                }
+                                const double c = cgnt[l3 / 2 + (lmax_cg + 1) * (lm2 + kkr_cg * lm1)];
+                                zlzl += c * step;
+                                if (lm2 == lm1 && zj_flag == 1) zljl += c * step;
+                            }
+                            pzz[lm2 + kkr * lm1] += zlzl * zlm1 * zlm2 * w;
+                        }
+                    }
+                    *pzj += zljl * zlm1 * jlm1 * w;
+                }
+            }
+        }
+}

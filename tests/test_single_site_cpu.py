@@ -277,3 +277,68 @@ def test_local_to_global_t_inverts_trgtol():
         ml = O.trgtol(16, 16, u, ud, CF.local_to_global_t(tu, td, ev))
         assert np.abs(ml[:, :, 0] - tu).max() < 1e-14 and np.abs(ml[:, :, 3] - td).max() < 1e-14
         assert np.abs(ml[:, :, 1]).max() < 1e-14 and np.abs(ml[:, :, 2]).max() < 1e-14
+
+
+# --------------------------------------------- interstitial cell integrals
+@pytest.mark.skipif(not RL.ss_available(), reason="oracle/_ref/libss_ref.so not built")
+@pytest.mark.parametrize("zj_flag", [0, 1])
+def test_interstitial_equals_compiled_inter_m(zj_flag):
+    """VORPOL/inter.f -> interstitial.f compiled from the reference: pzz and
+    pzj of a cubic cell (synthetic step-function data) bit for bit, with
+    different left and right single-site matrices (spin-canted use)."""
+    cell = CF.cube_cell(3, 5.0, 2.3)
+    rng = np.random.default_rng(6 + zj_flag)
+    ml = rng.normal(size=4) + 1j * rng.normal(size=4)
+    mr = rng.normal(size=4) + 1j * rng.normal(size=4)
+    cg = O.cgaunt(3)
+    for e in (0.35 + 0.02j, -0.2 + 0.4j):
+        p = CF.prel_of(e)
+        a = O.interstitial(zj_flag, 3, p, ml, mr, 2.3, cell, cg)
+        b = RL.inter_m(zj_flag, 3, p, ml, mr, 2.3, cell, cg)
+        assert np.array_equal(a[0], b[0])
+        assert a[1] == b[1]
+
+
+def test_interstitial_spherical_cell_closed_form():
+    """A spherical cell of radius R: sigma_00 = sqrt(4 pi), so pzz is diagonal
+    with pzz_LL = int_{r_sph}^R r^2 Z_l(r)^2 dr and pzj = sum_L
+    int_{r_sph}^R r^2 Z_l J_l dr, Z_l = m_l j_l(pr) - i p h_l(pr),
+    J_l = i p h_l(pr) / m_l (h_l = j_l + i y_l), by adaptive quadrature
+    with scipy's spherical Bessel functions.  The Gauss points also cover
+    [r_lo, r_sph) inside the muffin-tin sphere, which must add nothing (the
+    sphere is taken out of sigma_00)."""
+    import scipy.integrate as si
+    import scipy.special as sp
+    lmax, rs, R_ = 3, 2.2, 2.9
+    e = 0.45 + 0.03j
+    p = complex(CF.prel_of(e))
+    rng = np.random.default_rng(2)
+    ml = rng.normal(size=4) + 1j * rng.normal(size=4)
+    inner = CF.sphere_cell(lmax, 1.6, rs, ngauss=16)   # entirely inside the sphere
+    outer = CF.sphere_cell(lmax, rs, R_, ngauss=24)
+    pz_in, pj_in = O.interstitial(1, lmax, p, ml, ml, rs, inner)
+    assert np.abs(pz_in).max() < 1e-14 and abs(pj_in) < 1e-14
+    pzz, pzj = O.interstitial(1, lmax, p, ml, ml, rs, outer)
+
+    def zl(l, r):
+        x = p * r
+        h = sp.spherical_jn(l, x) + 1j * sp.spherical_yn(l, x)
+        return ml[l] * sp.spherical_jn(l, x) - 1j * p * h, 1j * p * h / ml[l]
+
+    def cquad(f):
+        re = si.quad(lambda r: f(r).real, rs, R_, epsabs=0, epsrel=1e-13, limit=200)[0]
+        im = si.quad(lambda r: f(r).imag, rs, R_, epsabs=0, epsrel=1e-13, limit=200)[0]
+        return re + 1j * im
+
+    ref_pzj = 0.0
+    lm = 0
+    for l in range(lmax + 1):
+        zz = cquad(lambda r: r * r * zl(l, r)[0] ** 2)
+        zj = cquad(lambda r: r * r * zl(l, r)[0] * zl(l, r)[1])
+        for m in range(-l, l + 1):
+            assert abs(pzz[lm, lm] - zz) < 1e-11 * abs(zz), (l, pzz[lm, lm], zz)
+            ref_pzj += zj
+            lm += 1
+    off = pzz - np.diag(np.diag(pzz))
+    assert np.abs(off).max() < 1e-13 * np.abs(pzz).max()
+    assert abs(pzj - ref_pzj) < 1e-11 * abs(ref_pzj)
