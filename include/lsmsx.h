@@ -129,7 +129,11 @@ int lsmsx_tau00_batched(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
 
 /* tau00 from a caller-built matrix m (n x n, central kkrsz_ns block first)
  * and t0 (k x k).  m is not modified.  Same contract as solveTau00zgetrf
- * (linearSolvers_CPU.cpp:98-123) with the matrix on the host. */
+ * (linearSolvers_CPU.cpp:98-123) with the matrix on the host.  Pivoting is
+ * inside 64-row diagonal tiles; when a tile meets a zero pivot the solve is
+ * repeated once with the non-central rows in a fixed pseudo-random order
+ * (tau00 does not change under it), and info reports the first attempt's
+ * pivot only if that also fails. */
 int lsmsx_solve_tau00_matrix(lsmsx_ctx *ctx, int n, int k, const double *m,
                              int ldm, const double *t0, double *tau00, int *info);
 

@@ -102,10 +102,13 @@ __global__ void k_rel_tau(double2 *, int, const double *, const double2 *);
 __global__ void k_rel_blocks(double2 *, long, int, const double *);
 __global__ void k_lu_strip_h(const SolveDesc *, int, int, int);
 
-// Copy a caller matrix (central block first, n x n) into the solver layout.
+// Copy a caller matrix (central block first, n x n) into the solver layout;
+// rowperm (or null) reorders the n - k non-central rows: solver row xr takes
+// non-central row rowperm[xr].  A row permutation that leaves the central
+// rows in place leaves tau00 = [M^-1]_00 t0 unchanged.
 __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict__ descs,
                                                     const double2 *__restrict__ src, long lds,
-                                                    int n, int k) {
+                                                    int n, int k, const int *__restrict__ rowperm) {
   const SolveDesc &d = descs[0];
   const int Ns = d.Ns, pad = d.pad;
   const int c = blockIdx.x;  // solver-layout column
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict_
       v = make_double2(r == c ? 1.0 : 0.0, 0.0);
     } else {
       int xr = r - pad, xc = c - pad;
-      int orr = xr < n - k ? xr + k : xr - (n - k);
+      int orr = xr < n - k ? (rowperm ? rowperm[xr] : xr) + k : xr - (n - k);
       int oc = xc < n - k ? xc + k : xc - (n - k);
       v = src[orr + lds * oc];
     }
@@ -280,6 +283,7 @@ struct lsmsx_ctx {
   int ir_max_sweeps = 12;
   // residual form of the mixed path (LSMSX_RESIDUAL_*) and the G-half buffers
   int residual_form = LSMSX_RESIDUAL_AUTO;
+  DevBuf<int> rowperm;  // retry order of lsmsx_solve_tau00_matrix
   DevBuf<double2> GH, YB, GP;
   // relativistic store (lsmsx_upload_tmats_rel): the solver store holds
   // t' = f U^T t U, f = prel^2/E per energy slot; rel_finv[slot] = 1/f
@@ -1419,6 +1423,7 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->gf_w.release();
   c->M.release();
   c->GH.release();
+  c->rowperm.release();
   c->YB.release();
   c->GP.release();
   c->inv.release();
@@ -1682,18 +1687,55 @@ int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ld
   d.tau00 = c->tau.p;
   d.t0 = c->t0buf.p;
   HIPCHK(hipMemcpyAsync(c->desc.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_init_status, dim3(1), dim3(64), 0, c->stream, c->desc.p, 1);
-  hipLaunchKernelGGL(k_permute_in, dim3(Ns), dim3(256), 0, c->stream, c->desc.p, c->tmpM.p,
-                     (long)n, n, k);
-  if (int r = check_launch("k_permute_in")) return r;
-  if (int r = launch_lu<double>(c, c->desc.p, 1, d.nt)) return r;
-  int hinfo = 0;
+  // attempt 0: the caller's row order; attempt 1, only if a diagonal tile
+  // met a zero pivot: the non-central rows reordered so that each column in
+  // turn gets the largest remaining entry on the diagonal (a greedy static
+  // pivot order).  Pivoting inside 64-row tiles cannot exchange rows across
+  // tiles, so a non-singular M can still give a singular tile; a row
+  // permutation that keeps the central rows leaves tau00 unchanged and brings
+  // a nonzero into every diagonal position.  The caller sees info of the
+  // first attempt only when the second fails too.
+  int hinfo = 0, first_info = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    const int *rp = nullptr;
+    if (attempt == 1) {
+      const int nn = n - k;
+      std::vector<int> perm(nn);
+      std::vector<char> used(nn, 0);
+      for (int j = 0; j < nn; j++) {  // column j of the non-central part: caller column j + k
+        const double *col = m + 2 * ((long)ldm * (j + k) + k);
+        int best = -1;
+        double bv = -1.0;
+        for (int r = 0; r < nn; r++)
+          if (!used[r]) {
+            const double v = std::fabs(col[2 * r]) + std::fabs(col[2 * r + 1]);
+            if (v > bv) {
+              bv = v;
+              best = r;
+            }
+          }
+        used[best] = 1;
+        perm[j] = best;
+      }
+      if (int r = c->rowperm.ensure((size_t)nn)) return r;
+      HIPCHK(hipMemcpy(c->rowperm.p, perm.data(), nn * sizeof(int), hipMemcpyHostToDevice));
+      rp = c->rowperm.p;
+    }
+    hipLaunchKernelGGL(k_init_status, dim3(1), dim3(64), 0, c->stream, c->desc.p, 1);
+    hipLaunchKernelGGL(k_permute_in, dim3(Ns), dim3(256), 0, c->stream, c->desc.p, c->tmpM.p,
+                       (long)n, n, k, rp);
+    if (int r = check_launch("k_permute_in")) return r;
+    if (int r = launch_lu<double>(c, c->desc.p, 1, d.nt)) return r;
+    HIPCHK(hipMemcpyAsync(&hinfo, c->info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (attempt == 0) first_info = hinfo;
+    if (hinfo <= 0 || n == k) break;
+  }
   HIPCHK(hipMemcpyAsync(tau00, c->tau.p, sizeof(double2) * k * k, hipMemcpyDeviceToHost,
                         c->stream));
-  HIPCHK(hipMemcpyAsync(&hinfo, c->info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   drain_profile(c);
-  if (info) *info = hinfo;
+  if (info) *info = hinfo > 0 ? first_info : hinfo;
   return 0;
 }
 
@@ -1730,7 +1772,7 @@ int lsmsx_solve_tau00_matrix_noalloc(lsmsx_ctx *c, int n, int k, const double *d
   hipLaunchKernelGGL(k_put_desc, dim3(1), dim3(64), 0, c->stream, d, dd);
   hipLaunchKernelGGL(k_init_status, dim3(1), dim3(64), 0, c->stream, dd, 1);
   hipLaunchKernelGGL(k_permute_in, dim3(Ns), dim3(256), 0, c->stream, dd, (const double2 *)d_m,
-                     (long)ldm, n, k);
+                     (long)ldm, n, k, nullptr);
   if (int r = check_launch("k_permute_in")) return r;
   // no profiling events here: recording them may create events
   const bool prof = c->profiling;

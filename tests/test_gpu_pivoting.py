@@ -97,3 +97,33 @@ def test_fcc135_at_im_e_1e_4(mode):
         print(f"Im E = 1e-4, mode {mode}, E = {s.energies[iies[n]]}: {err:.2e}, "
               f"sweeps {iters[n]}, min tile pivot {minpiv[n]:.3g}")
         assert err < TAU_RTOL
+
+
+def test_singular_diagonal_tile_of_a_nonsingular_matrix():
+    """Tile-level permutation blocks: the non-central part swaps 64-row tiles
+    (0, 1) and (2, 3), so the first diagonal tile is zero although M is far
+    from singular.  Pivoting inside the tile finds no pivot; the solve is
+    repeated with the greedy static row order and must meet the FP64 bar
+    against LAPACK with info 0."""
+    rng = np.random.default_rng(23)
+    k, nn = 16, 304
+    n = k + nn
+    a = np.zeros((nn, nn), dtype=complex)
+    I = np.eye(64)
+    a[0:64, 64:128] = I
+    a[64:128, 0:64] = I
+    a[128:192, 192:256] = I
+    a[192:256, 128:192] = I
+    a[256:, 256:] = np.eye(48) + 0.1 * rng.normal(size=(48, 48))
+    m = np.zeros((n, n), dtype=complex)
+    m[:k, :k] = np.eye(k) + 0.2 * (rng.normal(size=(k, k)) + 1j * rng.normal(size=(k, k)))
+    m[:k, k:] = 0.05 * (rng.normal(size=(k, nn)) + 1j * rng.normal(size=(k, nn)))
+    m[k:, :k] = 0.05 * (rng.normal(size=(nn, k)) + 1j * rng.normal(size=(nn, k)))
+    m[k:, k:] = a
+    t0 = rng.normal(size=(k, k)) + 1j * rng.normal(size=(k, k))
+    ctx = capi.Context(0)
+    tau, info = ctx.solve_tau00_matrix(m, t0)
+    ctx.close()
+    ref, _ = O.solve_tau00_zgetrf(m, t0)
+    assert info == 0
+    assert rel(tau, ref) < TAU_RTOL
