@@ -76,11 +76,15 @@ PROFILES = [os.path.join(ROOT, "profiles", f) for f in
             ("k_lu_trail_fcc135_N2160.md", "k_lu_trail_h_fcc135_N2160.md")]
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE (copy-calibrated) on this workload; the commands and the
-    calibration are recorded in the profile summaries."""
-    key = "traffic_bytes_per_launch[%s]:" % kernel
+def profiled_value(field, kernel):
+    """A per-launch figure of `kernel` on this workload recorded in the profile
+    summaries as a `field[kernel]: value` line: `traffic_bytes_per_launch`
+    (HBM bytes from the TCC_EA0_RDREQ request counts and WRITE_SIZE, checked
+    against a copy kernel of known size), `mfma_util_counter` (MfmaUtil =
+    SQ_VALU_MFMA_BUSY_CYCLES over GRBM_GUI_ACTIVE per XCD x 1024 SIMDs) and
+    `mfma_flops_executed_per_launch` (512 x SQ_INSTS_VALU_MFMA_MOPS_<type>).
+    The summaries hold the rocprofv3 commands these come from."""
+    key = "%s[%s]:" % (field, kernel)
     for path in PROFILES:
         try:
             for line in open(path):
@@ -408,6 +412,9 @@ def main():
     t_mfma = upd_flops / (peak * 1e12)
     t_hbm = upd_bytes / (HBM_PEAK_GBS * 1e9)
     bound = "mfma" if t_mfma >= t_hbm else "hbm"
+    c2 = (args.natoms, args.nenergies) == (32, 31)
+    prof_flops = profiled_value("mfma_flops_executed_per_launch", kernel) if c2 else None
+    t_launch = t_s / max(upd_n, 1)
     path_tf = solve_flops(N, k) * solves_per_s / 1e12
 
     out = {
@@ -437,9 +444,14 @@ def main():
             "frac": round(achieved_tf / peak, 4) if bound == "mfma" else round(achieved_gbs / HBM_PEAK_GBS, 4),
             "algorithmic_flops_per_launch": upd_flops / max(upd_n, 1),
             "algorithmic_bytes_per_launch": upd_bytes / max(upd_n, 1),
-            "mfma_frac_executed": round(achieved_tf * flop_mult / peak, 4),
+            # executed MFMA flops per launch read from SQ_INSTS_VALU_MFMA_MOPS
+            # (profile of this workload) over the live launch time
+            "mfma_frac_executed": (round(prof_flops / t_launch / 1e12 / peak, 4) if prof_flops
+                                   else round(achieved_tf * flop_mult / peak, 4)),
+            "mfma_frac_executed_from": "counter" if prof_flops else "flop count",
+            "mfma_util_counter": profiled_value("mfma_util_counter", kernel) if c2 else None,
             "hbm_frac_algorithmic": round(achieved_gbs / HBM_PEAK_GBS, 4),
-            "traffic": profiled_traffic(kernel) if (args.natoms, args.nenergies) == (32, 31) else None,
+            "traffic": profiled_value("traffic_bytes_per_launch", kernel) if c2 else None,
             "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4), "launches": upd_n,
         },
         # algorithmic FP64-equivalent flops of the whole path (8/3 N^3 + 8 N^2 k per
