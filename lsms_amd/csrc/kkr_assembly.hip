@@ -41,6 +41,8 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+// site pairs of one assembly workgroup whose D values share the LDS at once
+constexpr int kAsmPairs = 40;
 constexpr int kLmaxD = 2 * kMaxLmax;                       // l <= lend of D
 constexpr int kNd = (kLmaxD + 1) * (kLmaxD + 1);           // 49 (l,m) of D
 
@@ -131,6 +133,109 @@ __device__ __forceinline__ double2 angular_lane(const AngTab &tb, int l, int m, 
   if (m == 0) return make_double2(pv, 0.0);
   if (m < 0) return cscale(make_double2(pv * cm, pv * sm), (am & 1) ? -1.0 : 1.0);
   return make_double2(pv * cm, -pv * sm);
+}
+
+// D_{l,m}(R), l <= 6, of one site pair for the |m| values of one quarter of
+// the m range (PART 0: |m| = 0, 4; 1: 1, 5; 2: 2, 6; 3: 3), written to
+// dlp[l(l+1) + m] and dlp[l(l+1) - m].  The same recurrences as hankel_lane and
+// angular_lane in the same order of operations, evaluated once per pair and
+// |m| instead of once per (l, m): the Hankel recursion runs over all l in one
+// pass and each Pbar column over its l in one pass.
+template <int PART>
+__device__ __forceinline__ void d_pair_part(const AngTab &tb, double2 prel, bool prel0, double rx,
+                                            double ry, double rz, double r,
+                                            double2 *__restrict__ dlp) {
+  double2 h[kLmaxD + 1];
+  if (prel0) {  // static limit: hfn_l = i^l (2l-1)!! / r^(l+1)
+    double v = 1.0 / r;
+    h[0] = cmul(ipow_c(0), make_double2(v, 0.0));
+#pragma unroll
+    for (int l = 1; l <= kLmaxD; l++) {
+      v *= (2.0 * l - 1.0) / r;
+      h[l] = cmul(ipow_c(l), make_double2(v, 0.0));
+    }
+  } else {
+    const double2 z = make_double2(prel.x * r, prel.y * r);
+    const double zn = 1.0 / (z.x * z.x + z.y * z.y);
+    const double2 iz = make_double2(z.x * zn, -z.y * zn);  // 1/z
+    const double ex = exp(-z.y);
+    double sn, cs;
+    sincos(z.x, &sn, &cs);
+    const double2 e = make_double2(ex * cs, ex * sn);      // e^{iz}
+    const double2 er = cscale(e, 1.0 / r);
+    double2 pm = make_double2(0.0, -1.0);
+    double2 p = make_double2(-1.0 + iz.y, -iz.x);          // -1 - i/z
+    {
+      const double2 t = cmul(cmul(pm, er), ipow_c(1));
+      h[0] = make_double2(-t.x, -t.y);
+      const double2 t1 = cmul(cmul(p, er), ipow_c(2));
+      h[1] = make_double2(-t1.x, -t1.y);
+    }
+#pragma unroll
+    for (int k = 1; k < kLmaxD; k++) {
+      const double2 t = cscale(cmul(p, iz), 2.0 * k + 1.0);
+      const double2 pn = make_double2(t.x - pm.x, t.y - pm.y);
+      pm = p;
+      p = pn;
+      const double2 hh = cmul(cmul(p, er), ipow_c(k + 2));
+      h[k + 1] = make_double2(-hh.x, -hh.y);
+    }
+  }
+  const double y00 = 0.28209479177387814347;  // 1/sqrt(4 pi)
+  const double x = rz / r, y = sqrt(1.0 - x * x);
+  const double pmag = sqrt(rx * rx + ry * ry);
+  double c1 = 0.0, s1 = 0.0;
+  if (pmag >= 1.0e-6) {
+    c1 = rx / pmag;
+    s1 = ry / pmag;
+  }
+  // Pbar_{|m|,|m|} and cos/sin(|m| phi) by the recursions of angular_lane,
+  // carried from |m| = part to part + 4
+  double pmm = y00, cm = 1.0, sm = 0.0;
+#pragma unroll
+  for (int k = 1; k <= PART; k++) {
+    pmm = -tb.pmmf[k] * y * pmm;
+    const double cn = cm * c1 - sm * s1;
+    const double snn = sm * c1 + cm * s1;
+    cm = cn;
+    sm = snn;
+  }
+#pragma unroll
+  for (int am = PART; am <= kLmaxD; am += 4) {
+    if (am > PART)
+#pragma unroll
+      for (int k = am - 3; k <= am; k++) {
+        pmm = -tb.pmmf[k] * y * pmm;
+        const double cn = cm * c1 - sm * s1;
+        const double snn = sm * c1 + cm * s1;
+        cm = cn;
+        sm = snn;
+      }
+    const double sgn = (am & 1) ? -1.0 : 1.0;
+    double p0 = pmm, p1 = 0.0;
+#pragma unroll
+    for (int l = am; l <= kLmaxD; l++) {
+      double pv;
+      if (l == am) {
+        pv = pmm;
+      } else if (l == am + 1) {
+        p1 = tb.pm1f[am] * x * pmm;
+        pv = p1;
+      } else {
+        const double pn = tb.pa[l][am] * (x * p1 - tb.pb[l][am] * p0);
+        p0 = p1;
+        p1 = pn;
+        pv = p1;
+      }
+      const int c = l * (l + 1);
+      if (am == 0) {
+        dlp[c] = cmul(h[l], make_double2(pv, 0.0));
+      } else {
+        dlp[c + am] = cmul(h[l], make_double2(pv * cm, -pv * sm));
+        dlp[c - am] = cmul(h[l], cscale(make_double2(pv * cm, pv * sm), sgn));
+      }
+    }
+  }
 }
 
 // Per-workgroup LDS set-up shared by the assembly kernels: Gaunt and i^(l-l')
@@ -333,10 +438,15 @@ __device__ __forceinline__ void store_gh(const SolveDesc &d, const double2 (&g)[
 // from one evaluation of D and G: D_{l,m}(-R) = (-1)^l D_{l,m}(R) and the
 // Gaunt coefficients vanish unless l1 + l2 + l3 is even, so
 // G_ji(L2, L1) = (-1)^(l1+l2) G_ij(L2, L1) entry by entry (exact sign flips).
-// The workgroup stages the angular and Gaunt tables in LDS (16 KiB, so that
-// many workgroups share a CU and hide the latency of the special functions);
-// each wave keeps its MFMA operands of t_i in registers and builds the pairs
-// k = w+1, w+5, ... independently.  NSC = n_spin_cant.
+// The workgroup stages the angular and Gaunt tables in LDS (16 KiB) and takes
+// its pairs in chunks of at most kAsmPairs (31 KiB of D values):
+//   1. D of every pair of the chunk, one lane per pair and one wave per
+//      quarter of the m range, so the Hankel and Legendre recursions run once
+//      per pair and |m| rather than once per (l, m) lane of a wave;
+//   2. the pairs dealt round-robin to the waves: the Gaunt sums of G from
+//      the chunk's D, then both blocks on the MFMA (each wave keeps the MFMA
+//      operands of t_i in registers).
+// NSC = n_spin_cant.
 template <int NSC>
 __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ descs,
                                                   const double2 *__restrict__ tmat, long ldT,
@@ -353,7 +463,7 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
   __shared__ double pa[kLmaxD + 1][kLmaxD + 1], pb[kLmaxD + 1][kLmaxD + 1];
   __shared__ double pmmf[kLmaxD + 1], pm1f[kLmaxD + 1];
   __shared__ int lofk[kNd];
-  __shared__ double2 dl[4][kNd];
+  __shared__ double2 dl[kAsmPairs][kNd];
 
   const int lmi = d.lmax[i], kkri = (lmi + 1) * (lmi + 1), kkri_ns = kkri * NSC;
   load_tables(cgnt_g, illp_g, lmax_tab, cg, il, pa, pb, pmmf, pm1f, lofk);
@@ -382,37 +492,43 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
 #pragma unroll
   for (int u = 0; u < 4; u++) sg[u] = ((lofk[li] + lofk[4 * u + kq]) & 1) ? -1.0 : 1.0;
 
-  for (int k = 1 + w; 2 * k <= n; k += 4) {
-    const int j = (i + k) % n;
-    if (2 * k == n && j < i) continue;  // the middle pair of an even n, once
-    const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1);
-    const int offj = d.site_off[j];
-    double2 tvj[NSC][NSC][4];
-    load_t_operand<NSC>(d, tmat, ldT, j, kkrj, tvj);
-    const double rx = xi - d.pos[3 * j], ry = yi - d.pos[3 * j + 1], rz = zi - d.pos[3 * j + 2];
-    const double r = sqrt(rx * rx + ry * ry + rz * rz);
-    // --- D_{l,m}(R_ij), one (l,m) per lane, all l <= 6
-    if (lane < kNd) {
-      const int l = lofk[lane], m = lane - l * (l + 1);
-      double2 h;
-      if (prel0) {  // static limit: hfn_l = i^l (2l-1)!! / r^(l+1)
-        double v = 1.0 / r;
-        for (int kk = 1; kk <= l; kk++) v *= (2.0 * kk - 1.0) / r;
-        h = cmul(ipow_c(l), make_double2(v, 0.0));
-      } else {
-        h = hankel_lane(prel, r, l, nullptr);
+  // the pairs j = i + k (mod n), k = 1 .. n/2, the middle pair of an even n
+  // (k = n/2, the last) once; in chunks of at most kAsmPairs
+  const int half = n / 2;
+  const int npairs = half - ((n % 2 == 0 && (i + half) % n < i) ? 1 : 0);
+  const int nchunk = (npairs + kAsmPairs - 1) / kAsmPairs;
+  const int per = nchunk ? (npairs + nchunk - 1) / nchunk : 0;
+  for (int q0 = 0; q0 < npairs; q0 += per) {
+    const int np = min(per, npairs - q0);
+    // D of the chunk's pairs: lane = pair, wave w = quarter w of the m range
+    if (lane < np) {
+      const int j = (i + q0 + lane + 1) % n;
+      const double rx = xi - d.pos[3 * j], ry = yi - d.pos[3 * j + 1], rz = zi - d.pos[3 * j + 2];
+      const double r = sqrt(rx * rx + ry * ry + rz * rz);
+      switch (w) {
+        case 0: d_pair_part<0>(tb, prel, prel0, rx, ry, rz, r, dl[lane]); break;
+        case 1: d_pair_part<1>(tb, prel, prel0, rx, ry, rz, r, dl[lane]); break;
+        case 2: d_pair_part<2>(tb, prel, prel0, rx, ry, rz, r, dl[lane]); break;
+        default: d_pair_part<3>(tb, prel, prel0, rx, ry, rz, r, dl[lane]); break;
       }
-      dl[w][lane] = cmul(h, angular_lane(tb, l, m, rx, ry, rz, r));
     }
-    wave_sync();
-    double2 g[4];
-    gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[w], g);
-    if (d.gh) store_gh(d, g, sg, i, j, n);
-    store_block<NSC>(d, g, tvi, offi, kkri, offj, kkrj);
+    __syncthreads();
+    // G and both blocks of each pair, pairs dealt round-robin to the waves
+    for (int pq = w; pq < np; pq += 4) {
+      const int j = (i + q0 + pq + 1) % n;
+      const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1);
+      const int offj = d.site_off[j];
+      double2 tvj[NSC][NSC][4];
+      load_t_operand<NSC>(d, tmat, ldT, j, kkrj, tvj);
+      double2 g[4];
+      gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[pq], g);
+      if (d.gh) store_gh(d, g, sg, i, j, n);
+      store_block<NSC>(d, g, tvi, offi, kkri, offj, kkrj);
 #pragma unroll
-    for (int u = 0; u < 4; u++) g[u] = make_double2(sg[u] * g[u].x, sg[u] * g[u].y);
-    store_block<NSC>(d, g, tvj, offj, kkrj, offi, kkri);
-    wave_sync();  // dl[w] is rewritten for the next pair
+      for (int u = 0; u < 4; u++) g[u] = make_double2(sg[u] * g[u].x, sg[u] * g[u].y);
+      store_block<NSC>(d, g, tvj, offj, kkrj, offi, kkri);
+    }
+    __syncthreads();  // dl is rewritten by the next chunk
   }
 }
 
