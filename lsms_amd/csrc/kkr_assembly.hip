@@ -377,7 +377,7 @@ __device__ __forceinline__ void load_t_operand(const SolveDesc &d, const double2
 // stored to M (and the tile-major FP32 copy W).  Transposed product as in
 // the LU update, so register q of lane (li, kq) is element (li, kq + 4q) of
 // the block and each store is a run of 16 consecutive rows.
-template <int NSC>
+template <int NSC, bool WRITE_W>
 __device__ __forceinline__ void store_block(const SolveDesc &d, const double2 (&g)[4],
                                             const double2 (&tv)[NSC][NSC][4], int offr, int kkrr,
                                             int offc, int kkrc) {
@@ -404,12 +404,58 @@ __device__ __forceinline__ void store_block(const SolveDesc &d, const double2 (&
           if (jc < kkrc) {
             const double re = q1[q] - q2[q], im = q3[q] - q1[q] - q2[q];
             const int row = offr + bi * kkrr + li, col = offc + bj * kkrc + jc;
-            if (M) M[row + ld * col] = make_double2(-re, -im);
-            if (W) W[tiled_index(row, col, d.nt)] = make_float2((float)-re, (float)-im);
+            M[row + ld * col] = make_double2(-re, -im);
+            if (WRITE_W) W[tiled_index(row, col, d.nt)] = make_float2((float)-re, (float)-im);
           }
         }
       }
     }
+}
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Collinear block (r, c) = -t_r G of the FP32 copy W alone (the mixed path
+// without an FP64 M): the FP32 matrix is the factorization's input, and its
+// entries carry FP32 rounding either way, so the product runs on the FP32
+// matrix cores, complex 4M (two accumulators, no cancellation between
+// product sums).  g and tv as in store_block, rounded to FP32.  Register r
+// of lane (li, kq) is element (li, 4 kq + r) of the block (the FP32 MFMA
+// accumulator layout), stored as a run of 16 rows per column.
+__device__ __forceinline__ void store_block_w32(const SolveDesc &d, const float2 (&g)[4],
+                                                const float2 (&tv)[4], int offr, int kkrr,
+                                                int offc, int kkrc) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+  f4 re = {0.f, 0.f, 0.f, 0.f}, im = re;
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const float2 gg = g[u], t = tv[u];
+    re = __builtin_amdgcn_mfma_f32_16x16x4f32(gg.x, t.x, re, 0, 0, 0);
+    re = __builtin_amdgcn_mfma_f32_16x16x4f32(-gg.y, t.y, re, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f32_16x16x4f32(gg.x, t.y, im, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f32_16x16x4f32(gg.y, t.x, im, 0, 0, 0);
+  }
+  if (li < kkrr) {
+    float2 *__restrict__ W = d.w32;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int jc = 4 * kq + r;
+      if (jc < kkrc) W[tiled_index(offr + li, offc + jc, d.nt)] = make_float2(-re[r], -im[r]);
+    }
+  }
+}
+
+// t of site s in the b-operand layout of store_block_w32 (collinear), FP32.
+__device__ __forceinline__ void load_t_operand_w32(const SolveDesc &d,
+                                                   const double2 *__restrict__ tmat, long ldT,
+                                                   int s, int kkr, float2 (&tv)[4]) {
+  const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+  const double2 *T = tmat + ldT * d.store[s] + d.t_off;
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int ci = 4 * u + kq;
+    const double2 v = (li < kkr && ci < kkr) ? T[li + (long)d.ldt * ci] : make_double2(0.0, 0.0);
+    tv[u] = make_float2((float)v.x, (float)v.y);
+  }
 }
 
 // The pair's structure constants once (mixed path without an FP64 M), in the
@@ -446,9 +492,9 @@ __device__ __forceinline__ void store_gh(const SolveDesc &d, const double2 (&g)[
 //   2. the pairs dealt round-robin to the waves: the Gaunt sums of G from
 //      the chunk's D, then both blocks on the MFMA (each wave keeps the MFMA
 //      operands of t_i in registers).
-// NSC = n_spin_cant.
-template <int NSC>
-__global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ descs,
+// NSC = n_spin_cant; MODE = kAsmM, kAsmMW or kAsmGW (the outputs written).
+template <int NSC, int MODE>
+__global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const SolveDesc *__restrict__ descs,
                                                   const double2 *__restrict__ tmat, long ldT,
                                                   const double *__restrict__ cgnt_g,
                                                   const double2 *__restrict__ illp_g,
@@ -469,8 +515,14 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
   load_tables(cgnt_g, illp_g, lmax_tab, cg, il, pa, pb, pmmf, pm1f, lofk);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, kq = lane >> 4;
+  static_assert(MODE != kAsmGW || NSC == 1, "structure constants once per pair: collinear");
+  constexpr bool gw = MODE == kAsmGW;
+  // t_i stays in registers when collinear; the spin-canted operands (64
+  // registers) are read again from L2 for each pair instead
   double2 tvi[NSC][NSC][4];
-  load_t_operand<NSC>(d, tmat, ldT, i, kkri, tvi);
+  float2 tvi32[4];
+  if (gw) load_t_operand_w32(d, tmat, ldT, i, kkri, tvi32);
+  else if (NSC == 1) load_t_operand<NSC>(d, tmat, ldT, i, kkri, tvi);
   __syncthreads();
 
   const AngTab tb{pa, pb, pmmf, pm1f};
@@ -481,9 +533,9 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
   if (w == 0) {
     for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
       int rr = e % kkri_ns, cc = e / kkri_ns;
-      if (d.M)
+      if (MODE != kAsmGW)
         d.M[(offi + rr) + (long)d.ld * (offi + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
-      if (d.w32)
+      if (MODE != kAsmM)
         d.w32[tiled_index(offi + rr, offi + cc, d.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
     }
   }
@@ -518,24 +570,47 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc *__restrict__ 
       const int j = (i + q0 + pq + 1) % n;
       const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1);
       const int offj = d.site_off[j];
-      double2 tvj[NSC][NSC][4];
-      load_t_operand<NSC>(d, tmat, ldT, j, kkrj, tvj);
       double2 g[4];
+      if (gw) {
+        float2 tvj32[4];
+        load_t_operand_w32(d, tmat, ldT, j, kkrj, tvj32);
+        gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[pq], g);
+        store_gh(d, g, sg, i, j, n);
+        float2 gf[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) gf[u] = make_float2((float)g[u].x, (float)g[u].y);
+        store_block_w32(d, gf, tvi32, offi, kkri, offj, kkrj);
+#pragma unroll
+        for (int u = 0; u < 4; u++) gf[u] = make_float2((float)sg[u] * gf[u].x, (float)sg[u] * gf[u].y);
+        store_block_w32(d, gf, tvj32, offj, kkrj, offi, kkri);
+        continue;
+      }
+      double2 tvj[NSC][NSC][4];
+      if (NSC == 1) load_t_operand<NSC>(d, tmat, ldT, j, kkrj, tvj);
+      else load_t_operand<NSC>(d, tmat, ldT, i, kkri, tvi);
       gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[pq], g);
-      if (d.gh) store_gh(d, g, sg, i, j, n);
-      store_block<NSC>(d, g, tvi, offi, kkri, offj, kkrj);
+      store_block<NSC, MODE == kAsmMW>(d, g, tvi, offi, kkri, offj, kkrj);
+      if (NSC != 1) {  // spin-canted: t_j only once the block of t_i is done
+        __builtin_amdgcn_sched_barrier(0);
+        load_t_operand<NSC>(d, tmat, ldT, j, kkrj, tvj);
+      }
 #pragma unroll
       for (int u = 0; u < 4; u++) g[u] = make_double2(sg[u] * g[u].x, sg[u] * g[u].y);
-      store_block<NSC>(d, g, tvj, offj, kkrj, offi, kkri);
+      store_block<NSC, MODE == kAsmMW>(d, g, tvj, offj, kkrj, offi, kkri);
     }
     __syncthreads();  // dl is rewritten by the next chunk
   }
 }
 
-template __global__ void k_assemble<1>(const SolveDesc *, const double2 *, long, const double *,
-                                       const double2 *, int);
-template __global__ void k_assemble<2>(const SolveDesc *, const double2 *, long, const double *,
-                                       const double2 *, int);
+#define LSMSX_ASM_INST(NSC, MODE)                                                        \
+  template __global__ void k_assemble<NSC, MODE>(const SolveDesc *, const double2 *, long, \
+                                                 const double *, const double2 *, int);
+LSMSX_ASM_INST(1, kAsmM)
+LSMSX_ASM_INST(1, kAsmMW)
+LSMSX_ASM_INST(1, kAsmGW)
+LSMSX_ASM_INST(2, kAsmM)
+LSMSX_ASM_INST(2, kAsmMW)
+#undef LSMSX_ASM_INST
 
 // Energy derivative of the assembly contracted with the explicit inverse, one
 // workgroup per row site i = blockIdx.x of solve blockIdx.y:

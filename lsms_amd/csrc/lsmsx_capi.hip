@@ -28,7 +28,7 @@ static_assert(lsmsx::kInfoSweepStall == LSMSX_INFO_SWEEP_STALL, "info code");
 static_assert(lsmsx::kInfoRefineUnconverged == LSMSX_INFO_REFINE_UNCONVERGED, "info code");
 
 namespace lsmsx {
-template <int NSC>
+template <int NSC, int MODE>
 __global__ void k_assemble(const SolveDesc *, const double2 *, long, const double *,
                            const double2 *, int);
 __global__ void k_pad_identity(const SolveDesc *);
@@ -341,14 +341,25 @@ struct ProfScope {
   }
 };
 
-// KKR assembly of a batch (descriptors dd) with n_spin_cant = nsc.
-void launch_assemble(lsmsx_ctx *c, int nsc, dim3 grid, hipStream_t st, const SolveDesc *dd) {
-  if (nsc == 2)
-    hipLaunchKernelGGL(k_assemble<2>, grid, dim3(256), 0, st, dd, c->tmat.p, c->ldT, c->cgnt.p,
-                       c->illp.p, c->lmax_tab);
-  else
-    hipLaunchKernelGGL(k_assemble<1>, grid, dim3(256), 0, st, dd, c->tmat.p, c->ldT, c->cgnt.p,
-                       c->illp.p, c->lmax_tab);
+// KKR assembly of a batch (descriptors dd) with n_spin_cant = nsc, writing
+// the outputs `mode` names (kAsmM: M; kAsmMW: M and W; kAsmGW: the
+// structure constants once per site pair and W, collinear).
+void launch_assemble(lsmsx_ctx *c, int nsc, dim3 grid, hipStream_t st, const SolveDesc *dd,
+                     int mode = kAsmM) {
+#define LSMSX_ASM_LAUNCH(NSC, MODE)                                                         \
+  hipLaunchKernelGGL((k_assemble<NSC, MODE>), grid, dim3(256), 0, st, dd, c->tmat.p, c->ldT, \
+                     c->cgnt.p, c->illp.p, c->lmax_tab)
+  if (nsc == 2) {
+    if (mode == kAsmMW) LSMSX_ASM_LAUNCH(2, kAsmMW);
+    else LSMSX_ASM_LAUNCH(2, kAsmM);
+  } else if (mode == kAsmGW) {
+    LSMSX_ASM_LAUNCH(1, kAsmGW);
+  } else if (mode == kAsmMW) {
+    LSMSX_ASM_LAUNCH(1, kAsmMW);
+  } else {
+    LSMSX_ASM_LAUNCH(1, kAsmM);
+  }
+#undef LSMSX_ASM_LAUNCH
 }
 
 int drain_profile(lsmsx_ctx *c) {
@@ -850,7 +861,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       {
         ProfScope p(c, 0);
         hipLaunchKernelGGL(k_pad_identity, dim3(n), dim3(256), 0, st, dd);
-        launch_assemble(c, nsc, dim3(mliz, n), st, dd);
+        launch_assemble(c, nsc, dim3(mliz, n), st, dd,
+                        gmode ? kAsmGW : (mixed ? kAsmMW : kAsmM));
       }
       if (int r = check_launch("k_assemble")) return r;
       if (after_asm) HIPCHK(hipEventRecord(after_asm, st));

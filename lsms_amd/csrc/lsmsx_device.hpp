@@ -96,6 +96,13 @@ struct SolveDesc {
   long ldT;
 };
 
+// Outputs of one k_assemble launch: the FP64 M; M and its tile-major FP32
+// copy W; or the structure constants once per site pair (FP64) and W
+// (collinear only)
+constexpr int kAsmM = 0;
+constexpr int kAsmMW = 1;
+constexpr int kAsmGW = 2;
+
 // G-half residual: sites per band, waves per workgroup and sites per wave
 constexpr int kGBand = 16;
 constexpr int kGWaves = 8;
