@@ -272,10 +272,26 @@ int lsmsx_get_schedule(lsmsx_ctx *ctx, int *flags);
  * mixed path, which wait on convergence counts, stay outside the graph). */
 int lsmsx_set_graph(lsmsx_ctx *ctx, int on);
 
+/* Steps per trailing update of the split-FP16 path (LSMSX_PREC_MIXED_F16):
+ * 4, 8 (the default) or 16.  Larger groups read and write the trailing
+ * matrix less often and spend more in the cross and strip updates that join
+ * their halves; results of different group sizes differ by rounding only
+ * (each meets the refinement tolerance).  lsmsx_get_split_group returns the
+ * current size and, in ms[3] (may be NULL), the wall times of sizes 4, 8 and
+ * 16 measured by the last lsmsx_autotune at LSMSX_PREC_MIXED_F16 (-1 when
+ * not measured).  No reference counterpart (tuning of this path's
+ * factorization). */
+int lsmsx_set_split_group(lsmsx_ctx *ctx, int group);
+int lsmsx_get_split_group(lsmsx_ctx *ctx, int *group, double *ms);
+
 /* Autotuner: times the given batch (arguments as lsmsx_tau00_batched) under
  * each of the eight schedules (flag combinations), keeps the fastest on the
  * context and returns it in *best; ms[8] (may be NULL) receives the wall
- * times. */
+ * times.  At LSMSX_PREC_MIXED_F16 it then times the split group sizes 4, 8
+ * and 16 under that schedule and keeps the fastest (lsmsx_get_split_group).
+ * The tile width (64) and the update kernels' region shapes are fixed: the
+ * MFMA wave tiles, the LDS-resident diagonal LU and the tile-local pivoting
+ * are built on them. */
 int lsmsx_autotune(lsmsx_ctx *ctx, int nsolve, const int *atom_ids, const int *ispins,
                    const int *iies, const double *prel, int n_spin_cant, int n_spin_pola,
                    int precision, int *best, double *ms);

@@ -18,6 +18,7 @@ PREC_FP64_REFINED = 3
  KIND_IR_SOLVE, KIND_INVERSE, KIND_ADJOINT, KIND_CROSS) = range(10)
 
 EXPORTED = [
+    "lsmsx_set_split_group", "lsmsx_get_split_group",
     "lsmsx_last_error", "lsmsx_ctx_create", "lsmsx_ctx_destroy", "lsmsx_set_tables",
     "lsmsx_compute_tables", "lsmsx_upload_tmats", "lsmsx_set_liz", "lsmsx_build_kkr",
     "lsmsx_tau00", "lsmsx_tau00_batched", "lsmsx_solve_tau00_matrix",
@@ -485,6 +486,18 @@ class Context:
         _ck(load().lsmsx_autotune(self._h, n, _p(a), _p(s), _p(e), _p(pr), self.nsc, self.nspola,
                                   precision, C.byref(best), _p(ms)))
         return best.value, ms
+
+    def set_split_group(self, group):
+        """Steps per trailing update of the split-FP16 path: 4, 8 or 16."""
+        _ck(load().lsmsx_set_split_group(self._h, int(group)))
+
+    def split_group(self):
+        """(current split group, wall ms of groups 4, 8, 16 from the last
+        split-FP16 autotune, -1 where not measured)."""
+        g = C.c_int(0)
+        ms = np.zeros(3)
+        _ck(load().lsmsx_get_split_group(self._h, C.byref(g), _p(ms)))
+        return g.value, ms
 
     def set_residual_form(self, form):
         """RESIDUAL_AUTO (structure constants once per site pair where the

@@ -299,6 +299,8 @@ struct lsmsx_ctx {
   double pivot_guard = 1e-2;  // plain FP64 solves with a smaller tile pivot are refined
   // schedule (LSMSX_SCHED_* flags) and hipGraph replay of the batched path
   int sched = 0;
+  int split_group = 8;                     // steps per trailing update, split-FP16 path
+  double group_ms[3] = {-1.0, -1.0, -1.0};  // last autotune: split groups 4, 8, 16
   bool use_graph = false;
   std::map<std::vector<long>, hipGraphExec_t> graphs;
   // profiling
@@ -405,14 +407,15 @@ std::vector<int> site_offsets(const AtomDev &a, int nsc, bool solver) {
   return off;
 }
 
-// Steps per trailing update: eight with the split-FP16 update (taken as two
-// halves of four, joined by a strip update: the trailing matrix is read and
-// written once per eight steps while the cross updates stay at most three
-// steps deep), two otherwise.
-inline int lu_group(bool split_f16) { return split_f16 ? 8 : 2; }
+// Steps per trailing update: with the split-FP16 update the context's
+// split_group (4, 8 or 16, default 8; taken by recursive halving down to
+// blocks of four, joined by strip updates: the trailing matrix is read and
+// written once per group while the cross updates stay at most three steps
+// deep), two otherwise.
+inline int lu_group(const lsmsx_ctx *c, bool split_f16) { return split_f16 ? c->split_group : 2; }
 
 // Launch the blocked LU over a chunk of nb solves already in solver layout
-// (each descriptor's group = lu_group(split_f16)): steps taken in groups
+// (each descriptor's group = lu_group(c, split_f16)): steps taken in groups
 // g .. g+G-1 -- panels of g, then for each later step t of the group a cross
 // update of tile row/column t with the group's earlier steps and the panels
 // of t -- then one K = 64 G trailing update, so the trailing matrix is read
@@ -422,7 +425,7 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
               hipStream_t st = nullptr) {
   if (!st) st = c->stream;
   const int swz = (c->sched & LSMSX_SCHED_XCD_SWIZZLE) ? 1 : 0;
-  const int G = lu_group(split_f16);
+  const int G = lu_group(c, split_f16);
   auto diag_panel = [&](int s) -> int {
     const int nrest = nt_max - 1 - s;
     if (nrest <= 0) return 0;
@@ -689,7 +692,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     if (plain) bytes[i] += kTile * kTile * 16;
     if (mixed) bytes[i] += Ns * Ns * 8 + nt * kTile * kTile * 8 + Ns * kc * 24;
     if (refined64) bytes[i] += Ns * Ns * 16 + nt * kTile * kTile * 16 + Ns * kc * 32;
-    if (split) bytes[i] += 2 * lu_group(true) * nt * ((long)kPlaneHalves * 2 + 4);
+    if (split) bytes[i] += 2 * lu_group(c, true) * nt * ((long)kPlaneHalves * 2 + 4);
   }
   long cap = c->ws_limit;
   if (cap <= 0) {
@@ -715,7 +718,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       mtot += Ns * Ns;
       xtot += Ns * kc;
       ivtot += refining ? (Ns / kTile) * kTile * kTile : kTile * kTile;
-      htot += 2 * lu_group(split) * (Ns / kTile);
+      htot += 2 * lu_group(c, split) * (Ns / kTile);
       maxliz = std::max(maxliz, a.numLIZ);
       nt_max = std::max(nt_max, (int)(Ns / kTile));
       if (gmode) {
@@ -769,7 +772,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       if (split) {
         d.hp = c->hp.p + hoff * kPlaneHalves;
         d.he = c->he.p + hoff;
-        hoff += 2 * lu_group(split) * (Ns / kTile);
+        hoff += 2 * lu_group(c, split) * (Ns / kTile);
       }
       d.pos = a.d_pos;
       d.lmax = a.d_lmax;
@@ -804,7 +807,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       // solve groups its own steps (0..G-1), (G..2G-1), ... whatever its
       // size: the arithmetic of a solve never depends on the schedule or the
       // batch
-      d.group = lu_group(split);
+      d.group = lu_group(c, split);
       d.step0 = (c->sched & LSMSX_SCHED_RIGHT_ALIGN) ? ((nt_max - d.nt) & ~(d.group - 1)) : 0;
       d.perm = c->perm.p + (long)b * nt_max * kTile;
       d.info = c->info.p + b;
@@ -896,7 +899,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       std::memcpy(&tol_bits, &c->ir_tol, sizeof(tol_bits));
       std::vector<long> key = {nb, nt_max, maxliz, precision, c->sched, (long)c->desc.p,
                                (long)c->tmat.p, c->ldT, (long)c->cgnt.p, (long)c->illp.p,
-                               c->lmax_tab, c->ir_max_sweeps, tol_bits, gmode ? nbands_max : -1};
+                               c->lmax_tab, c->ir_max_sweeps, tol_bits, gmode ? nbands_max : -1,
+                               lu_group(c, split)};
       auto it = c->graphs.find(key);
       if (it == c->graphs.end()) {
         const bool prof = c->profiling;
@@ -1199,7 +1203,7 @@ int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       d.Ns = Ns;
       d.pad = Ns - N;
       d.nt = Ns / kTile;
-      d.group = lu_group(false);
+      d.group = lu_group(c, false);
       d.full_lu = 1;
       d.inv = c->inv.p + ioff;
       d.inv_stride = kTile * kTile;
@@ -1691,7 +1695,7 @@ int lsmsx_solve_tau00_matrix(lsmsx_ctx *c, int n, int k, const double *m, int ld
   d.Ns = Ns;
   d.pad = Ns - n;
   d.nt = Ns / kTile;
-  d.group = lu_group(false);
+  d.group = lu_group(c, false);
   d.inv = c->inv.p;
   d.perm = c->perm.p;
   d.info = c->info.p;
@@ -1772,7 +1776,7 @@ int lsmsx_solve_tau00_matrix_noalloc(lsmsx_ctx *c, int n, int k, const double *d
   d.Ns = Ns;
   d.pad = Ns - n;
   d.nt = Ns / kTile;
-  d.group = lu_group(false);
+  d.group = lu_group(c, false);
   d.inv = base + kNoAllocInvOff;
   d.inv_stride = 0;
   d.perm = (int *)(base + kNoAllocPermOff);
@@ -1930,7 +1934,7 @@ int lsmsx_tau_full(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double *
   d.Ns = Ns;
   d.pad = Ns - N;
   d.nt = nt;
-  d.group = lu_group(false);
+  d.group = lu_group(c, false);
   d.full_lu = 1;
   d.inv = c->inv.p;
   d.inv_stride = kTile * kTile;
@@ -1973,6 +1977,21 @@ int lsmsx_get_schedule(lsmsx_ctx *c, int *flags) {
   return 0;
 }
 
+int lsmsx_set_split_group(lsmsx_ctx *c, int group) {
+  if (!c) return fail(-1, "null context");
+  if (group != 4 && group != 8 && group != 16) return fail(-2, "split group must be 4, 8 or 16");
+  c->split_group = group;
+  return 0;
+}
+
+int lsmsx_get_split_group(lsmsx_ctx *c, int *group, double *ms) {
+  if (!c || !group) return fail(-1, "null argument");
+  *group = c->split_group;
+  if (ms)
+    for (int k = 0; k < 3; k++) ms[k] = c->group_ms[k];
+  return 0;
+}
+
 int lsmsx_set_graph(lsmsx_ctx *c, int on) {
   if (!c) return fail(-1, "null context");
   c->use_graph = on != 0;
@@ -2009,8 +2028,32 @@ int lsmsx_autotune(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *isp
       sbest = sch;
     }
   }
-  c->profiling = prof;
   c->sched = sbest;
+  // split-FP16 path: the steps per trailing update under the chosen schedule
+  if (precision == LSMSX_PREC_MIXED_F16 && rc == 0) {
+    const int groups[3] = {4, 8, 16};
+    int gbest = c->split_group;
+    double gt = 1e300;
+    for (int k = 0; k < 3 && rc == 0; k++) {
+      c->split_group = groups[k];
+      // first run: plane buffers of this size (and the graph, when on)
+      rc = run_batched(c, nsolve, atom_ids, ispins, iies, prel, nsc, nspola, precision,
+                       tau.data(), info.data());
+      if (rc) break;
+      auto t0 = std::chrono::steady_clock::now();
+      rc = run_batched(c, nsolve, atom_ids, ispins, iies, prel, nsc, nspola, precision,
+                       tau.data(), info.data());
+      const double dt =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      c->group_ms[k] = dt;
+      if (dt < gt) {
+        gt = dt;
+        gbest = groups[k];
+      }
+    }
+    c->split_group = gbest;
+  }
+  c->profiling = prof;
   if (best) *best = sbest;
   return rc;
 }

@@ -2,9 +2,9 @@
 //
 // The mixed path factors an FP32 copy of the KKR matrix (tau_lu.hip, T =
 // float) and refines against FP64.  Its dominant kernel is the trailing
-// update of a group of eight elimination steps, C -= sum_k L_k U_k (K = 512:
-// C is read and written once per eight steps).  Here the products run on the
-// FP16 matrix cores (v_mfma_f32_16x16x32_f16, FP32 accumulation): every FP32
+// update of a group of elimination steps (eight by default), C -= sum_k
+// L_k U_k (K = 512 for eight: C is read and written once per group).  Here
+// the products run on the FP16 matrix cores (v_mfma_f32_16x16x32_f16, FP32 accumulation): every FP32
 // panel entry was split by the panel kernel into x 2^-e = hi + lo (FP16 each,
 // one power-of-2 scale e per 64x64 tile), and each real product is
 //   x y ~ (hx hy + hx ly + lx hy) 2^(ex + ey)
@@ -60,12 +60,11 @@ __device__ __forceinline__ h8 neg8(h8 v) {
 // four groups of a 64-byte row by bits 2-3 of the row index.
 __device__ __forceinline__ int swz_group(int r, int g) { return g ^ ((r >> 2) & 3); }
 
-// Steps per trailing update on this path (the descriptors' group; a power of
-// two), taken by recursive halving down to blocks of kCrossH steps: the
-// crosses within a block apply at most kCrossH - 1 steps, and k_lu_strip_h
-// applies each first half to its second half's tile columns and rows before
-// they are factored.
-constexpr int kGroupH = 8;
+// Steps per trailing update on this path: the descriptors' group (4, 8 or
+// 16; the planes of step s sit in slot s mod group), taken by recursive
+// halving down to blocks of kCrossH steps: the crosses within a block apply
+// at most kCrossH - 1 steps, and k_lu_strip_h applies each first half to its
+// second half's tile columns and rows before they are factored.
 constexpr int kCrossH = 4;
 
 // C tiles (r0 + wt, c0 + ct), wt < nrow <= RT, ct < ncol <= NC (a 64 RT x
@@ -93,7 +92,7 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
   const long ld = tile_ld(d);
   const _Float16 *hp = (const _Float16 *)d.hp;
   auto tile_exp = [&](int s) {
-    const int par = s & (kGroupH - 1);
+    const int par = s & (d.group - 1);
     return d.he[(par * 2 + 0) * nt + ti] + d.he[(par * 2 + 1) * nt + tj];
   };
   GF2 *C = tile_ptr<float>(d, valid ? ti : r0, valid ? tj : c0) + ld * wc;
@@ -102,7 +101,7 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
   // issued together.  Tiles past the end are clamped to a real tile (their
   // waves skip the MFMAs).
   auto chunk_src = [&](int ch, int q) -> GH8 * {
-    const int s = s_first + (ch >> 1), kh = ch & 1, par = s & (kGroupH - 1);
+    const int s = s_first + (ch >> 1), kh = ch & 1, par = s & (d.group - 1);
     const int e8 = tid + NT * q;
     if (q < NQL) {
       const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
@@ -281,7 +280,7 @@ __global__ __launch_bounds__(512) void k_lu_trail_h(const SolveDesc *__restrict_
   const SolveDesc &d = descs[by];
   const int g0 = step - d.step0;
   if (g0 < 0) return;
-  const int ns = min(kGroupH, d.nt - 1 - g0);
+  const int ns = min(d.group, d.nt - 1 - g0);
   if (ns <= 0) return;
   if (d.conv && *d.conv) return;
   const int b = g0 + ns - 1;  // last step of the group

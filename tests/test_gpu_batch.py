@@ -141,6 +141,66 @@ def test_autotune_then_same_results(c4):
     ctx.set_schedule(0)
 
 
+@pytest.mark.parametrize("group", [4, 16, 8])
+def test_split_group_sizes(c4, group):
+    """Split-FP16 path with 4, 8 or 16 steps per trailing update (recursive
+    halves joined by strip updates): uneven solves meet LAPACK at 1e-10, the
+    schedules stay bitwise identical at each size, and a graph replay equals
+    the eager result."""
+    s, ctx = c4
+    atoms = pick(s, 6)
+    iies = [2, 19, 30, 7, 24, 13]
+    pr = [s.prel(i) for i in iies]
+    mixed = capi.PREC_MIXED_F16
+    ctx.set_split_group(group)
+    assert ctx.split_group()[0] == group
+    outs = []
+    for flags in (0, capi.SCHED_RIGHT_ALIGN | capi.SCHED_XCD_SWIZZLE):
+        ctx.set_schedule(flags)
+        tau, info = ctx.tau00_batched(atoms, iies, pr, 16, precision=mixed)
+        assert np.all(info == 0)
+        outs.append(tau)
+    assert np.array_equal(outs[0], outs[1])
+    ctx.set_graph(True)
+    g1, _ = ctx.tau00_batched(atoms, iies, pr, 16, precision=mixed)
+    g2, _ = ctx.tau00_batched(atoms, iies, pr, 16, precision=mixed)
+    ctx.set_graph(False)
+    assert np.array_equal(g1, outs[1]) and np.array_equal(g2, outs[1])
+    ctx.set_schedule(0)
+    ctx.set_split_group(8)
+    for b in range(len(atoms)):
+        err = rel(outs[0][b], lapack_tau00(s, atoms[b], iies[b]))
+        assert err < 1e-10, (group, b, err)
+
+
+def test_split_group_rejects_other_sizes(c4):
+    s, ctx = c4
+    for g in (0, 2, 6, 32):
+        with pytest.raises(capi.LsmsxError):
+            ctx.set_split_group(g)
+    assert ctx.split_group()[0] == 8
+
+
+def test_autotune_chooses_split_group(c4):
+    """At the split-FP16 precision the autotuner also times the group sizes
+    4, 8 and 16 under its best schedule and keeps the fastest."""
+    s, ctx = c4
+    atoms = pick(s, 8)
+    iies = list(range(8))
+    pr = [s.prel(i) for i in iies]
+    best, ms = ctx.autotune(atoms, iies, pr, precision=capi.PREC_MIXED_F16)
+    group, gms = ctx.split_group()
+    assert 0 <= best <= 7 and np.all(ms > 0)
+    assert group in (4, 8, 16) and np.all(gms > 0)
+    assert gms[(4, 8, 16).index(group)] == gms.min()
+    tau, info = ctx.tau00_batched(atoms, iies, pr, 16, precision=capi.PREC_MIXED_F16)
+    assert np.all(info == 0)
+    for b in (0, len(atoms) - 1):
+        assert rel(tau[b], lapack_tau00(s, atoms[b], iies[b])) < 1e-10
+    ctx.set_schedule(0)
+    ctx.set_split_group(8)
+
+
 def test_full_alloy_batch_graph(c4):
     """All 64 atoms x 32 energies (2048 uneven solves) in one graph launch
     sequence: a sample checked bitwise against small batches, and the
