@@ -20,7 +20,7 @@
 // per workgroup and read by two waves; the next chunk's loads are issued
 // before this chunk's MFMAs, so the L2 latency hides behind them.  The four
 // 16-byte k groups of a 64-byte LDS row are XOR-swizzled by the row, so the
-// 16 lanes of a 128-bit read hit distinct banks.  Complex products in 4M form
+// lanes of a 128-bit read hit distinct banks.  Complex products in 4M form
 // (Cr += (-Ar) Br + Ai Bi, Ci += (-Ar) Bi + (-Ai) Br) with two FP32
 // accumulator sets.  The MFMA computes D = C^T = U^T L^T: the a-operand is
 // a U plane (lane l: column l%16, k = 8(l/16)..+7), the b-operand an L plane
@@ -57,8 +57,10 @@ __device__ __forceinline__ h8 neg8(h8 v) {
 }
 
 // LDS k group of row r holding logical group g (8 halves): XOR swizzle of the
-// four groups of a 64-byte row by bits 2-3 of the row index.
-__device__ __forceinline__ int swz_group(int r, int g) { return g ^ ((r >> 2) & 3); }
+// four groups of a 64-byte row by (r >> 1) ^ (r >> 2), so that any 8 (and any
+// 16) consecutive rows read at one group cover distinct 16-byte bank groups:
+// rows of equal parity among 8 (equal r mod 4 among 16) get distinct masks.
+__device__ __forceinline__ int swz_group(int r, int g) { return g ^ (((r >> 1) ^ (r >> 2)) & 3); }
 
 // Steps per trailing update on this path: the descriptors' group (4, 8 or
 // 16; the planes of step s sit in slot s mod group), taken by recursive
