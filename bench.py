@@ -220,6 +220,8 @@ def main():
                          "precision (one stream, no graph, no autotuner, no CPU leg; --schedule "
                          "flags apply, except two streams): the command under rocprofv3 for the "
                          "kernel summaries")
+    ap.add_argument("--split-group", type=int, choices=(4, 8, 16), default=8,
+                    help="with --profile-step f16: steps per trailing update of the split path")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
@@ -274,13 +276,15 @@ def main():
                 "f16": capi.PREC_MIXED_F16}[args.profile_step]
         ctx.set_graph(False)
         ctx.set_schedule(max(args.schedule, 0) & ~capi.SCHED_TWO_STREAMS)
+        ctx.set_split_group(args.split_group)
         ctx.tau00_batched(atoms, iies, prels, k, precision=prec)
         t0 = time.perf_counter()
         tau, info = ctx.tau00_batched(atoms, iies, prels, k, precision=prec)
         el = time.perf_counter() - t0
         iters, _ = ctx.last_stats(nsolve)
         ctx.close()
-        print(json.dumps({"profile_step": args.profile_step, "solves": nsolve,
+        print(json.dumps({"profile_step": args.profile_step, "split_group": args.split_group,
+                          "solves": nsolve,
                           "ms_per_step": round(1e3 * el, 3), "info_nonzero": int(np.sum(info != 0)),
                           "mean_sweeps": round(float(np.mean(iters)), 3)}), flush=True)
         return
@@ -418,10 +422,11 @@ def main():
     t_mfma = upd_flops / (peak * 1e12)
     t_hbm = upd_bytes / (HBM_PEAK_GBS * 1e9)
     bound = "mfma" if t_mfma >= t_hbm else "hbm"
-    # the counter figures in profiles/ are of this workload with the default
-    # step groups (8 on the split path)
-    c2 = (args.natoms, args.nenergies) == (32, 31) and (headline != "split_f16_update" or group == 8)
-    prof_flops = profiled_value("mfma_flops_executed_per_launch", kernel) if c2 else None
+    # the counter figures in profiles/ are of this workload
+    c2 = (args.natoms, args.nenergies) == (32, 31)
+    # the split kernel's figures are recorded per step group (k_lu_trail_h@8 ...)
+    pkey = f"{kernel}@{group}" if headline == "split_f16_update" else kernel
+    prof_flops = profiled_value("mfma_flops_executed_per_launch", pkey) if c2 else None
     t_launch = t_s / max(upd_n, 1)
     path_tf = solve_flops(N, k) * solves_per_s / 1e12
 
@@ -457,9 +462,9 @@ def main():
             "mfma_frac_executed": (round(prof_flops / t_launch / 1e12 / peak, 4) if prof_flops
                                    else round(achieved_tf * flop_mult / peak, 4)),
             "mfma_frac_executed_from": "counter" if prof_flops else "flop count",
-            "mfma_util_counter": profiled_value("mfma_util_counter", kernel) if c2 else None,
+            "mfma_util_counter": profiled_value("mfma_util_counter", pkey) if c2 else None,
             "hbm_frac_algorithmic": round(achieved_gbs / HBM_PEAK_GBS, 4),
-            "traffic": profiled_value("traffic_bytes_per_launch", kernel) if c2 else None,
+            "traffic": profiled_value("traffic_bytes_per_launch", pkey) if c2 else None,
             "avg_launch_ms": round(upd_ms / max(upd_n, 1), 4), "launches": upd_n,
         },
         # algorithmic FP64-equivalent flops of the whole path (8/3 N^3 + 8 N^2 k per

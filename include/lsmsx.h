@@ -288,7 +288,8 @@ int lsmsx_get_split_group(lsmsx_ctx *ctx, int *group, double *ms);
  * each of the eight schedules (flag combinations), keeps the fastest on the
  * context and returns it in *best; ms[8] (may be NULL) receives the wall
  * times.  At LSMSX_PREC_MIXED_F16 it then times the split group sizes 4, 8
- * and 16 under that schedule and keeps the fastest (lsmsx_get_split_group).
+ * and 16 under that schedule and keeps the fastest (lsmsx_get_split_group),
+ * or 8 when that is within 2% of the fastest (one timed run per size).
  * The tile width (64) and the update kernels' region shapes are fixed: the
  * MFMA wave tiles, the LDS-resident diagonal LU and the tile-local pivoting
  * are built on them. */

@@ -2031,6 +2031,9 @@ int lsmsx_autotune(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *isp
   c->sched = sbest;
   // split-FP16 path: the steps per trailing update under the chosen schedule
   if (precision == LSMSX_PREC_MIXED_F16 && rc == 0) {
+    // a single timed run per size: another size replaces the default 8 only
+    // when it is faster by more than 2%, so that run-to-run noise does not
+    // decide
     const int groups[3] = {4, 8, 16};
     int gbest = c->split_group;
     double gt = 1e300;
@@ -2051,6 +2054,7 @@ int lsmsx_autotune(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *isp
         gbest = groups[k];
       }
     }
+    if (c->group_ms[1] <= 1.02 * gt) gbest = 8;
     c->split_group = gbest;
   }
   c->profiling = prof;
