@@ -227,14 +227,22 @@ __global__ __launch_bounds__(512) void k_ir_residual_g(const SolveDesc *__restri
       const int c = b + it / (kGSites * kGWaves);
       const int j = (w + st) % kGWaves, q = kGBand * c + kGSites * j + sq;
       if (it + 1 < nit) load_q(site_q(it + 1), gn, xqn);
+      // the predecessor's z sums: all four LDS reads issued together (st is
+      // uniform across the wave), one wait
+      double2 zv[4];
+      if (st > 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) zv[r] = zs[j][sq][r * 64 + lane];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) zv[r] = make_double2(0.0, 0.0);
+      }
       d4 z[3];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        double2 v = make_double2(0.0, 0.0);
-        if (st > 0) v = zs[j][sq][r * 64 + lane];
-        z[0][r] = v.x;
+        z[0][r] = zv[r].x;
         z[1][r] = 0.0;
-        z[2][r] = v.x + v.y;
+        z[2][r] = zv[r].x + zv[r].y;
       }
 #pragma unroll
       for (int sp = 0; sp < kGSites; sp++) {
