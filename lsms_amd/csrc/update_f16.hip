@@ -93,17 +93,26 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
   const int nt = d.nt;
   const long ld = tile_ld(d);
   const _Float16 *hp = (const _Float16 *)d.hp;
-  auto tile_exp = [&](int s) {
-    const int par = s & (d.group - 1);
-    return d.he[(par * 2 + 0) * nt + ti] + d.he[(par * 2 + 1) * nt + tj];
-  };
+  const int gmask = d.group - 1;  // plane slot of step s: s & gmask
+  // The tile exponents of every step of the range (ns <= 16), read once into
+  // the wave's LDS row: a global load per step inside the chunk loop would
+  // wait on every load in flight, the plane prefetches included.
+  __shared__ int exs[NT / 64][16];
+  if (lane < ns) {
+    const int par = (s_first + lane) & gmask;
+    exs[w][lane] = valid ? d.he[(par * 2 + 0) * nt + ti] + d.he[(par * 2 + 1) * nt + tj] : 0;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  auto tile_exp = [&](int s) { return exs[w][s - s_first]; };
   GF2 *C = tile_ptr<float>(d, valid ? ti : r0, valid ? tj : c0) + ld * wc;
   // Chunk ch = (step u = ch / 2, k half ch % 2): NQ loads of 8 halves per
   // thread (q < NQL: the two L row tiles, then the U column tiles), all
   // issued together.  Tiles past the end are clamped to a real tile (their
   // waves skip the MFMAs).
   auto chunk_src = [&](int ch, int q) -> GH8 * {
-    const int s = s_first + (ch >> 1), kh = ch & 1, par = s & (d.group - 1);
+    const int s = s_first + (ch >> 1), kh = ch & 1, par = s & gmask;
     const int e8 = tid + NT * q;
     if (q < NQL) {
       const int tt = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63, grp = e8 & 3;
