@@ -353,12 +353,13 @@ __device__ __forceinline__ void gaunt_lane(const double *cg, const double2 *il, 
 
 // t of site s packed as the F77 route does (calculateTauMatrix.cpp:155-183:
 // the leading kkr block of each spin block) in the b-operand layout:
-// tv[bi][bj][u] = t(bi kkrsz + li, bj kkrsz + 4u + kq), zero outside kkr.
+// tv[bi][bj][u] = t(bi kkrsz + li, bj kkrsz + 4u + kq), zero outside kkr;
+// T is the site's t block (store column and energy slot), ldt its leading
+// dimension.
 template <int NSC>
-__device__ __forceinline__ void load_t_operand(const SolveDesc &d, const double2 *__restrict__ tmat,
-                                               long ldT, int s, int kkr, double2 (&tv)[NSC][NSC][4]) {
+__device__ __forceinline__ void load_t_operand(const double2 *__restrict__ T, int kkrsz, int ldt,
+                                               int kkr, double2 (&tv)[NSC][NSC][4]) {
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
-  const double2 *T = tmat + ldT * d.store[s] + d.t_off;
 #pragma unroll
   for (int bi = 0; bi < NSC; bi++)
 #pragma unroll
@@ -367,10 +368,21 @@ __device__ __forceinline__ void load_t_operand(const SolveDesc &d, const double2
       for (int u = 0; u < 4; u++) {
         const int ci = 4 * u + kq;
         tv[bi][bj][u] = (li < kkr && ci < kkr)
-                            ? T[(bi * d.kkrsz + li) + (long)d.ldt * (bj * d.kkrsz + ci)]
+                            ? T[(bi * kkrsz + li) + (long)ldt * (bj * kkrsz + ci)]
                             : make_double2(0.0, 0.0);
       }
 }
+
+// Where the assembly writes, read from the descriptor once per workgroup
+// (a descriptor field read after a store is read again, and its wait would
+// take in every store in flight).
+struct AsmDst {
+  double2 *M;    // FP64 M (or null)
+  float2 *W;     // tile-major FP32 copy (or null)
+  double2 *gh;   // structure constants once per site pair (or null)
+  long ld;       // leading dimension of M
+  int nt;        // tiles per tile column of W
+};
 
 // Block (r, c) = -tpack_r diag_spin(G) (calculateTauMatrix.cpp:240-242) on
 // FP64 MFMA, 3M form, one 16 x 16 x 16 product per spin block pair (bi, bj),
@@ -378,13 +390,13 @@ __device__ __forceinline__ void load_t_operand(const SolveDesc &d, const double2
 // the LU update, so register q of lane (li, kq) is element (li, kq + 4q) of
 // the block and each store is a run of 16 consecutive rows.
 template <int NSC, bool WRITE_W>
-__device__ __forceinline__ void store_block(const SolveDesc &d, const double2 (&g)[4],
+__device__ __forceinline__ void store_block(const AsmDst &o, const double2 (&g)[4],
                                             const double2 (&tv)[NSC][NSC][4], int offr, int kkrr,
                                             int offc, int kkrc) {
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
-  double2 *__restrict__ M = d.M;
-  float2 *__restrict__ W = d.w32;
-  const long ld = d.ld;
+  double2 *__restrict__ M = o.M;
+  float2 *__restrict__ W = o.W;
+  const long ld = o.ld;
 #pragma unroll
   for (int bi = 0; bi < NSC; bi++)
 #pragma unroll
@@ -405,7 +417,7 @@ __device__ __forceinline__ void store_block(const SolveDesc &d, const double2 (&
             const double re = q1[q] - q2[q], im = q3[q] - q1[q] - q2[q];
             const int row = offr + bi * kkrr + li, col = offc + bj * kkrc + jc;
             M[row + ld * col] = make_double2(-re, -im);
-            if (WRITE_W) W[tiled_index(row, col, d.nt)] = make_float2((float)-re, (float)-im);
+            if (WRITE_W) W[tiled_index(row, col, o.nt)] = make_float2((float)-re, (float)-im);
           }
         }
       }
@@ -421,7 +433,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // product sums).  g and tv as in store_block, rounded to FP32.  Register r
 // of lane (li, kq) is element (li, 4 kq + r) of the block (the FP32 MFMA
 // accumulator layout), stored as a run of 16 rows per column.
-__device__ __forceinline__ void store_block_w32(const SolveDesc &d, const float2 (&g)[4],
+__device__ __forceinline__ void store_block_w32(const AsmDst &o, const float2 (&g)[4],
                                                 const float2 (&tv)[4], int offr, int kkrr,
                                                 int offc, int kkrc) {
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
@@ -435,25 +447,24 @@ __device__ __forceinline__ void store_block_w32(const SolveDesc &d, const float2
     im = __builtin_amdgcn_mfma_f32_16x16x4f32(gg.y, t.x, im, 0, 0, 0);
   }
   if (li < kkrr) {
-    float2 *__restrict__ W = d.w32;
+    float2 *__restrict__ W = o.W;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int jc = 4 * kq + r;
-      if (jc < kkrc) W[tiled_index(offr + li, offc + jc, d.nt)] = make_float2(-re[r], -im[r]);
+      if (jc < kkrc) W[tiled_index(offr + li, offc + jc, o.nt)] = make_float2(-re[r], -im[r]);
     }
   }
 }
 
-// t of site s in the b-operand layout of store_block_w32 (collinear), FP32.
-__device__ __forceinline__ void load_t_operand_w32(const SolveDesc &d,
-                                                   const double2 *__restrict__ tmat, long ldT,
-                                                   int s, int kkr, float2 (&tv)[4]) {
+// t of site s in the b-operand layout of store_block_w32 (collinear), FP32;
+// T and ldt as for load_t_operand.
+__device__ __forceinline__ void load_t_operand_w32(const double2 *__restrict__ T, int ldt, int kkr,
+                                                   float2 (&tv)[4]) {
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
-  const double2 *T = tmat + ldT * d.store[s] + d.t_off;
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int ci = 4 * u + kq;
-    const double2 v = (li < kkr && ci < kkr) ? T[li + (long)d.ldt * ci] : make_double2(0.0, 0.0);
+    const double2 v = (li < kkr && ci < kkr) ? T[li + (long)ldt * ci] : make_double2(0.0, 0.0);
     tv[u] = make_float2((float)v.x, (float)v.y);
   }
 }
@@ -462,17 +473,18 @@ __device__ __forceinline__ void load_t_operand_w32(const SolveDesc &d,
 // orientation of the smaller solver position: block gh_index(p, q), p < q,
 // holds G_pq (rows: the site at p), column-major.  Lane (li, kq) holds
 // g[u] = G_ij(4u + kq, li) (rows: site i); G_ji = (-1)^(l1+l2) G_ij entry by
-// entry (sg), so both orientations come from the same entries.  Collinear,
+// entry (bit u of sgm set: -1), so both orientations come from the same
+// entries.  Collinear,
 // every site with the central site's lmax = 3 (16 x 16 blocks).
-__device__ __forceinline__ void store_gh(const SolveDesc &d, const double2 (&g)[4],
-                                         const double (&sg)[4], int i, int j, int n) {
+__device__ __forceinline__ void store_gh(const AsmDst &o, const double2 (&g)[4],
+                                         int sgm, int i, int j, int n) {
   const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
   const int pi = solver_pos(i, n), pj = solver_pos(j, n);
   const bool fwd = pi < pj;
-  double2 *__restrict__ blk = d.gh + gh_index(fwd ? pi : pj, fwd ? pj : pi, n) * 256;
+  double2 *__restrict__ blk = o.gh + gh_index(fwd ? pi : pj, fwd ? pj : pi, n) * 256;
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    const double f = fwd ? 1.0 : sg[u];
+    const double f = (!fwd && ((sgm >> u) & 1)) ? -1.0 : 1.0;
     blk[(4 * u + kq) + 16 * li] = make_double2(f * g[u].x, f * g[u].y);
   }
 }
@@ -510,8 +522,13 @@ __global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const 
   __shared__ double pmmf[kLmaxD + 1], pm1f[kLmaxD + 1];
   __shared__ int lofk[kNd];
   __shared__ double2 dl[kAsmPairs][kNd];
+  __shared__ int pmeta[kAsmPairs][3];  // per pair of the chunk: kkr, block offset, store column
 
+  const AsmDst o{d.M, d.w32, d.gh, d.ld, d.nt};
+  const double2 *__restrict__ tbase = tmat + d.t_off;  // energy slot (and spin block) of the store
+  const int kkrsz = d.kkrsz, ldt = d.ldt;
   const int lmi = d.lmax[i], kkri = (lmi + 1) * (lmi + 1), kkri_ns = kkri * NSC;
+  const double2 *__restrict__ Ti = tbase + ldT * d.store[i];
   load_tables(cgnt_g, illp_g, lmax_tab, cg, il, pa, pb, pmmf, pm1f, lofk);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, kq = lane >> 4;
@@ -521,8 +538,8 @@ __global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const 
   // registers) are read again from L2 for each pair instead
   double2 tvi[NSC][NSC][4];
   float2 tvi32[4];
-  if (gw) load_t_operand_w32(d, tmat, ldT, i, kkri, tvi32);
-  else if (NSC == 1) load_t_operand<NSC>(d, tmat, ldT, i, kkri, tvi);
+  if (gw) load_t_operand_w32(Ti, ldt, kkri, tvi32);
+  else if (NSC == 1) load_t_operand<NSC>(Ti, kkrsz, ldt, kkri, tvi);
   __syncthreads();
 
   const AngTab tb{pa, pb, pmmf, pm1f};
@@ -530,19 +547,23 @@ __global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const 
   const double2 prel = make_double2(d.prel_re, d.prel_im);
   const bool prel0 = (prel.x == 0.0 && prel.y == 0.0);
   const int offi = d.site_off[i];
+  const int *__restrict__ lmaxp = d.lmax;
+  const int *__restrict__ offp = d.site_off;
+  const int *__restrict__ storep = d.store;
+  const double *__restrict__ posp = d.pos;
   if (w == 0) {
     for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
       int rr = e % kkri_ns, cc = e / kkri_ns;
       if (MODE != kAsmGW)
-        d.M[(offi + rr) + (long)d.ld * (offi + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
+        o.M[(offi + rr) + o.ld * (offi + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
       if (MODE != kAsmM)
-        d.w32[tiled_index(offi + rr, offi + cc, d.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
+        o.W[tiled_index(offi + rr, offi + cc, o.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
     }
   }
   // (-1)^(l1+l2) of the lane's four G entries
-  double sg[4];
+  int sgm = 0;  // bit u: (-1)^(l1+l2) = -1 for entry u
 #pragma unroll
-  for (int u = 0; u < 4; u++) sg[u] = ((lofk[li] + lofk[4 * u + kq]) & 1) ? -1.0 : 1.0;
+  for (int u = 0; u < 4; u++) sgm |= ((lofk[li] + lofk[4 * u + kq]) & 1) << u;
 
   // the pairs j = i + k (mod n), k = 1 .. n/2, the middle pair of an even n
   // (k = n/2, the last) once; in chunks of at most kAsmPairs
@@ -552,11 +573,19 @@ __global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const 
   const int per = nchunk ? (npairs + nchunk - 1) / nchunk : 0;
   for (int q0 = 0; q0 < npairs; q0 += per) {
     const int np = min(per, npairs - q0);
-    // D of the chunk's pairs: lane = pair, wave w = quarter w of the m range
+    // D of the chunk's pairs: lane = pair, wave w = quarter w of the m range;
+    // wave 0 also leaves each pair's site data in LDS, so that no global read
+    // (whose wait would take in the stores in flight) sits between the pairs
     if (lane < np) {
       const int j = (i + q0 + lane + 1) % n;
-      const double rx = xi - d.pos[3 * j], ry = yi - d.pos[3 * j + 1], rz = zi - d.pos[3 * j + 2];
+      const double rx = xi - posp[3 * j], ry = yi - posp[3 * j + 1], rz = zi - posp[3 * j + 2];
       const double r = sqrt(rx * rx + ry * ry + rz * rz);
+      if (w == 0) {
+        const int lmj = lmaxp[j];
+        pmeta[lane][0] = (lmj + 1) * (lmj + 1);
+        pmeta[lane][1] = offp[j];
+        pmeta[lane][2] = storep[j];
+      }
       switch (w) {
         case 0: d_pair_part<0>(tb, prel, prel0, rx, ry, rz, r, dl[lane]); break;
         case 1: d_pair_part<1>(tb, prel, prel0, rx, ry, rz, r, dl[lane]); break;
@@ -566,39 +595,46 @@ __global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const 
     }
     __syncthreads();
     // G and both blocks of each pair, pairs dealt round-robin to the waves
-    for (int pq = w; pq < np; pq += 4) {
-      const int j = (i + q0 + pq + 1) % n;
-      const int lmj = d.lmax[j], kkrj = (lmj + 1) * (lmj + 1);
-      const int offj = d.site_off[j];
-      double2 g[4];
-      if (gw) {
+    if (gw) {
+      for (int pq = w; pq < np; pq += 4) {
+        const int j = (i + q0 + pq + 1) % n;
+        const int kkrj = pmeta[pq][0], offj = pmeta[pq][1];
         float2 tvj32[4];
-        load_t_operand_w32(d, tmat, ldT, j, kkrj, tvj32);
+        load_t_operand_w32(tbase + ldT * pmeta[pq][2], ldt, kkrj, tvj32);
+        double2 g[4];
         gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[pq], g);
-        store_gh(d, g, sg, i, j, n);
+        store_gh(o, g, sgm, i, j, n);
         float2 gf[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) gf[u] = make_float2((float)g[u].x, (float)g[u].y);
-        store_block_w32(d, gf, tvi32, offi, kkri, offj, kkrj);
+        store_block_w32(o, gf, tvi32, offi, kkri, offj, kkrj);
 #pragma unroll
-        for (int u = 0; u < 4; u++) gf[u] = make_float2((float)sg[u] * gf[u].x, (float)sg[u] * gf[u].y);
-        store_block_w32(d, gf, tvj32, offj, kkrj, offi, kkri);
-        continue;
+        for (int u = 0; u < 4; u++)
+          if ((sgm >> u) & 1) gf[u] = make_float2(-gf[u].x, -gf[u].y);
+        store_block_w32(o, gf, tvj32, offj, kkrj, offi, kkri);
       }
-      double2 tvj[NSC][NSC][4];
-      if (NSC == 1) load_t_operand<NSC>(d, tmat, ldT, j, kkrj, tvj);
-      else load_t_operand<NSC>(d, tmat, ldT, i, kkri, tvi);
-      gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[pq], g);
-      store_block<NSC, MODE == kAsmMW>(d, g, tvi, offi, kkri, offj, kkrj);
-      if (NSC != 1) {  // spin-canted: t_j only once the block of t_i is done
-        __builtin_amdgcn_sched_barrier(0);
-        load_t_operand<NSC>(d, tmat, ldT, j, kkrj, tvj);
-      }
+    } else {
+      for (int pq = w; pq < np; pq += 4) {
+        const int j = (i + q0 + pq + 1) % n;
+        const int kkrj = pmeta[pq][0], offj = pmeta[pq][1];
+        const double2 *__restrict__ Tj = tbase + ldT * pmeta[pq][2];
+        double2 g[4];
+        double2 tvj[NSC][NSC][4];
+        if (NSC == 1) load_t_operand<NSC>(Tj, kkrsz, ldt, kkrj, tvj);
+        else load_t_operand<NSC>(Ti, kkrsz, ldt, kkri, tvi);
+        gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[pq], g);
+        store_block<NSC, MODE == kAsmMW>(o, g, tvi, offi, kkri, offj, kkrj);
+        if (NSC != 1) {  // spin-canted: t_j only once the block of t_i is done
+          __builtin_amdgcn_sched_barrier(0);
+          load_t_operand<NSC>(Tj, kkrsz, ldt, kkrj, tvj);
+        }
 #pragma unroll
-      for (int u = 0; u < 4; u++) g[u] = make_double2(sg[u] * g[u].x, sg[u] * g[u].y);
-      store_block<NSC, MODE == kAsmMW>(d, g, tvj, offj, kkrj, offi, kkri);
+        for (int u = 0; u < 4; u++)
+          if ((sgm >> u) & 1) g[u] = make_double2(-g[u].x, -g[u].y);
+        store_block<NSC, MODE == kAsmMW>(o, g, tvj, offj, kkrj, offi, kkri);
+      }
     }
-    __syncthreads();  // dl is rewritten by the next chunk
+    __syncthreads();  // dl and pmeta are rewritten by the next chunk
   }
 }
 
