@@ -272,6 +272,17 @@ int lsmsx_get_schedule(lsmsx_ctx *ctx, int *flags);
  * mixed path, which wait on convergence counts, stay outside the graph). */
 int lsmsx_set_graph(lsmsx_ctx *ctx, int on);
 
+/* Structure constants shared within a batch (on by default): solves of one
+ * chunk whose atoms have bitwise equal LIZ positions and LIZlmax (atoms of a
+ * lattice whose LIZs are built alike) and the same energy slot and prel
+ * share G_ij(E); the first of them evaluates it once per site pair and every
+ * solve builds its blocks M_ij = -t_i G_ij from the stored values, so the
+ * matrices (and results) are bitwise those of per-solve evaluation.
+ * Collinear, non-relativistic batches; off: every solve evaluates its own.
+ * No reference counterpart (the reference builds G per atom and energy,
+ * calculateTauMatrix.cpp:57-315). */
+int lsmsx_set_shared_structure_constants(lsmsx_ctx *ctx, int on);
+
 /* Steps per trailing update of the split-FP16 path (LSMSX_PREC_MIXED_F16):
  * 4, 8 (the default) or 16.  Larger groups read and write the trailing
  * matrix less often and spend more in the cross and strip updates that join

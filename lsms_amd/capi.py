@@ -18,7 +18,7 @@ PREC_FP64_REFINED = 3
  KIND_IR_SOLVE, KIND_INVERSE, KIND_ADJOINT, KIND_CROSS) = range(10)
 
 EXPORTED = [
-    "lsmsx_set_split_group", "lsmsx_get_split_group",
+    "lsmsx_set_split_group", "lsmsx_get_split_group", "lsmsx_set_shared_structure_constants",
     "lsmsx_last_error", "lsmsx_ctx_create", "lsmsx_ctx_destroy", "lsmsx_set_tables",
     "lsmsx_compute_tables", "lsmsx_upload_tmats", "lsmsx_set_liz", "lsmsx_build_kkr",
     "lsmsx_tau00", "lsmsx_tau00_batched", "lsmsx_solve_tau00_matrix",
@@ -486,6 +486,11 @@ class Context:
         _ck(load().lsmsx_autotune(self._h, n, _p(a), _p(s), _p(e), _p(pr), self.nsc, self.nspola,
                                   precision, C.byref(best), _p(ms)))
         return best.value, ms
+
+    def set_shared_structure_constants(self, on):
+        """Share G_ij(E) among the solves of a batch with equal LIZ geometry,
+        energy slot and prel (default on; results are bitwise the same)."""
+        _ck(load().lsmsx_set_shared_structure_constants(self._h, 1 if on else 0))
 
     def set_split_group(self, group):
         """Steps per trailing update of the split-FP16 path: 4, 8 or 16."""

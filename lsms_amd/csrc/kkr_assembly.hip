@@ -506,7 +506,7 @@ __device__ __forceinline__ void store_gh(const AsmDst &o, const double2 (&g)[4],
 //      operands of t_i in registers).
 // NSC = n_spin_cant; MODE = kAsmM, kAsmMW or kAsmGW (the outputs written).
 template <int NSC, int MODE>
-__global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const SolveDesc *__restrict__ descs,
+__global__ __launch_bounds__(256, (MODE == kAsmGW || MODE == kAsmG) ? 3 : 1) void k_assemble(const SolveDesc *__restrict__ descs,
                                                   const double2 *__restrict__ tmat, long ldT,
                                                   const double *__restrict__ cgnt_g,
                                                   const double2 *__restrict__ illp_g,
@@ -532,14 +532,16 @@ __global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const 
   load_tables(cgnt_g, illp_g, lmax_tab, cg, il, pa, pb, pmmf, pm1f, lofk);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, kq = lane >> 4;
-  static_assert(MODE != kAsmGW || NSC == 1, "structure constants once per pair: collinear");
+  static_assert((MODE != kAsmGW && MODE != kAsmG) || NSC == 1,
+                "structure constants once per pair: collinear");
   constexpr bool gw = MODE == kAsmGW;
+  constexpr bool g_only = MODE == kAsmG;
   // t_i stays in registers when collinear; the spin-canted operands (64
   // registers) are read again from L2 for each pair instead
   double2 tvi[NSC][NSC][4];
   float2 tvi32[4];
   if (gw) load_t_operand_w32(Ti, ldt, kkri, tvi32);
-  else if (NSC == 1) load_t_operand<NSC>(Ti, kkrsz, ldt, kkri, tvi);
+  else if (NSC == 1 && !g_only) load_t_operand<NSC>(Ti, kkrsz, ldt, kkri, tvi);
   __syncthreads();
 
   const AngTab tb{pa, pb, pmmf, pm1f};
@@ -551,7 +553,7 @@ __global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const 
   const int *__restrict__ offp = d.site_off;
   const int *__restrict__ storep = d.store;
   const double *__restrict__ posp = d.pos;
-  if (w == 0) {
+  if (w == 0 && !g_only) {
     for (int e = lane; e < kkri_ns * kkri_ns; e += 64) {
       int rr = e % kkri_ns, cc = e / kkri_ns;
       if (MODE != kAsmGW)
@@ -595,7 +597,13 @@ __global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const 
     }
     __syncthreads();
     // G and both blocks of each pair, pairs dealt round-robin to the waves
-    if (gw) {
+    if (g_only) {
+      for (int pq = w; pq < np; pq += 4) {
+        double2 g[4];
+        gaunt_lane(cg, il, lofk, lmax_tab, prel0, dl[pq], g);
+        store_gh(o, g, sgm, i, (i + q0 + pq + 1) % n, n);
+      }
+    } else if (gw) {
       for (int pq = w; pq < np; pq += 4) {
         const int j = (i + q0 + pq + 1) % n;
         const int kkrj = pmeta[pq][0], offj = pmeta[pq][1];
@@ -644,9 +652,97 @@ __global__ __launch_bounds__(256, MODE == kAsmGW ? 3 : 1) void k_assemble(const 
 LSMSX_ASM_INST(1, kAsmM)
 LSMSX_ASM_INST(1, kAsmMW)
 LSMSX_ASM_INST(1, kAsmGW)
+LSMSX_ASM_INST(1, kAsmG)
 LSMSX_ASM_INST(2, kAsmM)
 LSMSX_ASM_INST(2, kAsmMW)
 #undef LSMSX_ASM_INST
+
+// The blocks of one solve from structure constants already stored once per
+// site pair (d.gh, written by k_assemble<1, kAsmG> for the first solve of a
+// group that shares the LIZ geometry and the energy): for row site i =
+// blockIdx.x of solve blockIdx.y, the identity block (i, i) and, for the
+// pairs j = i + k (mod n) of k_assemble, both blocks
+//   M(i,j) = -t_i G_ij,  M(j,i) = -t_j G_ji,  G_ji = (-1)^(l1+l2) G_ij,
+// with G_ij read back in the orientation store_gh wrote it (the same FP64
+// values the solve would compute, so the blocks are bitwise those of
+// k_assemble).  Collinear; MODE = kAsmM, kAsmMW or kAsmGW (W alone).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_assemble_from_g(const SolveDesc *__restrict__ descs,
+                                                         const double2 *__restrict__ tmat,
+                                                         long ldT) {
+  const SolveDesc &d = descs[blockIdx.y];
+  const int i = blockIdx.x;
+  const int n = d.numLIZ;
+  if (i >= n) return;
+  const AsmDst o{d.M, d.w32, d.gh, d.ld, d.nt};
+  const double2 *__restrict__ tbase = tmat + d.t_off;
+  const int kkrsz = d.kkrsz, ldt = d.ldt;
+  const int *__restrict__ lmaxp = d.lmax;
+  const int *__restrict__ offp = d.site_off;
+  const int *__restrict__ storep = d.store;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, kq = lane >> 4;
+  const int lmi = lmaxp[i], kkri = (lmi + 1) * (lmi + 1);
+  const int offi = offp[i];
+  const double2 *__restrict__ Ti = tbase + ldT * storep[i];
+  constexpr bool w_only = MODE == kAsmGW;
+  double2 tvi[1][1][4];
+  float2 tvi32[4];
+  if (w_only) load_t_operand_w32(Ti, ldt, kkri, tvi32);
+  else load_t_operand<1>(Ti, kkrsz, ldt, kkri, tvi);
+  if (w == 0) {
+    for (int e = lane; e < kkri * kkri; e += 64) {
+      const int rr = e % kkri, cc = e / kkri;
+      if (MODE != kAsmGW) o.M[(offi + rr) + o.ld * (offi + cc)] = make_double2(rr == cc ? 1.0 : 0.0, 0.0);
+      if (MODE != kAsmM) o.W[tiled_index(offi + rr, offi + cc, o.nt)] = make_float2(rr == cc ? 1.f : 0.f, 0.f);
+    }
+  }
+  // (-1)^(l1+l2) of the lane's four G entries, l of an lm index = floor(sqrt(lm))
+  auto lof = [](int lm) { return lm < 1 ? 0 : lm < 4 ? 1 : lm < 9 ? 2 : 3; };
+  int sgm = 0;
+#pragma unroll
+  for (int u = 0; u < 4; u++) sgm |= ((lof(li) + lof(4 * u + kq)) & 1) << u;
+  const int pi = solver_pos(i, n);
+  const int half = n / 2;
+  const int npairs = half - ((n % 2 == 0 && (i + half) % n < i) ? 1 : 0);
+  for (int k = 1 + w; k <= npairs; k += 4) {
+    const int j = (i + k) % n;
+    const int lmj = lmaxp[j], kkrj = (lmj + 1) * (lmj + 1), offj = offp[j];
+    const double2 *__restrict__ Tj = tbase + ldT * storep[j];
+    const int pj = solver_pos(j, n);
+    const bool fwd = pi < pj;
+    const double2 *__restrict__ blk = o.gh + gh_index(fwd ? pi : pj, fwd ? pj : pi, n) * 256;
+    double2 g[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double2 v = blk[(4 * u + kq) + 16 * li];
+      g[u] = (!fwd && ((sgm >> u) & 1)) ? make_double2(-v.x, -v.y) : v;
+    }
+    if (w_only) {
+      float2 tvj32[4], gf[4];
+      load_t_operand_w32(Tj, ldt, kkrj, tvj32);
+#pragma unroll
+      for (int u = 0; u < 4; u++) gf[u] = make_float2((float)g[u].x, (float)g[u].y);
+      store_block_w32(o, gf, tvi32, offi, kkri, offj, kkrj);
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if ((sgm >> u) & 1) gf[u] = make_float2(-gf[u].x, -gf[u].y);
+      store_block_w32(o, gf, tvj32, offj, kkrj, offi, kkri);
+    } else {
+      double2 tvj[1][1][4];
+      load_t_operand<1>(Tj, kkrsz, ldt, kkrj, tvj);
+      store_block<1, MODE == kAsmMW>(o, g, tvi, offi, kkri, offj, kkrj);
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if ((sgm >> u) & 1) g[u] = make_double2(-g[u].x, -g[u].y);
+      store_block<1, MODE == kAsmMW>(o, g, tvj, offj, kkrj, offi, kkri);
+    }
+  }
+}
+
+template __global__ void k_assemble_from_g<kAsmM>(const SolveDesc *, const double2 *, long);
+template __global__ void k_assemble_from_g<kAsmMW>(const SolveDesc *, const double2 *, long);
+template __global__ void k_assemble_from_g<kAsmGW>(const SolveDesc *, const double2 *, long);
 
 // Energy derivative of the assembly contracted with the explicit inverse, one
 // workgroup per row site i = blockIdx.x of solve blockIdx.y:

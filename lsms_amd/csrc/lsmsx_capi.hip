@@ -31,6 +31,8 @@ namespace lsmsx {
 template <int NSC, int MODE>
 __global__ void k_assemble(const SolveDesc *, const double2 *, long, const double *,
                            const double2 *, int);
+template <int MODE>
+__global__ void k_assemble_from_g(const SolveDesc *, const double2 *, long);
 __global__ void k_pad_identity(const SolveDesc *);
 template <class T>
 __global__ void k_lu_diag(const SolveDesc *, int);
@@ -181,6 +183,8 @@ struct AtomDev {
   int nrmat = 0;
   int kkrsz = 0;
   std::vector<int> lmax, store;
+  std::vector<double> pos;  // host copy of LIZPos
+  int geo = -1;             // atoms with bitwise equal LIZPos and LIZlmax share this id
   double *d_pos = nullptr;
   int *d_lmax = nullptr, *d_store = nullptr;
   int *d_off[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [solver layout?][nsc-1]
@@ -257,7 +261,7 @@ struct lsmsx_ctx {
   DevBuf<double> minpiv, dxn;
   DevBuf<unsigned short> hp;  // split-FP16 panel planes
   DevBuf<int> he;
-  DevBuf<SolveDesc> desc, desc2;
+  DevBuf<SolveDesc> desc, desc2, desc_own, desc_asm;
   // explicit inverse / adjoint
   DevBuf<double2> Minv, gx, lpart, lloyd, vbar, gblk, gstore, tfull;
   DevBuf<int> keyi, colmap;
@@ -283,6 +287,11 @@ struct lsmsx_ctx {
   int ir_max_sweeps = 12;
   // residual form of the mixed path (LSMSX_RESIDUAL_*) and the G-half buffers
   int residual_form = LSMSX_RESIDUAL_AUTO;
+  // structure constants once per group of solves with the same LIZ geometry
+  // and energy (lsmsx_set_shared_structure_constants)
+  int share_g = 1;
+  int next_geo = 0;
+  DevBuf<double2> GS;  // their storage on the paths that keep no G otherwise
   DevBuf<int> rowperm;  // retry order of lsmsx_solve_tau00_matrix
   DevBuf<double2> GH, YB, GP;
   // relativistic store (lsmsx_upload_tmats_rel): the solver store holds
@@ -354,6 +363,8 @@ void launch_assemble(lsmsx_ctx *c, int nsc, dim3 grid, hipStream_t st, const Sol
   if (nsc == 2) {
     if (mode == kAsmMW) LSMSX_ASM_LAUNCH(2, kAsmMW);
     else LSMSX_ASM_LAUNCH(2, kAsmM);
+  } else if (mode == kAsmG) {
+    LSMSX_ASM_LAUNCH(1, kAsmG);
   } else if (mode == kAsmGW) {
     LSMSX_ASM_LAUNCH(1, kAsmGW);
   } else if (mode == kAsmMW) {
@@ -362,6 +373,18 @@ void launch_assemble(lsmsx_ctx *c, int nsc, dim3 grid, hipStream_t st, const Sol
     LSMSX_ASM_LAUNCH(1, kAsmM);
   }
 #undef LSMSX_ASM_LAUNCH
+}
+
+// Blocks of every solve of a batch from structure constants stored once per
+// group of solves (d.gh), collinear: mode kAsmM, kAsmMW or kAsmGW.
+void launch_assemble_from_g(lsmsx_ctx *c, dim3 grid, hipStream_t st, const SolveDesc *dd,
+                            int mode) {
+  if (mode == kAsmGW)
+    hipLaunchKernelGGL(k_assemble_from_g<kAsmGW>, grid, dim3(256), 0, st, dd, c->tmat.p, c->ldT);
+  else if (mode == kAsmMW)
+    hipLaunchKernelGGL(k_assemble_from_g<kAsmMW>, grid, dim3(256), 0, st, dd, c->tmat.p, c->ldT);
+  else
+    hipLaunchKernelGGL(k_assemble_from_g<kAsmM>, grid, dim3(256), 0, st, dd, c->tmat.p, c->ldT);
 }
 
 int drain_profile(lsmsx_ctx *c) {
@@ -842,6 +865,82 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       xoff += (long)Ns * kc;
       moff += (long)Ns * Ns;
     }
+    // Structure constants shared by the solves of the chunk that have the same
+    // LIZ geometry (bitwise equal positions and lmax), energy slot and prel:
+    // the first solve of each such group evaluates them once
+    // (k_assemble<1, kAsmG>) and every solve builds its blocks from them
+    // (k_assemble_from_g) -- the same FP64 values, so the same blocks.
+    // Collinear, non-relativistic, and only when some group has two solves.
+    std::vector<SolveDesc> own;
+    int own_liz = 1;
+    if (c->share_g && nsc == 1 && !c->rel) {
+      std::map<std::tuple<int, int, long, long>, int> first;
+      std::vector<int> owner(nb);
+      bool any = false;
+      for (int b = 0; b < nb; b++) {
+        const int i = done + b;
+        long pr = 0, pi = 0;
+        std::memcpy(&pr, &prel[2 * i], sizeof(pr));
+        std::memcpy(&pi, &prel[2 * i + 1], sizeof(pi));
+        auto key = std::make_tuple(c->atoms[atom_ids[i]].geo, iies[i], pr, pi);
+        auto it = first.find(key);
+        if (it == first.end()) {
+          first.emplace(key, b);
+          owner[b] = b;
+        } else {
+          owner[b] = it->second;
+          any = true;
+        }
+      }
+      if (any) {
+        if (!gmode) {  // G storage for the owners (the mixed path with G keeps its own)
+          long gs = 0;
+          for (int b = 0; b < nb; b++)
+            if (owner[b] == b) gs += g_blocks(hd[b].numLIZ) * 256;
+          if (int r = c->GS.ensure(std::max(gs, 1L))) return r;
+          long go = 0;
+          for (int b = 0; b < nb; b++)
+            if (owner[b] == b) {
+              hd[b].gh = c->GS.p + go;
+              go += g_blocks(hd[b].numLIZ) * 256;
+            }
+        }
+        for (int b = 0; b < nb; b++) {
+          if (owner[b] == b) {
+            own.push_back(hd[b]);
+            own_liz = std::max(own_liz, hd[b].numLIZ);
+          } else {
+            hd[b].gh = hd[owner[b]].gh;
+          }
+        }
+        if (int r = c->desc_own.ensure(own.size())) return r;
+        HIPCHK(hipMemcpyAsync(c->desc_own.p, own.data(), own.size() * sizeof(SolveDesc),
+                              hipMemcpyHostToDevice, c->stream));
+      }
+    }
+    const bool share = !own.empty();
+    const int asm_mode = gmode ? kAsmGW : (mixed ? kAsmMW : kAsmM);
+    // two halves on two streams (see enqueue below)
+    const bool two = (c->sched & LSMSX_SCHED_TWO_STREAMS) && nb >= 2 && !c->profiling;
+    if (share) {
+      // the blocks are built with the solves of a group next to each other
+      // (within each half), so that a group's stored structure constants are
+      // read from the caches rather than HBM once its first solve has read them
+      std::vector<SolveDesc> ha(nb);
+      const int h = two ? nb / 2 : nb;
+      for (int part = 0; part < (two ? 2 : 1); part++) {
+        const int p0 = part ? h : 0, p1 = part ? nb : h;
+        std::vector<int> order;
+        for (int b = p0; b < p1; b++) order.push_back(b);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+          return hd[x].gh < hd[y].gh;  // equal gh <=> one group
+        });
+        for (int k = 0; k < (int)order.size(); k++) ha[p0 + k] = hd[order[k]];
+      }
+      if (int r = c->desc_asm.ensure((size_t)nb)) return r;
+      HIPCHK(hipMemcpyAsync(c->desc_asm.p, ha.data(), nb * sizeof(SolveDesc),
+                            hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(hipMemcpyAsync(c->desc.p, hd.data(), nb * sizeof(SolveDesc), hipMemcpyHostToDevice,
                           c->stream));
     // assembly + factorization of the chunk: launched eagerly, or captured
@@ -864,8 +963,10 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       {
         ProfScope p(c, 0);
         hipLaunchKernelGGL(k_pad_identity, dim3(n), dim3(256), 0, st, dd);
-        launch_assemble(c, nsc, dim3(mliz, n), st, dd,
-                        gmode ? kAsmGW : (mixed ? kAsmMW : kAsmM));
+        if (share)
+          launch_assemble_from_g(c, dim3(mliz, n), st, c->desc_asm.p + b0, asm_mode);
+        else
+          launch_assemble(c, nsc, dim3(mliz, n), st, dd, asm_mode);
       }
       if (int r = check_launch("k_assemble")) return r;
       if (after_asm) HIPCHK(hipEventRecord(after_asm, st));
@@ -883,8 +984,12 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     // assembly is done: its assembly (FP64 VALU work) runs beside the first
     // half's factorization (MFMA and HBM work), and one half's latency-bound
     // steps (diagonal tiles, panels) overlap the other half's trailing updates
-    const bool two = (c->sched & LSMSX_SCHED_TWO_STREAMS) && nb >= 2 && !c->profiling;
     auto enqueue = [&]() -> int {
+      if (share) {  // the shared structure constants, before either part
+        ProfScope p(c, 0);
+        launch_assemble(c, 1, dim3(own_liz, (int)own.size()), c->stream, c->desc_own.p, kAsmG);
+        if (int r = check_launch("k_assemble (structure constants)")) return r;
+      }
       if (!two) return enqueue_part(c->stream, 0, nb, nullptr);
       const int h = nb / 2;
       if (int r = enqueue_part(c->stream, 0, h, c->ev_asm)) return r;
@@ -900,7 +1005,8 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       std::vector<long> key = {nb, nt_max, maxliz, precision, c->sched, (long)c->desc.p,
                                (long)c->tmat.p, c->ldT, (long)c->cgnt.p, (long)c->illp.p,
                                c->lmax_tab, c->ir_max_sweeps, tol_bits, gmode ? nbands_max : -1,
-                               lu_group(c, split)};
+                               lu_group(c, split), share ? (long)own.size() : -1,
+                               (long)c->desc_own.p, (long)c->desc_asm.p};
       auto it = c->graphs.find(key);
       if (it == c->graphs.end()) {
         const bool prof = c->profiling;
@@ -1571,6 +1677,17 @@ int lsmsx_set_liz(lsmsx_ctx *c, int atom_id, int numLIZ, const double *pos, cons
   a.numLIZ = numLIZ;
   a.lmax.assign(lmax, lmax + numLIZ);
   a.store.assign(store, store + numLIZ);
+  a.pos.assign(pos, pos + 3 * numLIZ);
+  a.geo = -1;
+  for (auto &kv : c->atoms) {
+    const AtomDev &b = kv.second;
+    if (kv.first == atom_id || b.geo < 0 || b.numLIZ != numLIZ || b.lmax != a.lmax) continue;
+    if (std::memcmp(b.pos.data(), a.pos.data(), sizeof(double) * 3 * numLIZ) == 0) {
+      a.geo = b.geo;
+      break;
+    }
+  }
+  if (a.geo < 0) a.geo = c->next_geo++;
   a.nrmat = 0;
   for (int s = 0; s < numLIZ; s++) {
     if (lmax[s] < 0 || lmax[s] > kMaxLmax) return fail(-2, "LIZlmax must be in 0..3");
@@ -1974,6 +2091,12 @@ int lsmsx_set_schedule(lsmsx_ctx *c, int flags) {
 int lsmsx_get_schedule(lsmsx_ctx *c, int *flags) {
   if (!c || !flags) return fail(-1, "null argument");
   *flags = c->sched;
+  return 0;
+}
+
+int lsmsx_set_shared_structure_constants(lsmsx_ctx *c, int on) {
+  if (!c) return fail(-1, "null context");
+  c->share_g = on ? 1 : 0;
   return 0;
 }
 

@@ -102,6 +102,10 @@ struct SolveDesc {
 constexpr int kAsmM = 0;
 constexpr int kAsmMW = 1;
 constexpr int kAsmGW = 2;
+// the structure constants alone (once per group of solves that share the
+// LIZ geometry and the energy; k_assemble_from_g then writes every solve's
+// blocks from them)
+constexpr int kAsmG = 3;
 
 // G-half residual: sites per band, waves per workgroup and sites per wave
 constexpr int kGBand = 16;
