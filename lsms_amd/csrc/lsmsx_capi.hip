@@ -261,7 +261,7 @@ struct lsmsx_ctx {
   DevBuf<double> minpiv, dxn;
   DevBuf<unsigned short> hp;  // split-FP16 panel planes
   DevBuf<int> he;
-  DevBuf<SolveDesc> desc, desc2, desc_own, desc_asm;
+  DevBuf<SolveDesc> desc, desc2, desc_own;
   // explicit inverse / adjoint
   DevBuf<double2> Minv, gx, lpart, lloyd, vbar, gblk, gstore, tfull;
   DevBuf<int> keyi, colmap;
@@ -923,24 +923,17 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     // two halves on two streams (see enqueue below)
     const bool two = (c->sched & LSMSX_SCHED_TWO_STREAMS) && nb >= 2 && !c->profiling;
     if (share) {
-      // the blocks are built with the solves of a group next to each other
-      // (within each half), so that a group's stored structure constants are
-      // read from the caches rather than HBM once its first solve has read them
-      std::vector<SolveDesc> ha(nb);
-      const int h = two ? nb / 2 : nb;
-      for (int part = 0; part < (two ? 2 : 1); part++) {
-        const int p0 = part ? h : 0, p1 = part ? nb : h;
-        std::vector<int> order;
-        for (int b = p0; b < p1; b++) order.push_back(b);
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-          return hd[x].gh < hd[y].gh;  // equal gh <=> one group
-        });
-        for (int k = 0; k < (int)order.size(); k++) ha[p0 + k] = hd[order[k]];
-      }
-      if (int r = c->desc_asm.ensure((size_t)nb)) return r;
-      HIPCHK(hipMemcpyAsync(c->desc_asm.p, ha.data(), nb * sizeof(SolveDesc),
-                            hipMemcpyHostToDevice, c->stream));
+      // the chunk's solves run with the solves of a group next to each other
+      // (each descriptor keeps its own output slots), so that a group's
+      // stored structure constants -- read by the block builder and by every
+      // refinement residual -- come from the caches once its first solve has
+      // read them
+      std::stable_sort(hd.begin(), hd.end(),
+                       [](const SolveDesc &x, const SolveDesc &y) { return x.gh < y.gh; });
     }
+    // position in hd of the chunk's solve b (its output slots are index b)
+    std::vector<int> pos_of(nb);
+    for (int k = 0; k < nb; k++) pos_of[hd[k].info - c->info.p] = k;
     HIPCHK(hipMemcpyAsync(c->desc.p, hd.data(), nb * sizeof(SolveDesc), hipMemcpyHostToDevice,
                           c->stream));
     // assembly + factorization of the chunk: launched eagerly, or captured
@@ -964,7 +957,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
         ProfScope p(c, 0);
         hipLaunchKernelGGL(k_pad_identity, dim3(n), dim3(256), 0, st, dd);
         if (share)
-          launch_assemble_from_g(c, dim3(mliz, n), st, c->desc_asm.p + b0, asm_mode);
+          launch_assemble_from_g(c, dim3(mliz, n), st, dd, asm_mode);
         else
           launch_assemble(c, nsc, dim3(mliz, n), st, dd, asm_mode);
       }
@@ -1006,7 +999,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
                                (long)c->tmat.p, c->ldT, (long)c->cgnt.p, (long)c->illp.p,
                                c->lmax_tab, c->ir_max_sweeps, tol_bits, gmode ? nbands_max : -1,
                                lu_group(c, split), share ? (long)own.size() : -1,
-                               (long)c->desc_own.p, (long)c->desc_asm.p};
+                               (long)c->desc_own.p};
       auto it = c->graphs.find(key);
       if (it == c->graphs.end()) {
         const bool prof = c->profiling;
@@ -1058,7 +1051,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
         std::vector<SolveDesc> fb;
         for (int b = 0; b < nb; b++)
           if (!hconv[b]) {
-            SolveDesc d = hd[b];
+            SolveDesc d = hd[pos_of[b]];
             d.step0 = 0;  // the fallback chunk has its own step count
             d.A = d.M;
             d.tiled = 0;

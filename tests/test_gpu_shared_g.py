@@ -107,3 +107,20 @@ def test_alloy_batch_with_partly_shared_geometry():
         assert np.array_equal(shared, alone), prec
     ctx.close()
     print(f"{len(geo)} distinct LIZ geometries among 24 alloy atoms")
+
+
+def test_shared_fallback_keeps_each_solve_in_its_slot(c2pair):
+    """FP64-refined solves allowed one sweep cannot converge, so each falls
+    back to its plain FP64 result (info -3): with sharing the chunk runs in
+    group order, and every fallback must still come from its own solve --
+    bitwise the plain FP64 result of that solve."""
+    s, ctx = c2pair
+    atoms = np.array([0, 1, 1, 0, 0, 1])
+    iies = np.array([3, 1, 3, 1, 0, 0])
+    prels = np.array([s.prel(i) for i in iies])
+    plain, _ = run(ctx, atoms, iies, prels, capi.PREC_FP64, False)
+    ctx.set_refinement(max_sweeps=1, tol=1e-12)
+    shared, info = run(ctx, atoms, iies, prels, capi.PREC_FP64_REFINED, True)
+    ctx.set_refinement()
+    assert np.all(info == capi.INFO_REFINE_UNCONVERGED)
+    assert np.array_equal(shared, plain)
