@@ -717,6 +717,10 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     if (refined64) bytes[i] += Ns * Ns * 16 + nt * kTile * kTile * 16 + Ns * kc * 32;
     if (split) bytes[i] += 2 * lu_group(c, true) * nt * ((long)kPlaneHalves * 2 + 4);
   }
+  // the explicit-inverse route's buffers are not used here: give them back
+  // before sizing the chunks by the free memory
+  c->Minv.release();
+  c->gblk.release();
   long cap = c->ws_limit;
   if (cap <= 0) {
     size_t fr = 0, tot = 0;
@@ -1226,6 +1230,18 @@ int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     bytes[i] = 2 * Ns * Ns * 16 + nt * kTile * kTile * 16 + 2 * Ns * kc * 16 +
                (tau_bar ? (long)a.numLIZ * kc * kc * 16 : 0);
   }
+  // the batched mixed and refined paths' buffers are not used here: give
+  // them back before sizing the chunks by the free memory
+  c->W.release();
+  c->R.release();
+  c->GH.release();
+  c->YB.release();
+  c->GP.release();
+  c->GS.release();
+  c->hp.release();
+  c->he.release();
+  c->A64.release();
+  c->R64.release();
   long cap = c->ws_limit;
   if (cap <= 0) {
     size_t fr = 0, tot = 0;
@@ -1550,6 +1566,8 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->minpiv.release();
   c->desc.release();
   c->desc2.release();
+  c->desc_own.release();
+  c->GS.release();
   c->X.release();
   c->A64.release();
   c->R64.release();
