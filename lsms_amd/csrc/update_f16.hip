@@ -154,6 +154,9 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
   };
   auto compute = [&](int buf) __attribute__((always_inline)) {
     if (!valid) return;
+    // the wave that reaches its MFMAs first keeps the issue slots while the
+    // other wave of its SIMD is still staging, so the two drift apart
+    __builtin_amdgcn_s_setprio(2);
     // U operands of the wave's two 16-column blocks, with -Re(U) formed once
     // here for the -Im(L) Re(U) term (cheaper than negating Im(L) per row
     // block; Im(L) (-Re U) = -(Im(L) Re(U)) exactly)
@@ -192,6 +195,7 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
         ci[bi][bj] = m;
       }
     }
+    __builtin_amdgcn_s_setprio(0);
   };
   auto load_c = [&]() __attribute__((always_inline)) {
     if (!valid) return;
