@@ -13,12 +13,14 @@
 // exactly as for the FP32 update; its contraction is set by the conditioning
 // of M, not by this update.
 //
-// A workgroup owns a 128 x 64 region (C tiles (ti, tj) and (ti+1, tj)); each
-// of its four waves computes a 64 x 32 part of one tile.  Per 32-wide k
-// chunk the split planes of the two L row tiles and of the U column tile
-// are staged in LDS (48 KiB), so every plane element is fetched from L2 once
-// per workgroup and read by two waves; the next chunk's loads are issued
-// before this chunk's MFMAs, so the L2 latency hides behind them.  The four
+// A workgroup of the trailing update owns a 128 x 128 region (C tiles ti,
+// ti+1 x tj, tj+1); each of its eight waves computes a 64 x 32 part of one
+// tile.  Per 32-wide k chunk the split planes of the two L row tiles and of
+// the two U column tiles are staged in LDS (64 KiB, two stages), so every
+// plane element is fetched from L2 once per workgroup and read by two or four
+// waves; the loads run two chunks ahead of the MFMAs, so the L2 latency hides
+// behind them.  (The cross update uses 128 x 64 and 64 x 128 regions of four
+// waves and one stage.)  The four
 // 16-byte k groups of a 64-byte LDS row are XOR-swizzled by the row, so the
 // lanes of a 128-bit read hit distinct banks.  Complex products in 4M form
 // (Cr += (-Ar) Br + Ai Bi, Ci += (-Ar) Bi + (-Ai) Br) with two FP32
@@ -26,11 +28,13 @@
 // a U plane (lane l: column l%16, k = 8(l/16)..+7), the b-operand an L plane
 // (lane l: row l%16, same k), and accumulator register r of lane l is
 // C(row l%16, column 4(l/16) + r) (layout measured on gfx950), so C moves
-// in 128 B runs.  C is scaled by the tile exponents on entry, between the
-// steps and on exit (exact powers of two).
+// in 128 B runs.  The accumulators start from zero and are scaled by the
+// tile exponents between the steps (exact powers of two); C is loaded during
+// the last k chunk, into the registers the plane prefetch no longer needs,
+// and added on exit, so no MFMA waits for C's HBM latency.
 //
-// Built without -amdgpu-mfma-vgpr-form: the 64 accumulators per lane live
-// in AGPRs.
+// Built without -amdgpu-mfma-vgpr-form, so that the compiler may keep the 64
+// accumulators per lane in AGPRs where the VGPRs run short.
 #include "lu_common.hpp"
 
 namespace lsmsx {
@@ -197,39 +201,51 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
     }
     __builtin_amdgcn_s_setprio(0);
   };
-  auto load_c = [&]() __attribute__((always_inline)) {
+  // C is added after the products: the accumulators start from zero and C's
+  // loads are issued during the last chunk, so no MFMA waits for them
+  fv2 cv[4][2][4];
+  auto issue_c = [&](int half) __attribute__((always_inline)) {
     if (!valid) return;
-    const float sc0 = ldexpf(1.f, -e);
 #pragma unroll
-    for (int bi = 0; bi < 4; bi++)
+    for (int bi = 2 * half; bi < 2 * half + 2; bi++)
 #pragma unroll
       for (int bj = 0; bj < 2; bj++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const fv2 c = __builtin_nontemporal_load((const fv2 *)(C + (16 * bi + l16) + ld * (16 * bj + 4 * g + r)));
-          cr[bi][bj][r] = c.x * sc0;
-          ci[bi][bj][r] = c.y * sc0;
-        }
+        for (int r = 0; r < 4; r++)
+          cv[bi][bj][r] = __builtin_nontemporal_load((const fv2 *)(C + (16 * bi + l16) + ld * (16 * bj + 4 * g + r)));
   };
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++) {
+      cr[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+      ci[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+    }
   const int nch = 2 * ns;
   if constexpr (DB == 1) {
     h8 pv[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) pv[q] = *chunk_src(0, q);
-    load_c();
 #pragma unroll 1
-    for (int ch = 0; ch < nch; ch++) {
+    for (int ch = 0; ch < nch - 1; ch++) {
       rescale(ch);
       __syncthreads();  // the previous chunk's readers are done
 #pragma unroll
       for (int q = 0; q < NQ; q++) *chunk_dst(0, q) = pv[q];
       __syncthreads();
-      if (ch + 1 < nch) {
 #pragma unroll
-        for (int q = 0; q < NQ; q++) pv[q] = *chunk_src(ch + 1, q);
-      }
+      for (int q = 0; q < NQ; q++) pv[q] = *chunk_src(ch + 1, q);
       compute(0);
     }
+    // the last chunk, with C's loads in the prefetch registers' place
+    rescale(nch - 1);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; q++) *chunk_dst(0, q) = pv[q];
+    __syncthreads();
+    issue_c(0);
+    issue_c(1);
+    compute(0);
   } else {
     // chunk c is staged in LDS buffer c & 1 and loaded into pa (even c) or pb
     // (odd c); nch is even
@@ -238,7 +254,6 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
     for (int q = 0; q < NQ; q++) pa[q] = *chunk_src(0, q);
 #pragma unroll
     for (int q = 0; q < NQ; q++) pb[q] = *chunk_src(1, q);
-    load_c();
 #pragma unroll
     for (int q = 0; q < NQ; q++) *chunk_dst(0, q) = pa[q];
     __syncthreads();
@@ -247,7 +262,7 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
       for (int q = 0; q < NQ; q++) pa[q] = *chunk_src(2, q);
     }
 #pragma unroll 1
-    for (int ch = 0; ch < nch; ch += 2) {
+    for (int ch = 0; ch < nch - 2; ch += 2) {
       rescale(ch);
       compute(0);
 #pragma unroll
@@ -258,16 +273,23 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
       }
       __syncthreads();  // buffer 1 staged; every wave is done with buffer 0
       compute(1);
-      if (ch + 2 < nch) {
 #pragma unroll
-        for (int q = 0; q < NQ; q++) *chunk_dst(0, q) = pa[q];
-        if (ch + 4 < nch) {
+      for (int q = 0; q < NQ; q++) *chunk_dst(0, q) = pa[q];
+      if (ch + 4 < nch) {
 #pragma unroll
-          for (int q = 0; q < NQ; q++) pa[q] = *chunk_src(ch + 4, q);
-        }
+        for (int q = 0; q < NQ; q++) pa[q] = *chunk_src(ch + 4, q);
       }
       __syncthreads();  // buffer 0 staged; every wave is done with buffer 1
     }
+    // the last two chunks, with C's loads in the prefetch registers' place
+    rescale(nch - 2);
+    issue_c(0);
+    compute(0);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) *chunk_dst(1, q) = pb[q];
+    issue_c(1);
+    __syncthreads();
+    compute(1);
   }
   if (valid) {
     const float sc2 = ldexpf(1.f, e);
@@ -277,7 +299,8 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
       for (int bj = 0; bj < 2; bj++)
 #pragma unroll
         for (int r = 0; r < 4; r++)
-          __builtin_nontemporal_store(fv2{cr[bi][bj][r] * sc2, ci[bi][bj][r] * sc2},
+          __builtin_nontemporal_store(fv2{cr[bi][bj][r] * sc2 + cv[bi][bj][r].x,
+                                          ci[bi][bj][r] * sc2 + cv[bi][bj][r].y},
                                       (fv2 *)(C + (16 * bi + l16) + ld * (16 * bj + 4 * g + r)));
   }
 }
