@@ -94,7 +94,6 @@ __global__ void k_grad_reduce(const double2 *, int, const int *, const long *, c
                               const int *, double2 *, long, int);
 __global__ void k_zero_c(double2 *, long);
 __global__ void k_lu_trail_h(const SolveDesc *, int, int);
-__global__ void k_lu_cross_h(const SolveDesc *, int, int, int);
 __global__ void k_interstitial(int, int, const double2 *, const double2 *, const double2 *,
                                const double *, const int *, const int *, int, int, const double *,
                                const double *, const double2 *, const double *, int,
@@ -102,7 +101,8 @@ __global__ void k_interstitial(int, int, const double2 *, const double2 *, const
 __global__ void k_rel_tstore(double2 *, long, long, int, const double *, const double2 *);
 __global__ void k_rel_tau(double2 *, int, const double *, const double2 *);
 __global__ void k_rel_blocks(double2 *, long, int, const double *);
-__global__ void k_lu_strip_h(const SolveDesc *, int, int, int);
+__global__ void k_lu_cross_diag_h(const SolveDesc *, int, int);
+__global__ void k_lu_panel_h(const SolveDesc *, int, int);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout;
 // rowperm (or null) reorders the n - k non-central rows: solver row xr takes
@@ -431,18 +431,19 @@ std::vector<int> site_offsets(const AtomDev &a, int nsc, bool solver) {
 }
 
 // Steps per trailing update: with the split-FP16 update the context's
-// split_group (4, 8 or 16, default 8; taken by recursive halving down to
-// blocks of four, joined by strip updates: the trailing matrix is read and
-// written once per group while the cross updates stay at most three steps
-// deep), two otherwise.
+// split_group (4, 8 or 16, default 8: the trailing matrix is read and
+// written once per group, the fused panels of a group's later steps apply
+// up to group - 1 earlier steps), two otherwise.
 inline int lu_group(const lsmsx_ctx *c, bool split_f16) { return split_f16 ? c->split_group : 2; }
 
 // Launch the blocked LU over a chunk of nb solves already in solver layout
 // (each descriptor's group = lu_group(c, split_f16)): steps taken in groups
 // g .. g+G-1 -- panels of g, then for each later step t of the group a cross
 // update of tile row/column t with the group's earlier steps and the panels
-// of t -- then one K = 64 G trailing update, so the trailing matrix is read
-// and written once per group.
+// of t (on the split-FP16 path the cross update is fused into the panel
+// kernel, and only the diagonal tile's has a kernel of its own) -- then one
+// K = 64 G trailing update, so the trailing matrix is read and written once
+// per group.
 template <class T>
 int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_f16 = false,
               hipStream_t st = nullptr) {
@@ -465,38 +466,58 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
   // the steps s .. s+n-1 of a group: n <= H, panels and crosses step by
   // step; otherwise the first half, the strip update of the second half's
   // columns and rows with the first half, then the second half
-  const int H = split_f16 ? 4 : G;
-  std::function<int(int, int)> block = [&](int s, int n) -> int {
-    if (s >= nt_max - 1) return 0;
-    if (n <= H) {
-      if (int r = diag_panel(s)) return r;
-      for (int u = 1; u < n && s + u < nt_max - 1; u++) {
-        {
+  if (split_f16) {
+    // split-FP16 path: for each step t of a group, the cross update of the
+    // diagonal tile (steps g .. t-1), its factorization, and the panel with
+    // the cross update of its column and row fused in; then the trailing
+    // update with the whole group
+    for (int s = 0; s < nt_max - 1; s += G) {
+      const int b = std::min(s + G - 1, nt_max - 2);
+      for (int t = s; t <= b; t++) {
+        const int u = t - s;
+        if (u > 0) {
           ProfScope p(c, 9);
-          const int n1 = nt_max - s - u;
-          if (split_f16)
-            hipLaunchKernelGGL(k_lu_cross_h, dim3((n1 + 1) / 2 + n1 / 2, nb), dim3(256), 0, st,
-                               dd, s, u, swz);
-          else
-            hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * n1 - 1, nb), dim3(256), 0, st, dd, s, u);
+          hipLaunchKernelGGL(k_lu_cross_diag_h, dim3(nb), dim3(128), 0, st, dd, s, u);
         }
-        if (int r = check_launch("k_lu_cross")) return r;
-        if (int r = diag_panel(s + u)) return r;
+        if (int r = check_launch("k_lu_cross_diag_h")) return r;
+        {
+          ProfScope p(c, 1);
+          hipLaunchKernelGGL(k_lu_diag<T>, dim3(nb), dim3(256), 0, st, dd, t);
+        }
+        {
+          ProfScope p(c, 2);
+          hipLaunchKernelGGL(k_lu_panel_h, dim3(2 * (nt_max - 1 - t), nb), dim3(256), 0, st, dd,
+                             t, u);
+        }
+        if (int r = check_launch("k_lu_diag/k_lu_panel_h")) return r;
       }
-      return 0;
+      const int nrest = nt_max - 1 - b;
+      if (nrest > 0) {
+        ProfScope p(c, 3);
+        hipLaunchKernelGGL(k_lu_trail_h, dim3(((nrest + 1) / 2) * ((nrest + 1) / 2), nb),
+                           dim3(512), 0, st, dd, s, swz);
+      }
+      if (int r = check_launch("k_lu_trail_h")) return r;
     }
-    const int h = n / 2;
-    if (int r = block(s, h)) return r;
-    if (s + h >= nt_max - 1) return 0;
     {
-      ProfScope p(c, 9);
-      const int hs = s + h, b = std::min(hs + h - 1, nt_max - 2);
-      const int w2 = (b - hs + 2) / 2, nr2 = (nt_max - hs + 1) / 2, nc2 = (nt_max - b) / 2;
-      hipLaunchKernelGGL(k_lu_strip_h, dim3(nr2 * w2 + w2 * nc2, nb), dim3(512), 0, st, dd, s, h,
-                         swz);
-      if (int r = check_launch("k_lu_strip_h")) return r;
+      ProfScope p(c, 4);
+      hipLaunchKernelGGL(k_lu_last<T>, dim3(nb), dim3(256), 0, st, dd);
     }
-    return block(s + h, h);
+    return check_launch("k_lu_last");
+  }
+  // steps s .. s+n-1 of a group: panels and crosses step by step
+  auto block = [&](int s, int n) -> int {
+    if (int r = diag_panel(s)) return r;
+    for (int u = 1; u < n && s + u < nt_max - 1; u++) {
+      {
+        ProfScope p(c, 9);
+        const int n1 = nt_max - s - u;
+        hipLaunchKernelGGL(k_lu_cross<T>, dim3(2 * n1 - 1, nb), dim3(256), 0, st, dd, s, u);
+      }
+      if (int r = check_launch("k_lu_cross")) return r;
+      if (int r = diag_panel(s + u)) return r;
+    }
+    return 0;
   };
   for (int s = 0; s < nt_max - 1; s += G) {
     if (int r = block(s, G)) return r;
@@ -504,12 +525,8 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
     const int nrest = nt_max - 1 - last;
     if (nrest > 0) {
       ProfScope p(c, 3);
-      if (split_f16)
-        hipLaunchKernelGGL(k_lu_trail_h, dim3(((nrest + 1) / 2) * ((nrest + 1) / 2), nb),
-                           dim3(512), 0, st, dd, s, swz);
-      else
-        hipLaunchKernelGGL(k_lu_trail<T>, dim3(nrest * nrest, nb), dim3(256), 0, st, dd, s,
-                           swz);
+      hipLaunchKernelGGL(k_lu_trail<T>, dim3(nrest * nrest, nb), dim3(256), 0, st, dd, s,
+                         swz);
     }
     if (int r = check_launch("k_lu_trail")) return r;
   }

@@ -233,6 +233,29 @@ __device__ __forceinline__ void zero_acc(Acc3<T> &q) {
         for (int r = 0; r < 4; r++) q.p[m][bj][bi][r] = T(0);
 }
 
+// Split-FP16 operands: eight halves of a 32-wide k chunk per lane, the
+// FP16 MFMA with FP32 accumulation, and a sign flip of eight halves (four
+// 32-bit XORs of the packed sign bits).
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) const h8 GH8;
+#else
+typedef const h8 GH8;
+#endif
+__device__ __forceinline__ f4 mfma_h(h8 a, h8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ h8 neg8(h8 v) {
+  return __builtin_bit_cast(h8, __builtin_bit_cast(u4v, v) ^ 0x80008000u);
+}
+
+// LDS k group of row r holding logical group g (8 halves): XOR swizzle of the
+// four groups of a 64-byte row by (r >> 1) ^ (r >> 2), so that any 8 (and any
+// 16) consecutive rows read at one group cover distinct 16-byte bank groups:
+// rows of equal parity among 8 (equal r mod 4 among 16) get distinct masks.
+__device__ __forceinline__ int swz_group(int r, int g) { return g ^ (((r >> 1) ^ (r >> 2)) & 3); }
+
 // XCD-aware workgroup order.  Workgroups are dispatched round-robin over
 // the 8 XCDs by linear id, so consecutive tiles of one solve would land on
 // 8 different L2 caches.  The remap gives each XCD a contiguous range of

@@ -14,7 +14,10 @@
 //   k_lu_panel   L_is = A_is U^-1 (tiles below), U_sj = L^-1 P A_sj (right)
 //   k_lu_update  A_ij -= L_is U_sj                     <- MFMA, dominant
 // with the steps taken in groups (k_lu_cross, k_lu_trail: the trailing
-// matrix is read and written once per group); then k_lu_last factors the last tile with the pivots of its non-central
+// matrix is read and written once per group; on the split-FP16 path
+// k_lu_panel_h takes the cross updates inside the panel and the trailing
+// update is k_lu_trail_h in update_f16.hip); then k_lu_last factors the last
+// tile with the pivots of its non-central
 // rows and of the central rows kept apart.  In FP64 it also solves
 // S tau00 = t0 with the Schur complement S (the plain path); in FP32 it
 // stores the last tile's factors for the refinement solves (mixed path).
@@ -288,31 +291,188 @@ __device__ void write_split_planes(float2 (*Ts)[kTile + 1], bool lower, _Float16
   }
 }
 
+// Cross update of tile (ti, tj) inside the split-FP16 panel, C = A - sum over
+// the steps s_first .. s_first+ns-1 of L(ti, s) U(s, tj), written to
+// Ts[col][rowpos[row]] (rowpos null: the identity).  The products run on the
+// split planes of those steps (slot s mod group) with the FP16 MFMAs of the
+// trailing update, in its orientation (a-operand a U plane, lane l: column
+// l%16; b-operand an L plane, lane l: row l%16; k = 8(l/16)..+7 of a 32-wide
+// chunk; register r of lane l is C(row l%16, column 4(l/16) + r)) and its
+// 4M x 3 split products, FP32 accumulators that start from zero and are
+// rescaled between the steps by the tile exponents (exact powers of two).
+// Wave w takes rows wr.., columns wc.. (32 x 32).  Per chunk the L and U
+// planes (16 KiB each) are staged in Ts's memory with the XOR-swizzled k
+// groups, the next chunk's loads in flight during this chunk's MFMAs (eight
+// 16-byte loads per thread).  A (the tile's FP32 values) is read during the
+// last chunk and added at the end.
+__device__ __forceinline__ void cross_to_lds(const SolveDesc &d, int ti, int tj, int s_first,
+                                             int ns, const GCx<float> *A, long ld,
+                                             float2 (*Ts)[kTile + 1], const int *rowpos) {
+  __shared__ int exs[16];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
+  const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
+  const int nt = d.nt, gmask = d.group - 1;
+  const _Float16 *hp = (const _Float16 *)d.hp;
+  h8(*Ls)[kTile][4] = reinterpret_cast<h8(*)[kTile][4]>(&Ts[0][0]);  // [plane][row][k group]
+  h8(*Us)[kTile][4] = Ls + 4;                                          // [plane][column][k group]
+  if (tid < ns) {
+    const int par = (s_first + tid) & gmask;
+    exs[tid] = d.he[(par * 2 + 0) * nt + ti] + d.he[(par * 2 + 1) * nt + tj];
+  }
+  __syncthreads();
+  // chunk ch = (step ch / 2, k half ch % 2); load q of a thread: side (L, U),
+  // plane, row (column), k group
+  auto src = [&](int ch, int q) __attribute__((always_inline)) -> GH8 * {
+    const int s = s_first + (ch >> 1), kh = ch & 1, par = s & gmask;
+    const int e8 = tid + 256 * q, side = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63,
+              grp = e8 & 3;
+    return (GH8 *)(hp + (long)((par * 2 + side) * nt + (side ? tj : ti)) * kPlaneHalves) +
+           ((plane * kTile + rr) * kTile + 32 * kh + 8 * grp) / 8;
+  };
+  auto dst = [&](int q) __attribute__((always_inline)) -> h8 * {
+    const int e8 = tid + 256 * q, side = e8 >> 10, plane = (e8 >> 8) & 3, rr = (e8 >> 2) & 63,
+              grp = e8 & 3;
+    return &(side ? Us : Ls)[plane][rr][swz_group(rr, grp)];
+  };
+  f4 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++) {
+      cr[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+      ci[bi][bj] = cr[bi][bj];
+    }
+  int e = exs[0];
+  auto rescale = [&](int ch) __attribute__((always_inline)) {
+    if (ch > 0 && !(ch & 1)) {
+      const int e2 = exs[ch >> 1];
+      if (e2 == e) return;
+      const float sc1 = ldexpf(1.f, e - e2);
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++) {
+          cr[bi][bj] *= sc1;
+          ci[bi][bj] *= sc1;
+        }
+      e = e2;
+    }
+  };
+  // Cr += (-Lr) Ur + Li Ui, Ci += (-Lr) Ui + Li (-Ur): the L planes hold -Re
+  auto compute = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++) {
+      const int col = wc + 16 * bj + l16, gu = swz_group(col, g);
+      const h8 ur_h = Us[0][col][gu], ur_l = Us[1][col][gu];
+      const h8 ui_h = Us[2][col][gu], ui_l = Us[3][col][gu];
+      const h8 nur_h = neg8(ur_h), nur_l = neg8(ur_l);
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) {
+        const int row = wr + 16 * bi + l16, gl = swz_group(row, g);
+        const h8 nar_h = Ls[0][row][gl], nar_l = Ls[1][row][gl];
+        const h8 ai_h = Ls[2][row][gl], ai_l = Ls[3][row][gl];
+        f4 r = cr[bi][bj], m = ci[bi][bj];
+        r = mfma_h(ur_h, nar_h, r);
+        r = mfma_h(ur_l, nar_h, r);
+        r = mfma_h(ur_h, nar_l, r);
+        r = mfma_h(ui_h, ai_h, r);
+        r = mfma_h(ui_l, ai_h, r);
+        r = mfma_h(ui_h, ai_l, r);
+        m = mfma_h(ui_h, nar_h, m);
+        m = mfma_h(ui_l, nar_h, m);
+        m = mfma_h(ui_h, nar_l, m);
+        m = mfma_h(nur_h, ai_h, m);
+        m = mfma_h(nur_l, ai_h, m);
+        m = mfma_h(nur_h, ai_l, m);
+        cr[bi][bj] = r;
+        ci[bi][bj] = m;
+      }
+    }
+  };
+  const int nch = 2 * ns;
+  h8 pv[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) pv[q] = *src(0, q);
+#pragma unroll 1
+  for (int ch = 0; ch < nch - 1; ch++) {
+    rescale(ch);
+    __syncthreads();  // the previous chunk's readers are done
+#pragma unroll
+    for (int q = 0; q < 8; q++) *dst(q) = pv[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q++) pv[q] = *src(ch + 1, q);
+    compute();
+  }
+  // the last chunk, with A's loads in the prefetch registers' place:
+  // A(wr + 16 bi + l16, wc + 16 bj + 4 g + r)
+  rescale(nch - 1);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 8; q++) *dst(q) = pv[q];
+  __syncthreads();
+  float2 av[2][2][4];
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        av[bi][bj][r] = ldg<float>(A + (wr + 16 * bi + l16) + ld * (wc + 16 * bj + 4 * g + r));
+  compute();
+  __syncthreads();  // every wave is done reading the staged planes
+  const float sc2 = ldexpf(1.f, e);
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = wr + 16 * bi + l16, col = wc + 16 * bj + 4 * g + r;
+        Ts[col][rowpos ? rowpos[row] : row] =
+            make_float2(av[bi][bj][r].x + cr[bi][bj][r] * sc2, av[bi][bj][r].y + ci[bi][bj][r] * sc2);
+      }
+}
+
 // Panel of step s: x < nrest -> L_is = A_is U^-1 ; else U_sj = L^-1 P A_sj.
 // The operand tile is staged in LDS (rows gathered by the pivot permutation
-// for U_sj) before it is overwritten in place.
-template <class T>
-__global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ descs, int step) {
+// for U_sj) before it is overwritten in place.  With CROSS (the split-FP16
+// path) the tile first takes the cross update with the ncross steps before s
+// in its group, on the way into LDS (cross_to_lds), so that no separate
+// cross kernel reads and writes it.
+template <class T, bool CROSS>
+__device__ __forceinline__ void panel_tile(const SolveDesc &d, int step, int ncross) {
   typedef CxT<T> V;
-  const SolveDesc &d = descs[blockIdx.y];
-  step -= d.step0;
   const int nrest = d.nt - 1 - step;
   const int x = blockIdx.x;
   if (step < 0 || nrest <= 0 || x >= 2 * nrest) return;
   if (d.conv && *d.conv) return;
-  __shared__ V Ts[kTile][kTile + 1];  // Ts[col][row]
+  __shared__ __align__(16) V Ts[kTile][kTile + 1];  // Ts[col][row]
   const int tid = threadIdx.x, w = tid >> 6;
   const int lane = tid & 63, li = lane & 15, kq = lane >> 4;
   const int wr = 32 * (w & 1), wc = 32 * (w >> 1);
   const long ld = tile_ld(d);
   const GCx<T> *__restrict__ inv = (const GCx<T> *)d.inv + step * d.inv_stride;
   const bool lower = x < nrest;
-  GCx<T> *tile = lower ? tile_ptr<T>(d, step + 1 + x, step) : tile_ptr<T>(d, step, step + 1 + x - nrest);
-  if (lower) {
-    stage_tile<T>(Ts, tile, ld, [](int r) { return r; });
-  } else {
-    const int *perm = d.perm + step * kTile;
-    stage_tile<T>(Ts, tile, ld, [&](int r) { return perm[r]; });
+  const int ti = lower ? step + 1 + x : step, tj = lower ? step : step + 1 + x - nrest;
+  GCx<T> *tile = tile_ptr<T>(d, ti, tj);
+  bool staged = false;
+  if constexpr (CROSS) {
+    if (ncross > 0) {
+      // the row of C that lands at LDS row r is perm[r] (U_sj): rowpos = perm^-1
+      __shared__ int rowpos[kTile];
+      if (!lower && tid < kTile) rowpos[d.perm[step * kTile + tid]] = tid;
+      cross_to_lds(d, ti, tj, step - ncross, ncross, tile, ld, Ts, lower ? nullptr : rowpos);
+      staged = true;
+    }
+  }
+  if (!staged) {
+    if (lower) {
+      stage_tile<T>(Ts, tile, ld, [](int r) { return r; });
+    } else {
+      const int *perm = d.perm + step * kTile;
+      stage_tile<T>(Ts, tile, ld, [&](int r) { return perm[r]; });
+    }
   }
   __syncthreads();
   Acc3<T> q;
@@ -358,13 +518,28 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
       }
   if constexpr (sizeof(T) == 4) {
     if (planes) {
-      // split-FP16 planes of this panel tile for k_lu_trail_h
+      // split-FP16 planes of this panel tile for the split updates
       const int parity = step % d.group, side = lower ? 0 : 1;
-      const int t = step + 1 + (lower ? x : x - nrest);
+      const int t = lower ? ti : tj;
       write_split_planes(Ts, lower, (_Float16 *)d.hp + (long)((parity * 2 + side) * d.nt + t) * kPlaneHalves,
                          d.he + (parity * 2 + side) * d.nt + t);
     }
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ descs, int step) {
+  const SolveDesc &d = descs[blockIdx.y];
+  panel_tile<T, false>(d, step - d.step0, 0);
+}
+
+// Panel of step `step` on the split-FP16 path with its cross update fused:
+// the tiles of column and row `step` first take the ncross steps before it
+// in its group (planes in the group's slots), then the panel.
+__global__ __launch_bounds__(256, 4) void k_lu_panel_h(const SolveDesc *__restrict__ descs, int step,
+                                                   int ncross) {
+  const SolveDesc &d = descs[blockIdx.y];
+  panel_tile<float, true>(d, step - d.step0, ncross);
 }
 
 // C(ti, tj) -= sum_{u < ns} L(ti, s0+u) U(s0+u, tj): one 64x64 tile per

@@ -19,9 +19,10 @@
 // the two U column tiles are staged in LDS (64 KiB, two stages), so every
 // plane element is fetched from L2 once per workgroup and read by two or four
 // waves; the loads run two chunks ahead of the MFMAs, so the L2 latency hides
-// behind them.  (The cross update uses 128 x 64 and 64 x 128 regions of four
-// waves and one stage.)  The four
-// 16-byte k groups of a 64-byte LDS row are XOR-swizzled by the row, so the
+// behind them.  (The cross update of a diagonal tile uses one 64 x 64 region
+// of two waves and one stage; the other tiles of a step's column and row
+// take their cross update inside the fused panel, k_lu_panel_h in
+// tau_lu.hip.)  The four 16-byte k groups of a 64-byte LDS row are XOR-swizzled by the row, so the
 // lanes of a 128-bit read hit distinct banks.  Complex products in 4M form
 // (Cr += (-Ar) Br + Ai Bi, Ci += (-Ar) Bi + (-Ai) Br) with two FP32
 // accumulator sets.  The MFMA computes D = C^T = U^T L^T: the a-operand is
@@ -39,39 +40,10 @@
 
 namespace lsmsx {
 
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef GCx<float> GF2;
 // C moves with non-temporal loads and stores: it is touched once per group,
 // and keeping it out of L2 leaves room for the split planes, which are re-read.
 typedef float fv2 __attribute__((ext_vector_type(2)));
-#if defined(__HIP_DEVICE_COMPILE__)
-typedef __attribute__((address_space(1))) const h8 GH8;
-#else
-typedef const h8 GH8;
-#endif
-
-__device__ __forceinline__ f4 mfma_h(h8 a, h8 b, f4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
-// Sign flip of eight halves as four 32-bit XORs of the packed sign bits.
-typedef unsigned u4v __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ h8 neg8(h8 v) {
-  return __builtin_bit_cast(h8, __builtin_bit_cast(u4v, v) ^ 0x80008000u);
-}
-
-// LDS k group of row r holding logical group g (8 halves): XOR swizzle of the
-// four groups of a 64-byte row by (r >> 1) ^ (r >> 2), so that any 8 (and any
-// 16) consecutive rows read at one group cover distinct 16-byte bank groups:
-// rows of equal parity among 8 (equal r mod 4 among 16) get distinct masks.
-__device__ __forceinline__ int swz_group(int r, int g) { return g ^ (((r >> 1) ^ (r >> 2)) & 3); }
-
-// Steps per trailing update on this path: the descriptors' group (4, 8 or
-// 16; the planes of step s sit in slot s mod group), taken by recursive
-// halving down to blocks of kCrossH steps: the crosses within a block apply
-// at most kCrossH - 1 steps, and k_lu_strip_h applies each first half to its
-// second half's tile columns and rows before they are factored.
-constexpr int kCrossH = 4;
 
 // C tiles (r0 + wt, c0 + ct), wt < nrow <= RT, ct < ncol <= NC (a 64 RT x
 // 64 NC region of 128 RT NC threads; tiles past nrow / ncol are skipped),
@@ -336,71 +308,20 @@ __global__ __launch_bounds__(512) void k_lu_trail_h(const SolveDesc *__restrict_
                  min(2, nrest - 2 * sc), g0, ns, Ls, Us);
 }
 
-// Strip update between the halves of a block of 2 ns steps g .. g+2ns-1
-// (local g = step - step0): the first half's steps g .. g+ns-1 applied to the
-// second half's tile columns h .. b (rows >= h) and tile rows h .. b
-// (columns > b), h = g + ns, b its last step, in 128 x 128 regions
-// (K = 64 ns), so that the second half starts from columns and rows updated
-// by the whole first half, like a block of its own.  The group's trailing
-// update then applies all of its steps to the tiles beyond the group.
-// Nothing to do when the block has no second half.  xcd_swizzle as in
-// k_lu_cross_h.
-__global__ __launch_bounds__(512) void k_lu_strip_h(const SolveDesc *__restrict__ descs, int step,
-                                                    int ns, int xcd_swizzle) {
-  __shared__ h8 Ls[2][2][4][kTile][4];
-  __shared__ h8 Us[2][2][4][kTile][4];
-  int bx = blockIdx.x, by = blockIdx.y;
-  if (xcd_swizzle) xcd_remap(bx, by);
-  const SolveDesc &d = descs[by];
-  const int g0 = step - d.step0, nt = d.nt;
-  if (g0 < 0) return;
-  if (d.conv && *d.conv) return;
-  const int h = g0 + ns;
-  if (h > nt - 2) return;  // no second half
-  const int b = min(h + ns - 1, nt - 2);
-  const int w2 = (b - h + 2) / 2;            // column (row) pairs of the strip
-  const int nr2 = (nt - h + 1) / 2;          // row pairs of its column part
-  const int nc2 = (nt - 1 - b + 1) / 2;      // column pairs of its row part
-  const int x = bx;
-  if (x < nr2 * w2) {
-    const int rp = x / w2, cp = x % w2;
-    const int r0 = h + 2 * rp, c0 = h + 2 * cp;
-    region_h<2, 2, 2>(d, r0, min(2, nt - r0), c0, min(2, b + 1 - c0), g0, ns, Ls, Us);
-  } else if (x < nr2 * w2 + w2 * nc2) {
-    const int y = x - nr2 * w2, rp = y / nc2, cp = y % nc2;
-    const int r0 = h + 2 * rp, c0 = b + 1 + 2 * cp;
-    region_h<2, 2, 2>(d, r0, min(2, b + 1 - r0), c0, min(2, nt - c0), g0, ns, Ls, Us);
-  }
-}
-
-// Cross update for step t = g + u of the block starting at global step
-// `step` (see k_lu_cross in tau_lu.hip) on the split planes of steps
-// g .. t-1, 256 threads per workgroup: tile column t (rows >= t) in 128 x 64
-// regions (two row tiles), then tile row t (columns > t) in 64 x 128 regions
-// (two column tiles, sharing the row's L planes), so every wave has a tile.
-// With xcd_swizzle the workgroups of one solve run on one XCD (xcd_remap), so
-// the planes its regions share are fetched into one L2.
-__global__ __launch_bounds__(256) void k_lu_cross_h(const SolveDesc *__restrict__ descs, int step,
-                                                    int u, int xcd_swizzle) {
-  // three tiles' planes: (L, L, U) for the column part, (L, U, U) for the row part
-  __shared__ h8 stage[3][4][kTile][4];
-  int bx = blockIdx.x, by = blockIdx.y;
-  if (xcd_swizzle) xcd_remap(bx, by);
-  const SolveDesc &d = descs[by];
+// Cross update of the diagonal tile of step t = g + u of the group starting
+// at global step `step` (g local), before the tile is factored: the steps
+// g .. t-1 on their split planes, one 64 x 64 region of 128 threads per
+// solve.  The other tiles of column and row t take the same update inside
+// the fused panel (k_lu_panel_h).
+__global__ __launch_bounds__(128) void k_lu_cross_diag_h(const SolveDesc *__restrict__ descs,
+                                                         int step, int u) {
+  __shared__ h8 Ls[1][1][4][kTile][4];
+  __shared__ h8 Us[1][1][4][kTile][4];
+  const SolveDesc &d = descs[blockIdx.x];
   const int g0 = step - d.step0, t = g0 + u;
-  if (g0 < 0 || u >= min(kCrossH, d.nt - 1 - g0)) return;
+  if (g0 < 0 || u <= 0 || t > d.nt - 2) return;
   if (d.conv && *d.conv) return;
-  const int n1 = d.nt - t, nc = (n1 + 1) / 2, nr = n1 / 2, x = bx;
-  if (x < nc) {
-    auto &Ls = *reinterpret_cast<h8(*)[1][2][4][kTile][4]>(&stage[0]);
-    auto &Us = *reinterpret_cast<h8(*)[1][1][4][kTile][4]>(&stage[2]);
-    region_h<2, 1, 1>(d, t + 2 * x, min(2, n1 - 2 * x), t, 1, g0, u, Ls, Us);
-  } else if (x < nc + nr) {
-    auto &Ls = *reinterpret_cast<h8(*)[1][1][4][kTile][4]>(&stage[0]);
-    auto &Us = *reinterpret_cast<h8(*)[1][2][4][kTile][4]>(&stage[1]);
-    const int c0 = t + 1 + 2 * (x - nc);
-    region_h<1, 2, 1>(d, t, 1, c0, min(2, d.nt - c0), g0, u, Ls, Us);
-  }
+  region_h<1, 1, 1>(d, t, 1, t, 1, g0, u, Ls, Us);
 }
 
 }  // namespace lsmsx

@@ -143,8 +143,9 @@ def test_autotune_then_same_results(c4):
 
 @pytest.mark.parametrize("group", [4, 16, 8])
 def test_split_group_sizes(c4, group):
-    """Split-FP16 path with 4, 8 or 16 steps per trailing update (recursive
-    halves joined by strip updates): uneven solves meet LAPACK at 1e-10, the
+    """Split-FP16 path with 4, 8 or 16 steps per trailing update (the fused
+    panels of a group apply up to 15 earlier steps): uneven solves meet
+    LAPACK at 1e-10, the
     schedules stay bitwise identical at each size, and a graph replay equals
     the eager result."""
     s, ctx = c4
