@@ -102,7 +102,7 @@ __global__ void k_rel_tstore(double2 *, long, long, int, const double *, const d
 __global__ void k_rel_tau(double2 *, int, const double *, const double2 *);
 __global__ void k_rel_blocks(double2 *, long, int, const double *);
 __global__ void k_lu_cross_diag_h(const SolveDesc *, int, int);
-__global__ void k_lu_panel_h(const SolveDesc *, int, int);
+__global__ void k_lu_panel_h(const SolveDesc *, int, int, int);
 
 // Copy a caller matrix (central block first, n x n) into the solver layout;
 // rowperm (or null) reorders the n - k non-central rows: solver row xr takes
@@ -487,7 +487,7 @@ int launch_lu(lsmsx_ctx *c, const SolveDesc *dd, int nb, int nt_max, bool split_
         {
           ProfScope p(c, 2);
           hipLaunchKernelGGL(k_lu_panel_h, dim3(2 * (nt_max - 1 - t), nb), dim3(256), 0, st, dd,
-                             t, u);
+                             t, u, swz);
         }
         if (int r = check_launch("k_lu_diag/k_lu_panel_h")) return r;
       }
@@ -732,7 +732,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
     if (plain) bytes[i] += kTile * kTile * 16;
     if (mixed) bytes[i] += Ns * Ns * 8 + nt * kTile * kTile * 8 + Ns * kc * 24;
     if (refined64) bytes[i] += Ns * Ns * 16 + nt * kTile * kTile * 16 + Ns * kc * 32;
-    if (split) bytes[i] += 2 * lu_group(c, true) * nt * ((long)kPlaneHalves * 2 + 4);
+    if (split) bytes[i] += (2 * lu_group(c, true) * nt + 2) * ((long)kPlaneHalves * 2 + 4);
   }
   // the explicit-inverse route's buffers are not used here: give them back
   // before sizing the chunks by the free memory
@@ -762,7 +762,7 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
       mtot += Ns * Ns;
       xtot += Ns * kc;
       ivtot += refining ? (Ns / kTile) * kTile * kTile : kTile * kTile;
-      htot += 2 * lu_group(c, split) * (Ns / kTile);
+      htot += 2 * lu_group(c, split) * (Ns / kTile) + 2;
       maxliz = std::max(maxliz, a.numLIZ);
       nt_max = std::max(nt_max, (int)(Ns / kTile));
       if (gmode) {
@@ -817,6 +817,9 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
         d.hp = c->hp.p + hoff * kPlaneHalves;
         d.he = c->he.p + hoff;
         hoff += 2 * lu_group(c, split) * (Ns / kTile);
+        d.ihp = c->hp.p + hoff * kPlaneHalves;
+        d.ihe = c->he.p + hoff;
+        hoff += 2;
       }
       d.pos = a.d_pos;
       d.lmax = a.d_lmax;

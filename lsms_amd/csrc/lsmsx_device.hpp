@@ -80,6 +80,11 @@ struct SolveDesc {
   // of a group as hi/lo FP16 planes with one power-of-2 scale per tile
   void *hp;               // (step % group) x (L, U) x nt tiles x 4 planes x 64 x 64 halves
   int *he;                // (step % group) x (L, U) x nt tile exponents
+  // the current diagonal tile's inverses as split planes for the fused panel:
+  // L^-1 (unit lower, as an L tile) then U^-1 (upper, as a U tile), and
+  // their two exponents; written by k_lu_diag<float> when set
+  void *ihp;              // 2 tiles x 4 planes x 64 x 64 halves
+  int *ihe;               // 2
   // explicit inverse (full tau, Lloyd term, adjoint of tau00)
   int full_lu;            // 1: the last tile is factored by k_lu_diag like every other tile
   double2 *Minv;          // Ns x Ns (leading dim ld): M^-1 in the solver layout

@@ -196,6 +196,83 @@ __device__ __forceinline__ void stage_tile(CxT<T> (*S)[kTile + 1], const GCx<T> 
   for (int q = 0; q < kTile / 4; q++) S[c0 + 4 * q][r] = v[q];
 }
 
+// Power-of-2 scale of a 64 x 64 tile for its split-FP16 planes, 256
+// threads: ex with max |Re|, |Im| of get(rc, k) times 2^-ex below 2^14
+// (inside the FP16 range with room for the lo part; 0 for a zero tile).
+template <class F>
+__device__ int plane_exponent(F get) {
+  __shared__ float wmax[4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  __syncthreads();  // the values were written by all four waves
+  float m = 0.f;
+  for (int e = tid; e < kTile * kTile; e += 256) {
+    const float2 v = get(e >> 6, e & 63);
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+  for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if (lane == 0) wmax[w] = m;
+  __syncthreads();
+  m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+  int ex = 0;
+  if (m > 0.f) {
+    frexpf(m, &ex);  // m < 2^ex
+    ex -= 14;
+  }
+  return ex;
+}
+
+// Eight complex values x 2^-ex (Re times sgn) split into hi/lo FP16 planes:
+// x 2^-ex = hi + lo to 22 bits.
+__device__ __forceinline__ void split8(const float2 (&v)[8], float sgn, float sc, h8 &rh, h8 &rl,
+                                       h8 &ih, h8 &il) {
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    const float re = sgn * v[u].x * sc, im = v[u].y * sc;
+    const _Float16 a = (_Float16)re, b = (_Float16)im;
+    rh[u] = a;
+    rl[u] = (_Float16)(re - (float)a);
+    ih[u] = b;
+    il[u] = (_Float16)(im - (float)b);
+  }
+}
+
+// Split the 64 x 64 values get(rc, k) (256 threads) into hi/lo FP16 planes
+// with one power-of-2 scale (plane_exponent) for the split-FP16 products.
+// Layout, k fastest:
+//   L tile (rc = row i, k = tile column):  planes -Re, -Re lo, Im, Im lo ([i][k]);
+//   U tile (k = tile row, rc = column j):  planes  Re,  Re lo, Im, Im lo ([j][k]).
+// The L planes carry the sign of the update C -= L U.
+template <class F>
+__device__ void write_planes(F get, bool lower, _Float16 *out, int *eout) {
+  const int tid = threadIdx.x;
+  const int ex = plane_exponent(get);
+  if (tid == 0) *eout = ex;
+  const float sc = ldexpf(1.f, -ex);
+  const float sgn = lower ? -1.f : 1.f;
+  for (int q = tid; q < kTile * 8; q += 256) {
+    const int rc = q >> 3, kg = q & 7;  // row i (L) or column j (U); k group
+    float2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = get(rc, 8 * kg + u);
+    h8 rh, rl, ih, il;
+    split8(v, sgn, sc, rh, rl, ih, il);
+    h8 *o = (h8 *)(out + rc * kTile + 8 * kg);
+    o[0] = rh;
+    o[(kTile * kTile) / 8] = rl;
+    o[2 * (kTile * kTile) / 8] = ih;
+    o[3 * (kTile * kTile) / 8] = il;
+  }
+}
+
+// The planes of an FP32 panel tile held in LDS as Ts[col][row].
+__device__ __forceinline__ void write_split_planes(float2 (*Ts)[kTile + 1], bool lower,
+                                                   _Float16 *out, int *eout) {
+  if (lower)
+    write_planes([&](int i, int k) { return Ts[k][i]; }, true, out, eout);
+  else
+    write_planes([&](int j, int k) { return Ts[j][k]; }, false, out, eout);
+}
+
 // ---------------------------------------------------------------------------
 // Diagonal tile of step s: factor, write back L\U, store L^-1 (strict lower)
 // and U^-1 (upper incl. diagonal) packed in one tile, and the permutation.
@@ -238,56 +315,27 @@ __global__ __launch_bounds__(256) void k_lu_diag(const SolveDesc *__restrict__ d
     int r = e & 63, c = e >> 6;
     inv[r + kTile * c] = A[c][r];
   }
-}
-
-// Split one FP32 panel tile (Ts[col][row], 256 threads) into hi/lo FP16
-// planes for the split-FP16 trailing update, with one power-of-2 scale e per
-// tile (max |x| 2^-e < 2^14, inside the FP16 range with room for the lo
-// part); x 2^-e = hi + lo to 22 bits.  Layout, k fastest:
-//   L tile (rows i, k = tile column):  planes -Re, -Re lo, Im, Im lo ([i][k]);
-//   U tile (k = tile row, columns j):  planes  Re,  Re lo, Im, Im lo ([j][k]).
-// The L planes carry the sign of the update C -= L U.
-__device__ void write_split_planes(float2 (*Ts)[kTile + 1], bool lower, _Float16 *out, int *eout) {
-  __shared__ float wmax[4];
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  __syncthreads();  // the tile was written to Ts by all four waves
-  float m = 0.f;
-  for (int e = tid; e < kTile * kTile; e += 256) {
-    const float2 v = Ts[e >> 6][e & 63];
-    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
-  }
-  for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-  if (lane == 0) wmax[w] = m;
-  __syncthreads();
-  m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-  int ex = 0;
-  if (m > 0.f) {
-    frexpf(m, &ex);  // m < 2^ex
-    ex -= 14;
-  }
-  if (tid == 0) *eout = ex;
-  const float sc = ldexpf(1.f, -ex);
-  const float sgn = lower ? -1.f : 1.f;
-  typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-  for (int q = tid; q < kTile * 8; q += 256) {
-    const int rc = q >> 3, kg = q & 7;  // row i (L) or column j (U); k group
-    h8 rh, rl, ih, il;
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int k = 8 * kg + u;
-      const float2 v = lower ? Ts[k][rc] : Ts[rc][k];
-      const float re = sgn * v.x * sc, im = v.y * sc;
-      const _Float16 a = (_Float16)re, b = (_Float16)im;
-      rh[u] = a;
-      rl[u] = (_Float16)(re - (float)a);
-      ih[u] = b;
-      il[u] = (_Float16)(im - (float)b);
+  if constexpr (sizeof(T) == 4) {
+    if (d.ihp) {
+      // split planes of L^-1 (unit lower, L layout) and U^-1 (U layout) for
+      // the fused panel's FP16 products
+      _Float16 *ip = (_Float16 *)d.ihp;
+      write_planes(
+          [&](int i, int k) {
+            V v = czero<V>();
+            if (i > k) v = A[k][i];
+            if (i == k) v.x = 1.f;
+            return v;
+          },
+          true, ip, d.ihe);
+      write_planes(
+          [&](int j, int k) {
+            V v = czero<V>();
+            if (k <= j) v = A[j][k];
+            return v;
+          },
+          false, ip + kPlaneHalves, d.ihe + 1);
     }
-    h8 *o = (h8 *)(out + rc * kTile + 8 * kg);
-    o[0] = rh;
-    o[(kTile * kTile) / 8] = rl;
-    o[2 * (kTile * kTile) / 8] = ih;
-    o[3 * (kTile * kTile) / 8] = il;
   }
 }
 
@@ -441,10 +489,9 @@ __device__ __forceinline__ void cross_to_lds(const SolveDesc &d, int ti, int tj,
 // in its group, on the way into LDS (cross_to_lds), so that no separate
 // cross kernel reads and writes it.
 template <class T, bool CROSS>
-__device__ __forceinline__ void panel_tile(const SolveDesc &d, int step, int ncross) {
+__device__ __forceinline__ void panel_tile(const SolveDesc &d, int step, int ncross, int x) {
   typedef CxT<T> V;
   const int nrest = d.nt - 1 - step;
-  const int x = blockIdx.x;
   if (step < 0 || nrest <= 0 || x >= 2 * nrest) return;
   if (d.conv && *d.conv) return;
   __shared__ __align__(16) V Ts[kTile][kTile + 1];  // Ts[col][row]
@@ -475,31 +522,121 @@ __device__ __forceinline__ void panel_tile(const SolveDesc &d, int step, int ncr
     }
   }
   __syncthreads();
-  Acc3<T> q;
-  zero_acc(q);
-  // operand k-steps in flight: three for FP64 (the inverse's L2 latency),
-  // one for FP32, where more registers would cost a wave per SIMD
-  constexpr int kPanelPF = sizeof(T) == 8 ? 3 : 1;
-  if (lower) {
-    // A(i,k) = Ts(wr+i, k); B(k,j) = U^-1(k, wc+j), upper incl. diagonal
-    wave_gemm3m<T, false, kPanelPF>([&](int i, int k) { return Ts[k][wr + i]; },
-                          [&](int k, int j) {
-                            int jj = wc + j;
-                            V v = czero<V>();
-                            if (k <= jj) v = inv[k + kTile * jj];
-                            return v;
-                          },
-                          q);
+  V ov[2][2][4];  // output (row wr + 16 bi + li, column wc + 16 bj + acc_row(kq, r))
+  if constexpr (CROSS) {
+    // split-FP16 product with the diagonal tile's inverse planes (written by
+    // k_lu_diag): the tile's side is split here with its own scale, the
+    // accumulators take -(A B) as in the cross update (b-operand rows with
+    // -Re, a-operand columns with +Re), two 32-wide k chunks
+    const int eC = plane_exponent([&](int a, int b) { return Ts[a][b]; });
+    const float sc = ldexpf(1.f, -eC);
+    const int g = kq, l16 = li;
+    GH8 *ip = (GH8 *)((const _Float16 *)d.ihp + (lower ? kPlaneHalves : 0));
+    const int eI = d.ihe[lower ? 1 : 0];
+    f4 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++) {
+        cr[bi][bj] = f4{0.f, 0.f, 0.f, 0.f};
+        ci[bi][bj] = cr[bi][bj];
+      }
+#pragma unroll 1
+    for (int kh = 0; kh < 2; kh++) {
+      // b-operands (rows wr + 16 bi ..): L_is = C U^-1 takes the rows of C
+      // from Ts[k][i], U_sj = L^-1 (P C) the rows of L^-1 from its planes
+      h8 lo[2][4];
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++) {
+        if (lower) {
+          float2 v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) v[u] = Ts[32 * kh + 8 * g + u][wr + 16 * bi + l16];
+          split8(v, -1.f, sc, lo[bi][0], lo[bi][1], lo[bi][2], lo[bi][3]);
+        } else {
+#pragma unroll
+          for (int p = 0; p < 4; p++)
+            lo[bi][p] = ip[((p * kTile + wr + 16 * bi + l16) * kTile + 32 * kh + 8 * g) / 8];
+        }
+      }
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++) {
+        // a-operand (columns wc + 16 bj ..): U^-1 from its planes, or P C from Ts[j][k]
+        h8 uo[4];
+        if (lower) {
+#pragma unroll
+          for (int p = 0; p < 4; p++)
+            uo[p] = ip[((p * kTile + wc + 16 * bj + l16) * kTile + 32 * kh + 8 * g) / 8];
+        } else {
+          float2 v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) v[u] = Ts[wc + 16 * bj + l16][32 * kh + 8 * g + u];
+          split8(v, 1.f, sc, uo[0], uo[1], uo[2], uo[3]);
+        }
+        const h8 nur_h = neg8(uo[0]), nur_l = neg8(uo[1]);
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++) {
+          f4 r = cr[bi][bj], m = ci[bi][bj];
+          r = mfma_h(uo[0], lo[bi][0], r);
+          r = mfma_h(uo[1], lo[bi][0], r);
+          r = mfma_h(uo[0], lo[bi][1], r);
+          r = mfma_h(uo[2], lo[bi][2], r);
+          r = mfma_h(uo[3], lo[bi][2], r);
+          r = mfma_h(uo[2], lo[bi][3], r);
+          m = mfma_h(uo[2], lo[bi][0], m);
+          m = mfma_h(uo[3], lo[bi][0], m);
+          m = mfma_h(uo[2], lo[bi][1], m);
+          m = mfma_h(nur_h, lo[bi][2], m);
+          m = mfma_h(nur_l, lo[bi][2], m);
+          m = mfma_h(nur_h, lo[bi][3], m);
+          cr[bi][bj] = r;
+          ci[bi][bj] = m;
+        }
+      }
+    }
+    const float so = -ldexpf(1.f, eC + eI);
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) ov[bj][bi][r] = make_float2(cr[bi][bj][r] * so, ci[bi][bj][r] * so);
   } else {
-    // A(i,k) = L^-1(wr+i, k), unit lower; B(k,j) = (P A_sj)(k, wc+j)
-    wave_gemm3m<T, false, kPanelPF>([&](int i, int k) {
-                            int ii = wr + i;
-                            V v = czero<V>();
-                            if (ii == k) v.x = T(1);
-                            if (ii > k) v = inv[ii + kTile * k];
-                            return v;
-                          },
-                          [&](int k, int j) { return Ts[wc + j][k]; }, q);
+    Acc3<T> q;
+    zero_acc(q);
+    // operand k-steps in flight: three for FP64 (the inverse's L2 latency),
+    // one for FP32, where more registers would cost a wave per SIMD
+    constexpr int kPanelPF = sizeof(T) == 8 ? 3 : 1;
+    if (lower) {
+      // A(i,k) = Ts(wr+i, k); B(k,j) = U^-1(k, wc+j), upper incl. diagonal
+      wave_gemm3m<T, false, kPanelPF>([&](int i, int k) { return Ts[k][wr + i]; },
+                                      [&](int k, int j) {
+                                        int jj = wc + j;
+                                        V v = czero<V>();
+                                        if (k <= jj) v = inv[k + kTile * jj];
+                                        return v;
+                                      },
+                                      q);
+    } else {
+      // A(i,k) = L^-1(wr+i, k), unit lower; B(k,j) = (P A_sj)(k, wc+j)
+      wave_gemm3m<T, false, kPanelPF>([&](int i, int k) {
+                                        int ii = wr + i;
+                                        V v = czero<V>();
+                                        if (ii == k) v.x = T(1);
+                                        if (ii > k) v = inv[ii + kTile * k];
+                                        return v;
+                                      },
+                                      [&](int k, int j) { return Ts[wc + j][k]; }, q);
+    }
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          ov[bj][bi][r].x = q.p[0][bj][bi][r] - q.p[1][bj][bi][r];
+          ov[bj][bi][r].y = q.p[2][bj][bi][r] - q.p[0][bj][bi][r] - q.p[1][bj][bi][r];
+        }
   }
   GCx<T> *C = tile + wr + ld * wc;
   const bool planes = sizeof(T) == 4 && d.hp != nullptr;
@@ -510,9 +647,7 @@ __device__ __forceinline__ void panel_tile(const SolveDesc &d, int step, int ncr
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        V o;
-        o.x = q.p[0][bj][bi][r] - q.p[1][bj][bi][r];
-        o.y = q.p[2][bj][bi][r] - q.p[0][bj][bi][r] - q.p[1][bj][bi][r];
+        const V o = ov[bj][bi][r];
         C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))] = o;
         if (planes) Ts[wc + 16 * bj + acc_row<T>(kq, r)][wr + 16 * bi + li] = o;
       }
@@ -530,16 +665,21 @@ __device__ __forceinline__ void panel_tile(const SolveDesc &d, int step, int ncr
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ descs, int step) {
   const SolveDesc &d = descs[blockIdx.y];
-  panel_tile<T, false>(d, step - d.step0, 0);
+  panel_tile<T, false>(d, step - d.step0, 0, blockIdx.x);
 }
 
 // Panel of step `step` on the split-FP16 path with its cross update fused:
 // the tiles of column and row `step` first take the ncross steps before it
-// in its group (planes in the group's slots), then the panel.
+// in its group (planes in the group's slots), then the panel.  With
+// xcd_swizzle the workgroups of one solve run on one XCD (xcd_remap), so the
+// planes its tiles share (the U planes of the step's column, the L planes of
+// its row) are fetched into one L2.
 __global__ __launch_bounds__(256, 4) void k_lu_panel_h(const SolveDesc *__restrict__ descs, int step,
-                                                   int ncross) {
-  const SolveDesc &d = descs[blockIdx.y];
-  panel_tile<float, true>(d, step - d.step0, ncross);
+                                                      int ncross, int xcd_swizzle) {
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (xcd_swizzle) xcd_remap(bx, by);
+  const SolveDesc &d = descs[by];
+  panel_tile<float, true>(d, step - d.step0, ncross, bx);
 }
 
 // C(ti, tj) -= sum_{u < ns} L(ti, s0+u) U(s0+u, tj): one 64x64 tile per
