@@ -674,7 +674,7 @@ __global__ __launch_bounds__(256) void k_lu_panel(const SolveDesc *__restrict__ 
 // xcd_swizzle the workgroups of one solve run on one XCD (xcd_remap), so the
 // planes its tiles share (the U planes of the step's column, the L planes of
 // its row) are fetched into one L2.
-__global__ __launch_bounds__(256, 4) void k_lu_panel_h(const SolveDesc *__restrict__ descs, int step,
+__global__ __launch_bounds__(256) void k_lu_panel_h(const SolveDesc *__restrict__ descs, int step,
                                                       int ncross, int xcd_swizzle) {
   int bx = blockIdx.x, by = blockIdx.y;
   if (xcd_swizzle) xcd_remap(bx, by);
