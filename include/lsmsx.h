@@ -251,10 +251,10 @@ int lsmsx_tau00_adjoint(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
  *                            late enough that all finish on the same step, so
  *                            the small final steps of every solve run together;
  *   LSMSX_SCHED_XCD_SWIZZLE  trailing-update workgroups (and, with split-FP16
- *                            updates, the cross and strip updates) are
- *                            remapped so that each of the 8 XCDs works on
- *                            whole solves and reads their panels into its
- *                            own L2;
+ *                            updates, the panels with their fused cross
+ *                            updates) are remapped so that each of the 8 XCDs
+ *                            works on whole solves and reads their panels
+ *                            into its own L2;
  *   LSMSX_SCHED_TWO_STREAMS  the two halves of a chunk run on two streams (of
  *                            default priority), so one half's diagonal-tile
  *                            and panel steps overlap the other's updates.
@@ -285,8 +285,9 @@ int lsmsx_set_shared_structure_constants(lsmsx_ctx *ctx, int on);
 
 /* Steps per trailing update of the split-FP16 path (LSMSX_PREC_MIXED_F16):
  * 4, 8 (the default) or 16.  Larger groups read and write the trailing
- * matrix less often and spend more in the cross and strip updates that join
- * their halves; results of different group sizes differ by rounding only
+ * matrix less often and spend more in the cross updates inside the panels,
+ * which apply up to group - 1 earlier steps of the group to each panel
+ * tile; results of different group sizes differ by rounding only
  * (each meets the refinement tolerance).  lsmsx_get_split_group returns the
  * current size and, in ms[3] (may be NULL), the wall times of sizes 4, 8 and
  * 16 measured by the last lsmsx_autotune at LSMSX_PREC_MIXED_F16 (-1 when
