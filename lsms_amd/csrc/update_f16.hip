@@ -131,8 +131,10 @@ __device__ __forceinline__ void region_h(const SolveDesc &d, int r0, int nrow, i
   auto compute = [&](int buf) __attribute__((always_inline)) {
     if (!valid) return;
     // the wave that reaches its MFMAs first keeps the issue slots while the
-    // other wave of its SIMD is still staging, so the two drift apart
+    // other wave of its SIMD is still staging, so the two drift apart; the
+    // scheduler interleaves the LDS operand reads with the MFMAs
     __builtin_amdgcn_s_setprio(2);
+    __builtin_amdgcn_iglp_opt(0);
     // U operands of the wave's two 16-column blocks, with -Re(U) formed once
     // here for the -Im(L) Re(U) term (cheaper than negating Im(L) per row
     // block; Im(L) (-Re U) = -(Im(L) Re(U)) exactly)
