@@ -114,13 +114,21 @@ int lsmsx_tau00(lsmsx_ctx *ctx, int atom_id, int ispin, int iie,
 
 /* Batched form: nsolve independent (atom, spin, energy slot) solves in one
  * grouped launch sequence.  prel holds 2*nsolve doubles; tau00 receives
- * nsolve blocks of kkrsz_ns^2 complex; info nsolve ints (0 ok, > 0 the
- * first singular pivot, -2 an internal wait bound of the mixed path's
- * refinement sweep was hit: never expected, the solve's tau00 is then not
- * valid; -3 the FP64 refinement of the solve -- LSMSX_PREC_FP64_REFINED, or
- * the redo of a guarded plain solve or of an unconverged mixed solve -- did
- * not converge within the sweep limit, and tau00 is the unrefined plain
- * FP64 result).  Replaces the OpenMP loop of calculateAllTauMatrices
+ * nsolve blocks of kkrsz_ns^2 complex; info nsolve ints (0 ok, > 0 M is
+ * singular: the first exactly singular U(i,i) of an LU with partial
+ * pivoting over whole columns, 1-based in the reference order; -2 an
+ * internal wait bound of the mixed path's refinement sweep was hit: never
+ * expected, the solve's tau00 is then not valid; -3 the FP64 refinement of
+ * the solve did not converge within the sweep limit and tau00 is the
+ * unrefined plain FP64 result -- only for relativistic stores, see below).
+ * Solves that pivoting inside the 64 x 64 diagonal tiles cannot factor (a
+ * singular diagonal tile of a non-singular M), and solves whose FP64
+ * refinement (LSMSX_PREC_FP64_REFINED, or the redo of a guarded plain solve
+ * or of an unconverged mixed solve) does not converge, are solved again by
+ * the reference's own route, LU with partial pivoting over whole columns of
+ * the FP64 M and M X = [t0; 0] (rocSOLVER zgetrf/zgetrs on the device);
+ * lsmsx_last_batch_stats marks them with iters = -2.  Relativistic stores
+ * keep the -3 result.  Replaces the OpenMP loop of calculateAllTauMatrices
  * (calculateTauMatrix.cpp:841-945). */
 int lsmsx_tau00_batched(lsmsx_ctx *ctx, int nsolve, const int *atom_ids,
                         const int *ispins, const int *iies, const double *prel,
@@ -185,8 +193,9 @@ int lsmsx_set_pivot_guard(lsmsx_ctx *ctx, double threshold);
 /* Per-solve statistics of the last batched / single call on this context:
  * iters[i] = refinement sweeps (0 on the plain FP64 path, > 0 for a refined
  * solve including the plain path's guarded ones, -1 when a mixed solve was
- * redone in FP64), minpiv[i] = smallest |pivot| of the factorization that
- * produced the result.  Either pointer may be NULL. */
+ * redone in FP64, -2 when the column-pivoted fallback produced the result),
+ * minpiv[i] = smallest |pivot| of the tile-pivoted factorization of the
+ * solve.  Either pointer may be NULL. */
 int lsmsx_last_batch_stats(lsmsx_ctx *ctx, int nsolve, int *iters, double *minpiv);
 
 /* Workspace cap in bytes for the batched matrix pool (0 = automatic:

@@ -59,7 +59,9 @@ def build(verbose=False):
                 print(r.stdout, r.stderr, file=sys.stderr)
     objs = [os.path.join(OUT, s + ".o") for s in HIP_SOURCES + CXX_SOURCES]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    # rocSOLVER (with rocBLAS for its handle): the column-pivoted fallback
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-L/opt/rocm/lib",
+          "-lrocsolver", "-lrocblas"])
     os.replace(tmp, LIB)
     # C++ driver of the LSMS-shaped host interface (run by the GPU tests)
     drv = os.path.join(os.path.dirname(HERE), "tests", "cpp", "host_mirror_driver.cpp")

@@ -514,7 +514,8 @@ class Context:
 
     def last_stats(self, n):
         """(refinement sweeps per solve: 0 plain, > 0 refined, -1 mixed solve redone in
-        FP64; smallest |pivot| of the factorization that produced each result)."""
+        FP64, -2 solved by the column-pivoted fallback; smallest |pivot| of the
+        tile-pivoted factorization of each solve)."""
         it = np.zeros(n, dtype=np.int32)
         mp = np.zeros(n, dtype=np.float64)
         _ck(load().lsmsx_last_batch_stats(self._h, n, _p(it), _p(mp)))

@@ -21,6 +21,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/lsmsx.h"
 #include "lsmsx_device.hpp"
 
@@ -128,6 +131,16 @@ __global__ __launch_bounds__(256) void k_permute_in(const SolveDesc *__restrict_
     }
     col[r] = v;
   }
+}
+
+// Right-hand side [t0; 0] of the column-pivoted fallback: B (N x kc, column
+// stride N) from the central site's t block t0 (column stride ldt).
+__global__ void k_fallback_rhs(double2 *__restrict__ B, int N, int kc,
+                               const double2 *__restrict__ t0, int ldt) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)N * kc) return;
+  const int r = (int)(e % N), col = (int)(e / N);
+  B[e] = r < kc ? t0[r + (long)ldt * col] : make_double2(0.0, 0.0);
 }
 
 __global__ void k_init_status(const SolveDesc *__restrict__ descs, int n) {
@@ -293,6 +306,12 @@ struct lsmsx_ctx {
   int next_geo = 0;
   DevBuf<double2> GS;  // their storage on the paths that keep no G otherwise
   DevBuf<int> rowperm;  // retry order of lsmsx_solve_tau00_matrix
+  // column-pivoted fallback of the batched calls (rocSOLVER getrf/getrs on
+  // the FP64 matrix in the reference layout)
+  rocblas_handle blas = nullptr;
+  DevBuf<double2> fbM, fbB;
+  DevBuf<int> fbpiv, fbinfo;
+  DevBuf<SolveDesc> fbdesc;
   DevBuf<double2> GH, YB, GP;
   // relativistic store (lsmsx_upload_tmats_rel): the solver store holds
   // t' = f U^T t U, f = prel^2/E per energy slot; rel_finv[slot] = 1/f
@@ -666,6 +685,13 @@ int validate_common(lsmsx_ctx *c, int nsc, int nspola) {
 //                   unconverged solves are redone with FP64_REFINED;
 //   FP64_REFINED    FP64 factors of an FP64 copy + refinement against M;
 //                   unconverged solves fall back to the plain FP64 result.
+int assemble_reference_layout(lsmsx_ctx *c, AtomDev &a, int ispin, int iie, const double *prel,
+                              int nsc, int nspola, double2 *dst, SolveDesc *ddev);
+int column_pivoted_fallback(lsmsx_ctx *c, AtomDev &a, int ispin, int iie, const double *prel,
+                            int nsc, int nspola, double *tau, int *info);
+// last_iters value of a solve whose result comes from the column-pivoted fallback
+constexpr int kItersColumnPivoted = -2;
+
 int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins,
                 const int *iies, const double *prel, int nsc, int nspola, int precision,
                 double *tau_out, int *info_out) {
@@ -1157,12 +1183,31 @@ int run_batched(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
                   sizeof(double) * 2 * kc * kc);
       info_out[i] = rinfo[j];
       // sweeps of the FP64 refinement (plain solves); -1 marks a mixed
-      // solve redone in FP64
-      outer_iters[i] = plain ? c->last_iters[j] : -1;
+      // solve redone in FP64, -2 one taken by the column-pivoted fallback
+      outer_iters[i] = c->last_iters[j] == kItersColumnPivoted ? kItersColumnPivoted
+                       : plain                                 ? c->last_iters[j]
+                                                               : -1;
       outer_minpiv[i] = c->last_minpiv[j];
     }
     c->last_iters = outer_iters;
     c->last_minpiv = outer_minpiv;
+  }
+  // Column-pivoted fallback (rocSOLVER on the reference-layout FP64 M) for
+  // the solves that are still singular under tile-local pivoting or whose
+  // FP64 refinement did not converge (relativistic stores excluded: their
+  // tau00 is post-processed in the kappa-mu basis)
+  if (!c->rel) {
+    for (int i = 0; i < nsolve; i++) {
+      if (!(info_out[i] > 0 || info_out[i] == kInfoRefineUnconverged)) continue;
+      AtomDev &a = c->atoms[atom_ids[i]];
+      if (nsc * a.kkrsz != kc) continue;
+      int finfo = 0;
+      if (int r = column_pivoted_fallback(c, a, ispins[i], iies[i], prel + 2 * i, nsc, nspola,
+                                          tau_out + (long)i * kc * kc * 2, &finfo))
+        return r;
+      info_out[i] = finfo;
+      c->last_iters[i] = kItersColumnPivoted;
+    }
   }
   return 0;
 }
@@ -1471,6 +1516,99 @@ int run_adjoint(lsmsx_ctx *c, int nsolve, const int *atom_ids, const int *ispins
   return 0;
 }
 
+// The FP64 KKR matrix M = 1 - t G of one (atom, spin, energy) in the
+// reference layout (central site first, column-major N x N, N = nsc nrmat)
+// into dst, on the context's stream; ddev is a device descriptor slot.  The
+// relativistic store's kappa-mu blocks U M'_ij U^T are formed as well.
+int assemble_reference_layout(lsmsx_ctx *c, AtomDev &a, int ispin, int iie, const double *prel,
+                              int nsc, int nspola, double2 *dst, SolveDesc *ddev) {
+  const int N = nsc * a.nrmat;
+  SolveDesc d{};
+  int kc = nsc * a.kkrsz;
+  d.pos = a.d_pos;
+  d.lmax = a.d_lmax;
+  d.store = a.d_store;
+  d.site_off = a.d_off[0][nsc - 1];
+  d.numLIZ = a.numLIZ;
+  d.n_spin_cant = nsc;
+  d.kkrsz = a.kkrsz;
+  d.kc = kc;
+  d.prel_re = prel[0];
+  d.prel_im = prel[1];
+  d.t_off = (long)iie * c->blk + ((nspola != nsc) ? (long)kc * kc * ispin : 0);
+  d.ldt = kc;
+  d.M = dst;
+  d.ld = N;
+  d.Ns = N;
+  d.pad = 0;
+  d.nt = 0;
+  d.A = d.M;
+  HIPCHK(hipMemcpyAsync(ddev, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+  {
+    ProfScope p(c, 0);
+    launch_assemble(c, nsc, dim3(a.numLIZ, 1), c->stream, ddev);
+  }
+  if (int r = check_launch("k_assemble")) return r;
+  if (c->rel) {  // the reference's kappa-mu blocks U M'_ij U^T
+    if (nsc != 2 || 2 * a.kkrsz != c->rel_kk2)
+      return fail(-6, "the relativistic store needs n_spin_cant = 2 and the store's lmax");
+    for (int s = 0; s < a.numLIZ; s++)
+      if (a.lmax[s] != a.lmax[0])
+        return fail(-6, "relativistic blocks need every LIZ site at the central lmax");
+    hipLaunchKernelGGL(k_rel_blocks, dim3(a.numLIZ, a.numLIZ), dim3(256), 0, c->stream, dst,
+                       (long)N, c->rel_kk2, c->relU.p);
+    if (int r = check_launch("k_rel_blocks")) return r;
+  }
+  return 0;
+}
+
+// Column-pivoted fallback of the batched calls: the reference's own route
+// (solveTau00zgetrf, linearSolvers_CPU.cpp:98-123: LU of M with partial
+// pivoting over whole columns, M X = [t0; 0], tau00 = the top kc x kc block
+// of X), with rocSOLVER's zgetrf/zgetrs on the FP64 matrix in the reference
+// layout.  For the solves that pivoting inside 64 x 64 diagonal tiles cannot
+// factor (a singular diagonal tile, info > 0) or whose refinement against M
+// does not converge.  tau (host, kc x kc column-major) and *info receive the
+// result; *info is getrf's: 0, or the first exactly singular U(i,i) (1-based,
+// reference order), when M itself is singular.
+int column_pivoted_fallback(lsmsx_ctx *c, AtomDev &a, int ispin, int iie, const double *prel,
+                            int nsc, int nspola, double *tau, int *info) {
+  const int N = nsc * a.nrmat, kc = nsc * a.kkrsz;
+  if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success)
+    return fail(-4, "rocblas_create_handle failed");
+  if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success)
+    return fail(-4, "rocblas_set_stream failed");
+  if (int r = c->fbM.ensure((size_t)N * N)) return r;
+  if (int r = c->fbB.ensure((size_t)N * kc)) return r;
+  if (int r = c->fbpiv.ensure((size_t)N)) return r;
+  if (int r = c->fbinfo.ensure(1)) return r;
+  if (int r = c->fbdesc.ensure(1)) return r;
+  if (int r = assemble_reference_layout(c, a, ispin, iie, prel, nsc, nspola, c->fbM.p,
+                                        c->fbdesc.p))
+    return r;
+  const long t_off = (long)iie * c->blk + ((nspola != nsc) ? (long)kc * kc * ispin : 0);
+  const double2 *t0 = c->tmat.p + t_off + c->ldT * a.store[0];
+  const long nb = (long)N * kc;
+  hipLaunchKernelGGL(k_fallback_rhs, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream,
+                     c->fbB.p, N, kc, t0, kc);
+  if (int r = check_launch("k_fallback_rhs")) return r;
+  auto *A = reinterpret_cast<rocblas_double_complex *>(c->fbM.p);
+  auto *B = reinterpret_cast<rocblas_double_complex *>(c->fbB.p);
+  if (rocsolver_zgetrf(c->blas, N, N, A, N, c->fbpiv.p, c->fbinfo.p) != rocblas_status_success)
+    return fail(-4, "rocsolver_zgetrf failed");
+  if (rocsolver_zgetrs(c->blas, rocblas_operation_none, N, kc, A, N, c->fbpiv.p, B, N) !=
+      rocblas_status_success)
+    return fail(-4, "rocsolver_zgetrs failed");
+  int hinfo = 0;
+  HIPCHK(hipMemcpy2DAsync(tau, sizeof(double2) * kc, c->fbB.p, sizeof(double2) * N,
+                          sizeof(double2) * kc, kc, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&hinfo, c->fbinfo.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drain_profile(c);
+  *info = hinfo;
+  return 0;
+}
+
 }  // namespace
 
 namespace {
@@ -1575,6 +1713,13 @@ int lsmsx_ctx_destroy(lsmsx_ctx *c) {
   c->M.release();
   c->GH.release();
   c->rowperm.release();
+  c->fbM.release();
+  c->fbB.release();
+  c->fbpiv.release();
+  c->fbinfo.release();
+  c->fbdesc.release();
+  if (c->blas) (void)rocblas_destroy_handle(c->blas);
+  c->blas = nullptr;
   c->YB.release();
   c->GP.release();
   c->inv.release();
@@ -1758,42 +1903,8 @@ int lsmsx_build_kkr(lsmsx_ctx *c, int atom_id, int ispin, int iie, const double 
   HIPCHK(hipSetDevice(c->device));
   if (int r = c->M.ensure((size_t)N * N)) return r;
   if (int r = c->desc.ensure(1)) return r;
-  SolveDesc d{};
-  int kc = nsc * a.kkrsz;
-  d.pos = a.d_pos;
-  d.lmax = a.d_lmax;
-  d.store = a.d_store;
-  d.site_off = a.d_off[0][nsc - 1];
-  d.numLIZ = a.numLIZ;
-  d.n_spin_cant = nsc;
-  d.kkrsz = a.kkrsz;
-  d.kc = kc;
-  d.prel_re = prel[0];
-  d.prel_im = prel[1];
-  d.t_off = (long)iie * c->blk + ((nspola != nsc) ? (long)kc * kc * ispin : 0);
-  d.ldt = kc;
-  d.M = c->M.p;
-  d.ld = N;
-  d.Ns = N;
-  d.pad = 0;
-  d.nt = 0;
-  d.A = d.M;
-  HIPCHK(hipMemcpyAsync(c->desc.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
-  {
-    ProfScope p(c, 0);
-    launch_assemble(c, nsc, dim3(a.numLIZ, 1), c->stream, c->desc.p);
-  }
-  if (int r = check_launch("k_assemble")) return r;
-  if (c->rel) {  // the reference's kappa-mu blocks U M'_ij U^T
-    if (nsc != 2 || 2 * a.kkrsz != c->rel_kk2)
-      return fail(-6, "the relativistic store needs n_spin_cant = 2 and the store's lmax");
-    for (int s = 0; s < a.numLIZ; s++)
-      if (a.lmax[s] != a.lmax[0])
-        return fail(-6, "relativistic blocks need every LIZ site at the central lmax");
-    hipLaunchKernelGGL(k_rel_blocks, dim3(a.numLIZ, a.numLIZ), dim3(256), 0, c->stream, c->M.p,
-                       (long)N, c->rel_kk2, c->relU.p);
-    if (int r = check_launch("k_rel_blocks")) return r;
-  }
+  if (int r = assemble_reference_layout(c, a, ispin, iie, prel, nsc, nspola, c->M.p, c->desc.p))
+    return r;
   HIPCHK(hipMemcpy2DAsync(m, sizeof(double2) * ldm, c->M.p, sizeof(double2) * N,
                           sizeof(double2) * N, N, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
