@@ -104,11 +104,12 @@ def test_pivot_guard_on_small_tile_pivot():
         assert rel(tau_adj[b], ref) < 1e-10, (a, ie)
 
 
-def test_unconverged_refinement_is_flagged():
+def test_unconverged_refinement_takes_the_column_pivoted_fallback():
     """A refinement held to one sweep cannot converge: the FP64-refined mode
-    then returns the unrefined FP64 tau00 with info INFO_REFINE_UNCONVERGED,
-    and so does a plain solve that the pivot guard sends there (a guard of
-    1e3 catches every solve); the values are still the plain FP64 ones."""
+    then solves again by LU with partial pivoting over whole columns of the
+    FP64 M (info 0, iters -2), and so does a plain solve that the pivot guard
+    sends there (a guard of 1e3 catches every solve); the values agree with
+    the plain FP64 ones and with LAPACK at 1e-10."""
     s = CF.config_c1()
     ctx = capi.Context(0)
     ctx.load_system(s)
@@ -126,9 +127,10 @@ def test_unconverged_refinement_is_flagged():
     tau_ok, info_ok = ctx.tau00_batched(atoms, iies, prels, 16, precision=REFINED)
     ctx.close()
     assert np.all(info0 == 0) and np.all(info_ok == 0)
-    assert np.all(info_r == capi.INFO_REFINE_UNCONVERGED) and np.all(it_r == -1)
-    assert np.all(info_g == capi.INFO_REFINE_UNCONVERGED)
+    assert np.all(info_r == 0) and np.all(it_r == -2)
+    assert np.all(info_g == 0)
     for n in range(3):
-        assert rel(tau_r[n], plain[n]) < 1e-13
-        assert rel(tau_g[n], plain[n]) < 1e-13
+        ref = lapack_refined_tau00(s, atoms[n], iies[n])
+        assert rel(tau_r[n], ref) < 1e-10
+        assert rel(tau_g[n], ref) < 1e-10
         assert rel(tau_ok[n], plain[n]) < 1e-10

@@ -110,17 +110,22 @@ def test_alloy_batch_with_partly_shared_geometry():
 
 
 def test_shared_fallback_keeps_each_solve_in_its_slot(c2pair):
-    """FP64-refined solves allowed one sweep cannot converge, so each falls
-    back to its plain FP64 result (info -3): with sharing the chunk runs in
-    group order, and every fallback must still come from its own solve --
-    bitwise the plain FP64 result of that solve."""
+    """FP64-refined solves allowed one sweep cannot converge, so each is
+    solved again by the column-pivoted fallback (info 0, iters -2): with
+    sharing the chunk runs in group order, and every fallback must still land
+    in its own slot -- each result meets LAPACK for its own (atom, energy),
+    bitwise the same with sharing on and off."""
     s, ctx = c2pair
     atoms = np.array([0, 1, 1, 0, 0, 1])
     iies = np.array([3, 1, 3, 1, 0, 0])
     prels = np.array([s.prel(i) for i in iies])
-    plain, _ = run(ctx, atoms, iies, prels, capi.PREC_FP64, False)
     ctx.set_refinement(max_sweeps=1, tol=1e-12)
     shared, info = run(ctx, atoms, iies, prels, capi.PREC_FP64_REFINED, True)
+    iters, _ = ctx.last_stats(len(atoms))
+    alone, _ = run(ctx, atoms, iies, prels, capi.PREC_FP64_REFINED, False)
     ctx.set_refinement()
-    assert np.all(info == capi.INFO_REFINE_UNCONVERGED)
-    assert np.array_equal(shared, plain)
+    assert np.all(info == 0) and np.all(iters == -2)
+    assert np.array_equal(shared, alone)
+    for b in range(len(atoms)):
+        ref = lapack_tau00(s, atoms[b], iies[b])
+        assert np.linalg.norm(shared[b] - ref) / np.linalg.norm(ref) < 1e-10
