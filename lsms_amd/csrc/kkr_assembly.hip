@@ -446,12 +446,17 @@ __device__ __forceinline__ void store_block_w32(const AsmDst &o, const float2 (&
     im = __builtin_amdgcn_mfma_f32_16x16x4f32(gg.x, t.y, im, 0, 0, 0);
     im = __builtin_amdgcn_mfma_f32_16x16x4f32(gg.y, t.x, im, 0, 0, 0);
   }
+  // non-temporal stores: W is written once here and read once by the
+  // factorization, far beyond what the caches hold
+  typedef float wv2 __attribute__((ext_vector_type(2)));
   if (li < kkrr) {
     float2 *__restrict__ W = o.W;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int jc = 4 * kq + r;
-      if (jc < kkrc) W[tiled_index(offr + li, offc + jc, o.nt)] = make_float2(-re[r], -im[r]);
+      if (jc < kkrc)
+        __builtin_nontemporal_store(wv2{-re[r], -im[r]},
+                                    (wv2 *)&W[tiled_index(offr + li, offc + jc, o.nt)]);
     }
   }
 }
