@@ -648,7 +648,15 @@ __device__ __forceinline__ void panel_tile(const SolveDesc &d, int step, int ncr
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const V o = ov[bj][bi][r];
-        C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))] = o;
+        if constexpr (CROSS) {
+          // the split path's FP32 factor tile is read again only by the
+          // refinement sweeps after the whole factorization: non-temporal
+          typedef float pv2 __attribute__((ext_vector_type(2)));
+          __builtin_nontemporal_store(pv2{o.x, o.y},
+                                      (pv2 *)&C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))]);
+        } else {
+          C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))] = o;
+        }
         if (planes) Ts[wc + 16 * bj + acc_row<T>(kq, r)][wr + 16 * bi + li] = o;
       }
   if constexpr (sizeof(T) == 4) {
