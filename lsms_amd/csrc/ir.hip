@@ -750,7 +750,13 @@ __global__ __launch_bounds__(kSweepWaves * 64) void k_ir_sweep(const SolveDesc *
   auto load_slot = [&](bool fwd, int u) __attribute__((always_inline)) {
     const GCx<float> *src = cb + kTile * 4 * pk + loff;
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++) rg[u][rb] = ldg<float>(src + 16 * rb);
+    for (int rb = 0; rb < 4; rb++) {
+      // the factor streams through once per sweep: non-temporal, so that the
+      // published y_s / x_s blocks the waves read from L2 stay there
+      typedef float sv2 __attribute__((ext_vector_type(2)));
+      const sv2 v = __builtin_nontemporal_load((const sv2 *)(src + 16 * rb));
+      rg[u][rb] = make_float2(v.x, v.y);
+    }
     if (pok && ++pk == 16) {
       pok = next_item(fwd, pt, ps);
       if (pok) {
