@@ -416,8 +416,14 @@ __device__ __forceinline__ void store_block(const AsmDst &o, const double2 (&g)[
           if (jc < kkrc) {
             const double re = q1[q] - q2[q], im = q3[q] - q1[q] - q2[q];
             const int row = offr + bi * kkrr + li, col = offc + bj * kkrc + jc;
-            M[row + ld * col] = make_double2(-re, -im);
-            if (WRITE_W) W[tiled_index(row, col, o.nt)] = make_float2((float)-re, (float)-im);
+            // non-temporal: M (and W) are written once and read once by the
+            // factorization, far beyond what the caches hold
+            typedef double mv2 __attribute__((ext_vector_type(2)));
+            typedef float wv2f __attribute__((ext_vector_type(2)));
+            __builtin_nontemporal_store(mv2{-re, -im}, (mv2 *)&M[row + ld * col]);
+            if (WRITE_W)
+              __builtin_nontemporal_store(wv2f{(float)-re, (float)-im},
+                                          (wv2f *)&W[tiled_index(row, col, o.nt)]);
           }
         }
       }
