@@ -710,7 +710,11 @@ __device__ __forceinline__ void update_tile(const SolveDesc &d, int ti, int tj, 
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        V c = C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))];
+        // C is read and written once per group: non-temporal, so that the
+        // panels the other tiles read stay in L2
+        typedef T cv2 __attribute__((ext_vector_type(2)));
+        const cv2 c = __builtin_nontemporal_load(
+            (const cv2 *)&C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))]);
         q.p[0][bj][bi][r] = c.x;
         q.p[1][bj][bi][r] = T(0);
         q.p[2][bj][bi][r] = c.x + c.y;
@@ -730,7 +734,9 @@ __device__ __forceinline__ void update_tile(const SolveDesc &d, int ti, int tj, 
         V o;
         o.x = q.p[0][bj][bi][r] - q.p[1][bj][bi][r];
         o.y = q.p[2][bj][bi][r] - q.p[0][bj][bi][r] - q.p[1][bj][bi][r];
-        C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))] = o;
+        typedef T cv2 __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(cv2{o.x, o.y},
+                                    (cv2 *)&C[(16 * bi + li) + ld * (16 * bj + acc_row<T>(kq, r))]);
       }
 }
 
